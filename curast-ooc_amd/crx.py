@@ -1,0 +1,581 @@
+"""Python binding of the gfx950 visibility-buffer rasterizer (ctypes over include/crx.h).
+
+Mirrors the host interface of CuRast's triangle path:
+
+* ``CMesh`` / ``RenderTarget`` / ``HugeTriangle`` are the reference's POD
+  structs (src/kernels/HostDeviceInterface.h:131-143, :196-234, :254-259),
+  byte-compatible with the C-ABI.
+* ``Context.clear_framebuffer``, ``Context.draw_triangles_visbuffer`` and
+  ``Context.resolve_visbuffer`` replace the launches of kernel_clearFramebuffer,
+  drawTrianglesVisbuffer and kernel_resolve_visbuffer_to_colorbuffer2D
+  (src/CuRast_render.h:211-264, :491-512, :580-592).
+* ``Scene`` / ``Mesh`` / ``create_projection_matrix`` mirror Scene, SNTriangles,
+  Mesh and Camera::createProjectionMatrix (src/scene/*.h, src/VKRenderer.h:141).
+
+The native library must be built (``python curast-ooc_amd/build.py``); there is
+no fallback path: importing this module without it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+_LIBDIR = _PKG / "lib"
+
+# ----------------------------------------------------------------------------
+# POD structs (must match include/crx.h byte for byte)
+# ----------------------------------------------------------------------------
+
+
+class Vec2(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Mat4(C.Structure):
+    """Column-major float 4x4 (glm::mat4): element (col c, row r) is m[4*c + r]."""
+    _fields_ = [("m", C.c_float * 16)]
+
+    @classmethod
+    def from_numpy(cls, a) -> "Mat4":
+        a = np.asarray(a, dtype=np.float32)
+        if a.shape == (4, 4):          # row-major math matrix -> column-major storage
+            a = a.T.reshape(16)
+        out = cls()
+        C.memmove(out.m, np.ascontiguousarray(a.reshape(16), dtype=np.float32).ctypes.data, 64)
+        return out
+
+    def to_numpy(self) -> np.ndarray:
+        """Row-major 4x4 (M[r, c])."""
+        return np.frombuffer(bytes(self.m), dtype=np.float32).reshape(4, 4).T.copy()
+
+
+class Box3(C.Structure):
+    _fields_ = [("min", Vec3), ("max", Vec3)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("data", C.c_void_p),
+                ("huffmanTables", C.c_void_p), ("quanttables", C.c_void_p),
+                ("mcuPositions", C.c_void_p), ("handle", C.c_uint32)]
+
+
+class _Instances(C.Structure):
+    _fields_ = [("offset", C.c_int32), ("count", C.c_int32)]
+
+
+class _Impostor(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("colors", C.c_void_p), ("numPoints", C.c_uint32)]
+
+
+class CMesh(C.Structure):
+    """CMesh (HostDeviceInterface.h:196-234), 280 bytes."""
+    _fields_ = [
+        ("numTriangles", C.c_uint32),
+        ("indices", C.c_void_p),
+        ("positions", C.c_void_p),
+        ("cummulativeTriangleCount", C.c_uint64),
+        ("aabb", Box3),
+        ("compressionFactor", Vec3),
+        ("index_min", C.c_uint32),
+        ("bitsPerIndex", C.c_uint32),
+        ("instances", _Instances),
+        ("world", Mat4),
+        ("id", C.c_int32),
+        ("uvs", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("colors", C.c_void_p),
+        ("firstTriangle", C.c_uint32),
+        ("numVertices", C.c_uint32),
+        ("index_max", C.c_uint32),
+        ("address", C.c_uint64),
+        ("texture", Texture),
+        ("isLoaded", C.c_uint8),
+        ("flipTriangles", C.c_uint8),
+        ("compressed", C.c_uint8),
+        ("impostor", _Impostor),
+    ]
+
+
+class RenderTarget(C.Structure):
+    """RenderTarget (HostDeviceInterface.h:131-143), 240 bytes."""
+    _fields_ = [
+        ("framebuffer", C.c_void_p),
+        ("colorbuffer", C.c_void_p),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("view", Mat4),
+        ("viewI", Mat4),
+        ("proj", Mat4),
+        ("cameraPos", Vec3),
+        ("f", C.c_float),
+        ("aspect", C.c_float),
+        ("debug", C.c_uint8),
+    ]
+
+
+class HugeTriangle(C.Structure):
+    _fields_ = [("meshIndex", C.c_int32), ("triangleIndex", C.c_int32),
+                ("tile_x", C.c_int32), ("tile_y", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("small_threshold", C.c_uint32), ("tile_size", C.c_uint32),
+                ("max_nontrivial", C.c_uint32), ("max_huge", C.c_uint32),
+                ("stage1_block", C.c_uint32), ("stage1_waves_per_cu", C.c_uint32),
+                ("stage2_waves_per_cu", C.c_uint32), ("stage3_waves_per_cu", C.c_uint32)]
+
+
+class DrawStats(C.Structure):
+    _fields_ = [("num_triangles", C.c_uint64), ("num_nontrivial", C.c_uint32),
+                ("num_huge", C.c_uint32), ("redraws", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("clear_ms", C.c_float), ("stage1_ms", C.c_float), ("stage2_ms", C.c_float),
+                ("stage3_ms", C.c_float), ("resolve_ms", C.c_float), ("pipeline_ms", C.c_float)]
+
+
+class View(C.Structure):
+    _fields_ = [("view", Mat4), ("proj", Mat4), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [("instanced", C.c_int32), ("frustum_culling", C.c_int32),
+                ("resolve", C.c_int32), ("attribute", C.c_int32)]
+
+
+# DisplayAttribute (HostDeviceInterface.h:277-286)
+DISPLAY_NONE, DISPLAY_TEXTURE, DISPLAY_UV, DISPLAY_NORMAL = 0, 1, 2, 3
+DISPLAY_VERTEX_COLORS, DISPLAY_TRIANGLE_ID, DISPLAY_MESH_ID, DISPLAY_STAGE = 4, 5, 6, 7
+
+CLEAR_PIXEL = 0xFFFFFFF000000000
+
+STATUS = {0: "CRX_OK", 1: "CRX_INVALID_ARG", 2: "CRX_OUT_OF_MEMORY", 3: "CRX_QUEUE_OVERFLOW",
+          4: "CRX_HIP_ERROR", 5: "CRX_NO_DEVICE", 6: "CRX_UNSUPPORTED", 7: "CRX_REJECTED"}
+
+assert C.sizeof(CMesh) == 280, C.sizeof(CMesh)
+assert C.sizeof(RenderTarget) == 240, C.sizeof(RenderTarget)
+
+
+class CrxError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+# ----------------------------------------------------------------------------
+# Library loading (fails loudly: there is no CPU fallback for the product path)
+# ----------------------------------------------------------------------------
+
+_lib = None
+_gen = None
+
+_VP = C.c_void_p
+_SIGS = {
+    "crx_abi_version": (C.c_int, []),
+    "crx_status_string": (C.c_char_p, [C.c_int]),
+    "crx_last_error": (C.c_char_p, []),
+    "crx_tuning_default": (None, [C.POINTER(Tuning)]),
+    "crx_context_create": (C.c_int, [C.c_int, C.POINTER(Tuning), C.POINTER(_VP)]),
+    "crx_context_destroy": (C.c_int, [_VP]),
+    "crx_context_set_tuning": (C.c_int, [_VP, C.POINTER(Tuning)]),
+    "crx_context_get_tuning": (C.c_int, [_VP, C.POINTER(Tuning)]),
+    "crx_context_num_cus": (C.c_int, [_VP, C.POINTER(C.c_int)]),
+    "crx_context_enable_timing": (C.c_int, [_VP, C.c_int]),
+    "crx_context_timings": (C.c_int, [_VP, C.POINTER(Timings)]),
+    "crx_device_alloc": (C.c_int, [_VP, C.c_size_t, C.POINTER(_VP)]),
+    "crx_device_free": (C.c_int, [_VP, _VP]),
+    "crx_memcpy_htod": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
+    "crx_memcpy_dtoh": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
+    "crx_stream_synchronize": (C.c_int, [_VP, _VP]),
+    "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
+    "crx_draw_triangles_visbuffer": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(CMesh),
+                                               C.c_uint32, C.POINTER(Mat4), C.POINTER(RenderTarget),
+                                               C.c_int, _VP, C.POINTER(DrawStats)]),
+    "crx_resolve_visbuffer": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(C.c_uint64),
+                                        C.POINTER(RenderTarget), C.c_int, _VP]),
+    "crx_camera_projection": (None, [C.c_float, C.c_float, C.c_float, C.POINTER(Mat4)]),
+    "crx_camera_look_at": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                  C.POINTER(Mat4)]),
+    "crx_mat4_inverse": (None, [C.POINTER(Mat4), C.POINTER(Mat4)]),
+    "crx_mat4_mul": (None, [C.POINTER(Mat4), C.POINTER(Mat4), C.POINTER(Mat4)]),
+    "crx_mesh_create": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, C.POINTER(_VP)]),
+    "crx_mesh_create_device": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, C.POINTER(Box3),
+                                         C.POINTER(_VP)]),
+    "crx_mesh_destroy": (C.c_int, [_VP]),
+    "crx_mesh_info": (C.c_int, [_VP, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(Box3)]),
+    "crx_mesh_buffers": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
+    "crx_scene_create": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "crx_scene_destroy": (C.c_int, [_VP]),
+    "crx_scene_add_node": (C.c_int, [_VP, _VP, C.POINTER(Mat4), C.POINTER(C.c_uint32)]),
+    "crx_scene_set_transform": (C.c_int, [_VP, C.c_uint32, C.POINTER(Mat4)]),
+    "crx_scene_num_nodes": (C.c_int, [_VP, C.POINTER(C.c_uint32)]),
+    "crx_render_settings_default": (None, [C.POINTER(RenderSettings)]),
+    "crx_render": (C.c_int, [_VP, _VP, C.POINTER(View), C.POINTER(RenderSettings), _VP, _VP, _VP,
+                             C.POINTER(DrawStats)]),
+}
+
+_GEN_SIGS = {
+    "crx_gen_splitmix64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "crx_gen_hash64": (C.c_uint64, [_VP, C.c_size_t]),
+    "crx_gen_sphere": (None, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint64, _VP, _VP, _VP]),
+    "crx_gen_heightfield": (None, [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint64,
+                                   _VP, _VP, _VP, C.c_int]),
+    "crx_gen_heightfield_indices": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                 C.c_uint32, _VP]),
+    "crx_gen_instances": (None, [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint64, C.POINTER(Mat4)]),
+}
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("CRX_LIBRARY", str(_LIBDIR / "libcrx.so")))
+
+
+def lib():
+    """The loaded libcrx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not path.exists():
+            raise ImportError(f"{path} is missing: build it with `python curast-ooc_amd/build.py`")
+        handle = C.CDLL(str(path))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def gen():
+    """The loaded libcrx_gen.so (synthetic workload generators)."""
+    global _gen
+    if _gen is None:
+        path = _LIBDIR / "libcrx_gen.so"
+        if not path.exists():
+            raise ImportError(f"{path} is missing: build it with `python curast-ooc_amd/build.py`")
+        handle = C.CDLL(str(path))
+        for name, (res, args) in _GEN_SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _gen = handle
+    return _gen
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise CrxError(status, lib().crx_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+# ----------------------------------------------------------------------------
+# Context, device memory
+# ----------------------------------------------------------------------------
+
+
+def default_tuning() -> Tuning:
+    t = Tuning()
+    lib().crx_tuning_default(C.byref(t))
+    return t
+
+
+class DeviceBuffer:
+    """Device allocation owned by a context (freed on close / garbage collection)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().crx_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, array: np.ndarray, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(array)
+        if a.nbytes > self.nbytes:
+            raise ValueError("upload larger than the buffer")
+        _check(lib().crx_memcpy_htod(self.ctx.handle, C.c_void_p(self.ptr), _ptr(a), a.nbytes, stream))
+        return self
+
+    def download(self, dtype, count: int | None = None, stream=None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dtype)
+        _check(lib().crx_memcpy_dtoh(self.ctx.handle, _ptr(out), C.c_void_p(self.ptr), out.nbytes, stream))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ctx.handle:
+            lib().crx_device_free(self.ctx.handle, C.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One rasterizer context on one device (queues, counters, staging)."""
+
+    def __init__(self, device: int = 0, tuning: Tuning | None = None):
+        h = C.c_void_p()
+        _check(lib().crx_context_create(device, C.byref(tuning) if tuning is not None else None, C.byref(h)))
+        self.handle = h.value
+        self._buffers: list[DeviceBuffer] = []
+
+    def close(self) -> None:
+        if self.handle:
+            for b in self._buffers:
+                b.free()
+            self._buffers.clear()
+            lib().crx_context_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().crx_context_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    @tuning.setter
+    def tuning(self, t: Tuning) -> None:
+        _check(lib().crx_context_set_tuning(self.handle, C.byref(t)))
+
+    @property
+    def num_cus(self) -> int:
+        n = C.c_int()
+        _check(lib().crx_context_num_cus(self.handle, C.byref(n)))
+        return n.value
+
+    def enable_timing(self, on: bool = True) -> None:
+        _check(lib().crx_context_enable_timing(self.handle, int(on)))
+
+    def timings(self) -> Timings:
+        t = Timings()
+        _check(lib().crx_context_timings(self.handle, C.byref(t)))
+        return t
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        b = DeviceBuffer(self, nbytes)
+        self._buffers.append(b)
+        return b
+
+    def to_device(self, array: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(array)
+        return self.alloc(max(a.nbytes, 1)).upload(a)
+
+    def synchronize(self, stream=None) -> None:
+        _check(lib().crx_stream_synchronize(self.handle, stream))
+
+    # --- kernel-level entry points ------------------------------------------------
+
+    def clear_framebuffer(self, fb: DeviceBuffer | int, num_pixels: int, stream=None) -> None:
+        ptr = fb.ptr if isinstance(fb, DeviceBuffer) else fb
+        _check(lib().crx_clear_framebuffer(C.c_void_p(ptr), num_pixels, stream))
+
+    def draw_triangles_visbuffer(self, meshes, instances, transforms, target: RenderTarget,
+                                 instanced: bool, stream=None) -> DrawStats:
+        """drawTrianglesVisbuffer (CuRast_render.h:211-264).
+
+        meshes / instances: ctypes arrays (or lists) of CMesh holding device
+        pointers; transforms: one world-to-view Mat4 per instance."""
+        m_arr = _as_array(CMesh, meshes)
+        i_arr = _as_array(CMesh, instances)
+        t_arr = _as_array(Mat4, transforms)
+        st = DrawStats()
+        _check(lib().crx_draw_triangles_visbuffer(
+            self.handle, m_arr, len(m_arr) if m_arr is not None else 0, i_arr,
+            len(i_arr) if i_arr is not None else 0, t_arr, C.byref(target), int(bool(instanced)),
+            stream, C.byref(st)))
+        return st
+
+    def resolve_visbuffer(self, instances, prefix, target: RenderTarget,
+                          attribute: int = DISPLAY_VERTEX_COLORS, stream=None) -> None:
+        i_arr = _as_array(CMesh, instances)
+        pre = np.ascontiguousarray(np.asarray(prefix, dtype=np.uint64))
+        n = len(i_arr) if i_arr is not None else 0
+        _check(lib().crx_resolve_visbuffer(self.handle, i_arr, n,
+                                           pre.ctypes.data_as(C.POINTER(C.c_uint64)) if n else None,
+                                           C.byref(target), int(attribute), stream))
+
+
+def _as_array(ctype, items):
+    if items is None:
+        return None
+    if isinstance(items, C.Array):
+        return items
+    items = list(items)
+    if not items:
+        return None
+    arr = (ctype * len(items))()
+    for i, it in enumerate(items):
+        arr[i] = it
+    return arr
+
+
+# ----------------------------------------------------------------------------
+# Camera / matrices (float, GLM order)
+# ----------------------------------------------------------------------------
+
+
+def create_projection_matrix(near: float, fovy_rad: float, aspect: float) -> Mat4:
+    """Camera::createProjectionMatrix (VKRenderer.h:141-148)."""
+    m = Mat4()
+    lib().crx_camera_projection(near, fovy_rad, aspect, C.byref(m))
+    return m
+
+
+def look_at(eye, center, up=(0.0, 1.0, 0.0)) -> Mat4:
+    e = (C.c_float * 3)(*eye)
+    c = (C.c_float * 3)(*center)
+    u = (C.c_float * 3)(*up)
+    m = Mat4()
+    lib().crx_camera_look_at(e, c, u, C.byref(m))
+    return m
+
+
+def mat4_inverse(m: Mat4) -> Mat4:
+    out = Mat4()
+    lib().crx_mat4_inverse(C.byref(m), C.byref(out))
+    return out
+
+
+def mat4_mul(a: Mat4, b: Mat4) -> Mat4:
+    out = Mat4()
+    lib().crx_mat4_mul(C.byref(a), C.byref(b), C.byref(out))
+    return out
+
+
+def identity() -> Mat4:
+    return Mat4.from_numpy(np.eye(4, dtype=np.float32))
+
+
+def make_target(fb, cb, width: int, height: int, view: Mat4, proj: Mat4) -> RenderTarget:
+    t = RenderTarget()
+    t.framebuffer = fb.ptr if isinstance(fb, DeviceBuffer) else fb
+    t.colorbuffer = (cb.ptr if isinstance(cb, DeviceBuffer) else cb) if cb is not None else None
+    t.width = width
+    t.height = height
+    t.view = view
+    t.viewI = mat4_inverse(view)
+    t.proj = proj
+    t.f = proj.m[5]
+    t.aspect = float(np.float32(width) / np.float32(height))
+    return t
+
+
+# ----------------------------------------------------------------------------
+# Scene API (Scene / SNTriangles / Mesh)
+# ----------------------------------------------------------------------------
+
+
+class Mesh:
+    """Mesh (Mesh.h:4-25): positions (V,3) float32, indices (T,3) uint32 or None,
+    colors (V,) uint32 RGBA8 or None; copied to device memory owned by the mesh."""
+
+    def __init__(self, ctx: Context, positions: np.ndarray, indices: np.ndarray | None,
+                 colors: np.ndarray | None = None, num_triangles: int | None = None):
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        idx = None if indices is None else np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        col = None if colors is None else np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+        nt = (len(idx) // 3 if idx is not None else len(pos) // 3) if num_triangles is None else num_triangles
+        h = C.c_void_p()
+        _check(lib().crx_mesh_create(ctx.handle, _ptr(pos), len(pos), _ptr(idx), nt, _ptr(col), C.byref(h)))
+        self.ctx = ctx
+        self.handle = h.value
+        self.num_triangles = nt
+        self.num_vertices = len(pos)
+
+    def buffers(self):
+        p, i, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().crx_mesh_buffers(self.handle, C.byref(p), C.byref(i), C.byref(c)))
+        return p.value, i.value, c.value
+
+    def aabb(self) -> Box3:
+        b = Box3()
+        _check(lib().crx_mesh_info(self.handle, None, None, C.byref(b)))
+        return b
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crx_mesh_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Scene graph of SNTriangles nodes (flat: each node carries its global transform)."""
+
+    def __init__(self, ctx: Context):
+        h = C.c_void_p()
+        _check(lib().crx_scene_create(ctx.handle, C.byref(h)))
+        self.ctx = ctx
+        self.handle = h.value
+        self.nodes: list[tuple[Mesh, Mat4]] = []
+
+    def add(self, mesh: Mesh, world: Mat4 | None = None) -> int:
+        w = identity() if world is None else world
+        nid = C.c_uint32()
+        _check(lib().crx_scene_add_node(self.handle, mesh.handle, C.byref(w), C.byref(nid)))
+        self.nodes.append((mesh, w))
+        return nid.value
+
+    def set_transform(self, node_id: int, world: Mat4) -> None:
+        _check(lib().crx_scene_set_transform(self.handle, node_id, C.byref(world)))
+        self.nodes[node_id] = (self.nodes[node_id][0], world)
+
+    def render(self, view: View, fb: DeviceBuffer, cb: DeviceBuffer | None,
+               settings: RenderSettings | None = None, stream=None) -> DrawStats:
+        """CuRast::draw for one view (CuRast_render.h:266-592)."""
+        st = DrawStats()
+        s = settings if settings is not None else render_settings()
+        _check(lib().crx_render(self.ctx.handle, self.handle, C.byref(view), C.byref(s),
+                                C.c_void_p(fb.ptr), C.c_void_p(cb.ptr) if cb is not None else None,
+                                stream, C.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crx_scene_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+def render_settings(**kw) -> RenderSettings:
+    s = RenderSettings()
+    lib().crx_render_settings_default(C.byref(s))
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def make_view(view: Mat4, proj: Mat4, width: int, height: int) -> View:
+    v = View()
+    v.view = view
+    v.proj = proj
+    v.width = width
+    v.height = height
+    return v

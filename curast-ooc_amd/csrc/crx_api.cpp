@@ -1,0 +1,992 @@
+// crx_api.cpp — host side of the C-ABI (include/crx.h).
+//
+// Builds the compact device records of crx_device.h from the reference's CMesh
+// arrays, owns queues / counters / staging per context, launches the kernels of
+// crx_kernels.hip and restates the frame preparation of CuRast::draw
+// (CuRast_render.h:266-519) for the scene API.  Host float arithmetic that
+// feeds the kernels (view*world, inverse(view), mirror flags) follows GLM's
+// evaluation order and is compiled without contraction, so it produces the
+// same bits as the device or the reference's GLM would under IEEE rules.
+#include "../../include/crx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "crx_device.h"
+#include "crx_internal.h"
+
+using namespace crx;
+
+namespace {
+
+thread_local std::string g_error;
+
+crx_status fail(crx_status s, const std::string& msg) {
+    g_error = msg;
+    return s;
+}
+
+#define CRX_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(CRX_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CRX_TRY(expr)                      \
+    do {                                   \
+        crx_status s_ = (expr);            \
+        if (s_ != CRX_OK) return s_;       \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t capacity = 0;
+    crx_status reserve(size_t bytes) {
+        if (bytes <= capacity) return CRX_OK;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            ptr = nullptr;
+            capacity = 0;
+            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipFree: ") + hipGetErrorString(e));
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return fail(CRX_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
+        }
+        capacity = want;
+        return CRX_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        capacity = 0;
+    }
+};
+
+// Pinned host staging with an event guarding reuse: the buffer is rewritten only
+// after the copy that last read it has completed.
+struct Staging {
+    void* ptr = nullptr;
+    size_t capacity = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    crx_status acquire(size_t bytes) {
+        if (pending) {
+            hipError_t e = hipEventSynchronize(done);
+            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+            pending = false;
+        }
+        if (!done) {
+            hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+        if (bytes <= capacity) return CRX_OK;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        capacity = 0;
+        size_t want = std::max<size_t>(bytes * 3 / 2, 65536);
+        hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return fail(CRX_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        capacity = want;
+        return CRX_OK;
+    }
+    crx_status mark(hipStream_t stream) {
+        hipError_t e = hipEventRecord(done, stream);
+        if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventRecord: ") + hipGetErrorString(e));
+        pending = true;
+        return CRX_OK;
+    }
+    void release() {
+        if (pending && done) (void)hipEventSynchronize(done);
+        if (ptr) (void)hipHostFree(ptr);
+        if (done) (void)hipEventDestroy(done);
+        ptr = nullptr;
+        done = nullptr;
+        capacity = 0;
+        pending = false;
+    }
+};
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+enum TimerSlot { T_CLEAR = 0, T_S1, T_S2, T_S3, T_S3_END, T_RESOLVE, T_RESOLVE_END, T_COUNT };
+
+}  // namespace
+
+struct crx_context {
+    int device = 0;
+    int numCUs = 0;
+    crx_tuning tuning{};
+    uint32_t* ctr = nullptr;          // device counters (CTR_COUNT)
+    uint32_t* ctrHost = nullptr;      // pinned readback
+    DeviceBuffer nontrivial;          // uint64 entries
+    DeviceBuffer huge;                // int4 entries
+    uint32_t ntCap = 0, hugeCap = 0;
+    DeviceBuffer drawMeta;            // items | instances | transforms
+    DeviceBuffer resolveMeta;         // instances | prefix
+    Staging drawStaging, resolveStaging;
+    bool timing = false;
+    hipEvent_t ev[T_COUNT] = {};
+    bool recorded[T_COUNT] = {};
+    uint64_t meshSeq = 0;
+};
+
+struct crx_mesh {
+    crx_context* ctx = nullptr;
+    float* positions = nullptr;
+    uint32_t* indices = nullptr;
+    uint32_t* colors = nullptr;
+    bool owned = false;
+    uint32_t numVertices = 0;
+    uint32_t numTriangles = 0;
+    crx_box3 aabb{};
+    uint64_t seq = 0;
+};
+
+struct crx_scene {
+    crx_context* ctx = nullptr;
+    struct Node {
+        crx_mesh* mesh;
+        crx_mat4 world;
+    };
+    std::vector<Node> nodes;
+    // frame-preparation scratch, kept across frames like the reference's statics
+    std::vector<crx_cmesh> unique, instances;
+    std::vector<crx_mat4> transforms;
+    std::vector<uint64_t> prefix;
+};
+
+namespace {
+
+// ---- host math in GLM 0.9.9 order --------------------------------------------
+
+// tmat4x4 * tmat4x4, type_mat4x4.inl:601-616
+crx_mat4 mat_mul(const crx_mat4& A, const crx_mat4& B) {
+    crx_mat4 R;
+    for (int j = 0; j < 4; j++) {
+        const float b0 = B.m[4 * j], b1 = B.m[4 * j + 1], b2 = B.m[4 * j + 2], b3 = B.m[4 * j + 3];
+        for (int r = 0; r < 4; r++) {
+            float acc = A.m[r] * b0 + A.m[4 + r] * b1;
+            acc = acc + A.m[8 + r] * b2;
+            acc = acc + A.m[12 + r] * b3;
+            R.m[4 * j + r] = acc;
+        }
+    }
+    return R;
+}
+
+// compute_inverse<tmat4x4>, func_matrix.inl:298-395
+crx_mat4 mat_inverse(const crx_mat4& M) {
+#define E(c, r) M.m[4 * (c) + (r)]
+    const float c00 = E(2,2) * E(3,3) - E(3,2) * E(2,3);
+    const float c02 = E(1,2) * E(3,3) - E(3,2) * E(1,3);
+    const float c03 = E(1,2) * E(2,3) - E(2,2) * E(1,3);
+    const float c04 = E(2,1) * E(3,3) - E(3,1) * E(2,3);
+    const float c06 = E(1,1) * E(3,3) - E(3,1) * E(1,3);
+    const float c07 = E(1,1) * E(2,3) - E(2,1) * E(1,3);
+    const float c08 = E(2,1) * E(3,2) - E(3,1) * E(2,2);
+    const float c10 = E(1,1) * E(3,2) - E(3,1) * E(1,2);
+    const float c11 = E(1,1) * E(2,2) - E(2,1) * E(1,2);
+    const float c12 = E(2,0) * E(3,3) - E(3,0) * E(2,3);
+    const float c14 = E(1,0) * E(3,3) - E(3,0) * E(1,3);
+    const float c15 = E(1,0) * E(2,3) - E(2,0) * E(1,3);
+    const float c16 = E(2,0) * E(3,2) - E(3,0) * E(2,2);
+    const float c18 = E(1,0) * E(3,2) - E(3,0) * E(1,2);
+    const float c19 = E(1,0) * E(2,2) - E(2,0) * E(1,2);
+    const float c20 = E(2,0) * E(3,1) - E(3,0) * E(2,1);
+    const float c22 = E(1,0) * E(3,1) - E(3,0) * E(1,1);
+    const float c23 = E(1,0) * E(2,1) - E(2,0) * E(1,1);
+    const float fac[6][4] = {{c00, c00, c02, c03}, {c04, c04, c06, c07}, {c08, c08, c10, c11},
+                             {c12, c12, c14, c15}, {c16, c16, c18, c19}, {c20, c20, c22, c23}};
+    const float vec[4][4] = {{E(1,0), E(0,0), E(0,0), E(0,0)}, {E(1,1), E(0,1), E(0,1), E(0,1)},
+                             {E(1,2), E(0,2), E(0,2), E(0,2)}, {E(1,3), E(0,3), E(0,3), E(0,3)}};
+    crx_mat4 I;
+    for (int i = 0; i < 4; i++) {
+        const float inv0 = (vec[1][i] * fac[0][i] - vec[2][i] * fac[1][i]) + vec[3][i] * fac[2][i];
+        const float inv1 = (vec[0][i] * fac[0][i] - vec[2][i] * fac[3][i]) + vec[3][i] * fac[4][i];
+        const float inv2 = (vec[0][i] * fac[1][i] - vec[1][i] * fac[3][i]) + vec[3][i] * fac[5][i];
+        const float inv3 = (vec[0][i] * fac[2][i] - vec[1][i] * fac[4][i]) + vec[2][i] * fac[5][i];
+        const float sa = (i & 1) ? -1.0f : 1.0f;   // SignA = (+,-,+,-)
+        const float sb = -sa;                       // SignB = (-,+,-,+)
+        I.m[i] = inv0 * sa;
+        I.m[4 + i] = inv1 * sb;
+        I.m[8 + i] = inv2 * sa;
+        I.m[12 + i] = inv3 * sb;
+    }
+    const float d0 = E(0,0) * I.m[0], d1 = E(0,1) * I.m[4], d2 = E(0,2) * I.m[8], d3 = E(0,3) * I.m[12];
+    const float det = (d0 + d1) + (d2 + d3);
+    const float oneOver = 1.0f / det;
+    for (int i = 0; i < 16; i++) I.m[i] = I.m[i] * oneOver;
+#undef E
+    return I;
+}
+
+// det(worldView[0..2]) = dot(cross(c0, c1), c2), triangles_visbuffer.cu:179-183
+bool mirror_flag(const crx_mat4& m) {
+    const f3 c0{m.m[0], m.m[1], m.m[2]};
+    const f3 c1{m.m[4], m.m[5], m.m[6]};
+    const f3 c2{m.m[8], m.m[9], m.m[10]};
+    return dot3(cross3(c0, c1), c2) < 0.0f;
+}
+
+crx_status check_tuning(const crx_tuning& t) {
+    if (t.small_threshold > 1u << 24) return fail(CRX_INVALID_ARG, "tuning.small_threshold too large");
+    if (t.tile_size < 16 || t.tile_size > 256 || (t.tile_size % 16) != 0)
+        return fail(CRX_INVALID_ARG, "tuning.tile_size must be a multiple of 16 in [16, 256]");
+    if (t.tile_size > 64 && (t.tile_size % 64) != 0)
+        return fail(CRX_INVALID_ARG, "tuning.tile_size above 64 must be a multiple of 64");
+    if (t.tile_size < 64 && (64 % t.tile_size) != 0)
+        return fail(CRX_INVALID_ARG, "tuning.tile_size below 64 must divide 64");
+    const uint32_t b = t.stage1_block;
+    if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
+        return fail(CRX_INVALID_ARG, "tuning.stage1_block must be 64, 128, 256, 512 or 1024");
+    if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
+    if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
+        t.stage3_waves_per_cu > 64 || t.stage1_waves_per_cu > 64)
+        return fail(CRX_INVALID_ARG, "tuning waves per CU out of range");
+    return CRX_OK;
+}
+
+crx_status check_target(const crx_render_target* t) {
+    if (!t) return fail(CRX_INVALID_ARG, "target is NULL");
+    if (!t->framebuffer) return fail(CRX_INVALID_ARG, "target->framebuffer is NULL");
+    if (t->width <= 0 || t->height <= 0) return fail(CRX_INVALID_ARG, "target size must be positive");
+    if ((int64_t)t->width * (int64_t)t->height > 0x7fffffffll)
+        return fail(CRX_INVALID_ARG, "target has more than 2^31-1 pixels");
+    return CRX_OK;
+}
+
+crx_status check_mesh(const crx_cmesh& m, const char* what, uint32_t k) {
+    if (m.compressed) return fail(CRX_UNSUPPORTED, std::string(what) + "[" + std::to_string(k) + "] is compressed (IX_PU16 is not built)");
+    if (m.numTriangles > 1431655765u)
+        return fail(CRX_INVALID_ARG, std::string(what) + "[" + std::to_string(k) + "] has more than 1431655765 triangles");
+    if (m.numTriangles > 0 && !m.positions)
+        return fail(CRX_INVALID_ARG, std::string(what) + "[" + std::to_string(k) + "] has no positions");
+    return CRX_OK;
+}
+
+InstanceRec make_instance(const crx_cmesh& m) {
+    InstanceRec r;
+    r.indices = m.indices;
+    r.positions = reinterpret_cast<const float*>(m.positions);
+    r.colors = m.colors;
+    r.normals = reinterpret_cast<const float*>(m.normals);
+    r.uvs = reinterpret_cast<const float*>(m.uvs);
+    r.cumm = m.cummulativeTriangleCount;
+    r.numTriangles = m.numTriangles;
+    r.transformIndex = (uint32_t)m.instances.offset;
+    r.id = m.id;
+    r.isLoaded = m.isLoaded ? 1u : 0u;
+    std::memcpy(r.world, m.world.m, sizeof(r.world));
+    return r;
+}
+
+crx_status ensure_queues(crx_context* ctx, uint32_t nt, uint32_t hg) {
+    if (nt > ctx->ntCap) {
+        CRX_TRY(ctx->nontrivial.reserve((size_t)nt * sizeof(uint64_t)));
+        ctx->ntCap = nt;
+    }
+    if (hg > ctx->hugeCap) {
+        CRX_TRY(ctx->huge.reserve((size_t)hg * sizeof(int4)));
+        ctx->hugeCap = hg;
+    }
+    return CRX_OK;
+}
+
+void record(crx_context* ctx, int slot, hipStream_t stream) {
+    if (!ctx->timing) return;
+    if (hipEventRecord(ctx->ev[slot], stream) == hipSuccess) ctx->recorded[slot] = true;
+}
+
+uint32_t grid_for(const crx_context* ctx, uint32_t wavesPerCU, uint32_t wavesPerBlock) {
+    uint64_t g = (uint64_t)ctx->numCUs * wavesPerCU / wavesPerBlock;
+    if (g < 8) g = 8;
+    return (uint32_t)g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int crx_abi_version(void) { return CRX_ABI_VERSION; }
+
+const char* crx_status_string(crx_status s) {
+    switch (s) {
+        case CRX_OK: return "CRX_OK";
+        case CRX_INVALID_ARG: return "CRX_INVALID_ARG";
+        case CRX_OUT_OF_MEMORY: return "CRX_OUT_OF_MEMORY";
+        case CRX_QUEUE_OVERFLOW: return "CRX_QUEUE_OVERFLOW";
+        case CRX_HIP_ERROR: return "CRX_HIP_ERROR";
+        case CRX_NO_DEVICE: return "CRX_NO_DEVICE";
+        case CRX_UNSUPPORTED: return "CRX_UNSUPPORTED";
+        case CRX_REJECTED: return "CRX_REJECTED";
+    }
+    return "CRX_UNKNOWN";
+}
+
+const char* crx_last_error(void) { return g_error.c_str(); }
+
+void crx_tuning_default(crx_tuning* out) {
+    if (!out) return;
+    out->small_threshold = 128;       // THRESHOLD_SMALL
+    out->tile_size = 64;              // TILE_SIZE
+    out->max_nontrivial = 5000000;    // MAX_NONTRIVIAL_TRIANGLES
+    out->max_huge = 5000000;          // MAX_HUGE_TRIANGLES
+    out->stage1_block = 256;          // TRIANGLES_PER_SWEEP
+    out->stage1_waves_per_cu = 32;
+    out->stage2_waves_per_cu = 16;
+    out->stage3_waves_per_cu = 16;
+}
+
+crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {
+    if (!out) return fail(CRX_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(CRX_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(CRX_INVALID_ARG, "device index out of range");
+    crx_tuning t;
+    crx_tuning_default(&t);
+    if (tuning) t = *tuning;
+    CRX_TRY(check_tuning(t));
+    CRX_HIP(hipSetDevice(device));
+    int cus = 0;
+    CRX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    crx_context* ctx = new crx_context();
+    ctx->device = device;
+    ctx->numCUs = cus;
+    ctx->tuning = t;
+    hipError_t e = hipMalloc(&ctx->ctr, CTR_COUNT * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(ctx->ctr, 0, CTR_COUNT * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->ctrHost, CTR_COUNT * sizeof(uint32_t), hipHostMallocDefault);
+    for (int i = 0; e == hipSuccess && i < T_COUNT; i++) e = hipEventCreate(&ctx->ev[i]);
+    if (e != hipSuccess) {
+        crx_context_destroy(ctx);
+        return fail(CRX_HIP_ERROR, std::string("context allocation: ") + hipGetErrorString(e));
+    }
+    *out = ctx;
+    return CRX_OK;
+}
+
+crx_status crx_context_destroy(crx_context* ctx) {
+    if (!ctx) return CRX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    ctx->drawStaging.release();
+    ctx->resolveStaging.release();
+    ctx->nontrivial.release();
+    ctx->huge.release();
+    ctx->drawMeta.release();
+    ctx->resolveMeta.release();
+    if (ctx->ctr) (void)hipFree(ctx->ctr);
+    if (ctx->ctrHost) (void)hipHostFree(ctx->ctrHost);
+    for (int i = 0; i < T_COUNT; i++)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    delete ctx;
+    return CRX_OK;
+}
+
+crx_status crx_context_set_tuning(crx_context* ctx, const crx_tuning* tuning) {
+    if (!ctx || !tuning) return fail(CRX_INVALID_ARG, "NULL argument");
+    CRX_TRY(check_tuning(*tuning));
+    ctx->tuning = *tuning;
+    return CRX_OK;
+}
+
+crx_status crx_context_get_tuning(const crx_context* ctx, crx_tuning* out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = ctx->tuning;
+    return CRX_OK;
+}
+
+crx_status crx_context_num_cus(const crx_context* ctx, int* out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = ctx->numCUs;
+    return CRX_OK;
+}
+
+crx_status crx_context_enable_timing(crx_context* ctx, int enable) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    ctx->timing = enable != 0;
+    for (bool& r : ctx->recorded) r = false;
+    return CRX_OK;
+}
+
+crx_status crx_context_timings(crx_context* ctx, crx_timings* out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    auto span = [&](int a, int b) -> float {
+        if (!ctx->recorded[a] || !ctx->recorded[b]) return -1.0f;
+        if (hipEventSynchronize(ctx->ev[b]) != hipSuccess) return -1.0f;
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return -1.0f;
+        return ms;
+    };
+    out->clear_ms = span(T_CLEAR, T_S1);
+    out->stage1_ms = span(T_S1, T_S2);
+    out->stage2_ms = span(T_S2, T_S3);
+    out->stage3_ms = span(T_S3, T_S3_END);
+    out->pipeline_ms = span(T_S1, T_S3_END);
+    out->resolve_ms = span(T_RESOLVE, T_RESOLVE_END);
+    return CRX_OK;
+}
+
+crx_status crx_device_alloc(crx_context* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    CRX_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(CRX_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return CRX_OK;
+}
+
+crx_status crx_device_free(crx_context* ctx, void* ptr) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    if (ptr) CRX_HIP(hipFree(ptr));
+    return CRX_OK;
+}
+
+crx_status crx_memcpy_htod(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (bytes == 0) return CRX_OK;
+    CRX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    CRX_HIP(hipStreamSynchronize(as_stream(stream)));
+    return CRX_OK;
+}
+
+crx_status crx_memcpy_dtoh(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (bytes == 0) return CRX_OK;
+    CRX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    CRX_HIP(hipStreamSynchronize(as_stream(stream)));
+    return CRX_OK;
+}
+
+crx_status crx_stream_synchronize(crx_context* ctx, void* stream) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    CRX_HIP(hipStreamSynchronize(as_stream(stream)));
+    return CRX_OK;
+}
+
+crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void* stream) {
+    if (!framebuffer && numPixels) return fail(CRX_INVALID_ARG, "framebuffer is NULL");
+    if (numPixels == 0) return CRX_OK;
+    int dev = 0, cus = 0;
+    CRX_HIP(hipGetDevice(&dev));
+    CRX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    CRX_HIP(launch_clear(framebuffer, numPixels, nullptr, cus, as_stream(stream)));
+    return CRX_OK;
+}
+
+crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes,
+                                        const crx_cmesh* instances, uint32_t numInstances,
+                                        const crx_mat4* transforms, const crx_render_target* target,
+                                        int instanced, void* stream_, crx_draw_stats* stats) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    CRX_TRY(check_target(target));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    // drawTrianglesVisbuffer returns early for an empty instance list (CuRast_render.h:231)
+    if (numInstances == 0) return CRX_OK;
+    if (!instances || !transforms) return fail(CRX_INVALID_ARG, "instances / transforms is NULL");
+    if (instanced && (!meshes || numMeshes == 0)) return fail(CRX_INVALID_ARG, "instanced draw without meshes");
+    hipStream_t stream = as_stream(stream_);
+    CRX_HIP(hipSetDevice(ctx->device));
+    const crx_tuning& tu = ctx->tuning;
+    const uint32_t block = tu.stage1_block;
+    uint32_t batchShift = 0;
+    while ((1u << batchShift) < block) batchShift++;
+
+    // ---- host records -----------------------------------------------------
+    const crx_cmesh* list = instanced ? meshes : instances;
+    const uint32_t numItems = instanced ? numMeshes : numInstances;
+    std::vector<DrawItem> items(numItems);
+    uint64_t batches = 0, tris = 0;
+    for (uint32_t k = 0; k < numItems; k++) {
+        const crx_cmesh& m = list[k];
+        CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));
+        const int64_t off = m.instances.offset;
+        const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
+        if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
+            return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
+        DrawItem& it = items[k];
+        it.indices = m.indices;
+        it.positions = reinterpret_cast<const float*>(m.positions);
+        it.cumm = m.cummulativeTriangleCount;
+        it.numTriangles = m.numTriangles;
+        it.instOffset = (uint32_t)off;
+        it.instCount = (uint32_t)cnt;
+        it.firstBatch = (uint32_t)batches;
+        batches += ((uint64_t)m.numTriangles + block - 1) >> batchShift;
+        tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
+        if (batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
+    }
+    std::vector<InstanceRec> inst(numInstances);
+    std::vector<Transform> xf(numInstances);
+    for (uint32_t i = 0; i < numInstances; i++) {
+        CRX_TRY(check_mesh(instances[i], "instances", i));
+        const int64_t ti = instances[i].instances.offset;
+        if (ti < 0 || ti >= (int64_t)numInstances)
+            return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
+        inst[i] = make_instance(instances[i]);
+        std::memcpy(xf[i].m, transforms[i].m, sizeof(xf[i].m));
+        xf[i].flip = mirror_flag(transforms[i]) ? 1u : 0u;
+        xf[i].pad[0] = xf[i].pad[1] = xf[i].pad[2] = 0u;
+    }
+    if (stats) stats->num_triangles = tris;
+    if (batches == 0) return CRX_OK;
+
+    const size_t offItems = 0;
+    const size_t offInst = align_up(offItems + items.size() * sizeof(DrawItem), 256);
+    const size_t offXf = align_up(offInst + inst.size() * sizeof(InstanceRec), 256);
+    const size_t metaBytes = offXf + xf.size() * sizeof(Transform);
+    CRX_TRY(ctx->drawMeta.reserve(metaBytes));
+    CRX_TRY(ctx->drawStaging.acquire(metaBytes));
+    char* stage = static_cast<char*>(ctx->drawStaging.ptr);
+    std::memcpy(stage + offItems, items.data(), items.size() * sizeof(DrawItem));
+    std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
+    std::memcpy(stage + offXf, xf.data(), xf.size() * sizeof(Transform));
+    CRX_HIP(hipMemcpyAsync(ctx->drawMeta.ptr, stage, metaBytes, hipMemcpyHostToDevice, stream));
+    CRX_TRY(ctx->drawStaging.mark(stream));
+    CRX_TRY(ensure_queues(ctx, tu.max_nontrivial, tu.max_huge));
+
+    char* meta = static_cast<char*>(ctx->drawMeta.ptr);
+    RasterParams P{};
+    P.items = reinterpret_cast<const DrawItem*>(meta + offItems);
+    P.inst = reinterpret_cast<const InstanceRec*>(meta + offInst);
+    P.transforms = reinterpret_cast<const Transform*>(meta + offXf);
+    P.fb = target->framebuffer;
+    P.ctr = ctx->ctr;
+    P.numItems = numItems;
+    P.numBatches = (uint32_t)batches;
+    P.W = target->width;
+    P.H = target->height;
+    P.Wf = (float)target->width;
+    P.Hf = (float)target->height;
+    P.Wm1 = (float)(target->width - 1);
+    P.Hm1 = (float)(target->height - 1);
+    P.f = target->proj.m[5];
+    P.aspect = (float)target->width / (float)target->height;
+    P.fa = P.f / P.aspect;
+    P.faI = 1.0f / (P.f / P.aspect);
+    P.fI = 1.0f / P.f;
+    P.smallThreshold = tu.small_threshold;
+    P.tile = (int32_t)tu.tile_size;
+    P.batchShift = batchShift;
+
+    uint32_t g1;
+    if (tu.stage1_waves_per_cu == 0) {
+        g1 = (uint32_t)std::min<uint64_t>(batches, 1ull << 31);
+    } else {
+        g1 = grid_for(ctx, tu.stage1_waves_per_cu, block / 64);
+        if (g1 > batches) g1 = (uint32_t)batches;
+    }
+    g1 = (g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
+    const uint32_t g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4);
+    const uint32_t g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
+
+    uint32_t redraws = 0;
+    for (;;) {
+        P.nontrivial = static_cast<uint64_t*>(ctx->nontrivial.ptr);
+        P.huge = static_cast<int4*>(ctx->huge.ptr);
+        P.ntCap = ctx->ntCap;
+        P.hugeCap = ctx->hugeCap;
+        CRX_HIP(launch_reset_counters(ctx->ctr, stream));
+        record(ctx, T_S1, stream);
+        CRX_HIP(launch_stage1(P, block, g1, stream));
+        record(ctx, T_S2, stream);
+        CRX_HIP(launch_stage2(P, g2, stream));
+        record(ctx, T_S3, stream);
+        CRX_HIP(launch_stage3(P, g3, stream));
+        record(ctx, T_S3_END, stream);
+        CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        CRX_HIP(hipStreamSynchronize(stream));
+        const uint32_t nNt = ctx->ctrHost[CTR_NONTRIVIAL];
+        const uint32_t nHuge = ctx->ctrHost[CTR_HUGE];
+        if (stats) {
+            stats->num_nontrivial = nNt;
+            stats->num_huge = nHuge;
+            stats->redraws = redraws;
+        }
+        if (nNt <= ctx->ntCap && nHuge <= ctx->hugeCap) break;
+        // Queue overflow: grow and draw again.  Every value the truncated pass
+        // wrote is also written by the full pass, so the minimum is unchanged.
+        if (++redraws > 8) return fail(CRX_QUEUE_OVERFLOW, "stage queues kept overflowing");
+        const uint64_t ntWant = std::max<uint64_t>(ctx->ntCap, (uint64_t)nNt + nNt / 4 + 1024);
+        uint64_t hgWant = ctx->hugeCap;
+        if (nHuge > ctx->hugeCap || nNt > ctx->ntCap) hgWant = std::max<uint64_t>(hgWant, (uint64_t)nHuge * 2 + 1024);
+        if (ntWant > 0xffffffffull || hgWant > 0xffffffffull)
+            return fail(CRX_QUEUE_OVERFLOW, "stage queue would exceed 2^32 entries");
+        CRX_TRY(ensure_queues(ctx, (uint32_t)ntWant, (uint32_t)hgWant));
+    }
+    return CRX_OK;
+}
+
+crx_status crx_resolve_visbuffer(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances,
+                                 const uint64_t* prefix, const crx_render_target* target,
+                                 crx_display_attribute attribute, void* stream_) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    CRX_TRY(check_target(target));
+    if (!target->colorbuffer) return fail(CRX_INVALID_ARG, "target->colorbuffer is NULL");
+    if (attribute == CRX_DISPLAY_TEXTURE || attribute == CRX_DISPLAY_STAGE)
+        return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
+    if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
+    if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
+    hipStream_t stream = as_stream(stream_);
+    CRX_HIP(hipSetDevice(ctx->device));
+    std::vector<InstanceRec> inst(numInstances);
+    for (uint32_t i = 0; i < numInstances; i++) {
+        CRX_TRY(check_mesh(instances[i], "instances", i));
+        inst[i] = make_instance(instances[i]);
+    }
+    const size_t offInst = 0;
+    const size_t offPrefix = align_up(inst.size() * sizeof(InstanceRec), 256);
+    const size_t bytes = offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8;
+    CRX_TRY(ctx->resolveMeta.reserve(bytes));
+    CRX_TRY(ctx->resolveStaging.acquire(bytes));
+    char* stage = static_cast<char*>(ctx->resolveStaging.ptr);
+    if (numInstances) {
+        std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
+        std::memcpy(stage + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
+        CRX_HIP(hipMemcpyAsync(ctx->resolveMeta.ptr, stage, bytes, hipMemcpyHostToDevice, stream));
+        CRX_TRY(ctx->resolveStaging.mark(stream));
+    }
+    char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
+    ResolveParams R{};
+    R.inst = reinterpret_cast<const InstanceRec*>(meta + offInst);
+    R.prefix = reinterpret_cast<const uint64_t*>(meta + offPrefix);
+    R.fb = target->framebuffer;
+    R.cb = target->colorbuffer;
+    R.numInstances = numInstances;
+    R.W = target->width;
+    R.H = target->height;
+    R.attribute = (int32_t)attribute;
+    R.Wf = (float)target->width;
+    R.Hf = (float)target->height;
+    R.p00I = 1.0f / target->proj.m[0];
+    R.p11I = 1.0f / target->proj.m[5];
+    const crx_mat4 viewI = mat_inverse(target->view);
+    std::memcpy(R.viewI, viewI.m, sizeof(R.viewI));
+    const f3 o = xform(viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
+    R.origin[0] = o.x;
+    R.origin[1] = o.y;
+    R.origin[2] = o.z;
+    record(ctx, T_RESOLVE, stream);
+    CRX_HIP(launch_resolve(R, stream));
+    record(ctx, T_RESOLVE_END, stream);
+    return CRX_OK;
+}
+
+// ---- camera / matrix helpers ---------------------------------------------
+
+void crx_camera_projection(float near_, float fovy, float aspect, crx_mat4* out) {
+    if (!out) return;
+    // Camera::createProjectionMatrix, VKRenderer.h:141-148
+    const float f = 1.0f / std::tan(fovy / 2.0f);
+    const float m[16] = {f / aspect, 0.0f, 0.0f, 0.0f, 0.0f, f, 0.0f, 0.0f,
+                         0.0f, 0.0f, 0.0f, -1.0f, 0.0f, 0.0f, near_, 0.0f};
+    std::memcpy(out->m, m, sizeof(m));
+}
+
+void crx_camera_look_at(const float eye[3], const float center[3], const float up[3], crx_mat4* out) {
+    if (!out || !eye || !center || !up) return;
+    // glm::lookAtRH (gtc/matrix_transform.inl)
+    const f3 e{eye[0], eye[1], eye[2]};
+    const f3 f = normalize3(sub3(f3{center[0], center[1], center[2]}, e));
+    const f3 s = normalize3(cross3(f, f3{up[0], up[1], up[2]}));
+    const f3 u = cross3(s, f);
+    float* m = out->m;
+    m[0] = s.x; m[4] = s.y; m[8] = s.z;
+    m[1] = u.x; m[5] = u.y; m[9] = u.z;
+    m[2] = -f.x; m[6] = -f.y; m[10] = -f.z;
+    m[3] = 0.0f; m[7] = 0.0f; m[11] = 0.0f;
+    m[12] = -dot3(s, e);
+    m[13] = -dot3(u, e);
+    m[14] = dot3(f, e);
+    m[15] = 1.0f;
+}
+
+void crx_mat4_inverse(const crx_mat4* m, crx_mat4* out) {
+    if (m && out) *out = mat_inverse(*m);
+}
+
+void crx_mat4_mul(const crx_mat4* a, const crx_mat4* b, crx_mat4* out) {
+    if (a && b && out) *out = mat_mul(*a, *b);
+}
+
+// ---- scene API --------------------------------------------------------------
+
+crx_status crx_mesh_create(crx_context* ctx, const float* positions, uint32_t numVertices,
+                           const uint32_t* indices, uint32_t numTriangles, const uint32_t* colors,
+                           crx_mesh** out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    if (numVertices > 0 && !positions) return fail(CRX_INVALID_ARG, "positions is NULL");
+    if (numTriangles > 1431655765u) return fail(CRX_INVALID_ARG, "more than 1431655765 triangles");
+    if (!indices && (uint64_t)numVertices < 3ull * numTriangles)
+        return fail(CRX_INVALID_ARG, "non-indexed mesh needs 3*numTriangles vertices");
+    if (indices) {
+        for (uint64_t i = 0; i < 3ull * numTriangles; i++) {
+            if (indices[i] >= numVertices) return fail(CRX_INVALID_ARG, "index " + std::to_string(i) + " out of range");
+        }
+    }
+    CRX_HIP(hipSetDevice(ctx->device));
+    crx_mesh* m = new crx_mesh();
+    m->ctx = ctx;
+    m->owned = true;
+    m->numVertices = numVertices;
+    m->numTriangles = numTriangles;
+    m->seq = ++ctx->meshSeq;
+    crx_box3 box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+    for (uint32_t v = 0; v < numVertices; v++) {
+        const float* p = positions + 3ull * v;
+        box.min.x = std::fmin(box.min.x, p[0]); box.max.x = std::fmax(box.max.x, p[0]);
+        box.min.y = std::fmin(box.min.y, p[1]); box.max.y = std::fmax(box.max.y, p[1]);
+        box.min.z = std::fmin(box.min.z, p[2]); box.max.z = std::fmax(box.max.z, p[2]);
+    }
+    m->aabb = box;
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> crx_status {
+        if (!src || bytes == 0) return CRX_OK;
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e != hipSuccess) { *dst = nullptr; return fail(CRX_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+        e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return CRX_OK;
+    };
+    crx_status s = upload((void**)&m->positions, positions, 12ull * numVertices);
+    if (s == CRX_OK && indices) s = upload((void**)&m->indices, indices, 12ull * numTriangles);
+    if (s == CRX_OK && colors) s = upload((void**)&m->colors, colors, 4ull * numVertices);
+    if (s != CRX_OK) {
+        crx_mesh_destroy(m);
+        return s;
+    }
+    *out = m;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_create_device(crx_context* ctx, const float* d_positions, uint32_t numVertices,
+                                  const uint32_t* d_indices, uint32_t numTriangles,
+                                  const uint32_t* d_colors, const crx_box3* aabb, crx_mesh** out) {
+    if (!ctx || !out || !aabb) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    if (numTriangles > 1431655765u) return fail(CRX_INVALID_ARG, "more than 1431655765 triangles");
+    if (numTriangles > 0 && !d_positions) return fail(CRX_INVALID_ARG, "positions is NULL");
+    crx_mesh* m = new crx_mesh();
+    m->ctx = ctx;
+    m->owned = false;
+    m->positions = const_cast<float*>(d_positions);
+    m->indices = const_cast<uint32_t*>(d_indices);
+    m->colors = const_cast<uint32_t*>(d_colors);
+    m->numVertices = numVertices;
+    m->numTriangles = numTriangles;
+    m->aabb = *aabb;
+    m->seq = ++ctx->meshSeq;
+    *out = m;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_destroy(crx_mesh* mesh) {
+    if (!mesh) return CRX_OK;
+    if (mesh->owned) {
+        if (mesh->positions) (void)hipFree(mesh->positions);
+        if (mesh->indices) (void)hipFree(mesh->indices);
+        if (mesh->colors) (void)hipFree(mesh->colors);
+    }
+    delete mesh;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_info(const crx_mesh* mesh, uint32_t* numTriangles, uint32_t* numVertices, crx_box3* aabb) {
+    if (!mesh) return fail(CRX_INVALID_ARG, "mesh is NULL");
+    if (numTriangles) *numTriangles = mesh->numTriangles;
+    if (numVertices) *numVertices = mesh->numVertices;
+    if (aabb) *aabb = mesh->aabb;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_buffers(const crx_mesh* mesh, const float** d_positions, const uint32_t** d_indices,
+                            const uint32_t** d_colors) {
+    if (!mesh) return fail(CRX_INVALID_ARG, "mesh is NULL");
+    if (d_positions) *d_positions = mesh->positions;
+    if (d_indices) *d_indices = mesh->indices;
+    if (d_colors) *d_colors = mesh->colors;
+    return CRX_OK;
+}
+
+crx_status crx_scene_create(crx_context* ctx, crx_scene** out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    crx_scene* s = new crx_scene();
+    s->ctx = ctx;
+    *out = s;
+    return CRX_OK;
+}
+
+crx_status crx_scene_destroy(crx_scene* scene) {
+    delete scene;
+    return CRX_OK;
+}
+
+crx_status crx_scene_add_node(crx_scene* scene, crx_mesh* mesh, const crx_mat4* world, uint32_t* node_id) {
+    if (!scene || !mesh || !world) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (mesh->ctx != scene->ctx) return fail(CRX_INVALID_ARG, "mesh belongs to another context");
+    scene->nodes.push_back(crx_scene::Node{mesh, *world});
+    if (node_id) *node_id = (uint32_t)(scene->nodes.size() - 1);
+    return CRX_OK;
+}
+
+crx_status crx_scene_set_transform(crx_scene* scene, uint32_t node_id, const crx_mat4* world) {
+    if (!scene || !world) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (node_id >= scene->nodes.size()) return fail(CRX_INVALID_ARG, "node_id out of range");
+    scene->nodes[node_id].world = *world;
+    return CRX_OK;
+}
+
+crx_status crx_scene_num_nodes(const crx_scene* scene, uint32_t* out) {
+    if (!scene || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = (uint32_t)scene->nodes.size();
+    return CRX_OK;
+}
+
+void crx_render_settings_default(crx_render_settings* out) {
+    if (!out) return;
+    out->instanced = 1;                                  // RASTERIZER_VISBUFFER_INSTANCED
+    out->frustum_culling = 1;                            // enableFrustumCulling
+    out->resolve = 1;
+    out->attribute = CRX_DISPLAY_VERTEX_COLORS;
+}
+
+crx_status crx_render(crx_context* ctx, crx_scene* scene, const crx_view* view,
+                      const crx_render_settings* settings_, uint64_t* framebuffer, uint64_t* colorbuffer,
+                      void* stream_, crx_draw_stats* stats) {
+    if (!ctx || !scene || !view || !framebuffer) return fail(CRX_INVALID_ARG, "NULL argument");
+    crx_render_settings settings;
+    crx_render_settings_default(&settings);
+    if (settings_) settings = *settings_;
+    if (settings.resolve && !colorbuffer) return fail(CRX_INVALID_ARG, "resolve needs a colorbuffer");
+    if (view->width <= 0 || view->height <= 0) return fail(CRX_INVALID_ARG, "view size must be positive");
+    hipStream_t stream = as_stream(stream_);
+
+    // SNTriangles::update (SNTriangles.h:29-105): frustum side planes of proj*view
+    const crx_mat4 viewProj = mat_mul(view->proj, view->view);
+    float pn[4][3], pd[4];
+    for (int p = 0; p < 4; p++) {
+        const int r = p < 2 ? 0 : 1;
+        const float sg = (p & 1) ? -1.0f : 1.0f;
+        float v[4];
+        for (int c = 0; c < 4; c++) {
+            const float r3 = viewProj.m[4 * c + 3], rr = viewProj.m[4 * c + r];
+            v[c] = sg > 0.0f ? r3 + rr : r3 - rr;
+        }
+        const f3 n{v[0], v[1], v[2]};
+        const float len = sqrtf(dot3(n, n));
+        pn[p][0] = n.x / len; pn[p][1] = n.y / len; pn[p][2] = n.z / len;
+        pd[p] = v[3] / len;
+    }
+    std::vector<uint32_t> visible;
+    visible.reserve(scene->nodes.size());
+    for (uint32_t i = 0; i < scene->nodes.size(); i++) {
+        const crx_scene::Node& nd = scene->nodes[i];
+        bool inside = true;
+        if (settings.frustum_culling) {
+            const crx_box3& b = nd.mesh->aabb;
+            const float cx[2] = {b.min.x, b.max.x}, cy[2] = {b.min.y, b.max.y}, cz[2] = {b.min.z, b.max.z};
+            f3 wmin{INFINITY, INFINITY, INFINITY}, wmax{-INFINITY, -INFINITY, -INFINITY};
+            for (int k = 0; k < 8; k++) {   // Box3::transform corner order
+                const f3 w = xform(nd.world.m, f3{cx[k & 1], cy[(k >> 1) & 1], cz[(k >> 2) & 1]}, 1.0f);
+                wmin.x = std::fmin(wmin.x, w.x); wmin.y = std::fmin(wmin.y, w.y); wmin.z = std::fmin(wmin.z, w.z);
+                wmax.x = std::fmax(wmax.x, w.x); wmax.y = std::fmax(wmax.y, w.y); wmax.z = std::fmax(wmax.z, w.z);
+            }
+            for (int p = 0; p < 4 && inside; p++) {
+                const f3 n{pn[p][0], pn[p][1], pn[p][2]};
+                const f3 v{n.x >= 0.0f ? wmax.x : wmin.x, n.y >= 0.0f ? wmax.y : wmin.y, n.z >= 0.0f ? wmax.z : wmin.z};
+                if (dot3(n, v) + pd[p] < 0.0f) inside = false;
+            }
+        }
+        if (inside) visible.push_back(i);
+    }
+    // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
+    // instances of one mesh keep node order.
+    std::stable_sort(visible.begin(), visible.end(), [&](uint32_t a, uint32_t b) {
+        const crx_mesh* A = scene->nodes[a].mesh;
+        const crx_mesh* B = scene->nodes[b].mesh;
+        if (A->numTriangles == B->numTriangles) return A->seq < B->seq;
+        return A->numTriangles > B->numTriangles;
+    });
+    const uint32_t n = (uint32_t)visible.size();
+    scene->unique.resize(n);
+    scene->instances.resize(n);
+    scene->transforms.resize(n);
+    scene->prefix.resize(n);
+    uint64_t sum = 0;
+    uint32_t u = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const crx_scene::Node& nd = scene->nodes[visible[i]];
+        crx_cmesh cm;
+        std::memset(&cm, 0, sizeof(cm));
+        cm.numTriangles = nd.mesh->numTriangles;
+        cm.indices = nd.mesh->indices;
+        cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
+        cm.colors = nd.mesh->colors;
+        cm.aabb = nd.mesh->aabb;
+        cm.world = nd.world;
+        cm.id = (int32_t)visible[i];
+        cm.numVertices = nd.mesh->numVertices;
+        cm.address = nd.mesh->seq;
+        cm.isLoaded = 1;
+        cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
+        cm.cummulativeTriangleCount = sum;
+        cm.instances.offset = (int32_t)i;
+        cm.instances.count = 0;
+        scene->transforms[i] = mat_mul(view->view, nd.world);
+        scene->prefix[i] = sum;
+        sum += cm.numTriangles;
+        scene->instances[i] = cm;
+        if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
+            scene->unique[u] = cm;
+            u++;
+        }
+        scene->unique[u - 1].instances.count++;
+    }
+
+    crx_render_target target;
+    std::memset(&target, 0, sizeof(target));
+    target.framebuffer = framebuffer;
+    target.colorbuffer = colorbuffer;
+    target.width = view->width;
+    target.height = view->height;
+    target.view = view->view;
+    target.viewI = mat_inverse(view->view);
+    target.proj = view->proj;
+    const f3 cam = xform(target.viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
+    target.cameraPos = crx_vec3{cam.x, cam.y, cam.z};
+    target.f = view->proj.m[5];
+    target.aspect = (float)view->width / (float)view->height;
+
+    CRX_HIP(hipSetDevice(ctx->device));
+    const uint32_t numPixels = (uint32_t)view->width * (uint32_t)view->height;
+    record(ctx, T_CLEAR, stream);
+    CRX_HIP(launch_clear(framebuffer, numPixels, nullptr, ctx->numCUs, stream));
+    if (settings.resolve) CRX_HIP(launch_clear(colorbuffer, numPixels, nullptr, ctx->numCUs, stream));
+    CRX_TRY(crx_draw_triangles_visbuffer(ctx, scene->unique.data(), u, scene->instances.data(), n,
+                                         scene->transforms.data(), &target, settings.instanced, stream_, stats));
+    if (settings.resolve) {
+        CRX_TRY(crx_resolve_visbuffer(ctx, scene->instances.data(), n, scene->prefix.data(), &target,
+                                      (crx_display_attribute)settings.attribute, stream_));
+    }
+    return CRX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,17 @@
+// crx_internal.h — launch wrappers shared by the kernels and the host glue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "crx_device.h"
+
+namespace crx {
+
+hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream);
+hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream);
+hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream);
+hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t stream);
+hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream);
+hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream);
+
+}  // namespace crx

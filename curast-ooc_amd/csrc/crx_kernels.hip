@@ -1,0 +1,789 @@
+// crx_kernels.hip — gfx950 kernels of the visibility-buffer rasterizer.
+//
+//   k_clear    framebuffer clear + counter reset   (kernel_clearFramebuffer, resolve.cu:213-232)
+//   k_stage1   one lane per small triangle         (stage1_drawSmallTriangles, triangles_visbuffer.cu:274-384)
+//   k_stage2   one wave64 per medium triangle      (stage2_drawMediumTriangles, :388-563)
+//   k_stage3   one wave64 per 64x64 tile           (stage3_drawHugeTriangles, :568-745)
+//   k_resolve  one lane per pixel                  (kernel_resolve_visbuffer_to_colorbuffer2D, resolve.cu:640-1141)
+//
+// Every kernel reproduces the reference arithmetic operation by operation
+// (IEEE binary32, GLM evaluation order, no contraction); what changes is the
+// schedule.  Cull predicates are independent early-outs, so they are evaluated
+// cheapest first; the frustum test is done without division by an exact
+// rewrite of "RN(p/d) > 1" (see frustum_outcode).
+#include <hip/hip_runtime.h>
+
+#include "crx_device.h"
+#include "crx_internal.h"
+
+namespace crx {
+
+namespace {
+
+constexpr int kWave = 64;
+
+// Orders LDS traffic between the lanes of one wave (no hardware barrier: the
+// waves of a workgroup run independent loops here).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void atomic_min_fb(uint64_t* fb, uint32_t pixel, uint64_t value) {
+    atomicMin((unsigned long long*)(fb + pixel), (unsigned long long)value);
+}
+
+__device__ __forceinline__ f3 load_pos(const float* __restrict__ positions, uint32_t i) {
+    const float* p = positions + 3ull * i;
+    return f3{p[0], p[1], p[2]};
+}
+
+// Frustum outcode of one view-space vertex with depth d > 0 (d >= 2^-102 or
+// +inf), without dividing.  With p = RN(fa*x) the reference tests
+// RN(p/d) > 1.  Round-to-nearest-even gives RN(q) > 1 <=> q > 1 + 2^-24, i.e.
+// p - d > d*2^-24.  d*2^-24 is exact (no underflow for d >= 2^-102) and so is
+// p - d wherever it matters: Sterbenz makes it exact for p in [d/2, 2d], and
+// outside that range the rounded difference cannot cross the threshold.  The
+// same argument gives RN(p/d) < -1 <=> p + d < -d*2^-24.  Infinite and NaN p
+// compare exactly like the quotient they stand for.
+__device__ __forceinline__ uint32_t frustum_outcode(float px, float py, float d) {
+    const float thr = d * 0x1p-24f;
+    uint32_t oc = 0;
+    oc |= (px - d > thr) ? 1u : 0u;
+    oc |= (px + d < -thr) ? 2u : 0u;
+    oc |= (py - d > thr) ? 4u : 0u;
+    oc |= (py + d < -thr) ? 8u : 0u;
+    return oc;
+}
+
+// rasterize<INDEXBUFFER, UNCOMPRESSED>, triangles_visbuffer.cu:83-271, for one
+// triangle-instance.  `id` is the packed triangle ID the reference forms at
+// :253 and `queueEntry` the stage-2 entry it forms at :193-195.
+__device__ __forceinline__ void stage1_triangle(const RasterParams& P, const float* __restrict__ m,
+                                                bool flip, f3 A, f3 B, f3 C,
+                                                uint64_t id, uint64_t queueEntry) {
+    const f3 av = xform_point(m, A);
+    const f3 bv = xform_point(m, B);
+    const f3 cv = xform_point(m, C);
+    const float da = -av.z, db = -bv.z, dc = -cv.z;   // ndc.z = depth = -view.z
+
+    // all vertices behind the camera (:119)
+    if (da <= 0.0f && db <= 0.0f && dc <= 0.0f) return;
+    bool nontrivial = (da <= 0.0f || db <= 0.0f || dc <= 0.0f);
+
+    // backface culling in view space (:168-188)
+    {
+        const f3 ab = sub3(bv, av);
+        const f3 ac = sub3(cv, av);
+        float res = dot3(av, cross3(ab, ac));
+        if (flip) res = -res;
+        if (res >= 0.0f) return;
+    }
+
+    const float fa = P.fa, f = P.f;
+    const float pxa = fa * av.x, pxb = fa * bv.x, pxc = fa * cv.x;
+    const float pya = f * av.y, pyb = f * bv.y, pyc = f * cv.y;
+
+    // frustum culling (:113-118), only when every depth is positive
+    if (!nontrivial) {
+        if (da >= 0x1p-102f && db >= 0x1p-102f && dc >= 0x1p-102f) {
+            const uint32_t oc = frustum_outcode(pxa, pya, da) & frustum_outcode(pxb, pyb, db) &
+                                frustum_outcode(pxc, pyc, dc);
+            if (oc) return;
+        } else {
+            const float ax = pxa / da, bx = pxb / db, cx = pxc / dc;
+            const float ay = pya / da, by = pyb / db, cy = pyc / dc;
+            if (ax > 1.0f && bx > 1.0f && cx > 1.0f) return;
+            if (ax < -1.0f && bx < -1.0f && cx < -1.0f) return;
+            if (ay > 1.0f && by > 1.0f && cy > 1.0f) return;
+            if (ay < -1.0f && by < -1.0f && cy < -1.0f) return;
+        }
+    }
+
+    // viewToNDC (rasterization_helpers.cuh:95-101) and ndcToScreen - 0.5 (:103-108, :132-134)
+    const float Wf = P.Wf, Hf = P.Hf;
+    const float ax = ((pxa / da) * 0.5f + 0.5f) * Wf - 0.5f;
+    const float bx = ((pxb / db) * 0.5f + 0.5f) * Wf - 0.5f;
+    const float cx = ((pxc / dc) * 0.5f + 0.5f) * Wf - 0.5f;
+    const float ay = ((pya / da) * 0.5f + 0.5f) * Hf - 0.5f;
+    const float by = ((pyb / db) * 0.5f + 0.5f) * Hf - 0.5f;
+    const float cy = ((pyc / dc) * 0.5f + 0.5f) * Hf - 0.5f;
+
+    float min_x = fminf(ax, fminf(bx, cx));
+    float max_x = fmaxf(ax, fmaxf(bx, cx));
+    float min_y = fminf(ay, fminf(by, cy));
+    float max_y = fmaxf(ay, fmaxf(by, cy));
+    min_x = fmaxf(min_x, 0.0f);
+    max_x = fminf(max_x, P.Wm1);
+    min_y = fmaxf(min_y, 0.0f);
+    max_y = fminf(max_y, P.Hm1);
+
+    // tiny-triangle cull (:153-158)
+    if (!nontrivial) {
+        const float sx = floorf(min_x);
+        const float sy = floorf(min_y);
+        if (min_x > sx + 0.0f && max_x < sx + 1.0f + 0.0f) return;
+        if (min_y > sy + 0.0f && max_y < sy + 1.0f + 0.0f) return;
+    }
+
+    const int size_x = f2i(ceilf(max_x) - ceilf(min_x));
+    const int size_y = f2i(ceilf(max_y) - ceilf(min_y));
+    const uint32_t numFragments = (uint32_t)size_x * (uint32_t)size_y;
+    if (numFragments > P.smallThreshold) nontrivial = true;   // unsigned compare, as the reference
+
+    if (nontrivial) {
+        const uint32_t slot = atomicAdd(P.ctr + CTR_NONTRIVIAL, 1u);
+        if (slot < P.ntCap) P.nontrivial[slot] = queueEntry;
+        return;
+    }
+
+    // per-lane incremental rasterisation (:200-269)
+    const float vabx = bx - ax, vaby = by - ay;
+    const float vacx = cx - ax, vacy = cy - ay;
+    const float factor = 1.0f / (vabx * vacy - vaby * vacx);
+    const float iza = 1.0f / da, izb = 1.0f / db, izc = 1.0f / dc;
+    const float ds_dx = vacy * factor;
+    const float ds_dy = -vacx * factor;
+    const float dt_dx = -vaby * factor;
+    const float dt_dy = vabx * factor;
+    const float start_x = ceilf(min_x);
+    const float start_y = ceilf(min_y);
+    const float s0x = start_x - ax + 0.0f;
+    const float s0y = start_y - ay + 0.0f;
+    float s_row = (s0x * vacy - s0y * vacx) * factor;
+    float t_row = (vabx * s0y - vaby * s0x) * factor;
+    uint32_t rowPix = (uint32_t)f2i(start_x) + (uint32_t)P.W * (uint32_t)f2i(start_y);
+    const uint32_t numPixels = (uint32_t)P.W * (uint32_t)P.H;
+    for (int fy = 0; fy < size_y; ++fy) {
+        float s = s_row, t = t_row;
+        uint32_t pix = rowPix;
+        for (int fx = 0; fx < size_x; ++fx) {
+            const float v = 1.0f - (s + t);
+            if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
+                const float inv_depth = (v * iza + s * izb) + t * izc;
+                const float depth = 1.0f / inv_depth;
+                atomic_min_fb(P.fb, pix, pack_pixel(depth, id));
+            }
+            s += ds_dx;
+            t += dt_dx;
+            pix++;
+        }
+        s_row += ds_dy;
+        t_row += dt_dy;
+        rowPix += (uint32_t)P.W;
+    }
+}
+
+}  // namespace
+
+// Clear the framebuffer to 0xFFFFFFF0'00000000 (depth +inf, ID 0), one pixel
+// per lane (a wave stores 512 contiguous bytes), and reset the draw counters
+// (graph-safe: no memset node).
+__global__ __launch_bounds__(256) void k_clear(uint64_t* __restrict__ fb, uint32_t numPixels,
+                                               uint32_t* __restrict__ ctr) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < numPixels; i += stride) fb[i] = kClearPixel;
+    if (blockIdx.x == 0 && ctr && threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(64) void k_reset_counters(uint32_t* __restrict__ ctr) {
+    if (threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
+}
+
+// Stage 1.  Each workgroup owns a contiguous range of triangle batches within
+// its XCD's share of the batch list (workgroups are dispatched round-robin over
+// the 8 XCDs, so workgroup b runs on XCD b % 8); neighbouring batches share
+// vertices, and keeping them on one XCD keeps those vertices in one L2.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
+    const uint32_t numBatches = P.numBatches;
+    const uint32_t G = gridDim.x;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t local = blockIdx.x >> 3;
+    const uint32_t perXcdBlocks = G >> 3;          // G is a multiple of 8
+    const uint32_t perXcd = (numBatches + 7u) >> 3;
+    const uint32_t begin = xcd * perXcd;
+    const uint32_t end = min(numBatches, begin + perXcd);
+    for (uint32_t batch = begin + local; batch < end; batch += perXcdBlocks) {
+        // draw item holding this batch: last item with firstBatch <= batch
+        uint32_t lo = 0, hi = P.numItems - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (P.items[mid].firstBatch <= batch) lo = mid; else hi = mid - 1u;
+        }
+        const DrawItem it = P.items[lo];
+        const uint32_t tri = ((batch - it.firstBatch) << P.batchShift) + threadIdx.x;
+        if (tri >= it.numTriangles) continue;
+        uint32_t i0, i1, i2;
+        if (it.indices) {
+            const uint32_t* ip = it.indices + 3ull * tri;
+            i0 = ip[0];
+            i1 = ip[1];
+            i2 = ip[2];
+        } else {
+            i0 = 3u * tri;
+            i1 = i0 + 1u;
+            i2 = i0 + 2u;
+        }
+        const f3 A = load_pos(it.positions, i0);
+        const f3 B = load_pos(it.positions, i1);
+        const f3 C = load_pos(it.positions, i2);
+        for (uint32_t j = 0; j < it.instCount; j++) {
+            const Transform& T = P.transforms[it.instOffset + j];
+            const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(j * it.numTriangles);
+            const uint64_t entry = ((uint64_t)(it.instOffset + j) << 32) | (uint64_t)tri;
+            stage1_triangle(P, T.m, T.flip != 0u, A, B, C, id, entry);
+        }
+    }
+}
+
+namespace {
+
+// SutherlandHodgman::windowClipping, rasterization_helpers.cuh:126-210,
+// extending the running box with the clipped polygon's vertices.
+__device__ __forceinline__ bool sh_inside(f2 p, int e) {
+    // edges: 0 x >= -1, 1 -x >= -1, 2 y >= -1, 3 -y >= -1  (sign*val >= sign*value)
+    const float val = (e < 2) ? p.x : p.y;
+    const float sign = (e & 1) ? -1.0f : 1.0f;
+    const float value = (e & 1) ? 1.0f : -1.0f;
+    return sign * val >= sign * value;
+}
+
+__device__ __forceinline__ f2 sh_intersect(f2 p1, f2 p2, int e) {
+    const float value = (e & 1) ? 1.0f : -1.0f;
+    if (e < 2) {
+        const float t = (value - p1.x) / (p2.x - p1.x);
+        return f2{value, p1.y + t * (p2.y - p1.y)};
+    }
+    const float t = (value - p1.y) / (p2.y - p1.y);
+    return f2{p1.x + t * (p2.x - p1.x), value};
+}
+
+__device__ void sh_clip_extend(f2 a, f2 b, f2 c, f2& bmin, f2& bmax) {
+    f2 bufA[8], bufB[8];
+    f2* in = bufA;
+    f2* out = bufB;
+    in[0] = a; in[1] = b; in[2] = c;
+    int inCount = 3;
+    for (int e = 0; e < 4; e++) {
+        int outCount = 0;
+        if (inCount == 0) break;
+        f2 S = in[inCount - 1];
+        for (int i = 0; i < inCount; i++) {
+            const f2 E = in[i];
+            const bool Ein = sh_inside(E, e);
+            const bool Sin = sh_inside(S, e);
+            if (Ein) {
+                if (!Sin) out[outCount++] = sh_intersect(S, E, e);
+                out[outCount++] = E;
+            } else if (Sin) {
+                out[outCount++] = sh_intersect(S, E, e);
+            }
+            S = E;
+        }
+        f2* tmp = in; in = out; out = tmp;
+        inCount = outCount;
+    }
+    for (int i = 0; i < inCount; i++) {
+        bmin.x = fminf(bmin.x, in[i].x);
+        bmin.y = fminf(bmin.y, in[i].y);
+        bmax.x = fmaxf(bmax.x, in[i].x);
+        bmax.y = fmaxf(bmax.y, in[i].y);
+    }
+}
+
+__device__ __forceinline__ f2 view_to_ndc_xy(f3 v, float f, float aspect) {
+    const float depth = -v.z;
+    return f2{(f / aspect) * v.x / depth, f * v.y / depth};
+}
+
+// computeScreenSpaceBoundingBox, rasterization_helpers.cuh:350-460 (mask != 7).
+__device__ void stage2_bbox(f3 av, f3 bv, f3 cv, float da, float db, float dc,
+                            f2 an, f2 bn, f2 cn, float f, float aspect, float W, float H,
+                            float& min_x, float& max_x, float& min_y, float& max_y) {
+    f2 bmin{1e10f, 1e10f};
+    f2 bmax{-1e10f, -1e10f};
+    const int mask = (da <= 0.0f ? 1 : 0) | (db <= 0.0f ? 2 : 0) | (dc <= 0.0f ? 4 : 0);
+    if (mask == 0) {
+        sh_clip_extend(an, bn, cn, bmin, bmax);
+    } else {
+        f3 v0, v1, v2;
+        if (mask == 1 || mask == 6) { v0 = av; v1 = bv; v2 = cv; }
+        else if (mask == 2 || mask == 5) { v0 = bv; v1 = cv; v2 = av; }
+        else { v0 = cv; v1 = av; v2 = bv; }
+        const float near_ = 0.01f;
+        const float t1 = (-near_ - v0.z) / (v1.z - v0.z);
+        const f3 I1 = add3(v0, scale3(sub3(v1, v0), t1));
+        const float t2 = (-near_ - v0.z) / (v2.z - v0.z);
+        const f3 I2 = add3(v0, scale3(sub3(v2, v0), t2));
+        if (mask == 1 || mask == 2 || mask == 4) {
+            const f2 p0 = view_to_ndc_xy(v1, f, aspect);
+            const f2 p1 = view_to_ndc_xy(v2, f, aspect);
+            const f2 pI1 = view_to_ndc_xy(I1, f, aspect);
+            const f2 pI2 = view_to_ndc_xy(I2, f, aspect);
+            sh_clip_extend(p0, p1, pI1, bmin, bmax);
+            sh_clip_extend(pI1, p1, pI2, bmin, bmax);
+        } else {
+            sh_clip_extend(view_to_ndc_xy(v0, f, aspect), view_to_ndc_xy(I1, f, aspect),
+                           view_to_ndc_xy(I2, f, aspect), bmin, bmax);
+        }
+    }
+    bmin.x = clampf(bmin.x, -1.0f, 1.0f);
+    bmax.x = clampf(bmax.x, -1.0f, 1.0f);
+    bmin.y = clampf(bmin.y, -1.0f, 1.0f);
+    bmax.y = clampf(bmax.y, -1.0f, 1.0f);
+    const float halfW = W * 0.5f, halfH = H * 0.5f;
+    min_x = bmin.x * halfW + halfW;
+    min_y = bmin.y * halfH + halfH;
+    max_x = bmax.x * halfW + halfW;
+    max_y = bmax.y * halfH + halfH;
+}
+
+// Per-entry setup of stage 2 (:418-471), computed by one lane for its entry.
+struct S2Setup {
+    float ax, ay, vabx, vaby, vacx, vacy, inv_det;
+    float iza, izb, izc;
+    float start_x, start_y;
+    int32_t size_x;
+    uint32_t numFragments;
+    uint32_t id_lo, id_hi;
+};
+
+}  // namespace
+
+// Stage 2.  A persistent grid of waves; each wave claims up to 64 queue
+// entries at once, every lane sets up one entry (bounding box with
+// Sutherland-Hodgman clipping), huge or near-plane triangles are split into
+// tiles appended to the stage-3 queue with one atomic per wave, and the
+// remaining entries are rasterised one after the other by all 64 lanes.
+__global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
+    __shared__ S2Setup sh_setup[4][kWave];
+    __shared__ uint32_t sh_tiles[4][kWave][4];   // tile_x, tile_y, tiles_x, numTiles
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    S2Setup* setup = sh_setup[wid];
+    const uint32_t count = min(P.ctr[CTR_NONTRIVIAL], P.ntCap);
+    const int TILE = P.tile;
+
+    while (true) {
+        wave_sync();
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(P.ctr + CTR_CURSOR2, (uint32_t)kWave);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t idx = base + lane;
+        bool raster = false;
+        uint32_t numTiles = 0;
+        int tile_x = 0, tile_y = 0, tiles_x = 0;
+        uint32_t meshIndex = 0, triangleIndex = 0;
+        if (idx < count) {
+            const uint64_t packed = P.nontrivial[idx];
+            meshIndex = (uint32_t)(packed >> 32);
+            triangleIndex = (uint32_t)packed;
+            const InstanceRec& mesh = P.inst[meshIndex];
+            const float* m = P.transforms[mesh.transformIndex].m;
+            uint32_t i0, i1, i2;
+            if (mesh.indices) {
+                i0 = mesh.indices[3u * triangleIndex + 0u];
+                i1 = mesh.indices[3u * triangleIndex + 1u];
+                i2 = mesh.indices[3u * triangleIndex + 2u];
+            } else {
+                i0 = 3u * triangleIndex; i1 = i0 + 1u; i2 = i0 + 2u;
+            }
+            const f3 av = xform_point(m, load_pos(mesh.positions, i0));
+            const f3 bv = xform_point(m, load_pos(mesh.positions, i1));
+            const f3 cv = xform_point(m, load_pos(mesh.positions, i2));
+            const float f = P.f, aspect = P.aspect;
+            const float da = -av.z, db = -bv.z, dc = -cv.z;
+            const f2 an = view_to_ndc_xy(av, f, aspect);
+            const f2 bn = view_to_ndc_xy(bv, f, aspect);
+            const f2 cn = view_to_ndc_xy(cv, f, aspect);
+            float min_x, max_x, min_y, max_y;
+            stage2_bbox(av, bv, cv, da, db, dc, an, bn, cn, f, aspect, P.Wf, P.Hf, min_x, max_x, min_y, max_y);
+            min_x -= 0.5f; min_y -= 0.5f;
+            max_x += 0.5f; max_y += 0.5f;
+            const float asx = (an.x * 0.5f + 0.5f) * P.Wf, asy = (an.y * 0.5f + 0.5f) * P.Hf;
+            const float bsx = (bn.x * 0.5f + 0.5f) * P.Wf, bsy = (bn.y * 0.5f + 0.5f) * P.Hf;
+            const float csx = (cn.x * 0.5f + 0.5f) * P.Wf, csy = (cn.y * 0.5f + 0.5f) * P.Hf;
+            const float vabx = bsx - asx, vaby = bsy - asy;
+            const float vacx = csx - asx, vacy = csy - asy;
+            const float inv_det = 1.0f / (vabx * vacy - vaby * vacx);
+            const int size_x = f2i(ceilf(max_x)) - f2i(floorf(min_x));
+            const int size_y = f2i(ceilf(max_y)) - f2i(floorf(min_y));
+            const uint32_t numFragments = (uint32_t)size_x * (uint32_t)size_y;
+            if (!(min_x >= max_x || min_y >= max_y)) {
+                if ((int32_t)numFragments >= TILE * TILE || da <= 0.0f || db <= 0.0f || dc <= 0.0f) {
+                    tile_x = f2i(min_x) / TILE;
+                    tile_y = f2i(min_y) / TILE;
+                    tiles_x = (f2i(max_x) + TILE - 1) / TILE - tile_x;
+                    const int tiles_y = (f2i(max_y) + TILE - 1) / TILE - tile_y;
+                    numTiles = (uint32_t)tiles_x * (uint32_t)tiles_y;
+                } else {
+                    raster = true;
+                    S2Setup s;
+                    s.ax = asx; s.ay = asy;
+                    s.vabx = vabx; s.vaby = vaby; s.vacx = vacx; s.vacy = vacy;
+                    s.inv_det = inv_det;
+                    s.iza = 1.0f / da; s.izb = 1.0f / db; s.izc = 1.0f / dc;
+                    s.start_x = floorf(min_x);
+                    s.start_y = floorf(min_y);
+                    s.size_x = size_x;
+                    s.numFragments = numFragments;
+                    // packed_id = (uint32)(uint32(cumm) + triangleIndex), :414-519
+                    const uint32_t id32 = (uint32_t)mesh.cumm + triangleIndex;
+                    s.id_lo = id32;
+                    s.id_hi = 0u;
+                    setup[lane] = s;
+                }
+            }
+        }
+
+        // LDS-staged append of tile entries: one atomic per wave for all lanes' tiles.
+        uint32_t prefix = numTiles;
+        #pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t n = __shfl_up(prefix, o);
+            if ((int)lane >= o) prefix += n;
+        }
+        const uint32_t total = __shfl(prefix, kWave - 1);
+        if (total > 0) {
+            uint32_t hbase = 0;
+            if (lane == kWave - 1) hbase = atomicAdd(P.ctr + CTR_HUGE, total);
+            hbase = __shfl(hbase, kWave - 1);
+            sh_tiles[wid][lane][0] = (uint32_t)tile_x;
+            sh_tiles[wid][lane][1] = (uint32_t)tile_y;
+            sh_tiles[wid][lane][2] = (uint32_t)tiles_x;
+            sh_tiles[wid][lane][3] = prefix - numTiles;
+            wave_sync();
+            uint64_t pending = __ballot(numTiles > 0);
+            while (pending) {
+                const int src = __builtin_ctzll(pending);
+                pending &= pending - 1;
+                const uint32_t n = __shfl(numTiles, src);
+                const uint32_t mi = __shfl(meshIndex, src);
+                const uint32_t ti = __shfl(triangleIndex, src);
+                const uint32_t tx0 = sh_tiles[wid][src][0];
+                const uint32_t ty0 = sh_tiles[wid][src][1];
+                const uint32_t ntx = sh_tiles[wid][src][2];
+                const uint32_t off = hbase + sh_tiles[wid][src][3];
+                for (uint32_t i = lane; i < n; i += kWave) {
+                    const uint32_t slot = off + i;
+                    if (slot < P.hugeCap) {
+                        P.huge[slot] = make_int4((int)mi, (int)ti, (int)(tx0 + i % ntx), (int)(ty0 + i / ntx));
+                    }
+                }
+            }
+        }
+
+        // Rasterise the medium triangles of this claim, all lanes per triangle (:516-561).
+        wave_sync();
+        uint64_t todo = __ballot(raster);
+        while (todo) {
+            const int src = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const S2Setup s = setup[src];
+            const uint64_t id = (uint64_t)s.id_lo;
+            for (uint32_t off = lane; off < s.numFragments; off += kWave) {
+                const int fragX = (int)off % s.size_x;
+                const int fragY = (int)off / s.size_x;
+                const float px = s.start_x + (float)fragX;
+                const float py = s.start_y + (float)fragY;
+                const float sx = px - s.ax + 0.5f;
+                const float sy = py - s.ay + 0.5f;
+                const float ss = (sx * s.vacy - sy * s.vacx) * s.inv_det;
+                const float tt = (s.vabx * sy - s.vaby * sx) * s.inv_det;
+                const float vv = 1.0f - (ss + tt);
+                if (ss >= 0.0f && tt >= 0.0f && vv >= 0.0f) {
+                    const int ix = f2i(px), iy = f2i(py);
+                    // samples outside the screen are discarded (the reference wraps them)
+                    if (ix >= 0 && ix < P.W && iy >= 0 && iy < P.H) {
+                        const float inv_depth = (vv * s.iza + ss * s.izb) + tt * s.izc;
+                        const float depth = 1.0f / inv_depth;
+                        atomic_min_fb(P.fb, (uint32_t)(ix + P.W * iy), pack_pixel(depth, id));
+                    }
+                }
+            }
+        }
+    }
+}
+
+namespace {
+
+// intersectTriangle_mt, double-sided branch (rasterization_helpers.cuh:213-270),
+// with orig = 0.  e1, e2, qvec and dot(e2, qvec) do not depend on the ray.
+struct MTTri {
+    f3 e1, e2, tvec, qvec;
+};
+
+__device__ __forceinline__ float intersect_mt(const MTTri& T, f3 dir) {
+    const f3 pvec = cross3(dir, T.e2);
+    const float det = dot3(T.e1, pvec);
+    if (det > -1e-7f && det < 1e-7f) return __builtin_inff();
+    const float inv_det = 1.0f / det;
+    const float u = dot3(T.tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return __builtin_inff();
+    const float v = dot3(dir, T.qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return __builtin_inff();
+    const float t = dot3(T.e2, T.qvec) * inv_det;
+    return t > 0.0f ? t : __builtin_inff();
+}
+
+}  // namespace
+
+// Stage 3.  A persistent grid of waves; each wave takes one tile entry at a
+// time and ray-casts its TILE x TILE pixels (RAYTRACE branch, :681-741).
+// With TILE <= 64 the 64 lanes cover 64/TILE rows per step, otherwise each
+// lane covers TILE/64 columns.
+__global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t count = min(P.ctr[CTR_HUGE], P.hugeCap);
+    const int TILE = P.tile;
+    const int lanesPerRow = TILE < kWave ? TILE : kWave;
+    const int rowsPerStep = kWave / lanesPerRow;
+    const int colsPerLane = TILE / lanesPerRow;
+    while (true) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(P.ctr + CTR_CURSOR3, 1u);
+        idx = __shfl(idx, 0);
+        if (idx >= count) break;
+        const int4 e = P.huge[idx];
+        const InstanceRec& mesh = P.inst[e.x];
+        const float* m = P.transforms[mesh.transformIndex].m;
+        const uint32_t tri = (uint32_t)e.y;
+        uint32_t i0, i1, i2;
+        if (mesh.indices) {
+            i0 = mesh.indices[3u * tri + 0u];
+            i1 = mesh.indices[3u * tri + 1u];
+            i2 = mesh.indices[3u * tri + 2u];
+        } else {
+            i0 = 3u * tri; i1 = i0 + 1u; i2 = i0 + 2u;
+        }
+        const f3 av = xform_point(m, load_pos(mesh.positions, i0));
+        const f3 bv = xform_point(m, load_pos(mesh.positions, i1));
+        const f3 cv = xform_point(m, load_pos(mesh.positions, i2));
+        MTTri T;
+        T.e1 = sub3(bv, av);
+        T.e2 = sub3(cv, av);
+        T.tvec = f3{-av.x, -av.y, -av.z};
+        T.qvec = cross3(T.tvec, T.e1);
+        const uint64_t id = mesh.cumm + (uint64_t)(int64_t)e.y;
+        const float min_x = clampf((float)(e.z * TILE), 0.0f, P.Wf);
+        const float min_y = clampf((float)(e.w * TILE), 0.0f, P.Hf);
+        const float fx0 = floorf(min_x), fy0 = floorf(min_y);
+        for (int c = 0; c < colsPerLane; c++) {
+            const int fragX = (int)(lane % (uint32_t)lanesPerRow) + c * lanesPerRow;
+            const int px = f2i(fx0 + (float)fragX);
+            if (px >= P.W) continue;   // the reference wraps these into the next row
+            const float u = 2.0f * ((float)px + 0.5f) / P.Wf - 1.0f;
+            const float rx = P.faI * u;
+            for (int y0 = 0; y0 < TILE; y0 += rowsPerStep) {
+                const int fragY = y0 + (int)(lane / (uint32_t)lanesPerRow);
+                const int py = f2i(fy0 + (float)fragY);
+                if (py >= P.H) break;
+                const float v = 2.0f * ((float)py + 0.5f) / P.Hf - 1.0f;
+                const f3 dir = normalize3(f3{rx, P.fI * v, -1.0f});
+                const float t = intersect_mt(T, dir);
+                if (t == __builtin_inff()) continue;
+                // depth = dot(t*dir, (0,0,-1)) (:734)
+                const float depth = ((t * dir.x) * 0.0f + (t * dir.y) * 0.0f) + (t * dir.z) * -1.0f;
+                atomic_min_fb(P.fb, (uint32_t)(px + P.W * py), pack_pixel(depth, id));
+            }
+        }
+    }
+}
+
+namespace {
+
+// intersectPlane, rasterization_helpers.cuh:318-335
+__device__ __forceinline__ float intersect_plane(f3 origin, f3 dir, f3 normal, float constant) {
+    const float denominator = dot3(normal, dir);
+    if (denominator == 0.0f) return __builtin_inff();
+    const float t = -(dot3(origin, normal) + constant) / denominator;
+    return (t >= 0.0f) ? t : __builtin_inff();
+}
+
+}  // namespace
+
+// Resolve: one lane per pixel, 64x4 pixels per workgroup (a wave reads 512
+// contiguous bytes of the visibility buffer).
+__global__ __launch_bounds__(256) void k_resolve(ResolveParams R) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= R.W || y >= R.H) return;
+    const uint32_t pixelID = (uint32_t)x + (uint32_t)R.W * (uint32_t)y;
+
+    const uint64_t pixel = R.fb[pixelID];
+    const uint64_t pixel_cb = R.cb[pixelID];
+    const float depth = unpack_depth(pixel);
+    const uint64_t total = pixel & 0xFFFFFFFFFull;
+    const float depth_cb = __builtin_bit_cast(float, (uint32_t)(pixel_cb >> 32));
+    if (depth_cb < depth) return;
+
+    uint32_t color = 0;
+    if (!__builtin_isinf(depth) && R.numInstances > 0) {
+        // binary search of the instance holding `total` (resolve.cu:697-721)
+        const uint32_t n = R.numInstances;
+        int left = 0, right = (int)(n - 1u), found = 0;
+        while (left <= right) {
+            const int mid = left + (right - left) / 2;
+            const int64_t before = (int64_t)R.prefix[mid];
+            const int64_t after = ((uint32_t)mid == n - 1u) ? 0xffffffffffll : (int64_t)R.prefix[mid + 1];
+            if (total >= (uint64_t)before && total < (uint64_t)after) { found = mid; break; }
+            else if ((uint64_t)after <= total) left = mid + 1;
+            else right = mid - 1;
+            if (mid == left) break;
+        }
+        const InstanceRec& mesh = R.inst[found];
+        uint32_t tri = (uint32_t)(total - mesh.cumm);
+        tri = min(tri, mesh.numTriangles - 1u);
+        // an empty mesh has no triangle to fetch: shade it like an unloaded one
+        if (!mesh.isLoaded || mesh.numTriangles == 0) {
+            color = 0xffff00ffu;
+        } else {
+            uint32_t i0, i1, i2;
+            if (mesh.indices) {
+                i0 = mesh.indices[3u * tri + 0u];
+                i1 = mesh.indices[3u * tri + 1u];
+                i2 = mesh.indices[3u * tri + 2u];
+            } else {
+                i0 = 3u * tri; i1 = i0 + 1u; i2 = i0 + 2u;
+            }
+            // ray through the pixel centre (computeRayDir, resolve.cu:662-680)
+            const f3 origin{R.origin[0], R.origin[1], R.origin[2]};
+            const float u = 2.0f * ((float)x + 0.5f) / R.Wf - 1.0f;
+            const float v = 2.0f * ((float)y + 0.5f) / R.Hf - 1.0f;
+            const f3 rview = normalize3(f3{R.p00I * u, R.p11I * v, -1.0f});
+            const f3 rayDir = normalize3(sub3(xform_point(R.viewI, rview), origin));
+
+            const f3 aw = xform_point(mesh.world, load_pos(mesh.positions, i0));
+            const f3 bw = xform_point(mesh.world, load_pos(mesh.positions, i1));
+            const f3 cw = xform_point(mesh.world, load_pos(mesh.positions, i2));
+            const f3 v0 = sub3(bw, aw);
+            const f3 v1 = sub3(cw, aw);
+            float t;
+            {   // resolve.cu:790-801
+                const f3 normal = normalize3(cross3(v0, v1));
+                const float d = dot3(aw, normal);
+                t = intersect_plane(origin, rayDir, normal, -d);
+            }
+            float stx, sty;
+            {   // resolve.cu:805-834
+                const float d00 = dot3(v0, v0);
+                const float d01 = dot3(v0, v1);
+                const float d11 = dot3(v1, v1);
+                const float denom = d00 * d11 - d01 * d01;
+                const float denomI = 1.0f / denom;
+                const f3 p = add3(origin, f3{t * rayDir.x, t * rayDir.y, t * rayDir.z});
+                const f3 v2 = sub3(p, aw);
+                const float d20 = dot3(v2, v0);
+                const float d21 = dot3(v2, v1);
+                stx = (d11 * d20 - d01 * d21) * denomI;
+                sty = (d00 * d21 - d01 * d20) * denomI;
+            }
+            const float w = 1.0f - stx - sty;
+            color = 0xffff00ffu;
+            const int attr = R.attribute;
+            if (attr == 0) {                       // NONE
+                color = 0xffffffffu;
+            } else if (attr == 5) {                // TRIANGLE_ID
+                color = tri * 12345678u;
+            } else if (attr == 6) {                // MESH_ID
+                color = ((uint32_t)mesh.id + 1u) * 12345678u;
+            } else if (attr == 2 && mesh.uvs) {    // UV
+                const float* ua = mesh.uvs + 2ull * i0;
+                const float* ub = mesh.uvs + 2ull * i1;
+                const float* uc = mesh.uvs + 2ull * i2;
+                float ux = (ua[0] * w + ub[0] * stx) + uc[0] * sty;
+                float uy = (ua[1] * w + ub[1] * stx) + uc[1] * sty;
+                ux = ux - floorf(ux);
+                uy = uy - floorf(uy);
+                const uint32_t r = f2u8(fmodf(ux - floorf(ux), 1.0f) * 256.0f);
+                const uint32_t g = f2u8(fmodf(uy - floorf(uy), 1.0f) * 256.0f);
+                color = (color & 0xff000000u) | r | (g << 8);
+            } else if (attr == 3) {                // NORMAL
+                f3 N;
+                if (mesh.normals) {
+                    const f3 n0 = xform(mesh.world, load_pos(mesh.normals, i0), 0.0f);
+                    const f3 n1 = xform(mesh.world, load_pos(mesh.normals, i1), 0.0f);
+                    const f3 n2 = xform(mesh.world, load_pos(mesh.normals, i2), 0.0f);
+                    N = normalize3(add3(add3(scale3(n0, w), scale3(n1, stx)), scale3(n2, sty)));
+                } else {
+                    N = normalize3(cross3(v0, v1));
+                }
+                const uint32_t r = f2u8(clampf(N.x * 255.0f, 0.0f, 255.0f));
+                const uint32_t g = f2u8(clampf(N.y * 255.0f, 0.0f, 255.0f));
+                const uint32_t b = f2u8(clampf(N.z * 255.0f, 0.0f, 255.0f));
+                color = (color & 0xff000000u) | r | (g << 8) | (b << 16);
+            } else if (attr == 4 && mesh.colors) { // VERTEX_COLORS (resolve.cu:952-973)
+                const uint32_t Ca = mesh.colors[i0], Cb = mesh.colors[i1], Cc = mesh.colors[i2];
+                uint32_t rgb = 0;
+                #pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float ca = (float)((Ca >> (8 * k)) & 0xffu);
+                    const float cb = (float)((Cb >> (8 * k)) & 0xffu);
+                    const float cc = (float)((Cc >> (8 * k)) & 0xffu);
+                    const float ch = (ca * w + cb * stx) + cc * sty;
+                    rgb |= f2u8(clampf(ch, 0.0f, 255.0f)) << (8 * k);
+                }
+                color = (Ca & 0xff000000u) | rgb;
+            } else {
+                color = 0xff0000ffu;
+            }
+            color |= 0xff000000u;
+        }
+    }
+    if (depth != __builtin_inff()) {
+        R.cb[pixelID] = ((uint64_t)__builtin_bit_cast(uint32_t, depth) << 32) | (uint64_t)color;
+    } else {
+        R.cb[pixelID] = ((uint64_t)0x7f800000u << 32);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers used by the host glue.
+// ---------------------------------------------------------------------------
+
+hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream) {
+    uint32_t blocks = (numPixels + 255u) / 256u;
+    const uint32_t cap = (uint32_t)numCUs * 16u;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, stream, fb, numPixels, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
+    switch (block) {
+        case 64:  hipLaunchKernelGGL(k_stage1<64>, dim3(grid), dim3(64), 0, stream, P); break;
+        case 128: hipLaunchKernelGGL(k_stage1<128>, dim3(grid), dim3(128), 0, stream, P); break;
+        case 256: hipLaunchKernelGGL(k_stage1<256>, dim3(grid), dim3(256), 0, stream, P); break;
+        case 512: hipLaunchKernelGGL(k_stage1<512>, dim3(grid), dim3(512), 0, stream, P); break;
+        case 1024: hipLaunchKernelGGL(k_stage1<1024>, dim3(grid), dim3(1024), 0, stream, P); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_stage2, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_stage3, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream) {
+    const dim3 grid((uint32_t)(R.W + 63) / 64u, (uint32_t)(R.H + 3) / 4u);
+    hipLaunchKernelGGL(k_resolve, grid, dim3(256), 0, stream, R);
+    return hipGetLastError();
+}
+
+}  // namespace crx

@@ -1,0 +1,341 @@
+/*
+ * crx.h — C-ABI of the MI355X (gfx950) visibility-buffer rasterizer.
+ *
+ * This is the drop-in boundary for CuRast's triangle visibility-buffer path:
+ * the three-stage `triangles_visbuffer` pipeline, the framebuffer clear and the
+ * visibility-buffer resolve, plus the scene / mesh / instance / camera surface
+ * that feeds them.  Every entry point is `extern "C"`, takes plain pointers and
+ * sizes, returns a crx_status and never exits the process.
+ *
+ * Reference interfaces replaced (paths relative to the CuRast-OOC source tree):
+ *   crx_cmesh              CMesh          src/kernels/HostDeviceInterface.h:196-234 (byte-compatible, 280 B)
+ *   crx_render_target      RenderTarget   src/kernels/HostDeviceInterface.h:131-143 (byte-compatible, 240 B)
+ *   crx_huge_triangle      HugeTriangle   src/kernels/HostDeviceInterface.h:254-259
+ *   crx_tuning             TILE_SIZE / TRIANGLES_PER_SWEEP / THRESHOLD_SMALL / MAX_* constants,
+ *                          src/kernels/HostDeviceInterface.h:261-266 (compile-time there, runtime here)
+ *   crx_clear_framebuffer  kernel_clearFramebuffer launch, src/CuRast_render.h:491-512,
+ *                          kernel src/kernels/resolve.cu:213-232
+ *   crx_draw_triangles_visbuffer
+ *                          drawTrianglesVisbuffer(...), src/CuRast_render.h:211-264, which launches
+ *                          kernel_stage{1,2,3}_* src/kernels/triangles_visbuffer.cu:755-835
+ *   crx_resolve_visbuffer  kernel_resolve_visbuffer_to_colorbuffer2D launch, src/CuRast_render.h:580-592,
+ *                          kernel src/kernels/resolve.cu:640-1141
+ *   crx_scene_* / crx_mesh_* / crx_render
+ *                          Scene / SNTriangles / Mesh and the frame preparation of CuRast::draw,
+ *                          src/scene/Scene.h:9-51, src/scene/SNTriangles.h:18-105, src/scene/Mesh.h:4-25,
+ *                          src/CuRast_render.h:266-519
+ *   crx_camera_projection  Camera::createProjectionMatrix, src/VKRenderer.h:141-148
+ *
+ * Arithmetic contract.  All device arithmetic is IEEE-754 binary32 with
+ * round-to-nearest, correctly rounded division and square root, denormals
+ * preserved and no contraction of a*b+c into fused multiply-adds.  Operations
+ * follow GLM 0.9.9's evaluation order (mat4*vec4 = (c0*x + c1*y) + (c2*z + c3*w),
+ * dot3 = (x+y)+z, cross, normalize = v * (1/sqrt(dot(v,v)))).  Where the
+ * reference uses __fdividef or fast-math division, the contract uses the IEEE
+ * quotient.  Under this contract the visibility buffer is a pure function of the
+ * inputs and the crx_tuning routing parameters (small_threshold, tile_size);
+ * scheduling, grid sizes and queue order cannot change a single bit because
+ * every write is a 64-bit atomic minimum.
+ *
+ * Deliberate normalisations of reference behaviour (every other quirk is kept):
+ *   - stage 2 and stage 3 discard samples outside [0,W)x[0,H) instead of
+ *     writing them to a wrapped or negative framebuffer index
+ *     (triangles_visbuffer.cu:548-549 and :694-731);
+ *   - queues are bounded; on overflow the draw grows its queues and redraws the
+ *     frame, so no triangle is ever dropped (triangles_visbuffer.cu:192-197, :503-511);
+ *   - a mesh may hold at most 1431655765 triangles (3*T must fit in uint32).
+ * Kept quirks: stage 1 never samples column W-1 or row H-1; the instanced
+ * stage-1 triangle ID adds (instance*numTriangles) mod 2^32; stage 2 forms its
+ * triangle ID in 32-bit arithmetic; stage 1 uses strict edge tests, stage 2
+ * inclusive ones and stage 3 the Moller-Trumbore [0,1] range.
+ */
+#ifndef CRX_H
+#define CRX_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRX_ABI_VERSION 1
+
+typedef enum crx_status {
+    CRX_OK = 0,
+    CRX_INVALID_ARG = 1,
+    CRX_OUT_OF_MEMORY = 2,
+    CRX_QUEUE_OVERFLOW = 3,
+    CRX_HIP_ERROR = 4,
+    CRX_NO_DEVICE = 5,
+    CRX_UNSUPPORTED = 6,
+    CRX_REJECTED = 7
+} crx_status;
+
+/* ------------------------------------------------------------------ */
+/* Plain-old-data types, GLM memory layout (tightly packed floats).    */
+/* ------------------------------------------------------------------ */
+
+typedef struct crx_vec2 { float x, y; } crx_vec2;
+typedef struct crx_vec3 { float x, y, z; } crx_vec3;
+/* Column-major 4x4 like glm::mat4: element (col c, row r) is m[4*c + r]. */
+typedef struct crx_mat4 { float m[16]; } crx_mat4;
+
+/* Box3, src/kernels/HostDeviceInterface.h:57-105 */
+typedef struct crx_box3 { crx_vec3 min, max; } crx_box3;
+
+/* Texture, src/kernels/HostDeviceInterface.h:177-185 (carried, not sampled) */
+typedef struct crx_texture {
+    int32_t width;
+    int32_t height;
+    uint32_t* data;
+    void* huffmanTables;
+    void* quanttables;
+    uint32_t* mcuPositions;
+    uint32_t handle;
+} crx_texture;
+
+/* CMesh, src/kernels/HostDeviceInterface.h:196-234.  Pointer members are
+ * device pointers.  Only the fields named in the comments are read. */
+typedef struct crx_cmesh {
+    uint32_t numTriangles;              /* read */
+    uint32_t* indices;                  /* read; NULL = non-indexed (vertex 3t+k) */
+    crx_vec3* positions;                /* read; xyz float, 12-byte stride */
+    uint64_t cummulativeTriangleCount;  /* read: ID of triangle 0 */
+    crx_box3 aabb;                      /* read by the host frustum cull */
+    crx_vec3 compressionFactor;
+    uint32_t index_min;
+    uint32_t bitsPerIndex;
+    struct { int32_t offset; int32_t count; } instances;  /* read */
+    crx_mat4 world;                     /* read by resolve */
+    int32_t id;                         /* read by resolve (MESH_ID) */
+    crx_vec2* uvs;
+    crx_vec3* normals;
+    uint32_t* colors;                   /* read by resolve (VERTEX_COLORS), RGBA8 */
+    uint32_t firstTriangle;
+    uint32_t numVertices;
+    uint32_t index_max;
+    uint64_t address;
+    crx_texture texture;
+    uint8_t isLoaded;                   /* read by resolve */
+    uint8_t flipTriangles;
+    uint8_t compressed;                 /* must be 0: the IX_PU16 variant is not built */
+    struct { crx_vec3* positions; uint32_t* colors; uint32_t numPoints; } impostor;
+} crx_cmesh;
+
+/* RenderTarget, src/kernels/HostDeviceInterface.h:131-143 */
+typedef struct crx_render_target {
+    uint64_t* framebuffer;   /* device, width*height packed depth|triangleID */
+    uint64_t* colorbuffer;   /* device, width*height depth<<32 | RGBA8 */
+    int32_t width;
+    int32_t height;
+    crx_mat4 view;
+    crx_mat4 viewI;
+    crx_mat4 proj;           /* proj[1][1] (m[5]) is the focal factor f */
+    crx_vec3 cameraPos;
+    float f;
+    float aspect;
+    uint8_t debug;
+} crx_render_target;
+
+/* HugeTriangle, src/kernels/HostDeviceInterface.h:254-259 */
+typedef struct crx_huge_triangle {
+    int32_t meshIndex;
+    int32_t triangleIndex;
+    int32_t tile_x;
+    int32_t tile_y;
+} crx_huge_triangle;
+
+/* DisplayAttribute, src/kernels/HostDeviceInterface.h:277-286 */
+typedef enum crx_display_attribute {
+    CRX_DISPLAY_NONE = 0,
+    CRX_DISPLAY_TEXTURE = 1,       /* not supported: crx_resolve_visbuffer returns CRX_UNSUPPORTED */
+    CRX_DISPLAY_UV = 2,
+    CRX_DISPLAY_NORMAL = 3,
+    CRX_DISPLAY_VERTEX_COLORS = 4,
+    CRX_DISPLAY_TRIANGLE_ID = 5,
+    CRX_DISPLAY_MESH_ID = 6,
+    CRX_DISPLAY_STAGE = 7          /* not supported: crx_resolve_visbuffer returns CRX_UNSUPPORTED */
+} crx_display_attribute;
+
+/* Runtime form of the reference's compile-time constants.
+ * small_threshold and tile_size route triangles between stages and so fix the
+ * bits of the result (the oracle takes the same struct); every other field only
+ * changes scheduling.  crx_tuning_default() returns the reference values. */
+typedef struct crx_tuning {
+    uint32_t small_threshold;     /* THRESHOLD_SMALL = 128: stage-1 fragment limit */
+    uint32_t tile_size;           /* TILE_SIZE = 64: stage-2 huge limit (tile^2) and stage-3 tile edge; 16..256, multiple of 16 */
+    uint32_t max_nontrivial;      /* MAX_NONTRIVIAL_TRIANGLES = 5e6: initial stage-2 queue capacity */
+    uint32_t max_huge;            /* MAX_HUGE_TRIANGLES = 5e6: initial stage-3 queue capacity */
+    uint32_t stage1_block;        /* threads per stage-1 workgroup (64..1024, multiple of 64) */
+    uint32_t stage1_waves_per_cu; /* stage-1 grid = CUs * this / (stage1_block/64); 0 = one wave per 256 triangles */
+    uint32_t stage2_waves_per_cu; /* stage-2 persistent grid size in waves per CU */
+    uint32_t stage3_waves_per_cu; /* stage-3 persistent grid size in waves per CU */
+} crx_tuning;
+
+/* Per-draw counters read back after a draw. */
+typedef struct crx_draw_stats {
+    uint64_t num_triangles;        /* triangle-instances submitted to stage 1 */
+    uint32_t num_nontrivial;       /* entries in the stage-2 queue */
+    uint32_t num_huge;             /* tile entries in the stage-3 queue */
+    uint32_t redraws;              /* redraws caused by queue growth */
+    uint32_t reserved;
+} crx_draw_stats;
+
+typedef struct crx_context crx_context;
+
+/* ------------------------------------------------------------------ */
+/* Library / context                                                   */
+/* ------------------------------------------------------------------ */
+
+int crx_abi_version(void);
+const char* crx_status_string(crx_status s);
+/* Message of the last failing call on this thread ("" if none). */
+const char* crx_last_error(void);
+void crx_tuning_default(crx_tuning* out);
+
+/* One context per device.  Owns queues, counters and staging for the draws
+ * issued through it; calls on one context are serialised by the caller. */
+crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out);
+crx_status crx_context_destroy(crx_context* ctx);
+crx_status crx_context_set_tuning(crx_context* ctx, const crx_tuning* tuning);
+crx_status crx_context_get_tuning(const crx_context* ctx, crx_tuning* out);
+/* Number of compute units of the context's device. */
+crx_status crx_context_num_cus(const crx_context* ctx, int* out);
+
+/* ------------------------------------------------------------------ */
+/* Kernel-level entry points (replace the reference's launches)        */
+/* `stream` is a hipStream_t (NULL = default stream).                  */
+/* ------------------------------------------------------------------ */
+
+/* kernel_clearFramebuffer (resolve.cu:213-232): fb[i] = 0xFFFFFFF0'00000000. */
+crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void* stream);
+
+/* drawTrianglesVisbuffer (CuRast_render.h:211-264).  `meshes` / `instances`
+ * are HOST arrays of CMesh whose buffers are device pointers; `transforms` is a
+ * HOST array of numInstances world-to-view matrices (view * world);
+ * `target` is a HOST RenderTarget whose framebuffer was cleared.  With
+ * instanced != 0 stage 1 walks the unique `meshes` and loops over their
+ * instances{offset,count} (the _instanced kernels); otherwise it walks
+ * `instances` as meshes.  Enqueues on `stream`; the call returns after
+ * the stage-2 and stage-3 queue sizes are known (one small readback). */
+crx_status crx_draw_triangles_visbuffer(crx_context* ctx,
+                                        const crx_cmesh* meshes, uint32_t numMeshes,
+                                        const crx_cmesh* instances, uint32_t numInstances,
+                                        const crx_mat4* transforms,
+                                        const crx_render_target* target,
+                                        int instanced, void* stream,
+                                        crx_draw_stats* stats /* nullable */);
+
+/* kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141).
+ * `instances` is a HOST array of CMesh, `prefix` a HOST array of the per-
+ * instance cumulative triangle counts (CuRast_render.h:412-435).  Writes
+ * target->colorbuffer. */
+crx_status crx_resolve_visbuffer(crx_context* ctx,
+                                 const crx_cmesh* instances, uint32_t numInstances,
+                                 const uint64_t* prefix,
+                                 const crx_render_target* target,
+                                 crx_display_attribute attribute, void* stream);
+
+/* Per-stage device time of the last draw / resolve issued on a context with
+ * timing enabled (hipEvent pairs around each launch, the role of the
+ * reference's Timer, src/Timer.h:61-113).  Milliseconds; -1 if not measured. */
+typedef struct crx_timings {
+    float clear_ms;
+    float stage1_ms;
+    float stage2_ms;
+    float stage3_ms;
+    float resolve_ms;
+    float pipeline_ms;   /* first stage-1 event to last stage-3 event ("<triangles visbuffer pipeline>") */
+} crx_timings;
+
+crx_status crx_context_enable_timing(crx_context* ctx, int enable);
+/* Synchronises on the recorded events and returns their durations. */
+crx_status crx_context_timings(crx_context* ctx, crx_timings* out);
+
+/* ------------------------------------------------------------------ */
+/* Device memory plumbing for C hosts (hipMalloc / hipMemcpy wrappers). */
+/* ------------------------------------------------------------------ */
+
+crx_status crx_device_alloc(crx_context* ctx, size_t bytes, void** out);
+crx_status crx_device_free(crx_context* ctx, void* ptr);
+crx_status crx_memcpy_htod(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
+crx_status crx_memcpy_dtoh(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
+crx_status crx_stream_synchronize(crx_context* ctx, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Camera helpers                                                       */
+/* ------------------------------------------------------------------ */
+
+/* Camera::createProjectionMatrix (VKRenderer.h:141-148): infinite far plane,
+ * f = 1/tan(fovy/2) in float; fovy in radians. */
+void crx_camera_projection(float near_, float fovy, float aspect, crx_mat4* out);
+/* GLM lookAt (right-handed), float arithmetic. */
+void crx_camera_look_at(const float eye[3], const float center[3], const float up[3], crx_mat4* out);
+/* GLM inverse / product in GLM's evaluation order (float). */
+void crx_mat4_inverse(const crx_mat4* m, crx_mat4* out);
+void crx_mat4_mul(const crx_mat4* a, const crx_mat4* b, crx_mat4* out);
+
+/* ------------------------------------------------------------------ */
+/* Scene API (replaces Scene / SNTriangles / Mesh and CuRast::draw)     */
+/* ------------------------------------------------------------------ */
+
+typedef struct crx_mesh crx_mesh;
+typedef struct crx_scene crx_scene;
+
+/* Mesh (Mesh.h:4-25).  Copies host arrays into device memory owned by the
+ * mesh: positions (3*numVertices floats), indices (3*numTriangles, or NULL for
+ * a non-indexed mesh with numVertices == 3*numTriangles), colors (numVertices
+ * RGBA8 words, nullable).  The AABB is computed from the positions. */
+crx_status crx_mesh_create(crx_context* ctx, const float* positions, uint32_t numVertices,
+                           const uint32_t* indices, uint32_t numTriangles,
+                           const uint32_t* colors, crx_mesh** out);
+/* Same, but wraps caller-owned device buffers (not freed by the mesh). */
+crx_status crx_mesh_create_device(crx_context* ctx, const float* d_positions, uint32_t numVertices,
+                                  const uint32_t* d_indices, uint32_t numTriangles,
+                                  const uint32_t* d_colors, const crx_box3* aabb, crx_mesh** out);
+crx_status crx_mesh_destroy(crx_mesh* mesh);
+crx_status crx_mesh_info(const crx_mesh* mesh, uint32_t* numTriangles, uint32_t* numVertices, crx_box3* aabb);
+/* Device pointers of the mesh buffers (colors may be NULL). */
+crx_status crx_mesh_buffers(const crx_mesh* mesh, const float** d_positions, const uint32_t** d_indices,
+                            const uint32_t** d_colors);
+
+crx_status crx_scene_create(crx_context* ctx, crx_scene** out);
+crx_status crx_scene_destroy(crx_scene* scene);
+/* SNTriangles node: mesh drawn with a world transform (column-major). */
+crx_status crx_scene_add_node(crx_scene* scene, crx_mesh* mesh, const crx_mat4* world, uint32_t* node_id);
+crx_status crx_scene_set_transform(crx_scene* scene, uint32_t node_id, const crx_mat4* world);
+crx_status crx_scene_num_nodes(const crx_scene* scene, uint32_t* out);
+
+/* View (VKRenderer.h:78-82): view matrix, projection and framebuffer size. */
+typedef struct crx_view {
+    crx_mat4 view;
+    crx_mat4 proj;
+    int32_t width;
+    int32_t height;
+} crx_view;
+
+/* Scene-level switches (CuRastSettings.h:31,49,55,60). */
+typedef struct crx_render_settings {
+    int32_t instanced;             /* RASTERIZER_VISBUFFER_INSTANCED vs RASTERIZER_VISBUFFER */
+    int32_t frustum_culling;       /* enableFrustumCulling (host AABB cull per node) */
+    int32_t resolve;               /* run the resolve pass */
+    int32_t attribute;             /* crx_display_attribute */
+} crx_render_settings;
+
+void crx_render_settings_default(crx_render_settings* out);
+
+/* CuRast::draw for one view: host node cull, sort/group instances,
+ * per-instance transforms and triangle prefix, clear, the three stages and
+ * (optionally) resolve.  framebuffer / colorbuffer are caller-owned device
+ * buffers of width*height uint64 (colorbuffer may be NULL when !resolve). */
+crx_status crx_render(crx_context* ctx, crx_scene* scene, const crx_view* view,
+                      const crx_render_settings* settings,
+                      uint64_t* framebuffer, uint64_t* colorbuffer, void* stream,
+                      crx_draw_stats* stats /* nullable */);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRX_H */

@@ -1,0 +1,128 @@
+"""Shared helpers of the test-suite: build GPU and oracle copies of one scene
+from the same host arrays and compare their buffers."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import crx
+import oracle as orc
+
+
+@dataclass
+class SceneSpec:
+    """Meshes and, per mesh, the world matrices of its instances."""
+    meshes: list = field(default_factory=list)        # list[orc.HostMesh]
+    worlds: list = field(default_factory=list)        # list[list[crx.Mat4]]
+
+    def add(self, mesh: "orc.HostMesh", worlds):
+        self.meshes.append(mesh)
+        self.worlds.append(list(worlds))
+        return self
+
+
+@dataclass
+class Frame:
+    """CMesh lists as CuRast::draw prepares them (CuRast_render.h:356-435)."""
+    unique: list
+    instances: list
+    transforms: list
+    prefix: np.ndarray
+
+
+def prepare(spec: SceneSpec, view: crx.Mat4, pointers, instanced_ids=True) -> Frame:
+    """pointers[k] = (positions, indices, colors) addresses of mesh k."""
+    unique, instances, transforms, prefix = [], [], [], []
+    total = 0
+    for k, (mesh, worlds) in enumerate(zip(spec.meshes, spec.worlds)):
+        first = len(instances)
+        for w in worlds:
+            m = crx.CMesh()
+            p, i, c = pointers[k]
+            m.numTriangles = mesh.num_triangles
+            m.positions = p
+            m.indices = i
+            m.colors = c
+            m.cummulativeTriangleCount = total
+            m.instances.offset = len(instances)
+            m.instances.count = 1
+            m.world = w
+            m.id = len(instances)
+            m.numVertices = len(mesh.positions)
+            m.isLoaded = 1
+            prefix.append(total)
+            total += mesh.num_triangles
+            instances.append(m)
+            transforms.append(crx.mat4_mul(view, w))
+        if worlds:
+            u = crx.CMesh.from_buffer_copy(instances[first])
+            u.instances.count = len(worlds)
+            unique.append(u)
+    return Frame(unique, instances, transforms, np.asarray(prefix, dtype=np.uint64))
+
+
+def host_pointers(spec: SceneSpec):
+    out = []
+    for m in spec.meshes:
+        out.append((m.positions.ctypes.data,
+                    m.indices.ctypes.data if m.indices is not None else None,
+                    m.colors.ctypes.data if m.colors is not None else None))
+    return out
+
+
+def oracle_render(spec: SceneSpec, W, H, view, proj, instanced=True, tuning=None, attribute=None,
+                  threads=0):
+    fr = prepare(spec, view, host_pointers(spec))
+    fb, st = orc.draw(crx, fr.unique, fr.instances, fr.transforms, W, H, view, proj, instanced,
+                      tuning=tuning, threads=threads)
+    cb = None
+    if attribute is not None:
+        cb = orc.resolve(crx, fr.instances, fr.prefix, fb, W, H, view, proj, attribute, threads=threads)
+    return fb, cb, st
+
+
+class GpuScene:
+    """Device copies of a SceneSpec's meshes on one context."""
+
+    def __init__(self, ctx: crx.Context, spec: SceneSpec):
+        self.ctx = ctx
+        self.spec = spec
+        self.pointers = []
+        for m in spec.meshes:
+            p = ctx.to_device(m.positions).ptr
+            i = ctx.to_device(m.indices).ptr if m.indices is not None else None
+            c = ctx.to_device(m.colors).ptr if m.colors is not None else None
+            self.pointers.append((p, i, c))
+
+    def render(self, W, H, view, proj, instanced=True, attribute=None, fb=None, cb=None, stream=None):
+        ctx = self.ctx
+        fr = prepare(self.spec, view, self.pointers)
+        fb = fb if fb is not None else ctx.alloc(8 * W * H)
+        cbuf = cb if cb is not None else (ctx.alloc(8 * W * H) if attribute is not None else None)
+        ctx.clear_framebuffer(fb, W * H, stream)
+        if cbuf is not None:
+            ctx.clear_framebuffer(cbuf, W * H, stream)
+        target = crx.make_target(fb, cbuf, W, H, view, proj)
+        st = ctx.draw_triangles_visbuffer(fr.unique, fr.instances, fr.transforms, target, instanced, stream)
+        out_cb = None
+        if attribute is not None:
+            ctx.resolve_visbuffer(fr.instances, fr.prefix, target, attribute, stream)
+            out_cb = cbuf.download(np.uint64, W * H, stream)
+        out_fb = fb.download(np.uint64, W * H, stream)
+        return out_fb, out_cb, st
+
+
+def describe_mismatch(a: np.ndarray, b: np.ndarray, W: int) -> str:
+    bad = np.nonzero(a != b)[0]
+    if bad.size == 0:
+        return "identical"
+    lines = [f"{bad.size} of {a.size} pixels differ"]
+    for i in bad[:8]:
+        lines.append(f"  px ({i % W},{i // W}): got {int(a[i]):#018x} want {int(b[i]):#018x}")
+    return "\n".join(lines)
+
+
+def rgba(cb: np.ndarray, W: int, H: int) -> np.ndarray:
+    return (cb & 0xFFFFFFFF).astype(np.uint32).view(np.uint8).reshape(H, W, 4)

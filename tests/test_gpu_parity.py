@@ -1,0 +1,215 @@
+"""Bit-exact parity of the gfx950 kernels with the CPU oracle.
+
+Every test renders the same host arrays through the C-ABI on the GPU and
+through the oracle on the CPU and requires identical visibility buffers (and
+colour buffers where resolved).  The configurations cover all three stages,
+both stage-1 walk orders, mirrored instances, near-plane crossings, widths
+that are not multiples of 64, non-default routing thresholds and queue
+overflow.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import crx
+import oracle as orc
+import workloads as wl
+from crxtest import GpuScene, SceneSpec, describe_mismatch, oracle_render
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = crx.Context(0)
+    yield c
+    c.close()
+
+
+def _check(ctx, spec, W, H, view, proj, instanced=True, attribute=crx.DISPLAY_VERTEX_COLORS,
+           tuning=None, expect_stage2=False, expect_stage3=False):
+    print(f"[case] W={W} H={H} instanced={instanced} attribute={attribute}", flush=True)
+    if tuning is not None:
+        ctx.tuning = tuning
+    try:
+        want_fb, want_cb, st = oracle_render(spec, W, H, view, proj, instanced=instanced,
+                                             tuning=tuning, attribute=attribute)
+        g = GpuScene(ctx, spec)
+        fb, cb, gst = g.render(W, H, view, proj, instanced=instanced, attribute=attribute)
+    finally:
+        ctx.tuning = crx.default_tuning()
+    assert gst.num_nontrivial == st.num_nontrivial
+    assert gst.num_huge == st.num_huge
+    if expect_stage2:
+        assert st.num_nontrivial > 0
+    if expect_stage3:
+        assert st.num_huge > 0
+    assert np.array_equal(fb, want_fb), describe_mismatch(fb, want_fb, W)
+    if attribute is not None:
+        assert np.array_equal(cb, want_cb), describe_mismatch(cb, want_cb, W)
+    return fb, st
+
+
+def test_sphere_1m_1080p(ctx):
+    """Config 1: 1,048,576-triangle displaced sphere, one 1920x1080 view, resolve to vertex colours."""
+    pos, idx, col = wl.sphere_mesh()
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
+    view, proj = wl.sphere_view(1920, 1080)
+    fb, st = _check(ctx, spec, 1920, 1080, view, proj)
+    assert int((fb != crx.CLEAR_PIXEL).sum()) > 300_000
+
+
+def test_sphere_non_instanced_odd_size(ctx):
+    pos, idx, col = wl.sphere_mesh(rings=128, segments=256)
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
+    view, proj = wl.sphere_view(1000, 701)
+    _check(ctx, spec, 1000, 701, view, proj, instanced=False)
+
+
+def test_instanced_with_mirrors(ctx):
+    """Instanced stage-1 walk (the _instanced kernels), including negative-scale instances."""
+    pos, idx, col = wl.sphere_mesh(rings=48, segments=96, seed=7)
+    worlds = list(wl.instance_transforms(60, extent=20.0, smin=0.5, smax=2.0, seed=11))
+    for k in range(0, 60, 5):                       # mirror every fifth instance
+        w = worlds[k]
+        for r in range(3):
+            w.m[r] = -w.m[r]
+    pos2, idx2, col2 = wl.sphere_mesh(rings=16, segments=32, radius=3.0, seed=9)
+    spec = (SceneSpec()
+            .add(orc.HostMesh(crx, pos, idx, col), worlds)
+            .add(orc.HostMesh(crx, pos2, idx2, col2), list(wl.instance_transforms(7, 20.0, seed=5))))
+    view, proj = wl.instances_view(extent=20.0, width=800, height=600, distance=30.0)
+    for instanced in (True, False):
+        _check(ctx, spec, 800, 600, view, proj, instanced=instanced)
+
+
+def _closeup_terrain():
+    pos, idx, col = wl.heightfield_mesh(129, spacing=1.0, amplitude=6.0, scale=32.0)
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
+    return spec
+
+
+def test_closeup_near_plane_stage2_stage3(ctx):
+    """Camera 1.5 units above a terrain: triangles span hundreds of pixels and
+    cross the camera plane, so stage 2 (wave per triangle) and stage 3 (tiles)
+    do most of the work."""
+    spec = _closeup_terrain()
+    W, H = 1280, 720
+    view = crx.look_at((64.0, 4.5, 20.0), (64.0, 2.0, 40.0), (0.0, 1.0, 0.0))
+    proj = wl.projection(W, H)
+    _check(ctx, spec, W, H, view, proj, expect_stage2=True, expect_stage3=True)
+
+
+@pytest.mark.parametrize("small,tile", [(32, 32), (8, 16), (256, 128), (128, 64)])
+def test_routing_tuning(ctx, small, tile):
+    """Non-default stage thresholds and tile sizes; the oracle routes identically."""
+    spec = _closeup_terrain()
+    W, H = 960, 540
+    view = crx.look_at((30.0, 12.0, 10.0), (64.0, 0.0, 64.0), (0.0, 1.0, 0.0))
+    proj = wl.projection(W, H)
+    t = crx.default_tuning()
+    t.small_threshold = small
+    t.tile_size = tile
+    _check(ctx, spec, W, H, view, proj, tuning=t)
+
+
+def test_queue_overflow_regrows(ctx):
+    spec = _closeup_terrain()
+    W, H = 1280, 720
+    view = crx.look_at((64.0, 4.5, 20.0), (64.0, 2.0, 40.0), (0.0, 1.0, 0.0))
+    proj = wl.projection(W, H)
+    want, _, st = oracle_render(spec, W, H, view, proj)
+    assert st.num_nontrivial > 8 and st.num_huge > 8
+    fresh = crx.Context(0)
+    try:
+        t = crx.default_tuning()
+        t.max_nontrivial = 4
+        t.max_huge = 4
+        fresh.tuning = t
+        fb, _, gst = GpuScene(fresh, spec).render(W, H, view, proj)
+    finally:
+        fresh.close()
+    assert gst.redraws >= 1
+    assert np.array_equal(fb, want), describe_mismatch(fb, want, W)
+
+
+def test_heightfield_oblique(ctx):
+    """Scaled-down config 2: 2049^2-vertex terrain (8.4M triangles), oblique view."""
+    n = 2049
+    pos, idx, col = wl.heightfield_mesh(n)
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
+    view, proj = wl.heightfield_view(n, 1920, 1080, altitude=150.0, setback=100.0)
+    _check(ctx, spec, 1920, 1080, view, proj)
+
+
+@pytest.mark.parametrize("attribute", [crx.DISPLAY_NONE, crx.DISPLAY_TRIANGLE_ID, crx.DISPLAY_MESH_ID,
+                                       crx.DISPLAY_NORMAL])
+def test_resolve_attributes(ctx, attribute):
+    pos, idx, col = wl.sphere_mesh(rings=64, segments=128)
+    worlds = list(wl.instance_transforms(5, extent=4.0, seed=3))
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), worlds)
+    view, proj = wl.instances_view(extent=4.0, width=640, height=480, distance=9.0)
+    _check(ctx, spec, 640, 480, view, proj, attribute=attribute)
+
+
+def test_resolve_rejects_texture(ctx):
+    fb = ctx.alloc(8 * 16)
+    cb = ctx.alloc(8 * 16)
+    view, proj = wl.sphere_view(4, 4)
+    target = crx.make_target(fb, cb, 4, 4, view, proj)
+    with pytest.raises(crx.CrxError) as e:
+        ctx.resolve_visbuffer([], [], target, crx.DISPLAY_TEXTURE)
+    assert e.value.status == 6
+
+
+def test_scene_api_matches_oracle_frame(ctx):
+    """crx_render (scene API: node cull, sort/group, prefix, clear, stages,
+    resolve) against the oracle's restatement of CuRast::draw."""
+    import ctypes as C
+    pos, idx, col = wl.sphere_mesh(rings=32, segments=64, seed=21)
+    pos2, idx2, col2 = wl.sphere_mesh(rings=24, segments=48, radius=2.0, seed=22)
+    meshes = [crx.Mesh(ctx, pos, idx, col), crx.Mesh(ctx, pos2, idx2, col2)]
+    hosts = [orc.HostMesh(crx, pos, idx, col), orc.HostMesh(crx, pos2, idx2, col2)]
+    scene = crx.Scene(ctx)
+    worlds = list(wl.instance_transforms(30, extent=30.0, seed=23))
+    order = []
+    for k, w in enumerate(worlds):
+        mk = k % 3 == 0
+        scene.add(meshes[1 if mk else 0], w)
+        order.append(1 if mk else 0)
+    W, H = 800, 450
+    view, proj = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
+    v = crx.make_view(view, proj, W, H)
+    fb = ctx.alloc(8 * W * H)
+    cb = ctx.alloc(8 * W * H)
+    scene.render(v, fb, cb, crx.render_settings())
+    got_fb = fb.download(np.uint64, W * H)
+    got_cb = cb.download(np.uint64, W * H)
+
+    L = orc.lib(crx)
+    n = len(worlds)
+    node_meshes = (crx.CMesh * n)()
+    node_world = (crx.Mat4 * n)()
+    for k in range(n):
+        hm = hosts[order[k]]
+        cm = hm.cmesh(crx)
+        b = meshes[order[k]].aabb()
+        cm.aabb = b
+        cm.address = order[k] + 1
+        node_meshes[k] = cm
+        node_world[k] = worlds[k]
+    uniq = (crx.CMesh * n)()
+    nu = C.c_uint32()
+    inst = (crx.CMesh * n)()
+    xf = (crx.Mat4 * n)()
+    pre = (C.c_uint64 * n)()
+    nv = L.crx_oracle_prepare_frame(node_meshes, node_world, n, C.byref(v), 1, 0, uniq, C.byref(nu), inst, xf, pre)
+    assert 0 < nv <= n
+    for k in range(nv):
+        inst[k].id = -1
+    fbw, _ = orc.draw(crx, list(uniq)[:nu.value], list(inst)[:nv], list(xf)[:nv], W, H, view, proj, True)
+    assert np.array_equal(got_fb, fbw), describe_mismatch(got_fb, fbw, W)
+    cbw = orc.resolve(crx, list(inst)[:nv], list(pre)[:nv], fbw, W, H, view, proj, crx.DISPLAY_VERTEX_COLORS)
+    assert np.array_equal(got_cb, cbw), describe_mismatch(got_cb, cbw, W)
+    scene.close()
