@@ -130,7 +130,8 @@ class Tuning(C.Structure):
     _fields_ = [("small_threshold", C.c_uint32), ("tile_size", C.c_uint32),
                 ("max_nontrivial", C.c_uint32), ("max_huge", C.c_uint32),
                 ("stage1_block", C.c_uint32), ("stage1_waves_per_cu", C.c_uint32),
-                ("stage2_waves_per_cu", C.c_uint32), ("stage3_waves_per_cu", C.c_uint32)]
+                ("stage2_waves_per_cu", C.c_uint32), ("stage3_waves_per_cu", C.c_uint32),
+                ("stage1_cull_order", C.c_uint32), ("stage1_order", C.c_uint32)]
 
 
 class DrawStats(C.Structure):
@@ -196,6 +197,7 @@ _SIGS = {
     "crx_memcpy_htod": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_memcpy_dtoh": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_stream_synchronize": (C.c_int, [_VP, _VP]),
+    "crx_copy_buffer": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
     "crx_draw_triangles_visbuffer": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(CMesh),
                                                C.c_uint32, C.POINTER(Mat4), C.POINTER(RenderTarget),
@@ -385,6 +387,9 @@ class Context:
 
     def synchronize(self, stream=None) -> None:
         _check(lib().crx_stream_synchronize(self.handle, stream))
+
+    def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int, stream=None) -> None:
+        _check(lib().crx_copy_buffer(self.handle, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, stream))
 
     # --- kernel-level entry points ------------------------------------------------
 

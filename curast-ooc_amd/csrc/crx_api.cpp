@@ -251,8 +251,10 @@ crx_status check_tuning(const crx_tuning& t) {
     if (t.tile_size < 64 && (64 % t.tile_size) != 0)
         return fail(CRX_INVALID_ARG, "tuning.tile_size below 64 must divide 64");
     const uint32_t b = t.stage1_block;
-    if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
-        return fail(CRX_INVALID_ARG, "tuning.stage1_block must be 64, 128, 256, 512 or 1024");
+    if (b != 128 && b != 256 && b != 512)
+        return fail(CRX_INVALID_ARG, "tuning.stage1_block must be 128, 256 or 512");
+    if (t.stage1_cull_order > 1) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0 or 1");
+    if (t.stage1_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_order must be 0, 1 or 2");
     if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
     if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
         t.stage3_waves_per_cu > 64 || t.stage1_waves_per_cu > 64)
@@ -349,6 +351,8 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage1_waves_per_cu = 32;
     out->stage2_waves_per_cu = 16;
     out->stage3_waves_per_cu = 16;
+    out->stage1_cull_order = 1;
+    out->stage1_order = 1;
 }
 
 crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {
@@ -482,6 +486,15 @@ crx_status crx_stream_synchronize(crx_context* ctx, void* stream) {
     return CRX_OK;
 }
 
+crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+    if (!ctx || !dst || !src) return fail(CRX_INVALID_ARG, "NULL argument");
+    if ((bytes % 16) != 0 || ((uintptr_t)dst % 16) != 0 || ((uintptr_t)src % 16) != 0)
+        return fail(CRX_INVALID_ARG, "crx_copy_buffer needs 16-byte aligned pointers and size");
+    CRX_HIP(hipSetDevice(ctx->device));
+    CRX_HIP(launch_copy(dst, src, bytes, ctx->numCUs, as_stream(stream)));
+    return CRX_OK;
+}
+
 crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void* stream) {
     if (!framebuffer && numPixels) return fail(CRX_INVALID_ARG, "framebuffer is NULL");
     if (numPixels == 0) return CRX_OK;
@@ -507,8 +520,9 @@ crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshe
     CRX_HIP(hipSetDevice(ctx->device));
     const crx_tuning& tu = ctx->tuning;
     const uint32_t block = tu.stage1_block;
-    uint32_t batchShift = 0;
+    uint32_t batchShift = 0;   // log2(triangles per batch)
     while ((1u << batchShift) < block) batchShift++;
+    const uint64_t batchTris = 1ull << batchShift;
 
     // ---- host records -----------------------------------------------------
     const crx_cmesh* list = instanced ? meshes : instances;
@@ -530,7 +544,7 @@ crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshe
         it.instOffset = (uint32_t)off;
         it.instCount = (uint32_t)cnt;
         it.firstBatch = (uint32_t)batches;
-        batches += ((uint64_t)m.numTriangles + block - 1) >> batchShift;
+        batches += ((uint64_t)m.numTriangles + batchTris - 1) >> batchShift;
         tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
         if (batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
     }
@@ -586,6 +600,8 @@ crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshe
     P.smallThreshold = tu.small_threshold;
     P.tile = (int32_t)tu.tile_size;
     P.batchShift = batchShift;
+    P.order = tu.stage1_order;
+    P.cullOrder = tu.stage1_cull_order;
 
     uint32_t g1;
     if (tu.stage1_waves_per_cu == 0) {
@@ -597,6 +613,8 @@ crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshe
     g1 = (g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
     const uint32_t g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4);
     const uint32_t g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
+    P.numWaves2 = g2 * 4u;
+    P.numWaves3 = g3 * 4u;
 
     uint32_t redraws = 0;
     for (;;) {

@@ -80,8 +80,11 @@ struct RasterParams {
     float faI, fI;           // 1/(f/aspect), 1/f (stage 3)
     uint32_t smallThreshold;
     int32_t tile;
-    uint32_t batchShift;     // log2(triangles per stage-1 batch)
-    uint32_t pad0;
+    uint32_t batchShift;     // log2(triangles per stage-1 batch = workgroup size)
+    uint32_t order;          // stage-1 batch order (crx_tuning::stage1_order)
+    uint32_t cullOrder;      // 0 = backface first, 1 = frustum first
+    uint32_t numWaves2;      // waves in the stage-2 grid
+    uint32_t numWaves3;      // waves in the stage-3 grid
 };
 
 struct ResolveParams {

@@ -13,5 +13,6 @@ hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, h
 hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t stream);
 hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream);
 hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, int numCUs, hipStream_t stream);
 
 }  // namespace crx

@@ -13,6 +13,8 @@
 // rewrite of "RN(p/d) > 1" (see frustum_outcode).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "crx_device.h"
 #include "crx_internal.h"
 
@@ -72,33 +74,36 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
     if (da <= 0.0f && db <= 0.0f && dc <= 0.0f) return;
     bool nontrivial = (da <= 0.0f || db <= 0.0f || dc <= 0.0f);
 
-    // backface culling in view space (:168-188)
-    {
-        const f3 ab = sub3(bv, av);
-        const f3 ac = sub3(cv, av);
-        float res = dot3(av, cross3(ab, ac));
-        if (flip) res = -res;
-        if (res >= 0.0f) return;
-    }
-
     const float fa = P.fa, f = P.f;
     const float pxa = fa * av.x, pxb = fa * bv.x, pxc = fa * cv.x;
     const float pya = f * av.y, pyb = f * bv.y, pyc = f * cv.y;
 
+    // backface culling in view space (:168-188)
+    auto backfacing = [&]() {
+        const f3 ab = sub3(bv, av);
+        const f3 ac = sub3(cv, av);
+        float res = dot3(av, cross3(ab, ac));
+        if (flip) res = -res;
+        return res >= 0.0f;
+    };
     // frustum culling (:113-118), only when every depth is positive
-    if (!nontrivial) {
+    auto outside = [&]() {
+        if (nontrivial) return false;
         if (da >= 0x1p-102f && db >= 0x1p-102f && dc >= 0x1p-102f) {
-            const uint32_t oc = frustum_outcode(pxa, pya, da) & frustum_outcode(pxb, pyb, db) &
-                                frustum_outcode(pxc, pyc, dc);
-            if (oc) return;
-        } else {
-            const float ax = pxa / da, bx = pxb / db, cx = pxc / dc;
-            const float ay = pya / da, by = pyb / db, cy = pyc / dc;
-            if (ax > 1.0f && bx > 1.0f && cx > 1.0f) return;
-            if (ax < -1.0f && bx < -1.0f && cx < -1.0f) return;
-            if (ay > 1.0f && by > 1.0f && cy > 1.0f) return;
-            if (ay < -1.0f && by < -1.0f && cy < -1.0f) return;
+            return (frustum_outcode(pxa, pya, da) & frustum_outcode(pxb, pyb, db) &
+                    frustum_outcode(pxc, pyc, dc)) != 0u;
         }
+        const float ax = pxa / da, bx = pxb / db, cx = pxc / dc;
+        const float ay = pya / da, by = pyb / db, cy = pyc / dc;
+        return (ax > 1.0f && bx > 1.0f && cx > 1.0f) || (ax < -1.0f && bx < -1.0f && cx < -1.0f) ||
+               (ay > 1.0f && by > 1.0f && cy > 1.0f) || (ay < -1.0f && by < -1.0f && cy < -1.0f);
+    };
+    if (P.cullOrder == 0) {
+        if (backfacing()) return;
+        if (outside()) return;
+    } else {
+        if (outside()) return;
+        if (backfacing()) return;
     }
 
     // viewToNDC (rasterization_helpers.cuh:95-101) and ndcToScreen - 0.5 (:103-108, :132-134)
@@ -191,50 +196,139 @@ __global__ __launch_bounds__(64) void k_reset_counters(uint32_t* __restrict__ ct
     if (threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
 }
 
-// Stage 1.  Each workgroup owns a contiguous range of triangle batches within
-// its XCD's share of the batch list (workgroups are dispatched round-robin over
-// the 8 XCDs, so workgroup b runs on XCD b % 8); neighbouring batches share
-// vertices, and keeping them on one XCD keeps those vertices in one L2.
+namespace {
+
+// Loads through the constant address space are scalar (s_load) when the index
+// is uniform; the draw records are read-only for the whole kernel.  Loads
+// through the global address space avoid flat instructions.  (The host pass
+// of the compiler sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* kconst(const T* p) {
+    return (const __attribute__((address_space(4))) T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* kglobal(const T* p) {
+    return (const __attribute__((address_space(1))) T*)(p);
+}
+#else
+template <typename T>
+__device__ __forceinline__ const T* kconst(const T* p) { return p; }
+template <typename T>
+__device__ __forceinline__ const T* kglobal(const T* p) { return p; }
+#endif
+
+// last draw item whose firstBatch <= batch
+__device__ __forceinline__ uint32_t find_item(const RasterParams& P, uint32_t batch) {
+    const auto items = kconst(P.items);
+    uint32_t lo = 0, hi = P.numItems - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (items[mid].firstBatch <= batch) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+struct S1Batch {
+    DrawItem it;         // uniform
+    uint32_t itemEnd;    // first batch past this item (uniform)
+    uint32_t item;       // uniform
+    uint32_t tri;        // this lane's triangle
+    uint32_t i0, i1, i2; // its vertex indices
+};
+
+// Locate `batch`'s item (advancing from `cur` when possible) and issue the
+// index loads of this lane's triangle.  Lanes past the item's last triangle
+// load the last triangle instead (a valid address) and are masked later.
+__device__ __forceinline__ void s1_fetch(const RasterParams& P, uint32_t batch, S1Batch& b, bool haveItem) {
+    if (!haveItem || batch >= b.itemEnd) {
+        b.item = find_item(P, batch);
+        const auto items = kconst(P.items);
+        b.it = items[b.item];
+        b.itemEnd = (b.item + 1u < P.numItems) ? items[b.item + 1u].firstBatch : P.numBatches;
+    }
+    b.tri = ((batch - b.it.firstBatch) << P.batchShift) + threadIdx.x;
+    const uint32_t t = min(b.tri, b.it.numTriangles - 1u);
+    if (b.it.indices) {
+        const auto ip = kglobal(b.it.indices) + 3ull * t;
+        b.i0 = ip[0];
+        b.i1 = ip[1];
+        b.i2 = ip[2];
+    } else {
+        b.i0 = 3u * t;
+        b.i1 = b.i0 + 1u;
+        b.i2 = b.i0 + 2u;
+    }
+}
+
+__device__ __forceinline__ f3 load_pos_g(const float* positions, uint32_t i) {
+    const auto p = kglobal(positions) + 3ull * i;
+    return f3{p[0], p[1], p[2]};
+}
+
+}  // namespace
+
+// Stage 1.  A batch is BLOCK consecutive triangles of one draw item, lane t of
+// the workgroup takes triangle t (a wave's index load reads 768 contiguous
+// bytes).  The loop is software-pipelined: the index loads of the next batch
+// are issued before the vertex loads of the current one complete, so one
+// index-fetch latency is hidden behind every batch.
+//
+// Batch order (P.order): 0 gives each XCD a contiguous share of the batch list
+// (workgroup b is dispatched to XCD b % 8, so blocks b, b+8, ... split the
+// share), keeping the vertices neighbouring batches share in one L2; 1 is a
+// plain grid-stride loop; 2 gives each workgroup one contiguous range.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
     const uint32_t numBatches = P.numBatches;
     const uint32_t G = gridDim.x;
-    const uint32_t xcd = blockIdx.x & 7u;
-    const uint32_t local = blockIdx.x >> 3;
-    const uint32_t perXcdBlocks = G >> 3;          // G is a multiple of 8
-    const uint32_t perXcd = (numBatches + 7u) >> 3;
-    const uint32_t begin = xcd * perXcd;
-    const uint32_t end = min(numBatches, begin + perXcd);
-    for (uint32_t batch = begin + local; batch < end; batch += perXcdBlocks) {
-        // draw item holding this batch: last item with firstBatch <= batch
-        uint32_t lo = 0, hi = P.numItems - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1u) >> 1;
-            if (P.items[mid].firstBatch <= batch) lo = mid; else hi = mid - 1u;
+    uint32_t begin, end, step;
+    if (P.order == 0) {
+        const uint32_t perXcd = (numBatches + 7u) >> 3;
+        begin = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
+        end = min(numBatches, (blockIdx.x & 7u) * perXcd + perXcd);
+        step = G >> 3;                                  // G is a multiple of 8
+    } else if (P.order == 1) {
+        begin = blockIdx.x;
+        end = numBatches;
+        step = G;
+    } else {
+        const uint32_t per = (numBatches + G - 1u) / G;
+        begin = blockIdx.x * per;
+        end = min(numBatches, begin + per);
+        step = 1;
+    }
+    if (begin >= end) return;
+    const auto transforms = kconst(P.transforms);
+
+    S1Batch cur;
+    s1_fetch(P, begin, cur, false);
+    for (uint32_t batch = begin;;) {
+        // vertex loads of this batch
+        const f3 A = load_pos_g(cur.it.positions, cur.i0);
+        const f3 B = load_pos_g(cur.it.positions, cur.i1);
+        const f3 C = load_pos_g(cur.it.positions, cur.i2);
+        // index loads of the next batch
+        const uint32_t next = batch + step;
+        const bool more = next < end;
+        S1Batch nxt = cur;
+        if (more) s1_fetch(P, next, nxt, true);
+
+        if (cur.tri < cur.it.numTriangles) {
+            const uint32_t tri = cur.tri;
+            for (uint32_t j = 0; j < cur.it.instCount; j++) {
+                const uint32_t inst = cur.it.instOffset + j;
+                const auto T = transforms + inst;
+                float m[16];
+                #pragma unroll
+                for (int e = 0; e < 16; e++) m[e] = T->m[e];
+                const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
+                stage1_triangle(P, m, T->flip != 0u, A, B, C, id, ((uint64_t)inst << 32) | (uint64_t)tri);
+            }
         }
-        const DrawItem it = P.items[lo];
-        const uint32_t tri = ((batch - it.firstBatch) << P.batchShift) + threadIdx.x;
-        if (tri >= it.numTriangles) continue;
-        uint32_t i0, i1, i2;
-        if (it.indices) {
-            const uint32_t* ip = it.indices + 3ull * tri;
-            i0 = ip[0];
-            i1 = ip[1];
-            i2 = ip[2];
-        } else {
-            i0 = 3u * tri;
-            i1 = i0 + 1u;
-            i2 = i0 + 2u;
-        }
-        const f3 A = load_pos(it.positions, i0);
-        const f3 B = load_pos(it.positions, i1);
-        const f3 C = load_pos(it.positions, i2);
-        for (uint32_t j = 0; j < it.instCount; j++) {
-            const Transform& T = P.transforms[it.instOffset + j];
-            const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(j * it.numTriangles);
-            const uint64_t entry = ((uint64_t)(it.instOffset + j) << 32) | (uint64_t)tri;
-            stage1_triangle(P, T.m, T.flip != 0u, A, B, C, id, entry);
-        }
+        if (!more) break;
+        batch = next;
+        cur = nxt;
     }
 }
 
@@ -365,12 +459,21 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
     S2Setup* setup = sh_setup[wid];
     const uint32_t count = min(P.ctr[CTR_NONTRIVIAL], P.ntCap);
     const int TILE = P.tile;
+    // first claim is static (wave w takes entries [64w, 64w+64)), later claims
+    // come from the shared cursor, so waves beyond the queue exit without an atomic
+    const uint32_t waveId = blockIdx.x * 4u + wid;
+    bool first = true;
 
     while (true) {
         wave_sync();
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(P.ctr + CTR_CURSOR2, (uint32_t)kWave);
-        base = __shfl(base, 0);
+        if (first) {
+            base = waveId * (uint32_t)kWave;
+            first = false;
+        } else {
+            if (lane == 0) base = P.numWaves2 * (uint32_t)kWave + atomicAdd(P.ctr + CTR_CURSOR2, (uint32_t)kWave);
+            base = __shfl(base, 0);
+        }
         if (base >= count) break;
         const uint32_t idx = base + lane;
         bool raster = false;
@@ -542,10 +645,18 @@ __global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
     const int lanesPerRow = TILE < kWave ? TILE : kWave;
     const int rowsPerStep = kWave / lanesPerRow;
     const int colsPerLane = TILE / lanesPerRow;
+    // static first claim (entry = wave id), then the shared cursor
+    const uint32_t waveId = blockIdx.x * 4u + (threadIdx.x / kWave);
+    bool first = true;
     while (true) {
         uint32_t idx = 0;
-        if (lane == 0) idx = atomicAdd(P.ctr + CTR_CURSOR3, 1u);
-        idx = __shfl(idx, 0);
+        if (first) {
+            idx = waveId;
+            first = false;
+        } else {
+            if (lane == 0) idx = P.numWaves3 + atomicAdd(P.ctr + CTR_CURSOR3, 1u);
+            idx = __shfl(idx, 0);
+        }
         if (idx >= count) break;
         const int4 e = P.huge[idx];
         const InstanceRec& mesh = P.inst[e.x];
@@ -740,6 +851,12 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveParams R) {
     }
 }
 
+// Plain streaming copy, 16 bytes per lane (calibrates the HBM traffic counters).
+__global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launch wrappers used by the host glue.
 // ---------------------------------------------------------------------------
@@ -760,11 +877,9 @@ hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream) {
 
 hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
     switch (block) {
-        case 64:  hipLaunchKernelGGL(k_stage1<64>, dim3(grid), dim3(64), 0, stream, P); break;
         case 128: hipLaunchKernelGGL(k_stage1<128>, dim3(grid), dim3(128), 0, stream, P); break;
         case 256: hipLaunchKernelGGL(k_stage1<256>, dim3(grid), dim3(256), 0, stream, P); break;
         case 512: hipLaunchKernelGGL(k_stage1<512>, dim3(grid), dim3(512), 0, stream, P); break;
-        case 1024: hipLaunchKernelGGL(k_stage1<1024>, dim3(grid), dim3(1024), 0, stream, P); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -777,6 +892,14 @@ hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t strea
 
 hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream) {
     hipLaunchKernelGGL(k_stage3, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, int numCUs, hipStream_t stream) {
+    const size_t n = bytes / 16;
+    uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)numCUs * 32);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, stream, (uint4*)dst, (const uint4*)src, n);
     return hipGetLastError();
 }
 

@@ -167,10 +167,14 @@ typedef struct crx_tuning {
     uint32_t tile_size;           /* TILE_SIZE = 64: stage-2 huge limit (tile^2) and stage-3 tile edge; 16..256, multiple of 16 */
     uint32_t max_nontrivial;      /* MAX_NONTRIVIAL_TRIANGLES = 5e6: initial stage-2 queue capacity */
     uint32_t max_huge;            /* MAX_HUGE_TRIANGLES = 5e6: initial stage-3 queue capacity */
-    uint32_t stage1_block;        /* threads per stage-1 workgroup (64..1024, multiple of 64) */
-    uint32_t stage1_waves_per_cu; /* stage-1 grid = CUs * this / (stage1_block/64); 0 = one wave per 256 triangles */
+    uint32_t stage1_block;        /* threads per stage-1 workgroup: 64, 128, 256, 512 or 1024 */
+    uint32_t stage1_waves_per_cu; /* stage-1 grid = CUs * this / (stage1_block/64); 0 = one workgroup per batch */
     uint32_t stage2_waves_per_cu; /* stage-2 persistent grid size in waves per CU */
     uint32_t stage3_waves_per_cu; /* stage-3 persistent grid size in waves per CU */
+    uint32_t stage1_cull_order;   /* 0 = backface test before the frustum test, 1 = frustum first
+                                     (both are exact early-outs; the order only changes cost) */
+    uint32_t stage1_order;        /* batch-to-workgroup order: 0 = contiguous share per XCD,
+                                     1 = grid-stride, 2 = contiguous share per workgroup */
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */
@@ -262,6 +266,10 @@ crx_status crx_device_free(crx_context* ctx, void* ptr);
 crx_status crx_memcpy_htod(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 crx_status crx_memcpy_dtoh(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 crx_status crx_stream_synchronize(crx_context* ctx, void* stream);
+/* Plain device-to-device copy kernel (16 bytes per lane).  No counterpart in
+ * the reference: it is the known-size kernel that calibrates HBM traffic
+ * counters (FETCH_SIZE / WRITE_SIZE) for the roofline measurement. */
+crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Camera helpers                                                       */
