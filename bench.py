@@ -109,6 +109,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run the 1 GiB copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--tuning", default="",
+                    help="comma-separated crx_tuning overrides, e.g. stage1_order=0,stage1_block=256")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -121,6 +123,13 @@ def main() -> int:
     pos, idx, col, worlds, view, proj, desc = build_workload(args.workload)
     log(f"[bench] generated {desc['workload']} in {time.perf_counter() - t0:.1f} s")
     ctx = crx.Context(0)
+    if args.tuning:
+        tu = ctx.tuning
+        for kv in args.tuning.split(","):
+            k, v = kv.split("=")
+            setattr(tu, k.strip(), int(v))
+        ctx.tuning = tu
+        log(f"[bench] tuning overrides: {args.tuning}")
     mesh = crx.Mesh(ctx, pos, idx, col)
     scene = crx.Scene(ctx)
     for w in worlds:

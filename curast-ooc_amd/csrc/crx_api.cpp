@@ -253,7 +253,8 @@ crx_status check_tuning(const crx_tuning& t) {
     const uint32_t b = t.stage1_block;
     if (b != 128 && b != 256 && b != 512)
         return fail(CRX_INVALID_ARG, "tuning.stage1_block must be 128, 256 or 512");
-    if (t.stage1_cull_order > 1) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0 or 1");
+    if (t.stage1_cull_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0, 1 or 2");
+    if (t.stage1_fast_tiny > 1) return fail(CRX_INVALID_ARG, "tuning.stage1_fast_tiny must be 0 or 1");
     if (t.stage1_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_order must be 0, 1 or 2");
     if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
     if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
@@ -353,6 +354,8 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage3_waves_per_cu = 16;
     out->stage1_cull_order = 1;
     out->stage1_order = 1;
+    out->stage1_fast_tiny = 1;
+    out->reserved = 0;
 }
 
 crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {
@@ -602,6 +605,10 @@ crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshe
     P.batchShift = batchShift;
     P.order = tu.stage1_order;
     P.cullOrder = tu.stage1_cull_order;
+    P.fastTiny = tu.stage1_fast_tiny;
+    // |v_rcp_f32 screen coordinate - IEEE one| <= 0.75*max(W,H)*2^-20 for |ndc| <= 2;
+    // the margin is 5x that plus an absolute 2^-6
+    P.tinyMargin = (float)std::max(target->width, target->height) * 0x1p-18f + 0x1p-6f;
 
     uint32_t g1;
     if (tu.stage1_waves_per_cu == 0) {

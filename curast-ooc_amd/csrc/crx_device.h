@@ -82,7 +82,9 @@ struct RasterParams {
     int32_t tile;
     uint32_t batchShift;     // log2(triangles per stage-1 batch = workgroup size)
     uint32_t order;          // stage-1 batch order (crx_tuning::stage1_order)
-    uint32_t cullOrder;      // 0 = backface first, 1 = frustum first
+    uint32_t cullOrder;      // 0 = backface first, 1 = frustum (y then x) first, 2 = frustum (x then y) first
+    uint32_t fastTiny;       // filtered tiny-triangle cull (crx_tuning::stage1_fast_tiny)
+    float tinyMargin;        // bound on |approximate - IEEE| screen coordinates, with slack
     uint32_t numWaves2;      // waves in the stage-2 grid
     uint32_t numWaves3;      // waves in the stage-3 grid
 };

@@ -49,61 +49,126 @@ __device__ __forceinline__ f3 load_pos(const float* __restrict__ positions, uint
 // outside that range the rounded difference cannot cross the threshold.  The
 // same argument gives RN(p/d) < -1 <=> p + d < -d*2^-24.  Infinite and NaN p
 // compare exactly like the quotient they stand for.
-__device__ __forceinline__ uint32_t frustum_outcode(float px, float py, float d) {
+__device__ __forceinline__ uint32_t outcode_axis(float p, float d) {
     const float thr = d * 0x1p-24f;
-    uint32_t oc = 0;
-    oc |= (px - d > thr) ? 1u : 0u;
-    oc |= (px + d < -thr) ? 2u : 0u;
-    oc |= (py - d > thr) ? 4u : 0u;
-    oc |= (py + d < -thr) ? 8u : 0u;
-    return oc;
+    return ((p - d > thr) ? 1u : 0u) | ((p + d < -thr) ? 2u : 0u);
+}
+
+// One view-space row of M * (x, y, z, 1): (r0*x + r1*y) + (r2*z + r3).
+CRX_HD float xform_row(const float* m, int r, f3 p) {
+    return (m[r] * p.x + m[4 + r] * p.y) + (m[8 + r] * p.z + m[12 + r]);
+}
+
+// Screen x (or y) of a vertex, -0.5 shifted (ndcToScreen, :103-108, :132-134),
+// from an NDC value.
+__device__ __forceinline__ float to_screen(float ndc, float size) {
+    return (ndc * 0.5f + 0.5f) * size - 0.5f;
+}
+
+// Exact tiny-triangle cull of one axis decided from approximate coordinates.
+// The reference culls the axis when its clipped bounding box [lo, hi] holds no
+// sample: lo > floor(lo) and hi < floor(lo) + 1.  lo' and hi' are within `m`
+// of lo and hi; if [lo'-m, lo'+m] lies strictly inside (k-1, k) with
+// k = ceil(lo'-m), then lo is not an integer and ceil(lo) = k, and
+// hi' + m < k gives hi < k: the reference culls the axis.
+__device__ __forceinline__ bool surely_culled_axis(float lo, float hi, float m) {
+    const float k = ceilf(lo - m);
+    return (lo + m < k) && (hi + m < k);
 }
 
 // rasterize<INDEXBUFFER, UNCOMPRESSED>, triangles_visbuffer.cu:83-271, for one
 // triangle-instance.  `id` is the packed triangle ID the reference forms at
 // :253 and `queueEntry` the stage-2 entry it forms at :193-195.
+//
+// The reference's early-outs (all behind, frustum, tiny, backface) are pure
+// predicates, so any order gives the same result; coordinates are computed
+// lazily in the order the cheapest test needs them.  Two of the tests are
+// decided without IEEE divisions when that is provably exact:
+//  * frustum: outcode_axis rewrites RN(p/d) > 1 as p - d > d*2^-24;
+//  * tiny cull (P.fastTiny): screen coordinates from v_rcp_f32 (1 ulp) are
+//    within P.tinyMargin of the IEEE ones for |ndc| <= 2 (the margin covers
+//    the quotient error 2^-21*|ndc| scaled by W/2 plus the roundings of the
+//    screen transform, with slack); only triangles that are culled for sure
+//    leave early, all others take the exact path.
 __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const float* __restrict__ m,
                                                 bool flip, f3 A, f3 B, f3 C,
                                                 uint64_t id, uint64_t queueEntry) {
-    const f3 av = xform_point(m, A);
-    const f3 bv = xform_point(m, B);
-    const f3 cv = xform_point(m, C);
-    const float da = -av.z, db = -bv.z, dc = -cv.z;   // ndc.z = depth = -view.z
-
+    // depth first: ndc.z = depth = -view.z
+    const float da = -xform_row(m, 2, A), db = -xform_row(m, 2, B), dc = -xform_row(m, 2, C);
     // all vertices behind the camera (:119)
     if (da <= 0.0f && db <= 0.0f && dc <= 0.0f) return;
     bool nontrivial = (da <= 0.0f || db <= 0.0f || dc <= 0.0f);
-
+    const bool safeDepth = (da >= 0x1p-102f && db >= 0x1p-102f && dc >= 0x1p-102f);
     const float fa = P.fa, f = P.f;
-    const float pxa = fa * av.x, pxb = fa * bv.x, pxc = fa * cv.x;
-    const float pya = f * av.y, pyb = f * bv.y, pyc = f * cv.y;
+
+    // frustum culling (:113-118), only when every depth is positive; the axis
+    // the view culls most is tested before the other coordinate is computed
+    const bool frustumFirst = P.cullOrder != 0;
+    float vya = 0.0f, vyb = 0.0f, vyc = 0.0f, vxa = 0.0f, vxb = 0.0f, vxc = 0.0f;
+    bool haveX = false, haveY = false;
+    if (frustumFirst && !nontrivial && safeDepth) {
+        if (P.cullOrder == 1) {
+            vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
+            haveY = true;
+            if (outcode_axis(f * vya, da) & outcode_axis(f * vyb, db) & outcode_axis(f * vyc, dc)) return;
+            vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C);
+            haveX = true;
+            if (outcode_axis(fa * vxa, da) & outcode_axis(fa * vxb, db) & outcode_axis(fa * vxc, dc)) return;
+        } else {
+            vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C);
+            haveX = true;
+            if (outcode_axis(fa * vxa, da) & outcode_axis(fa * vxb, db) & outcode_axis(fa * vxc, dc)) return;
+            vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
+            haveY = true;
+            if (outcode_axis(f * vya, da) & outcode_axis(f * vyb, db) & outcode_axis(f * vyc, dc)) return;
+        }
+    }
+    if (!haveX) { vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C); }
+    if (!haveY) { vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C); }
+    const f3 av{vxa, vya, -da}, bv{vxb, vyb, -db}, cv{vxc, vyc, -dc};
+    const float pxa = fa * vxa, pxb = fa * vxb, pxc = fa * vxc;
+    const float pya = f * vya, pyb = f * vyb, pyc = f * vyc;
 
     // backface culling in view space (:168-188)
-    auto backfacing = [&]() {
+    {
         const f3 ab = sub3(bv, av);
         const f3 ac = sub3(cv, av);
         float res = dot3(av, cross3(ab, ac));
         if (flip) res = -res;
-        return res >= 0.0f;
-    };
-    // frustum culling (:113-118), only when every depth is positive
-    auto outside = [&]() {
-        if (nontrivial) return false;
-        if (da >= 0x1p-102f && db >= 0x1p-102f && dc >= 0x1p-102f) {
-            return (frustum_outcode(pxa, pya, da) & frustum_outcode(pxb, pyb, db) &
-                    frustum_outcode(pxc, pyc, dc)) != 0u;
+        if (res >= 0.0f) return;
+    }
+    if (!frustumFirst && !nontrivial) {
+        if (safeDepth) {
+            if ((outcode_axis(pxa, da) & outcode_axis(pxb, db) & outcode_axis(pxc, dc)) ||
+                (outcode_axis(pya, da) & outcode_axis(pyb, db) & outcode_axis(pyc, dc)))
+                return;
         }
+    }
+    if (!nontrivial && !safeDepth) {
         const float ax = pxa / da, bx = pxb / db, cx = pxc / dc;
         const float ay = pya / da, by = pyb / db, cy = pyc / dc;
-        return (ax > 1.0f && bx > 1.0f && cx > 1.0f) || (ax < -1.0f && bx < -1.0f && cx < -1.0f) ||
-               (ay > 1.0f && by > 1.0f && cy > 1.0f) || (ay < -1.0f && by < -1.0f && cy < -1.0f);
-    };
-    if (P.cullOrder == 0) {
-        if (backfacing()) return;
-        if (outside()) return;
-    } else {
-        if (outside()) return;
-        if (backfacing()) return;
+        if ((ax > 1.0f && bx > 1.0f && cx > 1.0f) || (ax < -1.0f && bx < -1.0f && cx < -1.0f) ||
+            (ay > 1.0f && by > 1.0f && cy > 1.0f) || (ay < -1.0f && by < -1.0f && cy < -1.0f))
+            return;
+    }
+
+    // filtered tiny-triangle cull (see above); depths in [2^-100, 2^100]
+    if (P.fastTiny && !nontrivial && da >= 0x1p-100f && db >= 0x1p-100f && dc >= 0x1p-100f &&
+        da <= 0x1p100f && db <= 0x1p100f && dc <= 0x1p100f) {
+        const float ra = __builtin_amdgcn_rcpf(da), rb = __builtin_amdgcn_rcpf(db), rc = __builtin_amdgcn_rcpf(dc);
+        const float qxa = pxa * ra, qxb = pxb * rb, qxc = pxc * rc;
+        const float qya = pya * ra, qyb = pyb * rb, qyc = pyc * rc;
+        const float qmax = fmaxf(fmaxf(fmaxf(fabsf(qxa), fabsf(qxb)), fmaxf(fabsf(qxc), fabsf(qya))),
+                                 fmaxf(fabsf(qyb), fabsf(qyc)));
+        if (qmax <= 2.0f) {
+            const float sxa = to_screen(qxa, P.Wf), sxb = to_screen(qxb, P.Wf), sxc = to_screen(qxc, P.Wf);
+            const float sya = to_screen(qya, P.Hf), syb = to_screen(qyb, P.Hf), syc = to_screen(qyc, P.Hf);
+            const float lox = fmaxf(fminf(sxa, fminf(sxb, sxc)), 0.0f);
+            const float hix = fminf(fmaxf(sxa, fmaxf(sxb, sxc)), P.Wm1);
+            const float loy = fmaxf(fminf(sya, fminf(syb, syc)), 0.0f);
+            const float hiy = fminf(fmaxf(sya, fmaxf(syb, syc)), P.Hm1);
+            if (surely_culled_axis(lox, hix, P.tinyMargin) || surely_culled_axis(loy, hiy, P.tinyMargin)) return;
+        }
     }
 
     // viewToNDC (rasterization_helpers.cuh:95-101) and ndcToScreen - 0.5 (:103-108, :132-134)

@@ -172,9 +172,13 @@ typedef struct crx_tuning {
     uint32_t stage2_waves_per_cu; /* stage-2 persistent grid size in waves per CU */
     uint32_t stage3_waves_per_cu; /* stage-3 persistent grid size in waves per CU */
     uint32_t stage1_cull_order;   /* 0 = backface test before the frustum test, 1 = frustum first
-                                     (both are exact early-outs; the order only changes cost) */
+                                     testing y before x, 2 = frustum first testing x before y
+                                     (all are exact early-outs; the order only changes cost) */
     uint32_t stage1_order;        /* batch-to-workgroup order: 0 = contiguous share per XCD,
                                      1 = grid-stride, 2 = contiguous share per workgroup */
+    uint32_t stage1_fast_tiny;    /* 1 = decide sure tiny-triangle culls from v_rcp_f32 screen
+                                     coordinates with a proven error margin (exact either way) */
+    uint32_t reserved;
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */

@@ -1,0 +1,147 @@
+"""The C-ABI boundary without a GPU: the libraries load, export every symbol
+the headers declare, lay out the reference's structs byte for byte, run their
+host logic (matrices, tuning, error paths) and never link the oracle."""
+import ctypes as C
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import crx
+import oracle as orc
+
+ROOT = Path(__file__).resolve().parent.parent
+INCLUDE = ROOT / "include"
+
+
+def declared_functions(header: Path):
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(crx_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("crx.h", "libcrx.so"), ("crx_gen.h", "libcrx_gen.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    path = ROOT / "curast-ooc_amd" / "lib" / lib
+    handle = C.CDLL(str(path))
+    names = declared_functions(INCLUDE / header)
+    assert len(names) > 5
+    missing = [n for n in names if not hasattr(handle, n)]
+    assert not missing, missing
+
+
+def test_product_does_not_link_the_oracle():
+    out = subprocess.run(["readelf", "-d", str(ROOT / "curast-ooc_amd" / "lib" / "libcrx.so")],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout
+    assert "oracle" not in out
+    syms = subprocess.run(["nm", "-D", str(ROOT / "curast-ooc_amd" / "lib" / "libcrx.so")],
+                          stdout=subprocess.PIPE, text=True, check=True).stdout
+    assert "crx_oracle" not in syms
+
+
+# Offsets of the reference's CMesh / RenderTarget (src/kernels/HostDeviceInterface.h:131-143,
+# :196-234) with GLM's packed vec3/mat4 (4-byte alignment).
+CMESH_OFFSETS = {"numTriangles": 0, "indices": 8, "positions": 16, "cummulativeTriangleCount": 24, "aabb": 32,
+                 "compressionFactor": 56, "index_min": 68, "bitsPerIndex": 72, "instances": 76, "world": 84,
+                 "id": 148, "uvs": 152, "normals": 160, "colors": 168, "firstTriangle": 176, "numVertices": 180,
+                 "index_max": 184, "address": 192, "texture": 200, "isLoaded": 248, "flipTriangles": 249,
+                 "compressed": 250, "impostor": 256}
+TARGET_OFFSETS = {"framebuffer": 0, "colorbuffer": 8, "width": 16, "height": 20, "view": 24, "viewI": 88,
+                  "proj": 152, "cameraPos": 216, "f": 228, "aspect": 232, "debug": 236}
+
+
+def test_struct_layout_matches_reference(tmp_path):
+    fields = [("crx_cmesh", k) for k in CMESH_OFFSETS] + [("crx_render_target", k) for k in TARGET_OFFSETS]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "crx.h"', "int main(void){"]
+    src.append('printf("%zu %zu\\n", sizeof(crx_cmesh), sizeof(crx_render_target));')
+    for s, f in fields:
+        src.append(f'printf("%zu\\n", offsetof({s}, {f}));')
+    src.append("return 0;}")
+    (tmp_path / "probe.c").write_text("\n".join(src))
+    subprocess.run(["gcc", "-std=c99", f"-I{INCLUDE}", str(tmp_path / "probe.c"), "-o", str(tmp_path / "probe")],
+                   check=True)
+    out = subprocess.run([str(tmp_path / "probe")], stdout=subprocess.PIPE, text=True, check=True).stdout.split()
+    assert out[0] == "280" and out[1] == "240"
+    got = [int(x) for x in out[2:]]
+    want = list(CMESH_OFFSETS.values()) + list(TARGET_OFFSETS.values())
+    assert got == want
+    # the Python binding mirrors the same layout
+    for name, off in CMESH_OFFSETS.items():
+        assert getattr(crx.CMesh, name).offset == off
+    for name, off in TARGET_OFFSETS.items():
+        assert getattr(crx.RenderTarget, name).offset == off
+
+
+def test_tuning_defaults_are_the_reference_constants():
+    t = crx.default_tuning()
+    # TILE_SIZE, TRIANGLES_PER_SWEEP, THRESHOLD_SMALL, MAX_* (HostDeviceInterface.h:261-266)
+    assert (t.small_threshold, t.tile_size, t.max_nontrivial, t.max_huge) == (128, 64, 5_000_000, 5_000_000)
+
+
+def test_no_device_error_path():
+    """Without a GPU the context constructor reports CRX_NO_DEVICE with a
+    message instead of exiting (the reference exits, CURuntime.h:38-59)."""
+    try:
+        import torch  # noqa: F401
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(crx.CrxError) as e:
+        crx.Context(0)
+    assert e.value.status == 5
+    assert "device" in str(e.value)
+    assert crx.lib().crx_status_string(3) == b"CRX_QUEUE_OVERFLOW"
+    assert crx.lib().crx_abi_version() == 1
+
+
+def test_host_matrix_math_matches_oracle_bit_for_bit():
+    """The product's host GLM-order math and the oracle's independent
+    restatement agree bit for bit (view*world, inverse(view))."""
+    L = orc.lib(crx)
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        a = crx.Mat4.from_numpy(rng.normal(size=(4, 4)).astype(np.float32))
+        b = crx.Mat4.from_numpy(rng.normal(size=(4, 4)).astype(np.float32))
+        p1, p2 = crx.Mat4(), crx.Mat4()
+        crx.lib().crx_mat4_mul(C.byref(a), C.byref(b), C.byref(p1))
+        L.crx_oracle_mat4_mul(C.byref(a), C.byref(b), C.byref(p2))
+        assert bytes(p1) == bytes(p2)
+        i1, i2 = crx.Mat4(), crx.Mat4()
+        crx.lib().crx_mat4_inverse(C.byref(a), C.byref(i1))
+        L.crx_oracle_mat4_inverse(C.byref(a), C.byref(i2))
+        assert bytes(i1) == bytes(i2)
+
+
+def test_projection_matrix_formula():
+    """Camera::createProjectionMatrix: [f/a,0,0,0; 0,f,0,0; 0,0,0,-1; 0,0,near,0]
+    column-major with f = 1/tan(fovy/2) in float."""
+    fovy, aspect, near = np.float32(np.radians(60)), np.float32(16 / 9), np.float32(0.01)
+    m = np.frombuffer(bytes(crx.create_projection_matrix(near, fovy, aspect)), dtype=np.float32)
+    f = np.float32(1.0) / np.float32(np.tan(np.float32(fovy / np.float32(2))))
+    want = np.zeros(16, dtype=np.float32)
+    want[0], want[5], want[11], want[14] = f / aspect, f, -1.0, near
+    assert abs(float(m[5]) - float(f)) <= float(np.spacing(f))
+    np.testing.assert_allclose(m, want, rtol=1e-6)
+
+
+def test_look_at_is_a_rigid_view_matrix():
+    v = crx.look_at((3.0, 4.0, 5.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0)).to_numpy().astype(np.float64)
+    R = v[:3, :3]
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-6)
+    eye = v @ np.array([3, 4, 5, 1.0])
+    np.testing.assert_allclose(eye[:3], 0, atol=1e-5)
+    target = v @ np.array([0, 0, 0, 1.0])
+    assert target[2] < 0 and abs(target[0]) < 1e-5 and abs(target[1]) < 1e-5
+
+
+def test_cpu_suite_has_no_gpu_side_effects():
+    """The oracle and the generators run without the HIP runtime touching a device."""
+    import workloads as wl
+    pos, idx, col = wl.sphere_mesh(rings=8, segments=16)
+    assert pos.shape == (9 * 16, 3) and idx.shape == (2 * 8 * 16, 3)
+    assert idx.max() < len(pos)

@@ -213,3 +213,17 @@ def test_scene_api_matches_oracle_frame(ctx):
     cbw = orc.resolve(crx, list(inst)[:nv], list(pre)[:nv], fbw, W, H, view, proj, crx.DISPLAY_VERTEX_COLORS)
     assert np.array_equal(got_cb, cbw), describe_mismatch(got_cb, cbw, W)
     scene.close()
+
+
+@pytest.mark.parametrize("cull_order,fast_tiny,order,block", [
+    (0, 0, 0, 256), (0, 1, 1, 512), (1, 0, 2, 128), (1, 1, 1, 512), (2, 1, 0, 256), (2, 0, 1, 512)])
+def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, block):
+    """Cull order, the filtered tiny cull, batch order and workgroup size only
+    change the schedule: every variant must reproduce the oracle exactly."""
+    pos, idx, col = wl.sphere_mesh(rings=256, segments=512)
+    worlds = [crx.identity()] + list(wl.instance_transforms(6, extent=3.0, smin=0.2, smax=0.6, seed=17))
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), worlds)
+    view, proj = wl.sphere_view(1280, 720, radius=1.3)
+    t = crx.default_tuning()
+    t.stage1_cull_order, t.stage1_fast_tiny, t.stage1_order, t.stage1_block = cull_order, fast_tiny, order, block
+    _check(ctx, spec, 1280, 720, view, proj, tuning=t)

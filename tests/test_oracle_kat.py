@@ -1,0 +1,367 @@
+"""Known-answer tests of the CPU oracle, pinned by closed forms.
+
+Each expected value here comes from the mathematics of the case (integer
+pixel sets of axis-aligned right triangles, exact dyadic barycentrics, plane
+depths, textbook Moller-Trumbore, the bit layout of the packed pixel) and not
+from re-typing the oracle's formula, so a dropped term, a swapped operand or a
+wrong edge rule in the oracle fails one of them.
+"""
+import ctypes as C
+import math
+import struct
+
+import numpy as np
+import pytest
+
+import crx
+import oracle as orc
+
+L = orc.lib(crx)
+
+
+def f32(x):
+    return struct.unpack("<f", struct.pack("<f", x))[0]
+
+
+def fbits(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def identity_proj(f=1.0):
+    """proj with proj[1][1] = f (only f enters the stages) and an infinite far plane."""
+    m = np.zeros(16, dtype=np.float32)
+    m[0] = f
+    m[5] = f
+    m[11] = -1.0
+    m[14] = 0.01
+    return crx.Mat4.from_numpy(m)
+
+
+def screen_triangle_scene(pts, W, H, depth=1.0):
+    """A single triangle whose vertices land exactly on the given -0.5-shifted
+    screen coordinates (stage-1 space) with camera = identity, f = aspect = 1
+    (W == H), depth `depth`.  Screen s = (x/d * 0.5 + 0.5) * W - 0.5, so
+    x = ((s + 0.5) / W - 0.5) * 2 * d, exact for dyadic s and power-of-two W."""
+    assert W == H
+    verts = []
+    for sx, sy in pts:
+        x = ((sx + 0.5) / W - 0.5) * 2.0 * depth
+        y = ((sy + 0.5) / H - 0.5) * 2.0 * depth
+        verts.append((x, y, -depth))
+    pos = np.array(verts, dtype=np.float32)
+    # exact round trip of the chosen coordinates through the oracle's transform
+    idx = np.array([[0, 1, 2]], dtype=np.uint32)
+    return pos, idx
+
+
+def render_one(pos, idx, W, H, tuning=None, colors=None):
+    mesh = orc.HostMesh(crx, pos, idx, colors)
+    view = crx.identity()
+    proj = identity_proj(1.0)
+    inst = [mesh.cmesh(crx)]
+    fb, st = orc.draw(crx, inst, inst, [crx.mat4_mul(view, crx.identity())], W, H, view, proj, True,
+                      tuning=tuning, threads=1)
+    return fb, st, mesh, view, proj
+
+
+def covered(fb, W):
+    ids = np.nonzero(fb != crx.CLEAR_PIXEL)[0]
+    return {(int(i % W), int(i // W)) for i in ids}
+
+
+# ---------------------------------------------------------------------------
+# pixel packing (rasterization_helpers.cuh:35-66)
+# ---------------------------------------------------------------------------
+
+def test_pack_layout_and_clear_value():
+    # 28-bit depth in bits 36..63 (sign and 3 low mantissa bits dropped), 36-bit ID below
+    p = L.crx_oracle_pack_pixel(f32(1.0), 5)
+    assert p == ((0x3F800000 & 0x7FFFFFF8) << 33) | 5
+    assert L.crx_oracle_pack_pixel(float("inf"), 0) == 0xFFFFFFF000000000 == crx.CLEAR_PIXEL
+    d, i = C.c_float(), C.c_uint64()
+    L.crx_oracle_unpack_pixel(crx.CLEAR_PIXEL, C.byref(d), C.byref(i))
+    assert math.isinf(d.value) and i.value == 0
+    # the three low mantissa bits are dropped; negative depths pack by magnitude
+    x = struct.unpack("<f", struct.pack("<I", 0x40490FDB))[0]        # pi
+    L.crx_oracle_unpack_pixel(L.crx_oracle_pack_pixel(x, 0xFFFFFFFFF), C.byref(d), C.byref(i))
+    assert fbits(d.value) == 0x40490FD8 and i.value == 0xFFFFFFFFF
+    assert L.crx_oracle_pack_pixel(-2.0, 1) == L.crx_oracle_pack_pixel(2.0, 1)
+
+
+def test_pack_is_monotone_in_depth():
+    rng = np.random.default_rng(1)
+    depths = np.sort(rng.uniform(1e-3, 1e4, 2000).astype(np.float32))
+    packed = [L.crx_oracle_pack_pixel(float(d), 7) for d in depths]
+    assert all(a <= b for a, b in zip(packed, packed[1:]))
+
+
+# ---------------------------------------------------------------------------
+# GLM evaluation order: closed forms
+# ---------------------------------------------------------------------------
+
+def _m(rows):
+    return crx.Mat4.from_numpy(np.array(rows, dtype=np.float32))
+
+
+def test_mat_vec_closed_forms():
+    T = _m([[1, 0, 0, 5], [0, 1, 0, -3], [0, 0, 1, 2], [0, 0, 0, 1]])
+    out = (C.c_float * 4)()
+    L.crx_oracle_mat4_vec4(C.byref(T), (C.c_float * 4)(1, 2, 3, 1), out)
+    assert list(out) == [6.0, -1.0, 5.0, 1.0]
+    M = _m([[1, 1, 1, 1], [0, 0, 0, 0], [0, 0, 0, 0], [0, 0, 0, 0]])
+    L.crx_oracle_mat4_vec4(C.byref(M), (C.c_float * 4)(2.0 ** 24, 1.0, 1.0, -(2.0 ** 24)), out)
+    # GLM pairs (x+y) + (z+w): RN(2^24 + 1) + (1 - 2^24) = 2^24 - (2^24 - 1) = 1;
+    # left-to-right evaluation would give ((2^24 + 1) + 1) - 2^24 = 0
+    assert out[0] == 1.0
+
+
+def test_mat_mul_and_inverse_closed_forms():
+    R = _m([[0, -1, 0, 0], [1, 0, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])      # 90 degrees about z
+    Tm = _m([[1, 0, 0, 3], [0, 1, 0, 4], [0, 0, 1, 5], [0, 0, 0, 1]])
+    out = crx.Mat4()
+    L.crx_oracle_mat4_mul(C.byref(Tm), C.byref(R), C.byref(out))
+    np.testing.assert_array_equal(out.to_numpy(), Tm.to_numpy() @ R.to_numpy())
+    inv = crx.Mat4()
+    L.crx_oracle_mat4_inverse(C.byref(out), C.byref(inv))
+    # inverse of a rigid transform is exact here (entries 0/+-1 and small integers)
+    np.testing.assert_array_equal(inv.to_numpy(), np.linalg.inv(out.to_numpy().astype(np.float64)).astype(np.float32))
+    # inverse of a general matrix agrees with double precision to a few ulp
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(4, 4)).astype(np.float32) + 4 * np.eye(4, dtype=np.float32)
+    L.crx_oracle_mat4_inverse(C.byref(crx.Mat4.from_numpy(A)), C.byref(inv))
+    np.testing.assert_allclose(inv.to_numpy(), np.linalg.inv(A.astype(np.float64)), rtol=2e-5, atol=2e-6)
+
+
+def test_transform_within_one_ulp_of_double_when_exact_terms():
+    """view = worldView * p and ndc = (f/a * x / -z, f * y / -z, -z): with
+    matrices whose products are exact, every component equals the correctly
+    rounded double-precision value (<= 0.5 ulp)."""
+    rng = np.random.default_rng(5)
+    f, aspect = f32(1.7320508), f32(16 / 9)
+    wv = _m([[1, 0, 0, 0.5], [0, 1, 0, -0.25], [0, 0, 1, -10], [0, 0, 0, 1]])
+    view, ndc = (C.c_float * 3)(), (C.c_float * 3)()
+    for _ in range(200):
+        p = rng.integers(-512, 512, 3).astype(np.float32) / 8
+        L.crx_oracle_transform(C.byref(wv), (C.c_float * 3)(*p), f, aspect, view, ndc)
+        vx, vy, vz = p[0] + 0.5, p[1] - 0.25, p[2] - 10
+        assert tuple(view) == (vx, vy, vz)
+        d = -vz
+        fa = np.float32(f) / np.float32(aspect)
+        want_x = np.float32(np.float64(np.float32(fa * np.float32(vx))) / d)
+        want_y = np.float32(np.float64(np.float32(np.float32(f) * np.float32(vy))) / d)
+        assert ndc[0] == want_x and ndc[1] == want_y and ndc[2] == d
+
+
+# ---------------------------------------------------------------------------
+# stage 1 / stage 2 edge rules on exact right triangles
+# ---------------------------------------------------------------------------
+
+def test_stage1_strict_edges_integer_right_triangle():
+    """Vertices on sample positions (10,10), (18,10), (10,18): bbox 8x8 = 64
+    fragments -> stage 1, which uses strict tests s>0, t>0, v>0: exactly the
+    samples with x > 10, y > 10, x + y < 28 are covered."""
+    W = H = 64
+    pos, idx = screen_triangle_scene([(10, 10), (18, 10), (10, 18)], W, H)
+    fb, st, *_ = render_one(pos, idx, W, H)
+    assert st.num_nontrivial == 0
+    want = {(x, y) for x in range(64) for y in range(64) if x > 10 and y > 10 and x + y < 28}
+    assert covered(fb, W) == want and len(want) == 21
+
+
+def test_stage1_dyadic_offsets_right_triangle():
+    """Vertices off the sample grid at (10.25,10.25), (20.25,10.25), (10.25,20.25):
+    covered samples are x >= 11, y >= 11, x + y <= 30 (45 samples)."""
+    W = H = 64
+    pos, idx = screen_triangle_scene([(10.25, 10.25), (20.25, 10.25), (10.25, 20.25)], W, H)
+    fb, st, *_ = render_one(pos, idx, W, H)
+    want = {(x, y) for x in range(64) for y in range(64) if x >= 11 and y >= 11 and x + y <= 30}
+    assert covered(fb, W) == want and len(want) == 45
+
+
+def test_stage2_inclusive_edges_integer_right_triangle():
+    """Legs of 16 samples: bbox 16x16 = 256 > 128 fragments -> stage 2, whose
+    tests are inclusive (s,t,v >= 0): x >= 10, y >= 10, x + y <= 36 (153)."""
+    W = H = 64
+    pos, idx = screen_triangle_scene([(10, 10), (26, 10), (10, 26)], W, H)
+    fb, st, *_ = render_one(pos, idx, W, H)
+    assert st.num_nontrivial == 1 and st.num_huge == 0
+    want = {(x, y) for x in range(64) for y in range(64) if x >= 10 and y >= 10 and x + y <= 36}
+    assert covered(fb, W) == want and len(want) == 153
+
+
+def test_stage1_never_samples_last_column_or_row():
+    """Reference quirk kept: stage 1 clips the box to W-1 and iterates up to
+    ceil(max)-1, so column W-1 and row H-1 stay empty for small triangles."""
+    W = H = 32
+    pos, idx = screen_triangle_scene([(26.5, 26.5), (34.5, 26.5), (26.5, 34.5)], W, H)
+    fb, st, *_ = render_one(pos, idx, W, H)
+    cov = covered(fb, W)
+    assert cov and all(x < W - 1 and y < H - 1 for x, y in cov)
+    want = {(x, y) for x in range(27, W - 1) for y in range(27, H - 1) if x + y < 61}
+    assert cov == want
+
+
+def test_backface_is_culled_and_winding_matters():
+    W = H = 64
+    pos, idx = screen_triangle_scene([(10, 10), (18, 10), (10, 18)], W, H)
+    fb, *_ = render_one(pos, idx[:, ::-1].copy(), W, H)
+    assert not covered(fb, W)
+
+
+def test_depth_of_fronto_parallel_triangle():
+    W = H = 64
+    for depth in (0.5, 3.0, 1000.0):
+        pos, idx = screen_triangle_scene([(10.25, 10.25), (20.25, 10.25), (10.25, 20.25)], W, H, depth)
+        fb, *_ = render_one(pos, idx, W, H)
+        d = C.c_float()
+        i = C.c_uint64()
+        for px in np.nonzero(fb != crx.CLEAR_PIXEL)[0]:
+            L.crx_oracle_unpack_pixel(int(fb[px]), C.byref(d), C.byref(i))
+            assert abs(d.value - depth) <= depth * 2e-6 and i.value == 0
+
+
+def test_stage3_huge_triangle_matches_exact_ray_cast():
+    """A triangle spanning most of a 256x256 view goes to stage 3 (bbox >=
+    64^2); away from its edges every pixel centre is covered, outside none is,
+    and the depth is the plane's (fronto-parallel at depth 2)."""
+    W = H = 256
+    pts = [(20.0, 20.0), (230.0, 30.0), (40.0, 220.0)]
+    pos, idx = screen_triangle_scene(pts, W, H, depth=2.0)
+    fb, st, *_ = render_one(pos, idx, W, H)
+    assert st.num_nontrivial == 1 and st.num_huge > 0
+    cov = covered(fb, W)
+    (ax, ay), (bx, by), (cx, cy) = pts
+
+    def edge(px, py, x0, y0, x1, y1):
+        return (x1 - x0) * (py - y0) - (y1 - y0) * (px - x0)
+
+    area = edge(cx, cy, ax, ay, bx, by)
+    for y in range(H):
+        for x in range(W):
+            # pixel (x, y) is sampled at shifted-space (x, y); e_i / area are its barycentrics
+            b = [edge(x, y, ax, ay, bx, by) / area, edge(x, y, bx, by, cx, cy) / area,
+                 edge(x, y, cx, cy, ax, ay) / area]
+            if min(b) > 0.01:
+                assert (x, y) in cov
+            elif min(b) < -0.01:
+                assert (x, y) not in cov
+    d = C.c_float()
+    i = C.c_uint64()
+    for px in list(np.nonzero(fb != crx.CLEAR_PIXEL)[0])[::97]:
+        L.crx_oracle_unpack_pixel(int(fb[px]), C.byref(d), C.byref(i))
+        assert abs(d.value - 2.0) < 2e-5
+
+
+def test_moller_trumbore_textbook_cases():
+    tri = [(C.c_float * 3)(-1, -1, -5), (C.c_float * 3)(1, -1, -5), (C.c_float * 3)(0, 1, -5)]
+    t = L.crx_oracle_intersect_mt((C.c_float * 3)(0, 0, -1), *tri)
+    assert t == 5.0
+    assert math.isinf(L.crx_oracle_intersect_mt((C.c_float * 3)(0, 0, 1), *tri))      # behind
+    assert math.isinf(L.crx_oracle_intersect_mt((C.c_float * 3)(1, 0, 0), *tri))      # parallel
+    d = np.array([0.9, 0.0, -5.0]) / np.linalg.norm([0.9, 0.0, -5.0])
+    assert math.isinf(L.crx_oracle_intersect_mt((C.c_float * 3)(*d.astype(np.float32)), *tri))  # outside
+
+
+def test_stage2_bbox_contains_the_clipped_triangle():
+    """The Sutherland-Hodgman box of stage 2 against a dense sampling of the
+    view-space triangle clipped to the near plane and the screen."""
+    rng = np.random.default_rng(11)
+    f, aspect, W, H = f32(1.0), f32(1.0), 200.0, 200.0
+    box = (C.c_float * 4)()
+    for _ in range(200):
+        v = rng.uniform(-3, 3, (3, 3)).astype(np.float32)
+        v[:, 2] = rng.uniform(-4, 1, 3)
+        if (v[:, 2] >= -0.01).all():
+            continue
+        L.crx_oracle_stage2_bbox((C.c_float * 3)(*v[0]), (C.c_float * 3)(*v[1]), (C.c_float * 3)(*v[2]),
+                                 f, aspect, W, H, box)
+        mn_x, mx_x, mn_y, mx_y = box
+        # sample the triangle, keep points in front of the near plane and on screen
+        w = rng.dirichlet([1, 1, 1], 4000)
+        p = w @ v.astype(np.float64)
+        p = p[p[:, 2] < -0.0101]
+        if len(p) == 0:
+            continue
+        d = -p[:, 2]
+        sx = (p[:, 0] / d * 0.5 + 0.5) * W
+        sy = (p[:, 1] / d * 0.5 + 0.5) * H
+        on = (sx >= 0) & (sx <= W) & (sy >= 0) & (sy <= H)
+        if not on.any():
+            continue
+        assert (sx[on] >= mn_x - 1e-3).all() and (sx[on] <= mx_x + 1e-3).all()
+        assert (sy[on] >= mn_y - 1e-3).all() and (sy[on] <= mx_y + 1e-3).all()
+
+
+# ---------------------------------------------------------------------------
+# BitEdit (src/BitEdit.h:8-49)
+# ---------------------------------------------------------------------------
+
+def test_bitedit_round_trip_brute_force():
+    rng = np.random.default_rng(2)
+    for bits in range(1, 33):
+        buf = np.zeros(8, dtype=np.uint32)
+        vals = []
+        for k in range((8 * 32) // bits - 1):
+            v = int(rng.integers(0, 1 << bits))
+            L.crx_oracle_bitedit_write(buf.ctypes.data, k * bits, bits, v)
+            vals.append(v)
+        for k, v in enumerate(vals):
+            assert L.crx_oracle_bitedit_read(buf.ctypes.data, k * bits, bits) == v
+        # the whole buffer equals the little-endian concatenation of the values
+        acc = 0
+        for k, v in enumerate(vals):
+            acc |= v << (k * bits)
+        assert int.from_bytes(buf.tobytes(), "little") & ((1 << (len(vals) * bits)) - 1) == acc
+    assert L.crx_oracle_bitedit_read(np.zeros(2, np.uint32).ctypes.data, 0, 0) == 0
+    assert L.crx_oracle_bitedit_read(np.zeros(2, np.uint32).ctypes.data, 0, 33) == 0
+
+
+# ---------------------------------------------------------------------------
+# resolve (resolve.cu:640-1141)
+# ---------------------------------------------------------------------------
+
+def test_resolve_constant_and_blended_vertex_colors():
+    W = H = 64
+    pts = [(10.0, 10.0), (42.0, 10.0), (10.0, 42.0)]   # 32 legs -> stage 2 (inclusive)
+    pos, idx = screen_triangle_scene(pts, W, H, depth=2.0)
+    red, green, blue = 0xFF0000FF, 0xFF00FF00, 0xFFFF0000
+    cols = np.array([red, green, blue], dtype=np.uint32)
+    fb, st, mesh, view, proj = render_one(pos, idx, W, H, colors=cols)
+    inst = [mesh.cmesh(crx)]
+    cb = orc.resolve(crx, inst, [0], fb, W, H, view, proj, crx.DISPLAY_VERTEX_COLORS, threads=1)
+    rgba = (cb & 0xFFFFFFFF).astype(np.uint32).view(np.uint8).reshape(H, W, 4)
+    # sample (x, y) sits at barycentrics s = (x-10)/32 (towards B), t = (y-10)/32 (towards C)
+    for (x, y) in [(10, 10), (26, 10), (10, 26), (18, 18), (20, 14)]:
+        s, t = (x - 10) / 32, (y - 10) / 32
+        want = np.array([255 * (1 - s - t), 255 * s, 255 * t])
+        got = rgba[y, x, :3].astype(float)
+        assert np.all(np.abs(got - want) <= 1.0 + 1e-9), ((x, y), got, want)
+        assert rgba[y, x, 3] == 255
+        assert abs(struct.unpack("<f", struct.pack("<I", int(cb[y * W + x]) >> 32))[0] - 2.0) < 1e-5
+    # background: depth +inf, colour 0
+    assert int(cb[0]) == 0x7F80000000000000
+
+
+def test_resolve_triangle_id_recovers_linear_scan():
+    """TRIANGLE_ID colours encode tri*12345678: recover tri for each pixel and
+    compare with a linear scan of the instance prefix on the visbuffer ID."""
+    import workloads as wl
+    pos, idx, col = wl.sphere_mesh(rings=16, segments=32)
+    mesh = orc.HostMesh(crx, pos, idx, col)
+    worlds = list(wl.instance_transforms(5, extent=4.0, seed=3))
+    view, proj = wl.instances_view(extent=4.0, width=160, height=120, distance=9.0)
+    inst = [mesh.cmesh(crx, cumm=k * mesh.num_triangles, offset=k, world=w) for k, w in enumerate(worlds)]
+    uniq = [mesh.cmesh(crx, cumm=0, offset=0, count=len(worlds))]
+    xf = [crx.mat4_mul(view, w) for w in worlds]
+    fb, _ = orc.draw(crx, uniq, inst, xf, 160, 120, view, proj, True, threads=1)
+    prefix = [k * mesh.num_triangles for k in range(len(worlds))]
+    cb = orc.resolve(crx, inst, prefix, fb, 160, 120, view, proj, crx.DISPLAY_TRIANGLE_ID, threads=1)
+    checked = 0
+    for px in np.nonzero(fb != crx.CLEAR_PIXEL)[0]:
+        tid = int(fb[px]) & 0xFFFFFFFFF
+        k = max(j for j in range(len(prefix)) if prefix[j] <= tid)
+        tri = tid - prefix[k]
+        color = int(cb[px]) & 0xFFFFFFFF
+        assert color == ((tri * 12345678) & 0xFFFFFFFF) | 0xFF000000
+        checked += 1
+    assert checked > 100
