@@ -205,6 +205,14 @@ _SIGS = {
                                                C.c_int, _VP, C.POINTER(DrawStats)]),
     "crx_resolve_visbuffer": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(C.c_uint64),
                                         C.POINTER(RenderTarget), C.c_int, _VP]),
+    "crx_draw_triangles_visbuffer_views": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(CMesh),
+                                                     C.c_uint32, C.POINTER(Mat4), C.POINTER(RenderTarget),
+                                                     C.c_uint32, C.c_int, _VP, C.POINTER(DrawStats)]),
+    "crx_resolve_visbuffer_views": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(C.c_uint64),
+                                              C.POINTER(RenderTarget), C.c_uint32, C.c_int, _VP]),
+    "crx_clear_views": (C.c_int, [_VP, C.POINTER(RenderTarget), C.c_uint32, C.c_int, _VP]),
+    "crx_render_views": (C.c_int, [_VP, _VP, C.POINTER(View), C.c_uint32, C.POINTER(RenderSettings),
+                                   C.POINTER(_VP), C.POINTER(_VP), _VP, C.POINTER(DrawStats)]),
     "crx_camera_projection": (None, [C.c_float, C.c_float, C.c_float, C.POINTER(Mat4)]),
     "crx_camera_look_at": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   C.POINTER(Mat4)]),
@@ -423,6 +431,41 @@ class Context:
                                            pre.ctypes.data_as(C.POINTER(C.c_uint64)) if n else None,
                                            C.byref(target), int(attribute), stream))
 
+    # --- batched multi-view ---------------------------------------------------------
+
+    def draw_triangles_visbuffer_views(self, meshes, instances, transforms, targets,
+                                       instanced: bool, stream=None) -> DrawStats:
+        """One pass over the geometry for len(targets) views.
+
+        transforms: len(targets) * len(instances) world-to-view matrices,
+        view-major (transforms[v * n + i] = view_v * world_i)."""
+        m_arr = _as_array(CMesh, meshes)
+        i_arr = _as_array(CMesh, instances)
+        t_arr = _as_array(Mat4, transforms)
+        g_arr = _as_array(RenderTarget, targets)
+        st = DrawStats()
+        _check(lib().crx_draw_triangles_visbuffer_views(
+            self.handle, m_arr, len(m_arr) if m_arr is not None else 0, i_arr,
+            len(i_arr) if i_arr is not None else 0, t_arr, g_arr, len(g_arr) if g_arr is not None else 0,
+            int(bool(instanced)), stream, C.byref(st)))
+        return st
+
+    def resolve_visbuffer_views(self, instances, prefix, targets, attribute: int = DISPLAY_VERTEX_COLORS,
+                                stream=None) -> None:
+        i_arr = _as_array(CMesh, instances)
+        g_arr = _as_array(RenderTarget, targets)
+        pre = np.ascontiguousarray(np.asarray(prefix, dtype=np.uint64))
+        n = len(i_arr) if i_arr is not None else 0
+        _check(lib().crx_resolve_visbuffer_views(self.handle, i_arr, n,
+                                                 pre.ctypes.data_as(C.POINTER(C.c_uint64)) if n else None,
+                                                 g_arr, len(g_arr) if g_arr is not None else 0,
+                                                 int(attribute), stream))
+
+    def clear_views(self, targets, colorbuffers: bool = False, stream=None) -> None:
+        g_arr = _as_array(RenderTarget, targets)
+        _check(lib().crx_clear_views(self.handle, g_arr, len(g_arr) if g_arr is not None else 0,
+                                     int(bool(colorbuffers)), stream))
+
 
 def _as_array(ctype, items):
     if items is None:
@@ -562,6 +605,20 @@ class Scene:
         _check(lib().crx_render(self.ctx.handle, self.handle, C.byref(view), C.byref(s),
                                 C.c_void_p(fb.ptr), C.c_void_p(cb.ptr) if cb is not None else None,
                                 stream, C.byref(st)))
+        return st
+
+    def render_views(self, views, fbs, cbs=None, settings: RenderSettings | None = None,
+                     stream=None) -> DrawStats:
+        """All views in one batched pass; each view's buffers equal a
+        single-view draw of the shared (union-culled) instance list."""
+        st = DrawStats()
+        s = settings if settings is not None else render_settings()
+        v_arr = _as_array(View, views)
+        n = len(v_arr)
+        fb_arr = (C.c_void_p * n)(*[b.ptr for b in fbs])
+        cb_arr = (C.c_void_p * n)(*[b.ptr for b in cbs]) if cbs is not None else None
+        _check(lib().crx_render_views(self.ctx.handle, self.handle, v_arr, n, C.byref(s), fb_arr, cb_arr,
+                                      stream, C.byref(st)))
         return st
 
     def close(self) -> None:

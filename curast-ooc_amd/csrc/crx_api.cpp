@@ -136,8 +136,9 @@ struct crx_context {
     DeviceBuffer huge;                // int4 entries
     uint32_t ntCap = 0, hugeCap = 0;
     DeviceBuffer drawMeta;            // items | instances | transforms
-    DeviceBuffer resolveMeta;         // instances | prefix
-    Staging drawStaging, resolveStaging;
+    DeviceBuffer resolveMeta;         // instances | prefix | per-view records
+    DeviceBuffer clearMeta;           // per-buffer clear records
+    Staging drawStaging, resolveStaging, clearStaging;
     bool timing = false;
     hipEvent_t ev[T_COUNT] = {};
     bool recorded[T_COUNT] = {};
@@ -320,6 +321,446 @@ uint32_t grid_for(const crx_context* ctx, uint32_t wavesPerCU, uint32_t wavesPer
     return (uint32_t)g;
 }
 
+// Per-view record of a draw: size, projection and the view's transforms.
+ViewParams make_view_params(const crx_render_target& t, const Transform* transforms) {
+    ViewParams V{};
+    V.fb = t.framebuffer;
+    V.transforms = transforms;
+    V.W = t.width;
+    V.H = t.height;
+    V.Wf = (float)t.width;
+    V.Hf = (float)t.height;
+    V.Wm1 = (float)(t.width - 1);
+    V.Hm1 = (float)(t.height - 1);
+    V.f = t.proj.m[5];
+    V.aspect = (float)t.width / (float)t.height;
+    V.fa = V.f / V.aspect;
+    V.faI = 1.0f / (V.f / V.aspect);
+    V.fI = 1.0f / V.f;
+    // |v_rcp_f32 screen coordinate - IEEE one| <= 0.75*max(W,H)*2^-20 for |ndc| <= 2;
+    // the margin is 5x that plus an absolute 2^-6
+    V.tinyMargin = (float)std::max(t.width, t.height) * 0x1p-18f + 0x1p-6f;
+    return V;
+}
+
+// drawTrianglesVisbuffer (CuRast_render.h:211-264) for numViews >= 1 views
+// sharing one instance list: transforms[v * numInstances + i] is view v's
+// view * world of instance i.  One view runs the plain stage 1; several views
+// run the batched stage 1, which fetches each triangle once for all views.
+crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
+                      uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
+                      uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    if (numViews == 0 || numViews > kMaxViews)
+        return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
+    if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
+    for (uint32_t v = 0; v < numViews; v++) CRX_TRY(check_target(targets + v));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    // drawTrianglesVisbuffer returns early for an empty instance list (CuRast_render.h:231)
+    if (numInstances == 0) return CRX_OK;
+    if (!instances || !transforms) return fail(CRX_INVALID_ARG, "instances / transforms is NULL");
+    if (instanced && (!meshes || numMeshes == 0)) return fail(CRX_INVALID_ARG, "instanced draw without meshes");
+    if (numViews > 1 && numInstances > kInstanceMask + 1u)
+        return fail(CRX_INVALID_ARG, "a multi-view draw takes at most 2^24 instances");
+    CRX_HIP(hipSetDevice(ctx->device));
+    const crx_tuning& tu = ctx->tuning;
+    const uint32_t block = tu.stage1_block;
+    uint32_t batchShift = 0;   // log2(triangles per batch)
+    while ((1u << batchShift) < block) batchShift++;
+    const uint64_t batchTris = 1ull << batchShift;
+
+    // ---- host records -----------------------------------------------------
+    const crx_cmesh* list = instanced ? meshes : instances;
+    const uint32_t numItems = instanced ? numMeshes : numInstances;
+    std::vector<DrawItem> items(numItems);
+    uint64_t batches = 0, tris = 0;
+    for (uint32_t k = 0; k < numItems; k++) {
+        const crx_cmesh& m = list[k];
+        CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));
+        const int64_t off = m.instances.offset;
+        const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
+        if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
+            return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
+        DrawItem& it = items[k];
+        it.indices = m.indices;
+        it.positions = reinterpret_cast<const float*>(m.positions);
+        it.cumm = m.cummulativeTriangleCount;
+        it.numTriangles = m.numTriangles;
+        it.instOffset = (uint32_t)off;
+        it.instCount = (uint32_t)cnt;
+        it.firstBatch = (uint32_t)batches;
+        batches += ((uint64_t)m.numTriangles + batchTris - 1) >> batchShift;
+        tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
+        if (batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
+    }
+    std::vector<InstanceRec> inst(numInstances);
+    for (uint32_t i = 0; i < numInstances; i++) {
+        CRX_TRY(check_mesh(instances[i], "instances", i));
+        const int64_t ti = instances[i].instances.offset;
+        if (ti < 0 || ti >= (int64_t)numInstances)
+            return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
+        inst[i] = make_instance(instances[i]);
+    }
+    const size_t numXf = (size_t)numViews * numInstances;
+    std::vector<Transform> xf(numXf);
+    for (size_t i = 0; i < numXf; i++) {
+        std::memcpy(xf[i].m, transforms[i].m, sizeof(xf[i].m));
+        xf[i].flip = mirror_flag(transforms[i]) ? 1u : 0u;
+        xf[i].pad[0] = xf[i].pad[1] = xf[i].pad[2] = 0u;
+    }
+    if (stats) stats->num_triangles = tris * numViews;
+    if (batches == 0) return CRX_OK;
+
+    const size_t offItems = 0;
+    const size_t offInst = align_up(offItems + items.size() * sizeof(DrawItem), 256);
+    const size_t offViews = align_up(offInst + inst.size() * sizeof(InstanceRec), 256);
+    const size_t offXf = align_up(offViews + (size_t)numViews * sizeof(ViewParams), 256);
+    const size_t metaBytes = offXf + xf.size() * sizeof(Transform);
+    CRX_TRY(ctx->drawMeta.reserve(metaBytes));
+    char* meta = static_cast<char*>(ctx->drawMeta.ptr);
+    const Transform* xfDev = reinterpret_cast<const Transform*>(meta + offXf);
+    std::vector<ViewParams> views(numViews);
+    for (uint32_t v = 0; v < numViews; v++) views[v] = make_view_params(targets[v], xfDev + (size_t)v * numInstances);
+    CRX_TRY(ctx->drawStaging.acquire(metaBytes));
+    char* stage = static_cast<char*>(ctx->drawStaging.ptr);
+    std::memcpy(stage + offItems, items.data(), items.size() * sizeof(DrawItem));
+    std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
+    std::memcpy(stage + offViews, views.data(), views.size() * sizeof(ViewParams));
+    std::memcpy(stage + offXf, xf.data(), xf.size() * sizeof(Transform));
+    CRX_HIP(hipMemcpyAsync(ctx->drawMeta.ptr, stage, metaBytes, hipMemcpyHostToDevice, stream));
+    CRX_TRY(ctx->drawStaging.mark(stream));
+    CRX_TRY(ensure_queues(ctx, tu.max_nontrivial, tu.max_huge));
+
+    RasterParams P{};
+    P.items = reinterpret_cast<const DrawItem*>(meta + offItems);
+    P.inst = reinterpret_cast<const InstanceRec*>(meta + offInst);
+    P.views = reinterpret_cast<const ViewParams*>(meta + offViews);
+    P.view0 = views[0];
+    P.numViews = numViews;
+    P.ctr = ctx->ctr;
+    P.numItems = numItems;
+    P.numBatches = (uint32_t)batches;
+    P.smallThreshold = tu.small_threshold;
+    P.tile = (int32_t)tu.tile_size;
+    P.batchShift = batchShift;
+    P.order = tu.stage1_order;
+    P.cullOrder = tu.stage1_cull_order;
+    P.fastTiny = tu.stage1_fast_tiny;
+
+    uint32_t g1;
+    if (tu.stage1_waves_per_cu == 0) {
+        g1 = (uint32_t)std::min<uint64_t>(batches, 1ull << 31);
+    } else {
+        g1 = grid_for(ctx, tu.stage1_waves_per_cu, block / 64);
+        if (g1 > batches) g1 = (uint32_t)batches;
+    }
+    g1 = (g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
+    const uint32_t g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4);
+    const uint32_t g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
+    P.numWaves2 = g2 * 4u;
+    P.numWaves3 = g3 * 4u;
+
+    uint32_t redraws = 0;
+    for (;;) {
+        P.nontrivial = static_cast<uint64_t*>(ctx->nontrivial.ptr);
+        P.huge = static_cast<int4*>(ctx->huge.ptr);
+        P.ntCap = ctx->ntCap;
+        P.hugeCap = ctx->hugeCap;
+        CRX_HIP(launch_reset_counters(ctx->ctr, stream));
+        record(ctx, T_S1, stream);
+        if (numViews == 1) CRX_HIP(launch_stage1(P, block, g1, stream));
+        else CRX_HIP(launch_stage1_views(P, block, g1, stream));
+        record(ctx, T_S2, stream);
+        CRX_HIP(launch_stage2(P, g2, stream));
+        record(ctx, T_S3, stream);
+        CRX_HIP(launch_stage3(P, g3, stream));
+        record(ctx, T_S3_END, stream);
+        CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        CRX_HIP(hipStreamSynchronize(stream));
+        const uint32_t nNt = ctx->ctrHost[CTR_NONTRIVIAL];
+        const uint32_t nHuge = ctx->ctrHost[CTR_HUGE];
+        if (stats) {
+            stats->num_nontrivial = nNt;
+            stats->num_huge = nHuge;
+            stats->redraws = redraws;
+        }
+        if (nNt <= ctx->ntCap && nHuge <= ctx->hugeCap) break;
+        // Queue overflow: grow and draw again.  Every value the truncated pass
+        // wrote is also written by the full pass, so the minimum is unchanged.
+        if (++redraws > 8) return fail(CRX_QUEUE_OVERFLOW, "stage queues kept overflowing");
+        const uint64_t ntWant = std::max<uint64_t>(ctx->ntCap, (uint64_t)nNt + nNt / 4 + 1024);
+        uint64_t hgWant = ctx->hugeCap;
+        if (nHuge > ctx->hugeCap || nNt > ctx->ntCap) hgWant = std::max<uint64_t>(hgWant, (uint64_t)nHuge * 2 + 1024);
+        if (ntWant > 0xffffffffull || hgWant > 0xffffffffull)
+            return fail(CRX_QUEUE_OVERFLOW, "stage queue would exceed 2^32 entries");
+        CRX_TRY(ensure_queues(ctx, (uint32_t)ntWant, (uint32_t)hgWant));
+    }
+    return CRX_OK;
+}
+
+// Per-view resolve record (the instance list and prefix are shared).
+ResolveParams make_resolve_params(const crx_render_target& t, const InstanceRec* inst, const uint64_t* prefix,
+                                  uint32_t numInstances, crx_display_attribute attribute) {
+    ResolveParams R{};
+    R.inst = inst;
+    R.prefix = prefix;
+    R.fb = t.framebuffer;
+    R.cb = t.colorbuffer;
+    R.numInstances = numInstances;
+    R.W = t.width;
+    R.H = t.height;
+    R.attribute = (int32_t)attribute;
+    R.Wf = (float)t.width;
+    R.Hf = (float)t.height;
+    R.p00I = 1.0f / t.proj.m[0];
+    R.p11I = 1.0f / t.proj.m[5];
+    const crx_mat4 viewI = mat_inverse(t.view);
+    std::memcpy(R.viewI, viewI.m, sizeof(R.viewI));
+    const f3 o = xform(viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
+    R.origin[0] = o.x;
+    R.origin[1] = o.y;
+    R.origin[2] = o.z;
+    return R;
+}
+
+// kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141) for
+// numViews >= 1 targets sharing one instance list, in one launch.
+crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                         const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                         hipStream_t stream) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    if (numViews == 0 || numViews > kMaxViews)
+        return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
+    if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
+    for (uint32_t v = 0; v < numViews; v++) {
+        CRX_TRY(check_target(targets + v));
+        if (!targets[v].colorbuffer) return fail(CRX_INVALID_ARG, "target colorbuffer is NULL");
+    }
+    if (attribute == CRX_DISPLAY_TEXTURE || attribute == CRX_DISPLAY_STAGE)
+        return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
+    if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
+    if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
+    CRX_HIP(hipSetDevice(ctx->device));
+    std::vector<InstanceRec> inst(numInstances);
+    for (uint32_t i = 0; i < numInstances; i++) {
+        CRX_TRY(check_mesh(instances[i], "instances", i));
+        inst[i] = make_instance(instances[i]);
+    }
+    const size_t offInst = 0;
+    const size_t offPrefix = align_up(inst.size() * sizeof(InstanceRec), 256);
+    const size_t offViews = align_up(offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8, 256);
+    const size_t bytes = offViews + (numViews > 1 ? (size_t)numViews * sizeof(ResolveParams) : 0);
+    CRX_TRY(ctx->resolveMeta.reserve(bytes));
+    char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
+    const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(meta + offInst);
+    const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(meta + offPrefix);
+    std::vector<ResolveParams> rv(numViews);
+    uint64_t blocks = 0;
+    for (uint32_t v = 0; v < numViews; v++) {
+        rv[v] = make_resolve_params(targets[v], instDev, prefixDev, numInstances, attribute);
+        rv[v].firstBlock = (uint32_t)blocks;
+        blocks += (uint64_t)((targets[v].width + 63) / 64) * (uint64_t)((targets[v].height + 3) / 4);
+    }
+    if (blocks >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
+    if (numInstances || numViews > 1) {
+        CRX_TRY(ctx->resolveStaging.acquire(bytes));
+        char* stage = static_cast<char*>(ctx->resolveStaging.ptr);
+        std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
+        std::memcpy(stage + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
+        if (numViews > 1) std::memcpy(stage + offViews, rv.data(), rv.size() * sizeof(ResolveParams));
+        CRX_HIP(hipMemcpyAsync(ctx->resolveMeta.ptr, stage, bytes, hipMemcpyHostToDevice, stream));
+        CRX_TRY(ctx->resolveStaging.mark(stream));
+    }
+    record(ctx, T_RESOLVE, stream);
+    if (numViews == 1) {
+        CRX_HIP(launch_resolve(rv[0], stream));
+    } else {
+        CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews,
+                                     (uint32_t)blocks, stream));
+    }
+    record(ctx, T_RESOLVE_END, stream);
+    return CRX_OK;
+}
+
+// kernel_clearFramebuffer (resolve.cu:213-232) over several buffers in one launch.
+crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,
+                         hipStream_t stream) {
+    std::vector<ClearView> cv;
+    cv.reserve(n);
+    uint64_t blocks = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        if (numPixels[k] == 0) continue;
+        if (!buffers[k]) return fail(CRX_INVALID_ARG, "buffer to clear is NULL");
+        ClearView c;
+        c.fb = buffers[k];
+        c.numPixels = numPixels[k];
+        c.firstBlock = (uint32_t)blocks;
+        cv.push_back(c);
+        blocks += (numPixels[k] + kClearChunk - 1) / kClearChunk;
+    }
+    if (cv.empty()) return CRX_OK;
+    if (blocks >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one clear");
+    const size_t bytes = cv.size() * sizeof(ClearView);
+    CRX_TRY(ctx->clearMeta.reserve(bytes));
+    CRX_TRY(ctx->clearStaging.acquire(bytes));
+    std::memcpy(ctx->clearStaging.ptr, cv.data(), bytes);
+    CRX_HIP(hipMemcpyAsync(ctx->clearMeta.ptr, ctx->clearStaging.ptr, bytes, hipMemcpyHostToDevice, stream));
+    CRX_TRY(ctx->clearStaging.mark(stream));
+    CRX_HIP(launch_clear_views(static_cast<const ClearView*>(ctx->clearMeta.ptr), (uint32_t)cv.size(),
+                               (uint32_t)blocks, nullptr, stream));
+    return CRX_OK;
+}
+
+// CuRast::render / CuRast::draw (CuRast_render.h:266-519, :873-904) for
+// numViews >= 1 views: node culling (SNTriangles::update, SNTriangles.h:29-105)
+// keeps a node visible in any view, the visible nodes are sorted and grouped
+// once, and every view is drawn from that one instance list.
+crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
+                        const crx_render_settings* settings_, uint64_t* const* framebuffers,
+                        uint64_t* const* colorbuffers, hipStream_t stream, crx_draw_stats* stats) {
+    if (!ctx || !scene || !views || !framebuffers) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (numViews == 0 || numViews > kMaxViews)
+        return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
+    crx_render_settings settings;
+    crx_render_settings_default(&settings);
+    if (settings_) settings = *settings_;
+    if (settings.resolve && !colorbuffers) return fail(CRX_INVALID_ARG, "resolve needs colorbuffers");
+    for (uint32_t v = 0; v < numViews; v++) {
+        if (views[v].width <= 0 || views[v].height <= 0) return fail(CRX_INVALID_ARG, "view size must be positive");
+        if (!framebuffers[v]) return fail(CRX_INVALID_ARG, "framebuffer is NULL");
+        if (settings.resolve && !colorbuffers[v]) return fail(CRX_INVALID_ARG, "resolve needs a colorbuffer");
+    }
+
+    // SNTriangles::update (SNTriangles.h:29-105): frustum side planes of proj*view
+    std::vector<float> planes((size_t)numViews * 16);
+    for (uint32_t v = 0; v < numViews; v++) {
+        const crx_mat4 viewProj = mat_mul(views[v].proj, views[v].view);
+        for (int p = 0; p < 4; p++) {
+            const int r = p < 2 ? 0 : 1;
+            const float sg = (p & 1) ? -1.0f : 1.0f;
+            float q[4];
+            for (int c = 0; c < 4; c++) {
+                const float r3 = viewProj.m[4 * c + 3], rr = viewProj.m[4 * c + r];
+                q[c] = sg > 0.0f ? r3 + rr : r3 - rr;
+            }
+            const f3 n{q[0], q[1], q[2]};
+            const float len = sqrtf(dot3(n, n));
+            float* out = &planes[(size_t)v * 16 + 4 * p];
+            out[0] = n.x / len; out[1] = n.y / len; out[2] = n.z / len;
+            out[3] = q[3] / len;
+        }
+    }
+    std::vector<uint32_t> visible;
+    visible.reserve(scene->nodes.size());
+    for (uint32_t i = 0; i < scene->nodes.size(); i++) {
+        const crx_scene::Node& nd = scene->nodes[i];
+        bool keep = true;
+        if (settings.frustum_culling) {
+            const crx_box3& b = nd.mesh->aabb;
+            const float cx[2] = {b.min.x, b.max.x}, cy[2] = {b.min.y, b.max.y}, cz[2] = {b.min.z, b.max.z};
+            f3 wmin{INFINITY, INFINITY, INFINITY}, wmax{-INFINITY, -INFINITY, -INFINITY};
+            for (int k = 0; k < 8; k++) {   // Box3::transform corner order
+                const f3 w = xform(nd.world.m, f3{cx[k & 1], cy[(k >> 1) & 1], cz[(k >> 2) & 1]}, 1.0f);
+                wmin.x = std::fmin(wmin.x, w.x); wmin.y = std::fmin(wmin.y, w.y); wmin.z = std::fmin(wmin.z, w.z);
+                wmax.x = std::fmax(wmax.x, w.x); wmax.y = std::fmax(wmax.y, w.y); wmax.z = std::fmax(wmax.z, w.z);
+            }
+            keep = false;
+            for (uint32_t v = 0; v < numViews && !keep; v++) {
+                bool inside = true;
+                for (int p = 0; p < 4 && inside; p++) {
+                    const float* pl = &planes[(size_t)v * 16 + 4 * p];
+                    const f3 n{pl[0], pl[1], pl[2]};
+                    const f3 c{n.x >= 0.0f ? wmax.x : wmin.x, n.y >= 0.0f ? wmax.y : wmin.y,
+                               n.z >= 0.0f ? wmax.z : wmin.z};
+                    if (dot3(n, c) + pl[3] < 0.0f) inside = false;
+                }
+                keep = inside;
+            }
+        }
+        if (keep) visible.push_back(i);
+    }
+    // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
+    // instances of one mesh keep node order.
+    std::stable_sort(visible.begin(), visible.end(), [&](uint32_t a, uint32_t b) {
+        const crx_mesh* A = scene->nodes[a].mesh;
+        const crx_mesh* B = scene->nodes[b].mesh;
+        if (A->numTriangles == B->numTriangles) return A->seq < B->seq;
+        return A->numTriangles > B->numTriangles;
+    });
+    const uint32_t n = (uint32_t)visible.size();
+    scene->unique.resize(n);
+    scene->instances.resize(n);
+    scene->transforms.resize((size_t)n * numViews);
+    scene->prefix.resize(n);
+    uint64_t sum = 0;
+    uint32_t u = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const crx_scene::Node& nd = scene->nodes[visible[i]];
+        crx_cmesh cm;
+        std::memset(&cm, 0, sizeof(cm));
+        cm.numTriangles = nd.mesh->numTriangles;
+        cm.indices = nd.mesh->indices;
+        cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
+        cm.colors = nd.mesh->colors;
+        cm.aabb = nd.mesh->aabb;
+        cm.world = nd.world;
+        cm.id = (int32_t)visible[i];
+        cm.numVertices = nd.mesh->numVertices;
+        cm.address = nd.mesh->seq;
+        cm.isLoaded = 1;
+        cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
+        cm.cummulativeTriangleCount = sum;
+        cm.instances.offset = (int32_t)i;
+        cm.instances.count = 0;
+        for (uint32_t v = 0; v < numViews; v++) scene->transforms[(size_t)v * n + i] = mat_mul(views[v].view, nd.world);
+        scene->prefix[i] = sum;
+        sum += cm.numTriangles;
+        scene->instances[i] = cm;
+        if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
+            scene->unique[u] = cm;
+            u++;
+        }
+        scene->unique[u - 1].instances.count++;
+    }
+
+    std::vector<crx_render_target> targets(numViews);
+    std::vector<uint64_t*> clearBufs;
+    std::vector<uint32_t> clearPx;
+    for (uint32_t v = 0; v < numViews; v++) {
+        crx_render_target& t = targets[v];
+        std::memset(&t, 0, sizeof(t));
+        t.framebuffer = framebuffers[v];
+        t.colorbuffer = settings.resolve ? colorbuffers[v] : nullptr;
+        t.width = views[v].width;
+        t.height = views[v].height;
+        t.view = views[v].view;
+        t.viewI = mat_inverse(views[v].view);
+        t.proj = views[v].proj;
+        const f3 cam = xform(t.viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
+        t.cameraPos = crx_vec3{cam.x, cam.y, cam.z};
+        t.f = views[v].proj.m[5];
+        t.aspect = (float)views[v].width / (float)views[v].height;
+        const uint32_t px = (uint32_t)views[v].width * (uint32_t)views[v].height;
+        clearBufs.push_back(t.framebuffer);
+        clearPx.push_back(px);
+        if (settings.resolve) {
+            clearBufs.push_back(t.colorbuffer);
+            clearPx.push_back(px);
+        }
+    }
+
+    CRX_HIP(hipSetDevice(ctx->device));
+    record(ctx, T_CLEAR, stream);
+    CRX_TRY(clear_buffers(ctx, clearBufs.data(), clearPx.data(), (uint32_t)clearBufs.size(), stream));
+    CRX_TRY(draw_views(ctx, scene->unique.data(), u, scene->instances.data(), n, scene->transforms.data(),
+                       targets.data(), numViews, settings.instanced, stream, stats));
+    if (settings.resolve) {
+        CRX_TRY(resolve_views(ctx, scene->instances.data(), n, scene->prefix.data(), targets.data(), numViews,
+                              (crx_display_attribute)settings.attribute, stream));
+    }
+    return CRX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -393,10 +834,12 @@ crx_status crx_context_destroy(crx_context* ctx) {
     (void)hipDeviceSynchronize();
     ctx->drawStaging.release();
     ctx->resolveStaging.release();
+    ctx->clearStaging.release();
     ctx->nontrivial.release();
     ctx->huge.release();
     ctx->drawMeta.release();
     ctx->resolveMeta.release();
+    ctx->clearMeta.release();
     if (ctx->ctr) (void)hipFree(ctx->ctr);
     if (ctx->ctrHost) (void)hipHostFree(ctx->ctrHost);
     for (int i = 0; i < T_COUNT; i++)
@@ -511,208 +954,52 @@ crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void
 crx_status crx_draw_triangles_visbuffer(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes,
                                         const crx_cmesh* instances, uint32_t numInstances,
                                         const crx_mat4* transforms, const crx_render_target* target,
-                                        int instanced, void* stream_, crx_draw_stats* stats) {
-    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
-    CRX_TRY(check_target(target));
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    // drawTrianglesVisbuffer returns early for an empty instance list (CuRast_render.h:231)
-    if (numInstances == 0) return CRX_OK;
-    if (!instances || !transforms) return fail(CRX_INVALID_ARG, "instances / transforms is NULL");
-    if (instanced && (!meshes || numMeshes == 0)) return fail(CRX_INVALID_ARG, "instanced draw without meshes");
-    hipStream_t stream = as_stream(stream_);
-    CRX_HIP(hipSetDevice(ctx->device));
-    const crx_tuning& tu = ctx->tuning;
-    const uint32_t block = tu.stage1_block;
-    uint32_t batchShift = 0;   // log2(triangles per batch)
-    while ((1u << batchShift) < block) batchShift++;
-    const uint64_t batchTris = 1ull << batchShift;
+                                        int instanced, void* stream, crx_draw_stats* stats) {
+    if (!target) return fail(CRX_INVALID_ARG, "target is NULL");
+    return draw_views(ctx, meshes, numMeshes, instances, numInstances, transforms, target, 1, instanced,
+                      as_stream(stream), stats);
+}
 
-    // ---- host records -----------------------------------------------------
-    const crx_cmesh* list = instanced ? meshes : instances;
-    const uint32_t numItems = instanced ? numMeshes : numInstances;
-    std::vector<DrawItem> items(numItems);
-    uint64_t batches = 0, tris = 0;
-    for (uint32_t k = 0; k < numItems; k++) {
-        const crx_cmesh& m = list[k];
-        CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));
-        const int64_t off = m.instances.offset;
-        const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
-        if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
-            return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
-        DrawItem& it = items[k];
-        it.indices = m.indices;
-        it.positions = reinterpret_cast<const float*>(m.positions);
-        it.cumm = m.cummulativeTriangleCount;
-        it.numTriangles = m.numTriangles;
-        it.instOffset = (uint32_t)off;
-        it.instCount = (uint32_t)cnt;
-        it.firstBatch = (uint32_t)batches;
-        batches += ((uint64_t)m.numTriangles + batchTris - 1) >> batchShift;
-        tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
-        if (batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
-    }
-    std::vector<InstanceRec> inst(numInstances);
-    std::vector<Transform> xf(numInstances);
-    for (uint32_t i = 0; i < numInstances; i++) {
-        CRX_TRY(check_mesh(instances[i], "instances", i));
-        const int64_t ti = instances[i].instances.offset;
-        if (ti < 0 || ti >= (int64_t)numInstances)
-            return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
-        inst[i] = make_instance(instances[i]);
-        std::memcpy(xf[i].m, transforms[i].m, sizeof(xf[i].m));
-        xf[i].flip = mirror_flag(transforms[i]) ? 1u : 0u;
-        xf[i].pad[0] = xf[i].pad[1] = xf[i].pad[2] = 0u;
-    }
-    if (stats) stats->num_triangles = tris;
-    if (batches == 0) return CRX_OK;
-
-    const size_t offItems = 0;
-    const size_t offInst = align_up(offItems + items.size() * sizeof(DrawItem), 256);
-    const size_t offXf = align_up(offInst + inst.size() * sizeof(InstanceRec), 256);
-    const size_t metaBytes = offXf + xf.size() * sizeof(Transform);
-    CRX_TRY(ctx->drawMeta.reserve(metaBytes));
-    CRX_TRY(ctx->drawStaging.acquire(metaBytes));
-    char* stage = static_cast<char*>(ctx->drawStaging.ptr);
-    std::memcpy(stage + offItems, items.data(), items.size() * sizeof(DrawItem));
-    std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
-    std::memcpy(stage + offXf, xf.data(), xf.size() * sizeof(Transform));
-    CRX_HIP(hipMemcpyAsync(ctx->drawMeta.ptr, stage, metaBytes, hipMemcpyHostToDevice, stream));
-    CRX_TRY(ctx->drawStaging.mark(stream));
-    CRX_TRY(ensure_queues(ctx, tu.max_nontrivial, tu.max_huge));
-
-    char* meta = static_cast<char*>(ctx->drawMeta.ptr);
-    RasterParams P{};
-    P.items = reinterpret_cast<const DrawItem*>(meta + offItems);
-    P.inst = reinterpret_cast<const InstanceRec*>(meta + offInst);
-    P.transforms = reinterpret_cast<const Transform*>(meta + offXf);
-    P.fb = target->framebuffer;
-    P.ctr = ctx->ctr;
-    P.numItems = numItems;
-    P.numBatches = (uint32_t)batches;
-    P.W = target->width;
-    P.H = target->height;
-    P.Wf = (float)target->width;
-    P.Hf = (float)target->height;
-    P.Wm1 = (float)(target->width - 1);
-    P.Hm1 = (float)(target->height - 1);
-    P.f = target->proj.m[5];
-    P.aspect = (float)target->width / (float)target->height;
-    P.fa = P.f / P.aspect;
-    P.faI = 1.0f / (P.f / P.aspect);
-    P.fI = 1.0f / P.f;
-    P.smallThreshold = tu.small_threshold;
-    P.tile = (int32_t)tu.tile_size;
-    P.batchShift = batchShift;
-    P.order = tu.stage1_order;
-    P.cullOrder = tu.stage1_cull_order;
-    P.fastTiny = tu.stage1_fast_tiny;
-    // |v_rcp_f32 screen coordinate - IEEE one| <= 0.75*max(W,H)*2^-20 for |ndc| <= 2;
-    // the margin is 5x that plus an absolute 2^-6
-    P.tinyMargin = (float)std::max(target->width, target->height) * 0x1p-18f + 0x1p-6f;
-
-    uint32_t g1;
-    if (tu.stage1_waves_per_cu == 0) {
-        g1 = (uint32_t)std::min<uint64_t>(batches, 1ull << 31);
-    } else {
-        g1 = grid_for(ctx, tu.stage1_waves_per_cu, block / 64);
-        if (g1 > batches) g1 = (uint32_t)batches;
-    }
-    g1 = (g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
-    const uint32_t g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4);
-    const uint32_t g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
-    P.numWaves2 = g2 * 4u;
-    P.numWaves3 = g3 * 4u;
-
-    uint32_t redraws = 0;
-    for (;;) {
-        P.nontrivial = static_cast<uint64_t*>(ctx->nontrivial.ptr);
-        P.huge = static_cast<int4*>(ctx->huge.ptr);
-        P.ntCap = ctx->ntCap;
-        P.hugeCap = ctx->hugeCap;
-        CRX_HIP(launch_reset_counters(ctx->ctr, stream));
-        record(ctx, T_S1, stream);
-        CRX_HIP(launch_stage1(P, block, g1, stream));
-        record(ctx, T_S2, stream);
-        CRX_HIP(launch_stage2(P, g2, stream));
-        record(ctx, T_S3, stream);
-        CRX_HIP(launch_stage3(P, g3, stream));
-        record(ctx, T_S3_END, stream);
-        CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-        CRX_HIP(hipStreamSynchronize(stream));
-        const uint32_t nNt = ctx->ctrHost[CTR_NONTRIVIAL];
-        const uint32_t nHuge = ctx->ctrHost[CTR_HUGE];
-        if (stats) {
-            stats->num_nontrivial = nNt;
-            stats->num_huge = nHuge;
-            stats->redraws = redraws;
-        }
-        if (nNt <= ctx->ntCap && nHuge <= ctx->hugeCap) break;
-        // Queue overflow: grow and draw again.  Every value the truncated pass
-        // wrote is also written by the full pass, so the minimum is unchanged.
-        if (++redraws > 8) return fail(CRX_QUEUE_OVERFLOW, "stage queues kept overflowing");
-        const uint64_t ntWant = std::max<uint64_t>(ctx->ntCap, (uint64_t)nNt + nNt / 4 + 1024);
-        uint64_t hgWant = ctx->hugeCap;
-        if (nHuge > ctx->hugeCap || nNt > ctx->ntCap) hgWant = std::max<uint64_t>(hgWant, (uint64_t)nHuge * 2 + 1024);
-        if (ntWant > 0xffffffffull || hgWant > 0xffffffffull)
-            return fail(CRX_QUEUE_OVERFLOW, "stage queue would exceed 2^32 entries");
-        CRX_TRY(ensure_queues(ctx, (uint32_t)ntWant, (uint32_t)hgWant));
-    }
-    return CRX_OK;
+crx_status crx_draw_triangles_visbuffer_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes,
+                                              const crx_cmesh* instances, uint32_t numInstances,
+                                              const crx_mat4* transforms, const crx_render_target* targets,
+                                              uint32_t numViews, int instanced, void* stream, crx_draw_stats* stats) {
+    return draw_views(ctx, meshes, numMeshes, instances, numInstances, transforms, targets, numViews, instanced,
+                      as_stream(stream), stats);
 }
 
 crx_status crx_resolve_visbuffer(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances,
                                  const uint64_t* prefix, const crx_render_target* target,
-                                 crx_display_attribute attribute, void* stream_) {
+                                 crx_display_attribute attribute, void* stream) {
+    if (!target) return fail(CRX_INVALID_ARG, "target is NULL");
+    return resolve_views(ctx, instances, numInstances, prefix, target, 1, attribute, as_stream(stream));
+}
+
+crx_status crx_resolve_visbuffer_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances,
+                                       const uint64_t* prefix, const crx_render_target* targets, uint32_t numViews,
+                                       crx_display_attribute attribute, void* stream) {
+    return resolve_views(ctx, instances, numInstances, prefix, targets, numViews, attribute, as_stream(stream));
+}
+
+crx_status crx_clear_views(crx_context* ctx, const crx_render_target* targets, uint32_t numViews, int colorbuffers,
+                           void* stream) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
-    CRX_TRY(check_target(target));
-    if (!target->colorbuffer) return fail(CRX_INVALID_ARG, "target->colorbuffer is NULL");
-    if (attribute == CRX_DISPLAY_TEXTURE || attribute == CRX_DISPLAY_STAGE)
-        return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
-    if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
-    if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
-    hipStream_t stream = as_stream(stream_);
+    if (numViews > 0 && !targets) return fail(CRX_INVALID_ARG, "targets is NULL");
+    std::vector<uint64_t*> bufs;
+    std::vector<uint32_t> px;
+    for (uint32_t v = 0; v < numViews; v++) {
+        CRX_TRY(check_target(targets + v));
+        const uint32_t n = (uint32_t)targets[v].width * (uint32_t)targets[v].height;
+        bufs.push_back(targets[v].framebuffer);
+        px.push_back(n);
+        if (colorbuffers) {
+            if (!targets[v].colorbuffer) return fail(CRX_INVALID_ARG, "target colorbuffer is NULL");
+            bufs.push_back(targets[v].colorbuffer);
+            px.push_back(n);
+        }
+    }
     CRX_HIP(hipSetDevice(ctx->device));
-    std::vector<InstanceRec> inst(numInstances);
-    for (uint32_t i = 0; i < numInstances; i++) {
-        CRX_TRY(check_mesh(instances[i], "instances", i));
-        inst[i] = make_instance(instances[i]);
-    }
-    const size_t offInst = 0;
-    const size_t offPrefix = align_up(inst.size() * sizeof(InstanceRec), 256);
-    const size_t bytes = offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8;
-    CRX_TRY(ctx->resolveMeta.reserve(bytes));
-    CRX_TRY(ctx->resolveStaging.acquire(bytes));
-    char* stage = static_cast<char*>(ctx->resolveStaging.ptr);
-    if (numInstances) {
-        std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
-        std::memcpy(stage + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
-        CRX_HIP(hipMemcpyAsync(ctx->resolveMeta.ptr, stage, bytes, hipMemcpyHostToDevice, stream));
-        CRX_TRY(ctx->resolveStaging.mark(stream));
-    }
-    char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
-    ResolveParams R{};
-    R.inst = reinterpret_cast<const InstanceRec*>(meta + offInst);
-    R.prefix = reinterpret_cast<const uint64_t*>(meta + offPrefix);
-    R.fb = target->framebuffer;
-    R.cb = target->colorbuffer;
-    R.numInstances = numInstances;
-    R.W = target->width;
-    R.H = target->height;
-    R.attribute = (int32_t)attribute;
-    R.Wf = (float)target->width;
-    R.Hf = (float)target->height;
-    R.p00I = 1.0f / target->proj.m[0];
-    R.p11I = 1.0f / target->proj.m[5];
-    const crx_mat4 viewI = mat_inverse(target->view);
-    std::memcpy(R.viewI, viewI.m, sizeof(R.viewI));
-    const f3 o = xform(viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
-    R.origin[0] = o.x;
-    R.origin[1] = o.y;
-    R.origin[2] = o.z;
-    record(ctx, T_RESOLVE, stream);
-    CRX_HIP(launch_resolve(R, stream));
-    record(ctx, T_RESOLVE_END, stream);
-    return CRX_OK;
+    return clear_buffers(ctx, bufs.data(), px.data(), (uint32_t)bufs.size(), as_stream(stream));
 }
 
 // ---- camera / matrix helpers ---------------------------------------------
@@ -894,124 +1181,16 @@ void crx_render_settings_default(crx_render_settings* out) {
 }
 
 crx_status crx_render(crx_context* ctx, crx_scene* scene, const crx_view* view,
-                      const crx_render_settings* settings_, uint64_t* framebuffer, uint64_t* colorbuffer,
-                      void* stream_, crx_draw_stats* stats) {
-    if (!ctx || !scene || !view || !framebuffer) return fail(CRX_INVALID_ARG, "NULL argument");
-    crx_render_settings settings;
-    crx_render_settings_default(&settings);
-    if (settings_) settings = *settings_;
-    if (settings.resolve && !colorbuffer) return fail(CRX_INVALID_ARG, "resolve needs a colorbuffer");
-    if (view->width <= 0 || view->height <= 0) return fail(CRX_INVALID_ARG, "view size must be positive");
-    hipStream_t stream = as_stream(stream_);
+                      const crx_render_settings* settings, uint64_t* framebuffer, uint64_t* colorbuffer,
+                      void* stream, crx_draw_stats* stats) {
+    if (!view) return fail(CRX_INVALID_ARG, "view is NULL");
+    return render_views(ctx, scene, view, 1, settings, &framebuffer, &colorbuffer, as_stream(stream), stats);
+}
 
-    // SNTriangles::update (SNTriangles.h:29-105): frustum side planes of proj*view
-    const crx_mat4 viewProj = mat_mul(view->proj, view->view);
-    float pn[4][3], pd[4];
-    for (int p = 0; p < 4; p++) {
-        const int r = p < 2 ? 0 : 1;
-        const float sg = (p & 1) ? -1.0f : 1.0f;
-        float v[4];
-        for (int c = 0; c < 4; c++) {
-            const float r3 = viewProj.m[4 * c + 3], rr = viewProj.m[4 * c + r];
-            v[c] = sg > 0.0f ? r3 + rr : r3 - rr;
-        }
-        const f3 n{v[0], v[1], v[2]};
-        const float len = sqrtf(dot3(n, n));
-        pn[p][0] = n.x / len; pn[p][1] = n.y / len; pn[p][2] = n.z / len;
-        pd[p] = v[3] / len;
-    }
-    std::vector<uint32_t> visible;
-    visible.reserve(scene->nodes.size());
-    for (uint32_t i = 0; i < scene->nodes.size(); i++) {
-        const crx_scene::Node& nd = scene->nodes[i];
-        bool inside = true;
-        if (settings.frustum_culling) {
-            const crx_box3& b = nd.mesh->aabb;
-            const float cx[2] = {b.min.x, b.max.x}, cy[2] = {b.min.y, b.max.y}, cz[2] = {b.min.z, b.max.z};
-            f3 wmin{INFINITY, INFINITY, INFINITY}, wmax{-INFINITY, -INFINITY, -INFINITY};
-            for (int k = 0; k < 8; k++) {   // Box3::transform corner order
-                const f3 w = xform(nd.world.m, f3{cx[k & 1], cy[(k >> 1) & 1], cz[(k >> 2) & 1]}, 1.0f);
-                wmin.x = std::fmin(wmin.x, w.x); wmin.y = std::fmin(wmin.y, w.y); wmin.z = std::fmin(wmin.z, w.z);
-                wmax.x = std::fmax(wmax.x, w.x); wmax.y = std::fmax(wmax.y, w.y); wmax.z = std::fmax(wmax.z, w.z);
-            }
-            for (int p = 0; p < 4 && inside; p++) {
-                const f3 n{pn[p][0], pn[p][1], pn[p][2]};
-                const f3 v{n.x >= 0.0f ? wmax.x : wmin.x, n.y >= 0.0f ? wmax.y : wmin.y, n.z >= 0.0f ? wmax.z : wmin.z};
-                if (dot3(n, v) + pd[p] < 0.0f) inside = false;
-            }
-        }
-        if (inside) visible.push_back(i);
-    }
-    // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
-    // instances of one mesh keep node order.
-    std::stable_sort(visible.begin(), visible.end(), [&](uint32_t a, uint32_t b) {
-        const crx_mesh* A = scene->nodes[a].mesh;
-        const crx_mesh* B = scene->nodes[b].mesh;
-        if (A->numTriangles == B->numTriangles) return A->seq < B->seq;
-        return A->numTriangles > B->numTriangles;
-    });
-    const uint32_t n = (uint32_t)visible.size();
-    scene->unique.resize(n);
-    scene->instances.resize(n);
-    scene->transforms.resize(n);
-    scene->prefix.resize(n);
-    uint64_t sum = 0;
-    uint32_t u = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const crx_scene::Node& nd = scene->nodes[visible[i]];
-        crx_cmesh cm;
-        std::memset(&cm, 0, sizeof(cm));
-        cm.numTriangles = nd.mesh->numTriangles;
-        cm.indices = nd.mesh->indices;
-        cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
-        cm.colors = nd.mesh->colors;
-        cm.aabb = nd.mesh->aabb;
-        cm.world = nd.world;
-        cm.id = (int32_t)visible[i];
-        cm.numVertices = nd.mesh->numVertices;
-        cm.address = nd.mesh->seq;
-        cm.isLoaded = 1;
-        cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
-        cm.cummulativeTriangleCount = sum;
-        cm.instances.offset = (int32_t)i;
-        cm.instances.count = 0;
-        scene->transforms[i] = mat_mul(view->view, nd.world);
-        scene->prefix[i] = sum;
-        sum += cm.numTriangles;
-        scene->instances[i] = cm;
-        if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
-            scene->unique[u] = cm;
-            u++;
-        }
-        scene->unique[u - 1].instances.count++;
-    }
-
-    crx_render_target target;
-    std::memset(&target, 0, sizeof(target));
-    target.framebuffer = framebuffer;
-    target.colorbuffer = colorbuffer;
-    target.width = view->width;
-    target.height = view->height;
-    target.view = view->view;
-    target.viewI = mat_inverse(view->view);
-    target.proj = view->proj;
-    const f3 cam = xform(target.viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
-    target.cameraPos = crx_vec3{cam.x, cam.y, cam.z};
-    target.f = view->proj.m[5];
-    target.aspect = (float)view->width / (float)view->height;
-
-    CRX_HIP(hipSetDevice(ctx->device));
-    const uint32_t numPixels = (uint32_t)view->width * (uint32_t)view->height;
-    record(ctx, T_CLEAR, stream);
-    CRX_HIP(launch_clear(framebuffer, numPixels, nullptr, ctx->numCUs, stream));
-    if (settings.resolve) CRX_HIP(launch_clear(colorbuffer, numPixels, nullptr, ctx->numCUs, stream));
-    CRX_TRY(crx_draw_triangles_visbuffer(ctx, scene->unique.data(), u, scene->instances.data(), n,
-                                         scene->transforms.data(), &target, settings.instanced, stream_, stats));
-    if (settings.resolve) {
-        CRX_TRY(crx_resolve_visbuffer(ctx, scene->instances.data(), n, scene->prefix.data(), &target,
-                                      (crx_display_attribute)settings.attribute, stream_));
-    }
-    return CRX_OK;
+crx_status crx_render_views(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
+                            const crx_render_settings* settings, uint64_t* const* framebuffers,
+                            uint64_t* const* colorbuffers, void* stream, crx_draw_stats* stats) {
+    return render_views(ctx, scene, views, numViews, settings, framebuffers, colorbuffers, as_stream(stream), stats);
 }
 
 }  // extern "C"

@@ -60,33 +60,49 @@ enum Counter : int {
     CTR_COUNT = 16
 };
 
-struct RasterParams {
-    const DrawItem* items;
-    const InstanceRec* inst;
-    const Transform* transforms;
+// One render target of a draw.  A plain draw has one; a multi-view draw has
+// one per view, all sharing the draw items and instances.  Stage-2 and
+// stage-3 queue entries carry the view index in bits 24..31 of their
+// instance field.
+struct ViewParams {
     uint64_t* fb;
-    uint32_t* ctr;
-    uint64_t* nontrivial;
-    int4* huge;
-    uint32_t numItems;
-    uint32_t numBatches;
-    uint32_t ntCap;
-    uint32_t hugeCap;
+    const Transform* transforms;   // this view's world-to-view transform per instance
     int32_t W, H;
     float Wf, Hf;            // float(W), float(H)
     float Wm1, Hm1;          // float(W - 1), float(H - 1)
     float f, aspect;         // proj[1][1], float(W)/float(H)
     float fa;                // f / aspect
     float faI, fI;           // 1/(f/aspect), 1/f (stage 3)
+    float tinyMargin;        // bound on |approximate - IEEE| screen coordinates, with slack
+    uint32_t pad[2];
+};
+
+constexpr uint32_t kViewShift = 24;                 // instance field: view << 24 | instance
+constexpr uint32_t kInstanceMask = (1u << 24) - 1u;
+constexpr uint32_t kMaxViews = 256;
+
+struct RasterParams {
+    const DrawItem* items;
+    const InstanceRec* inst;
+    const ViewParams* views;
+    uint32_t* ctr;
+    uint64_t* nontrivial;
+    int4* huge;
+    uint32_t numItems;
+    uint32_t numBatches;
+    uint32_t numViews;
+    uint32_t ntCap;
+    uint32_t hugeCap;
     uint32_t smallThreshold;
     int32_t tile;
     uint32_t batchShift;     // log2(triangles per stage-1 batch = workgroup size)
     uint32_t order;          // stage-1 batch order (crx_tuning::stage1_order)
     uint32_t cullOrder;      // 0 = backface first, 1 = frustum (y then x) first, 2 = frustum (x then y) first
     uint32_t fastTiny;       // filtered tiny-triangle cull (crx_tuning::stage1_fast_tiny)
-    float tinyMargin;        // bound on |approximate - IEEE| screen coordinates, with slack
     uint32_t numWaves2;      // waves in the stage-2 grid
     uint32_t numWaves3;      // waves in the stage-3 grid
+    uint32_t pad0;
+    ViewParams view0;        // copy of views[0] (kernel argument: lands in SGPRs)
 };
 
 struct ResolveParams {
@@ -101,7 +117,17 @@ struct ResolveParams {
     float p00I, p11I;          // 1/proj[0][0], 1/proj[1][1]
     float viewI[16];           // inverse(view), GLM order, computed once per frame
     float origin[3];           // viewI * (0,0,0,1)
+    uint32_t firstBlock;       // multi-view resolve: first 64x4 tile block of this view
 };
+
+// One framebuffer of a multi-view clear.
+struct ClearView {
+    uint64_t* fb;
+    uint32_t numPixels;
+    uint32_t firstBlock;       // first block of this view (kClearChunk pixels per block)
+};
+
+constexpr uint32_t kClearChunk = 4096;
 
 constexpr uint64_t kClearPixel = 0xFFFFFFF000000000ull;
 

@@ -90,7 +90,8 @@ __device__ __forceinline__ bool surely_culled_axis(float lo, float hi, float m) 
 //    the quotient error 2^-21*|ndc| scaled by W/2 plus the roundings of the
 //    screen transform, with slack); only triangles that are culled for sure
 //    leave early, all others take the exact path.
-__device__ __forceinline__ void stage1_triangle(const RasterParams& P, const float* __restrict__ m,
+__device__ __forceinline__ void stage1_triangle(const RasterParams& P, const ViewParams& V,
+                                                const float* __restrict__ m,
                                                 bool flip, f3 A, f3 B, f3 C,
                                                 uint64_t id, uint64_t queueEntry) {
     // depth first: ndc.z = depth = -view.z
@@ -99,7 +100,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
     if (da <= 0.0f && db <= 0.0f && dc <= 0.0f) return;
     bool nontrivial = (da <= 0.0f || db <= 0.0f || dc <= 0.0f);
     const bool safeDepth = (da >= 0x1p-102f && db >= 0x1p-102f && dc >= 0x1p-102f);
-    const float fa = P.fa, f = P.f;
+    const float fa = V.fa, f = V.f;
 
     // frustum culling (:113-118), only when every depth is positive; the axis
     // the view culls most is tested before the other coordinate is computed
@@ -161,18 +162,18 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
         const float qmax = fmaxf(fmaxf(fmaxf(fabsf(qxa), fabsf(qxb)), fmaxf(fabsf(qxc), fabsf(qya))),
                                  fmaxf(fabsf(qyb), fabsf(qyc)));
         if (qmax <= 2.0f) {
-            const float sxa = to_screen(qxa, P.Wf), sxb = to_screen(qxb, P.Wf), sxc = to_screen(qxc, P.Wf);
-            const float sya = to_screen(qya, P.Hf), syb = to_screen(qyb, P.Hf), syc = to_screen(qyc, P.Hf);
+            const float sxa = to_screen(qxa, V.Wf), sxb = to_screen(qxb, V.Wf), sxc = to_screen(qxc, V.Wf);
+            const float sya = to_screen(qya, V.Hf), syb = to_screen(qyb, V.Hf), syc = to_screen(qyc, V.Hf);
             const float lox = fmaxf(fminf(sxa, fminf(sxb, sxc)), 0.0f);
-            const float hix = fminf(fmaxf(sxa, fmaxf(sxb, sxc)), P.Wm1);
+            const float hix = fminf(fmaxf(sxa, fmaxf(sxb, sxc)), V.Wm1);
             const float loy = fmaxf(fminf(sya, fminf(syb, syc)), 0.0f);
-            const float hiy = fminf(fmaxf(sya, fmaxf(syb, syc)), P.Hm1);
-            if (surely_culled_axis(lox, hix, P.tinyMargin) || surely_culled_axis(loy, hiy, P.tinyMargin)) return;
+            const float hiy = fminf(fmaxf(sya, fmaxf(syb, syc)), V.Hm1);
+            if (surely_culled_axis(lox, hix, V.tinyMargin) || surely_culled_axis(loy, hiy, V.tinyMargin)) return;
         }
     }
 
     // viewToNDC (rasterization_helpers.cuh:95-101) and ndcToScreen - 0.5 (:103-108, :132-134)
-    const float Wf = P.Wf, Hf = P.Hf;
+    const float Wf = V.Wf, Hf = V.Hf;
     const float ax = ((pxa / da) * 0.5f + 0.5f) * Wf - 0.5f;
     const float bx = ((pxb / db) * 0.5f + 0.5f) * Wf - 0.5f;
     const float cx = ((pxc / dc) * 0.5f + 0.5f) * Wf - 0.5f;
@@ -185,9 +186,9 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
     float min_y = fminf(ay, fminf(by, cy));
     float max_y = fmaxf(ay, fmaxf(by, cy));
     min_x = fmaxf(min_x, 0.0f);
-    max_x = fminf(max_x, P.Wm1);
+    max_x = fminf(max_x, V.Wm1);
     min_y = fmaxf(min_y, 0.0f);
-    max_y = fminf(max_y, P.Hm1);
+    max_y = fminf(max_y, V.Hm1);
 
     // tiny-triangle cull (:153-158)
     if (!nontrivial) {
@@ -223,8 +224,8 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
     const float s0y = start_y - ay + 0.0f;
     float s_row = (s0x * vacy - s0y * vacx) * factor;
     float t_row = (vabx * s0y - vaby * s0x) * factor;
-    uint32_t rowPix = (uint32_t)f2i(start_x) + (uint32_t)P.W * (uint32_t)f2i(start_y);
-    const uint32_t numPixels = (uint32_t)P.W * (uint32_t)P.H;
+    uint32_t rowPix = (uint32_t)f2i(start_x) + (uint32_t)V.W * (uint32_t)f2i(start_y);
+    const uint32_t numPixels = (uint32_t)V.W * (uint32_t)V.H;
     for (int fy = 0; fy < size_y; ++fy) {
         float s = s_row, t = t_row;
         uint32_t pix = rowPix;
@@ -233,7 +234,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
             if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
                 const float inv_depth = (v * iza + s * izb) + t * izc;
                 const float depth = 1.0f / inv_depth;
-                atomic_min_fb(P.fb, pix, pack_pixel(depth, id));
+                atomic_min_fb(V.fb, pix, pack_pixel(depth, id));
             }
             s += ds_dx;
             t += dt_dx;
@@ -241,7 +242,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const flo
         }
         s_row += ds_dy;
         t_row += dt_dy;
-        rowPix += (uint32_t)P.W;
+        rowPix += (uint32_t)V.W;
     }
 }
 
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
         step = 1;
     }
     if (begin >= end) return;
-    const auto transforms = kconst(P.transforms);
+    const auto transforms = kconst(P.view0.transforms);
 
     S1Batch cur;
     s1_fetch(P, begin, cur, false);
@@ -388,7 +389,215 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                 #pragma unroll
                 for (int e = 0; e < 16; e++) m[e] = T->m[e];
                 const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
-                stage1_triangle(P, m, T->flip != 0u, A, B, C, id, ((uint64_t)inst << 32) | (uint64_t)tri);
+                stage1_triangle(P, P.view0, m, T->flip != 0u, A, B, C, id, ((uint64_t)inst << 32) | (uint64_t)tri);
+            }
+        }
+        if (!more) break;
+        batch = next;
+        cur = nxt;
+    }
+}
+
+namespace {
+
+// DPP move of x (lanes outside rowMask keep `old`).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f(float old, float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, x),
+                                                                 CTRL, ROWMASK, 0xF, false));
+}
+
+// Minimum / maximum over the 64 lanes (all active), as a scalar: quad
+// exchanges, row rotations, then row broadcasts leave the result in lane 63.
+__device__ __forceinline__ float wave_min(float x) {
+    x = fminf(x, dpp_f<0xB1, 0xF>(x, x));    // quad_perm [1,0,3,2]
+    x = fminf(x, dpp_f<0x4E, 0xF>(x, x));    // quad_perm [2,3,0,1]
+    x = fminf(x, dpp_f<0x124, 0xF>(x, x));   // row_ror:4
+    x = fminf(x, dpp_f<0x128, 0xF>(x, x));   // row_ror:8
+    x = fminf(x, dpp_f<0x142, 0xA>(x, x));   // row_bcast:15 into rows 1, 3
+    x = fminf(x, dpp_f<0x143, 0xC>(x, x));   // row_bcast:31 into rows 2, 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
+
+__device__ __forceinline__ float wave_max(float x) {
+    x = fmaxf(x, dpp_f<0xB1, 0xF>(x, x));
+    x = fmaxf(x, dpp_f<0x4E, 0xF>(x, x));
+    x = fmaxf(x, dpp_f<0x124, 0xF>(x, x));
+    x = fmaxf(x, dpp_f<0x128, 0xF>(x, x));
+    x = fmaxf(x, dpp_f<0x142, 0xA>(x, x));
+    x = fmaxf(x, dpp_f<0x143, 0xC>(x, x));
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
+
+// Object-space box of a wave's 64 triangles.
+struct WaveBox {
+    f3 lo, hi;
+    float M;   // max |coordinate| over the box
+};
+
+// Culling planes of one view for one instance, from its world-to-view
+// transform m: g(p) = a.p + b, with g > 0 meaning "outside".
+//   x+ : fa*x_view + z_view    (RN(fa*x/d) > 1,  d = -z_view)
+//   x- : -fa*x_view + z_view   (RN(fa*x/d) < -1)
+//   y+ / y- : the same with f and y_view
+//   z  : z_view                (d < 0: behind the camera)
+// slope/cnst bound the magnitudes that enter the rounding errors:
+// S_x(p) = sum_k |m_xk||p_k| + |m_x3| <= A_x*M + B_x.
+struct ViewCull {
+    float px[4], nx[4], py[4], ny[4], z[4];
+    float slopeX, cnstX, slopeY, cnstY, slopeZ, cnstZ;
+};
+
+__device__ __forceinline__ ViewCull view_cull(const float* m, float fa, float f) {
+    ViewCull c;
+    #pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float x = m[4 * k], y = m[4 * k + 1], z = m[4 * k + 2];
+        c.px[k] = fa * x + z;
+        c.nx[k] = -(fa * x) + z;
+        c.py[k] = f * y + z;
+        c.ny[k] = -(f * y) + z;
+        c.z[k] = z;
+    }
+    const float A0 = (fabsf(m[0]) + fabsf(m[4])) + fabsf(m[8]);
+    const float A1 = (fabsf(m[1]) + fabsf(m[5])) + fabsf(m[9]);
+    const float A2 = (fabsf(m[2]) + fabsf(m[6])) + fabsf(m[10]);
+    c.slopeX = fabsf(fa) * A0 + A2;
+    c.cnstX = fabsf(fa) * fabsf(m[12]) + fabsf(m[14]);
+    c.slopeY = fabsf(f) * A1 + A2;
+    c.cnstY = fabsf(f) * fabsf(m[13]) + fabsf(m[14]);
+    c.slopeZ = A2;
+    c.cnstZ = fabsf(m[14]);
+    return c;
+}
+
+// min over the box of a.p + b: per coordinate the smaller of the two corners.
+__device__ __forceinline__ float box_min(const float* a, const WaveBox& B) {
+    return ((fminf(a[0] * B.lo.x, a[0] * B.hi.x) + fminf(a[1] * B.lo.y, a[1] * B.hi.y)) +
+            fminf(a[2] * B.lo.z, a[2] * B.hi.z)) + a[3];
+}
+
+// False only when every triangle inside the box is culled by the exact
+// per-triangle tests of stage1_triangle for this view: the box lies behind the
+// camera, or in front of it and beyond one side plane, by a margin.
+//
+// Why the margin suffices: with u = 2^-24, the float plane value at the chosen
+// corner is within 8u*(|fa|*S_x + S_z) of the exact affine minimum over the
+// box (coefficient roundings, products, three sums), and the per-triangle
+// code's own x_view, z_view and fa*x_view are within 4u*S_x, 4u*S_z and one
+// more rounding of the exact values; the cull there fires once
+// fa*x_view/d > 1 + 2^-22.  All of this is below 22u*(|fa|*S_x + S_z);
+// the margin is 2^-16 = 256u times that bound, plus 2^-90 for underflow.
+// The side planes are only used when every depth is above that margin too,
+// so each vertex has d >= 2^-102 and the division-free frustum test applies.
+__device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B) {
+    const float mz = (c.slopeZ * B.M + c.cnstZ) * 0x1p-16f + 0x1p-90f;
+    if (box_min(c.z, B) > mz) return false;                 // behind the camera
+    const float negz[4] = {-c.z[0], -c.z[1], -c.z[2], -c.z[3]};
+    if (!(box_min(negz, B) > mz)) return true;              // some depth may be small or negative
+    const float mx = (c.slopeX * B.M + c.cnstX) * 0x1p-16f + 0x1p-90f;
+    const float my = (c.slopeY * B.M + c.cnstY) * 0x1p-16f + 0x1p-90f;
+    if (box_min(c.px, B) > mx || box_min(c.nx, B) > mx) return false;
+    if (box_min(c.py, B) > my || box_min(c.ny, B) > my) return false;
+    return true;
+}
+
+__device__ __forceinline__ ViewCull lane_view_cull(const RasterParams& P, uint32_t view, uint32_t inst) {
+    const ViewParams* V = P.views + view;
+    const float* m = V->transforms[inst].m;
+    float mm[16];
+    #pragma unroll
+    for (int e = 0; e < 16; e++) mm[e] = m[e];
+    return view_cull(mm, V->fa, V->f);
+}
+
+}  // namespace
+
+// Stage 1 of a multi-view draw.  Each batch of triangles is fetched once and
+// drawn into every view that may see it: a wave reduces the object-space box
+// of its 64 triangles, each lane tests that box against one view
+// (view_may_see: conservative, so it never drops a triangle the view would
+// draw), and the wave runs the exact per-triangle code of the single-view
+// stage 1 for the views left in the ballot.  Queue entries carry the view in
+// the top 8 bits of their instance field.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
+    const uint32_t numBatches = P.numBatches;
+    const uint32_t G = gridDim.x;
+    uint32_t begin, end, step;
+    if (P.order == 0) {
+        const uint32_t perXcd = (numBatches + 7u) >> 3;
+        begin = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
+        end = min(numBatches, (blockIdx.x & 7u) * perXcd + perXcd);
+        step = G >> 3;
+    } else if (P.order == 1) {
+        begin = blockIdx.x;
+        end = numBatches;
+        step = G;
+    } else {
+        const uint32_t per = (numBatches + G - 1u) / G;
+        begin = blockIdx.x * per;
+        end = min(numBatches, begin + per);
+        step = 1;
+    }
+    if (begin >= end) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t numViews = P.numViews;
+    const bool cacheCull = numViews <= (uint32_t)kWave;   // lane v keeps view v's planes
+    ViewCull cull{};
+    uint32_t cullInst = 0xffffffffu;
+
+    S1Batch cur;
+    s1_fetch(P, begin, cur, false);
+    for (uint32_t batch = begin;;) {
+        const f3 A = load_pos_g(cur.it.positions, cur.i0);
+        const f3 B = load_pos_g(cur.it.positions, cur.i1);
+        const f3 C = load_pos_g(cur.it.positions, cur.i2);
+        const uint32_t next = batch + step;
+        const bool more = next < end;
+        S1Batch nxt = cur;
+        if (more) s1_fetch(P, next, nxt, true);
+
+        // object-space box of the wave's triangles (lanes past the item's end
+        // hold its last triangle, which is inside the mesh anyway)
+        WaveBox box;
+        box.lo = f3{wave_min(fminf(A.x, fminf(B.x, C.x))), wave_min(fminf(A.y, fminf(B.y, C.y))),
+                    wave_min(fminf(A.z, fminf(B.z, C.z)))};
+        box.hi = f3{wave_max(fmaxf(A.x, fmaxf(B.x, C.x))), wave_max(fmaxf(A.y, fmaxf(B.y, C.y))),
+                    wave_max(fmaxf(A.z, fmaxf(B.z, C.z)))};
+        box.M = fmaxf(fmaxf(fmaxf(fabsf(box.lo.x), fabsf(box.hi.x)), fmaxf(fabsf(box.lo.y), fabsf(box.hi.y))),
+                      fmaxf(fabsf(box.lo.z), fabsf(box.hi.z)));
+        // a NaN or infinite coordinate anywhere: no box, every view draws the wave
+        const float s = (((A.x + A.y) + (A.z + B.x)) + ((B.y + B.z) + (C.x + C.y))) + C.z;
+        const bool unbounded = __ballot(!(s - s == 0.0f)) != 0ull;
+
+        const bool valid = cur.tri < cur.it.numTriangles;
+        const uint32_t tri = cur.tri;
+        for (uint32_t j = 0; j < cur.it.instCount; j++) {
+            const uint32_t inst = cur.it.instOffset + j;
+            const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
+            for (uint32_t vb = 0; vb < numViews; vb += kWave) {
+                const uint32_t v = vb + lane;
+                bool maybe = false;
+                if (v < numViews) {
+                    if (!cacheCull || inst != cullInst) cull = lane_view_cull(P, v, inst);
+                    maybe = unbounded || view_may_see(cull, box);
+                }
+                if (cacheCull) cullInst = inst;
+                uint64_t mask = __ballot(maybe);
+                while (mask) {
+                    const uint32_t vi = vb + (uint32_t)__builtin_ctzll(mask);
+                    mask &= mask - 1ull;
+                    const ViewParams V = kconst(P.views)[vi];
+                    const auto T = kconst(V.transforms) + inst;
+                    float m[16];
+                    #pragma unroll
+                    for (int e = 0; e < 16; e++) m[e] = T->m[e];
+                    if (valid) {
+                        const uint64_t entry = ((uint64_t)((vi << kViewShift) | inst) << 32) | (uint64_t)tri;
+                        stage1_triangle(P, V, m, T->flip != 0u, A, B, C, id, entry);
+                    }
+                }
             }
         }
         if (!more) break;
@@ -507,7 +716,20 @@ struct S2Setup {
     int32_t size_x;
     uint32_t numFragments;
     uint32_t id_lo, id_hi;
+    uint64_t* fb;            // the entry's view
+    int32_t W, H;
 };
+
+// Instance field of a queue entry: the instance index for a plain draw,
+// view << 24 | instance for a multi-view draw.
+__device__ __forceinline__ uint32_t entry_instance(const RasterParams& P, uint32_t field) {
+    return P.numViews == 1u ? field : (field & kInstanceMask);
+}
+
+// Index of the view of a queue entry (0 for a plain draw).
+__device__ __forceinline__ uint32_t entry_view(const RasterParams& P, uint32_t field) {
+    return P.numViews == 1u ? 0u : (field >> kViewShift);
+}
 
 }  // namespace
 
@@ -547,10 +769,11 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
         uint32_t meshIndex = 0, triangleIndex = 0;
         if (idx < count) {
             const uint64_t packed = P.nontrivial[idx];
-            meshIndex = (uint32_t)(packed >> 32);
+            meshIndex = (uint32_t)(packed >> 32);    // view << 24 | instance, kept for the tile entries
             triangleIndex = (uint32_t)packed;
-            const InstanceRec& mesh = P.inst[meshIndex];
-            const float* m = P.transforms[mesh.transformIndex].m;
+            const InstanceRec& mesh = P.inst[entry_instance(P, meshIndex)];
+            const ViewParams& V = P.views[entry_view(P, meshIndex)];
+            const float* m = V.transforms[mesh.transformIndex].m;
             uint32_t i0, i1, i2;
             if (mesh.indices) {
                 i0 = mesh.indices[3u * triangleIndex + 0u];
@@ -562,18 +785,18 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
             const f3 av = xform_point(m, load_pos(mesh.positions, i0));
             const f3 bv = xform_point(m, load_pos(mesh.positions, i1));
             const f3 cv = xform_point(m, load_pos(mesh.positions, i2));
-            const float f = P.f, aspect = P.aspect;
+            const float f = V.f, aspect = V.aspect;
             const float da = -av.z, db = -bv.z, dc = -cv.z;
             const f2 an = view_to_ndc_xy(av, f, aspect);
             const f2 bn = view_to_ndc_xy(bv, f, aspect);
             const f2 cn = view_to_ndc_xy(cv, f, aspect);
             float min_x, max_x, min_y, max_y;
-            stage2_bbox(av, bv, cv, da, db, dc, an, bn, cn, f, aspect, P.Wf, P.Hf, min_x, max_x, min_y, max_y);
+            stage2_bbox(av, bv, cv, da, db, dc, an, bn, cn, f, aspect, V.Wf, V.Hf, min_x, max_x, min_y, max_y);
             min_x -= 0.5f; min_y -= 0.5f;
             max_x += 0.5f; max_y += 0.5f;
-            const float asx = (an.x * 0.5f + 0.5f) * P.Wf, asy = (an.y * 0.5f + 0.5f) * P.Hf;
-            const float bsx = (bn.x * 0.5f + 0.5f) * P.Wf, bsy = (bn.y * 0.5f + 0.5f) * P.Hf;
-            const float csx = (cn.x * 0.5f + 0.5f) * P.Wf, csy = (cn.y * 0.5f + 0.5f) * P.Hf;
+            const float asx = (an.x * 0.5f + 0.5f) * V.Wf, asy = (an.y * 0.5f + 0.5f) * V.Hf;
+            const float bsx = (bn.x * 0.5f + 0.5f) * V.Wf, bsy = (bn.y * 0.5f + 0.5f) * V.Hf;
+            const float csx = (cn.x * 0.5f + 0.5f) * V.Wf, csy = (cn.y * 0.5f + 0.5f) * V.Hf;
             const float vabx = bsx - asx, vaby = bsy - asy;
             const float vacx = csx - asx, vacy = csy - asy;
             const float inv_det = 1.0f / (vabx * vacy - vaby * vacx);
@@ -602,6 +825,9 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
                     const uint32_t id32 = (uint32_t)mesh.cumm + triangleIndex;
                     s.id_lo = id32;
                     s.id_hi = 0u;
+                    s.fb = V.fb;
+                    s.W = V.W;
+                    s.H = V.H;
                     setup[lane] = s;
                 }
             }
@@ -665,10 +891,10 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
                 if (ss >= 0.0f && tt >= 0.0f && vv >= 0.0f) {
                     const int ix = f2i(px), iy = f2i(py);
                     // samples outside the screen are discarded (the reference wraps them)
-                    if (ix >= 0 && ix < P.W && iy >= 0 && iy < P.H) {
+                    if (ix >= 0 && ix < s.W && iy >= 0 && iy < s.H) {
                         const float inv_depth = (vv * s.iza + ss * s.izb) + tt * s.izc;
                         const float depth = 1.0f / inv_depth;
-                        atomic_min_fb(P.fb, (uint32_t)(ix + P.W * iy), pack_pixel(depth, id));
+                        atomic_min_fb(s.fb, (uint32_t)(ix + s.W * iy), pack_pixel(depth, id));
                     }
                 }
             }
@@ -724,8 +950,11 @@ __global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
         }
         if (idx >= count) break;
         const int4 e = P.huge[idx];
-        const InstanceRec& mesh = P.inst[e.x];
-        const float* m = P.transforms[mesh.transformIndex].m;
+        // the entry is the same for all lanes: its view record is read by scalar loads
+        const uint32_t field = (uint32_t)__builtin_amdgcn_readfirstlane(e.x);
+        const InstanceRec& mesh = P.inst[entry_instance(P, field)];
+        const auto& V = kconst(P.views)[entry_view(P, field)];
+        const float* m = V.transforms[mesh.transformIndex].m;
         const uint32_t tri = (uint32_t)e.y;
         uint32_t i0, i1, i2;
         if (mesh.indices) {
@@ -744,26 +973,26 @@ __global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
         T.tvec = f3{-av.x, -av.y, -av.z};
         T.qvec = cross3(T.tvec, T.e1);
         const uint64_t id = mesh.cumm + (uint64_t)(int64_t)e.y;
-        const float min_x = clampf((float)(e.z * TILE), 0.0f, P.Wf);
-        const float min_y = clampf((float)(e.w * TILE), 0.0f, P.Hf);
+        const float min_x = clampf((float)(e.z * TILE), 0.0f, V.Wf);
+        const float min_y = clampf((float)(e.w * TILE), 0.0f, V.Hf);
         const float fx0 = floorf(min_x), fy0 = floorf(min_y);
         for (int c = 0; c < colsPerLane; c++) {
             const int fragX = (int)(lane % (uint32_t)lanesPerRow) + c * lanesPerRow;
             const int px = f2i(fx0 + (float)fragX);
-            if (px >= P.W) continue;   // the reference wraps these into the next row
-            const float u = 2.0f * ((float)px + 0.5f) / P.Wf - 1.0f;
-            const float rx = P.faI * u;
+            if (px >= V.W) continue;   // the reference wraps these into the next row
+            const float u = 2.0f * ((float)px + 0.5f) / V.Wf - 1.0f;
+            const float rx = V.faI * u;
             for (int y0 = 0; y0 < TILE; y0 += rowsPerStep) {
                 const int fragY = y0 + (int)(lane / (uint32_t)lanesPerRow);
                 const int py = f2i(fy0 + (float)fragY);
-                if (py >= P.H) break;
-                const float v = 2.0f * ((float)py + 0.5f) / P.Hf - 1.0f;
-                const f3 dir = normalize3(f3{rx, P.fI * v, -1.0f});
+                if (py >= V.H) break;
+                const float v = 2.0f * ((float)py + 0.5f) / V.Hf - 1.0f;
+                const f3 dir = normalize3(f3{rx, V.fI * v, -1.0f});
                 const float t = intersect_mt(T, dir);
                 if (t == __builtin_inff()) continue;
                 // depth = dot(t*dir, (0,0,-1)) (:734)
                 const float depth = ((t * dir.x) * 0.0f + (t * dir.y) * 0.0f) + (t * dir.z) * -1.0f;
-                atomic_min_fb(P.fb, (uint32_t)(px + P.W * py), pack_pixel(depth, id));
+                atomic_min_fb(V.fb, (uint32_t)(px + V.W * py), pack_pixel(depth, id));
             }
         }
     }
@@ -779,14 +1008,8 @@ __device__ __forceinline__ float intersect_plane(f3 origin, f3 dir, f3 normal, f
     return (t >= 0.0f) ? t : __builtin_inff();
 }
 
-}  // namespace
-
-// Resolve: one lane per pixel, 64x4 pixels per workgroup (a wave reads 512
-// contiguous bytes of the visibility buffer).
-__global__ __launch_bounds__(256) void k_resolve(ResolveParams R) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= R.W || y >= R.H) return;
+// Resolve of one pixel (x, y) < (W, H).
+__device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int y) {
     const uint32_t pixelID = (uint32_t)x + (uint32_t)R.W * (uint32_t)y;
 
     const uint64_t pixel = R.fb[pixelID];
@@ -916,6 +1139,56 @@ __global__ __launch_bounds__(256) void k_resolve(ResolveParams R) {
     }
 }
 
+// last view whose first block is <= b (firstBlock read through `get`)
+template <typename Get>
+__device__ __forceinline__ uint32_t find_view(uint32_t numViews, uint32_t b, Get get) {
+    uint32_t lo = 0, hi = numViews - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (get(mid) <= b) lo = mid; else hi = mid - 1u;
+    }
+    return lo;
+}
+
+}  // namespace
+
+// Resolve: one lane per pixel, 64x4 pixels per workgroup (a wave reads 512
+// contiguous bytes of the visibility buffer).
+__global__ __launch_bounds__(256) void k_resolve(ResolveParams R) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= R.W || y >= R.H) return;
+    resolve_pixel(R, x, y);
+}
+
+// Resolve of several views in one launch: block b resolves 64x4 tile
+// b - firstBlock[v] of view v, whose record is read by scalar loads.
+__global__ __launch_bounds__(256) void k_resolve_views(const ResolveParams* __restrict__ views, uint32_t numViews) {
+    const auto rv = kconst(views);
+    const uint32_t v = find_view(numViews, blockIdx.x, [&](uint32_t i) { return rv[i].firstBlock; });
+    const ResolveParams R = rv[v];
+    const uint32_t local = blockIdx.x - R.firstBlock;
+    const uint32_t tilesX = ((uint32_t)R.W + 63u) / 64u;
+    const int x = (int)(local % tilesX) * 64 + (int)(threadIdx.x & 63);
+    const int y = (int)(local / tilesX) * 4 + (int)(threadIdx.x >> 6);
+    if (x >= R.W || y >= R.H) return;
+    resolve_pixel(R, x, y);
+}
+
+// Clear of several framebuffers in one launch: block b clears the 4096-pixel
+// chunk b - firstBlock[v] of view v (each wave stores 512 contiguous bytes per
+// step), and block 0 resets the draw counters.
+__global__ __launch_bounds__(256) void k_clear_views(const ClearView* __restrict__ views, uint32_t numViews,
+                                                     uint32_t* __restrict__ ctr) {
+    const auto cv = kconst(views);
+    const uint32_t v = find_view(numViews, blockIdx.x, [&](uint32_t i) { return cv[i].firstBlock; });
+    const ClearView c = cv[v];
+    const uint32_t base = (blockIdx.x - c.firstBlock) * kClearChunk;
+    const uint32_t end = min(c.numPixels, base + kClearChunk);
+    for (uint32_t i = base + threadIdx.x; i < end; i += 256u) c.fb[i] = kClearPixel;
+    if (blockIdx.x == 0 && ctr && threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
+}
+
 // Plain streaming copy, 16 bytes per lane (calibrates the HBM traffic counters).
 __global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -947,6 +1220,29 @@ hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, h
         case 512: hipLaunchKernelGGL(k_stage1<512>, dim3(grid), dim3(512), 0, stream, P); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
+    switch (block) {
+        case 128: hipLaunchKernelGGL(k_stage1_views<128>, dim3(grid), dim3(128), 0, stream, P); break;
+        case 256: hipLaunchKernelGGL(k_stage1_views<256>, dim3(grid), dim3(256), 0, stream, P); break;
+        case 512: hipLaunchKernelGGL(k_stage1_views<512>, dim3(grid), dim3(512), 0, stream, P); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, hipStream_t stream) {
+    if (numViews == 0 || numBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resolve_views, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_views(const ClearView* views, uint32_t numViews, uint32_t numBlocks, uint32_t* ctr,
+                              hipStream_t stream) {
+    if (numViews == 0 || numBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_views, dim3(numBlocks), dim3(256), 0, stream, views, numViews, ctr);
     return hipGetLastError();
 }
 

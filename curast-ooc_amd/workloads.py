@@ -106,3 +106,50 @@ def instances_view(extent: float = 200.0, width: int = 1920, height: int = 1080,
     d = distance / math.sqrt(3.0)
     view = crx.look_at((c + d, c + d, c + d), (c, c, c), (0.0, 1.0, 0.0))
     return view, projection(width, height)
+
+
+MULTIVIEW_SIZES = ((256, 256), (512, 512), (1280, 720), (1920, 1080), (3840, 2160))
+
+
+def _look_dir(yaw: float, pitch: float):
+    """Unit direction with heading `yaw` (about +y) tilted down by `pitch`."""
+    cp = math.cos(pitch)
+    return (cp * math.sin(yaw), -math.sin(pitch), cp * math.cos(yaw))
+
+
+def multiview_views(n: int, positions: np.ndarray, count: int = 64, closeup_fraction: float = 0.25,
+                    sizes=MULTIVIEW_SIZES, seed: int = SEED_VIEWS, spacing: float = 1.0):
+    """Seeded camera poses over an n x n heightfield (positions = its vertex
+    array, used for the ground height under each camera).
+
+    Resolutions are drawn uniformly from `sizes`.  round(count*closeup_fraction)
+    views are close-ups: the eye 1.5-4 units above the ground looking down
+    30-70 degrees, so one cell spans several hundred pixels and its triangles
+    go through stages 2 and 3.  The others hover 100-600 units above the ground
+    at a random point, heading and 20-60 degree pitch.
+    Returns a list of (view, proj, width, height, is_closeup)."""
+    rng = np.random.default_rng(seed)
+    pos = positions.reshape(-1, 3)
+    n_close = int(round(count * closeup_fraction))
+    close = np.zeros(count, dtype=bool)
+    close[rng.permutation(count)[:n_close]] = True
+    out = []
+    for k in range(count):
+        W, H = sizes[int(rng.integers(len(sizes)))]
+        yaw = float(rng.uniform(0.0, 2.0 * math.pi))
+        if close[k]:
+            c = int(rng.integers(8, n - 9))
+            r = int(rng.integers(8, n - 9))
+            ground = float(pos[r * n + c, 1])
+            eye = ((c + 0.5) * spacing, ground + float(rng.uniform(1.5, 4.0)), (r + 0.5) * spacing)
+            pitch = math.radians(float(rng.uniform(30.0, 70.0)))
+        else:
+            c = int(rng.integers(0, n))
+            r = int(rng.integers(0, n))
+            ground = float(pos[r * n + c, 1])
+            eye = (c * spacing, ground + float(rng.uniform(100.0, 600.0)), r * spacing)
+            pitch = math.radians(float(rng.uniform(20.0, 60.0)))
+        d = _look_dir(yaw, pitch)
+        view = crx.look_at(eye, (eye[0] + d[0], eye[1] + d[1], eye[2] + d[2]), (0.0, 1.0, 0.0))
+        out.append((view, projection(W, H), W, H, bool(close[k])))
+    return out

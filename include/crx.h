@@ -245,6 +245,41 @@ crx_status crx_resolve_visbuffer(crx_context* ctx,
                                  const crx_render_target* target,
                                  crx_display_attribute attribute, void* stream);
 
+/* ---- Batched multi-view (no single reference counterpart: it generalises the
+ * instanced stage-1 loop of triangles_visbuffer.cu:373-379, which fetches a
+ * triangle once and loops over transforms, to a loop over views). ---- */
+
+#define CRX_MAX_VIEWS 256
+
+/* drawTrianglesVisbuffer for numViews (1..CRX_MAX_VIEWS) render targets that
+ * share one mesh / instance list, in one pass over the geometry.
+ * `transforms` holds numViews * numInstances matrices, view-major:
+ * transforms[v * numInstances + i] = view_v * world_i.  targets[v] gives view
+ * v's cleared framebuffer, size and projection; sizes may differ per view.
+ * The visibility buffer of every view is bit-identical to a single-view
+ * crx_draw_triangles_visbuffer of the same lists into that target.  A
+ * multi-view draw takes at most 2^24 instances.  stats->num_triangles counts
+ * triangle-instances times views. */
+crx_status crx_draw_triangles_visbuffer_views(crx_context* ctx,
+                                              const crx_cmesh* meshes, uint32_t numMeshes,
+                                              const crx_cmesh* instances, uint32_t numInstances,
+                                              const crx_mat4* transforms,
+                                              const crx_render_target* targets, uint32_t numViews,
+                                              int instanced, void* stream,
+                                              crx_draw_stats* stats /* nullable */);
+
+/* Resolve of numViews targets sharing one instance list, in one launch. */
+crx_status crx_resolve_visbuffer_views(crx_context* ctx,
+                                       const crx_cmesh* instances, uint32_t numInstances,
+                                       const uint64_t* prefix,
+                                       const crx_render_target* targets, uint32_t numViews,
+                                       crx_display_attribute attribute, void* stream);
+
+/* Clear every target's framebuffer (and, if colorbuffers != 0, colorbuffer)
+ * to 0xFFFFFFF0'00000000 in one launch. */
+crx_status crx_clear_views(crx_context* ctx, const crx_render_target* targets, uint32_t numViews,
+                           int colorbuffers, void* stream);
+
 /* Per-stage device time of the last draw / resolve issued on a context with
  * timing enabled (hipEvent pairs around each launch, the role of the
  * reference's Timer, src/Timer.h:61-113).  Milliseconds; -1 if not measured. */
@@ -345,6 +380,16 @@ crx_status crx_render(crx_context* ctx, crx_scene* scene, const crx_view* view,
                       const crx_render_settings* settings,
                       uint64_t* framebuffer, uint64_t* colorbuffer, void* stream,
                       crx_draw_stats* stats /* nullable */);
+
+/* crx_render for numViews (1..CRX_MAX_VIEWS) views in one batched pass.  The
+ * node cull keeps a node that is visible in ANY view, so all views share one
+ * instance list (and one triangle-ID numbering); each view's buffers equal a
+ * single-view draw of that list.  framebuffers[v] / colorbuffers[v] are
+ * device buffers of views[v].width * views[v].height uint64. */
+crx_status crx_render_views(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
+                            const crx_render_settings* settings,
+                            uint64_t* const* framebuffers, uint64_t* const* colorbuffers, void* stream,
+                            crx_draw_stats* stats /* nullable */);
 
 #ifdef __cplusplus
 }

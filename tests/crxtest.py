@@ -113,6 +113,26 @@ class GpuScene:
         out_fb = fb.download(np.uint64, W * H, stream)
         return out_fb, out_cb, st
 
+    def render_views(self, views, instanced=True, attribute=None, stream=None):
+        """One batched draw (and resolve) of all `views` = [(W, H, view, proj)].
+        Returns per-view framebuffers, colour buffers (or None) and the stats."""
+        ctx = self.ctx
+        frames = [prepare(self.spec, v, self.pointers) for (_, _, v, _) in views]
+        fr0 = frames[0]
+        transforms = [t for fr in frames for t in fr.transforms]
+        fbs = [ctx.alloc(8 * W * H) for (W, H, _, _) in views]
+        cbs = [ctx.alloc(8 * W * H) for (W, H, _, _) in views] if attribute is not None else None
+        targets = [crx.make_target(fbs[k], cbs[k] if cbs else None, W, H, v, p)
+                   for k, (W, H, v, p) in enumerate(views)]
+        ctx.clear_views(targets, colorbuffers=cbs is not None, stream=stream)
+        st = ctx.draw_triangles_visbuffer_views(fr0.unique, fr0.instances, transforms, targets, instanced, stream)
+        out_cb = None
+        if attribute is not None:
+            ctx.resolve_visbuffer_views(fr0.instances, fr0.prefix, targets, attribute, stream)
+            out_cb = [cbs[k].download(np.uint64, W * H, stream) for k, (W, H, _, _) in enumerate(views)]
+        out_fb = [fbs[k].download(np.uint64, W * H, stream) for k, (W, H, _, _) in enumerate(views)]
+        return out_fb, out_cb, st
+
 
 def describe_mismatch(a: np.ndarray, b: np.ndarray, W: int) -> str:
     bad = np.nonzero(a != b)[0]
