@@ -79,6 +79,155 @@ def load_traffic(workload: str):
         return None
 
 
+def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int = 64):
+    """Batched multi-view workload: `count` seeded views of the heightfield
+    (mixed resolutions, a quarter close-ups), each a full frame (clear, three
+    stages, resolve).  Batched = one crx_render_views call per step; plain =
+    crx_render once per view.  Returns views/s of both."""
+    import crx
+    import workloads as wl
+    vs = wl.multiview_views(n, pos, count=count)
+    views = [crx.make_view(v, p, W, H) for (v, p, W, H, _) in vs]
+    fbs = [ctx.alloc(8 * v.width * v.height) for v in views]
+    cbs = [ctx.alloc(8 * v.width * v.height) for v in views]
+    settings = crx.render_settings(instanced=1, frustum_culling=1, resolve=1, attribute=crx.DISPLAY_VERTEX_COLORS)
+    pixels = sum(v.width * v.height for v in views)
+
+    def batched():
+        return scene.render_views(views, fbs, cbs, settings)
+
+    def plain():
+        for k, v in enumerate(views):
+            scene.render(v, fbs[k], cbs[k], settings)
+
+    def timed(fn, reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    for _ in range(warmup):
+        batched()
+    # the batched frame must equal the plain frames bit for bit
+    batched()
+    ctx.synchronize()
+    got = [b.download(np.uint64, v.width * v.height) for b, v in zip(fbs, views)]
+    plain()
+    ctx.synchronize()
+    same = all(np.array_equal(g, b.download(np.uint64, v.width * v.height)) for g, b, v in zip(got, fbs, views))
+    ctx.enable_timing(True)
+    st = batched()
+    t = ctx.timings()
+    ctx.enable_timing(False)
+    stage = {"clear": t.clear_ms, "stage1": t.stage1_ms, "stage2": t.stage2_ms, "stage3": t.stage3_ms,
+             "resolve": t.resolve_ms}
+    reps_b = max(3, steps // 10)
+    tb = timed(batched, reps_b)
+    reps_p = max(2, steps // 40)
+    tp = timed(plain, reps_p)
+    out = {
+        "views": count, "pixels": pixels, "closeups": int(sum(c for *_, c in vs)),
+        "batched_views_per_s": count * reps_b / tb, "batched_ms_per_frame": 1e3 * tb / reps_b,
+        "plain_views_per_s": count * reps_p / tp, "plain_ms_per_frame": 1e3 * tp / reps_p,
+        "speedup": (count * reps_b / tb) / (count * reps_p / tp),
+        "bit_identical_to_plain": bool(same),
+        "batched_stage_ms": {k: round(x, 4) for k, x in stage.items()},
+        "batched_nontrivial": st.num_nontrivial, "batched_huge": st.num_huge,
+    }
+    for b in fbs + cbs:
+        b.free()
+    return out
+
+
+def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: int, batch: int, plain: bool = True):
+    """Out-of-core serving workload: an n x n heightfield cut into tile x tile
+    nodes in pinned host memory, half of them device-resident at start, the
+    device node cache holding half of the scene; a seeded Poisson request
+    stream (`rate` per second for `duration` seconds, 8 moving cameras, mixed
+    resolutions) is served by crx_server with continuous batching, then by the
+    same server limited to one view per batch and one lane (the plain path).
+    Reports throughput and p50 / p99 latency (submit to completion) of both."""
+    import threading
+    import crx
+    import workloads as wl
+    t0 = time.perf_counter()
+    nodes, bufs = wl.tiled_heightfield(ctx, n, tile, threads=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    node_bytes = [(nd["nv"] * 12 + 255) // 256 * 256 + (nd["nt"] * 12 + 255) // 256 * 256 +
+                  (nd["nv"] * 4 + 255) // 256 * 256 for nd in nodes]
+    total_tris = sum(nd["nt"] for nd in nodes)
+    log(f"[serve] {len(nodes)} nodes, {total_tris} triangles, {sum(node_bytes) / 1e9:.1f} GB pinned "
+        f"in {time.perf_counter() - t0:.1f} s")
+    ground = wl.ground_lookup(nodes, tile)
+    reqs = wl.serving_requests(n, ground, duration=duration, rate=rate)
+    views = [crx.make_view(v, p, W, H) for (_, v, p, W, H, _) in reqs]
+    arrivals = [t for (t, *_) in reqs]
+
+    def run(max_batch, nlanes, paced=True):
+        cfg = crx.server_config(node_cache_bytes=max(node_bytes) * (len(nodes) // 2), frame_bytes=8 << 30,
+                                max_batch_views=max_batch, lanes=nlanes, resolve=1)
+        srv = crx.Server(ctx, cfg)
+        for k, nd in enumerate(nodes):
+            srv.add_node(nd["pos"], nd["nv"], nd["idx"], nd["nt"], nd["col"], nd["aabb"], resident=(k % 2 == 0))
+        t1 = time.perf_counter()
+        srv.start()
+        log(f"[serve] server started (batch {max_batch}, lanes {nlanes}) in {time.perf_counter() - t1:.1f} s")
+        tickets = []
+        lat, batches, results = [], [], {}
+        done = threading.Event()
+
+        def reaper():
+            k = 0
+            while k < len(views):
+                while k >= len(tickets):
+                    time.sleep(0.0005)
+                r = srv.wait(tickets[k])
+                results[k] = r.status
+                lat.append(r.done_time - r.submit_time)
+                batches.append(r.batch_views)
+                srv.release(tickets[k])
+                k += 1
+            done.set()
+
+        th = threading.Thread(target=reaper)
+        th.start()
+        base = time.perf_counter()
+        for k, v in enumerate(views):
+            delay = arrivals[k] - (time.perf_counter() - base) if paced else 0.0
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(v))
+        th.join()
+        wall = time.perf_counter() - base
+        st = srv.stats()
+        srv.close()
+        lat_ms = np.array(lat) * 1e3
+        ok = sum(1 for s in results.values() if s == 0)
+        return {"requests": len(views), "ok": ok, "wall_s": round(wall, 3), "views_per_s": len(views) / wall,
+                "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                "mean_ms": round(float(lat_ms.mean()), 3), "mean_batch": round(float(np.mean(batches)), 2),
+                "max_batch": st.max_batch, "uploads": st.uploads, "upload_gb": round(st.upload_bytes / 1e9, 3),
+                "evictions": st.evictions, "cache_slots": st.cache_slots}
+
+    out = {"workload": f"tiled-heightfield-{n}x{n}-{len(nodes)}nodes", "triangles": total_tris,
+           "pinned_gb": round(sum(node_bytes) / 1e9, 2), "rate_per_s": rate, "duration_s": duration,
+           "batched": run(batch, lanes)}
+    log(f"[serve] batched {out['batched']}")
+    # capacity: every request queued at once
+    out["batched_saturated"] = run(batch, lanes, paced=False)
+    log(f"[serve] batched saturated {out['batched_saturated']}")
+    if plain:
+        out["plain"] = run(1, 1)
+        log(f"[serve] plain {out['plain']}")
+        out["plain_saturated"] = run(1, 1, paced=False)
+        log(f"[serve] plain saturated {out['plain_saturated']}")
+        out["capacity_ratio"] = out["batched_saturated"]["views_per_s"] / out["plain_saturated"]["views_per_s"]
+    for b in bufs:
+        b.free()
+    return out
+
+
 def cpu_baseline(pos, idx, col, worlds, view, proj, threads: int):
     """The CPU oracle rendering the same frame (stages + resolve), repeated for >= 10 s."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -107,6 +256,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="heightfield", choices=["heightfield", "sphere", "instances"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-multiview", action="store_true",
+                    help="skip the 64-view batched vs plain measurement on the heightfield")
+    ap.add_argument("--multiview-only", action="store_true",
+                    help="only the 64-view batched vs plain measurement (heightfield workload)")
+    ap.add_argument("--serve", action="store_true",
+                    help="only the out-of-core serving workload (Poisson requests, p50/p99 latency)")
+    ap.add_argument("--serve-n", type=int, default=31623, help="serving terrain grid size (vertices per side)")
+    ap.add_argument("--serve-tile", type=int, default=707, help="cells per node side (707 -> ~1M triangles)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run the 1 GiB copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--tuning", default="",
@@ -119,6 +278,20 @@ def main() -> int:
         return 2
 
     import crx
+    if args.serve:
+        ctx = crx.Context(0)
+        sv = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate, lanes=2, batch=64)
+        b = sv["batched"]
+        print(json.dumps({
+            "metric": "served views/sec with p50/p99 latency (out-of-core, Poisson arrivals)",
+            "value": b["views_per_s"], "unit": "views/s", "n_gpus": 1, "steps": b["requests"], "warmup": 0,
+            "ms_per_step": 1e3 * b["wall_s"] / max(1, b["requests"]), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": sv["workload"], "triangles": sv["triangles"], "pinned_gb": sv["pinned_gb"],
+                       "rate_per_s": sv["rate_per_s"], "duration_s": sv["duration_s"]},
+            "serving": sv}), flush=True)
+        ctx.close()
+        return 0
     t0 = time.perf_counter()
     pos, idx, col, worlds, view, proj, desc = build_workload(args.workload)
     log(f"[bench] generated {desc['workload']} in {time.perf_counter() - t0:.1f} s")
@@ -142,6 +315,20 @@ def main() -> int:
     T = len(idx) * len(worlds)
     V = len(pos)
     log(f"[bench] scene resident: {T} triangle-instances, {V} vertices; warmup {args.warmup}")
+
+    if args.multiview_only:
+        if args.workload != "heightfield":
+            raise SystemExit("--multiview-only needs the heightfield workload")
+        mv = bench_multiview(ctx, scene, pos, desc["grid"], args.steps, args.warmup)
+        log(f"[bench] multi-view {mv}")
+        print(json.dumps({
+            "metric": "batched views/sec (64 mixed-resolution views, full frames)",
+            "value": mv["batched_views_per_s"], "unit": "views/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": mv["batched_ms_per_frame"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": dict(desc, workload=f"{desc['workload']}-64views"), "batched_multiview": mv}), flush=True)
+        ctx.close()
+        return 0
 
     for _ in range(args.warmup):
         scene.render(v, fb, cb, settings)
@@ -192,6 +379,12 @@ def main() -> int:
                 "kernel": "k_stage1", "algorithmic_bytes_per_launch": algo,
                 "stage1_ms_mean": round(mean_s1, 4)}
 
+    multiview = None
+    if args.workload == "heightfield" and not args.no_multiview:
+        log("[bench] batched multi-view: 64 seeded views, batched vs plain ...")
+        multiview = bench_multiview(ctx, scene, pos, desc["grid"], args.steps, args.warmup)
+        log(f"[bench] multi-view {multiview}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
@@ -218,6 +411,7 @@ def main() -> int:
         "pipeline_triangles_per_s": T / (med["pipeline"] * 1e-3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_multiview": multiview,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -61,14 +61,14 @@ def build(verbose: bool = False) -> None:
     obj_dir = LIB / "obj"
     obj_dir.mkdir(exist_ok=True)
     objs = []
-    for src in ("crx_kernels.hip", "crx_api.cpp"):
+    for src in ("crx_kernels.hip", "crx_api.cpp", "crx_server.cpp"):
         obj = obj_dir / (Path(src).stem + ".o")
-        cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-pthread", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(str(obj))
-    _atomic_link([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs, LIB / "libcrx.so")
+    _atomic_link([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread"] + objs, LIB / "libcrx.so")
     gen_cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
                f"-I{INCLUDE}", str(CSRC / "crx_gen.cpp")]
     _atomic_link(gen_cmd, LIB / "libcrx_gen.so")

@@ -140,6 +140,31 @@ class DrawStats(C.Structure):
                 ("num_huge", C.c_uint32), ("redraws", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("tuning", Tuning), ("start_ms", C.c_float), ("best_ms", C.c_float),
+                ("trials", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+TUNE_SCHEDULE, TUNE_ROUTING, TUNE_QUEUES = 1, 2, 4
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("node_cache_bytes", C.c_uint64), ("frame_bytes", C.c_uint64), ("max_batch_views", C.c_uint32),
+                ("lanes", C.c_uint32), ("resolve", C.c_int32), ("reserved", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("framebuffer", C.c_void_p), ("colorbuffer", C.c_void_p), ("submit_time", C.c_double),
+                ("start_time", C.c_double), ("done_time", C.c_double), ("batch_views", C.c_uint32),
+                ("uploads", C.c_uint32), ("status", C.c_int), ("reserved", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("uploads", C.c_uint64),
+                ("upload_bytes", C.c_uint64), ("evictions", C.c_uint64), ("resident_nodes", C.c_uint32),
+                ("cache_slots", C.c_uint32), ("max_batch", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class Timings(C.Structure):
     _fields_ = [("clear_ms", C.c_float), ("stage1_ms", C.c_float), ("stage2_ms", C.c_float),
                 ("stage3_ms", C.c_float), ("resolve_ms", C.c_float), ("pipeline_ms", C.c_float)]
@@ -213,6 +238,23 @@ _SIGS = {
     "crx_clear_views": (C.c_int, [_VP, C.POINTER(RenderTarget), C.c_uint32, C.c_int, _VP]),
     "crx_render_views": (C.c_int, [_VP, _VP, C.POINTER(View), C.c_uint32, C.POINTER(RenderSettings),
                                    C.POINTER(_VP), C.POINTER(_VP), _VP, C.POINTER(DrawStats)]),
+    "crx_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "crx_host_alloc": (C.c_int, [_VP, C.c_size_t, C.POINTER(_VP)]),
+    "crx_host_free": (C.c_int, [_VP, _VP]),
+    "crx_server_create": (C.c_int, [_VP, C.POINTER(ServerConfig), C.POINTER(_VP)]),
+    "crx_server_destroy": (C.c_int, [_VP]),
+    "crx_server_add_node": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, C.POINTER(Box3),
+                                      C.POINTER(Mat4), C.c_int, C.POINTER(C.c_uint32)]),
+    "crx_server_start": (C.c_int, [_VP]),
+    "crx_server_submit": (C.c_int, [_VP, C.POINTER(View), C.POINTER(C.c_uint64)]),
+    "crx_server_wait": (C.c_int, [_VP, C.c_uint64, C.POINTER(RequestResult)]),
+    "crx_server_release": (C.c_int, [_VP, C.c_uint64]),
+    "crx_server_clock": (C.c_double, [_VP]),
+    "crx_server_get_stats": (C.c_int, [_VP, C.POINTER(ServerStats)]),
+    "crx_server_view_nodes": (C.c_int, [_VP, C.POINTER(View), C.POINTER(C.c_uint32), C.c_uint32,
+                                        C.POINTER(C.c_uint32)]),
+    "crx_autotune": (C.c_int, [_VP, _VP, C.POINTER(View), C.c_uint32, C.POINTER(_VP), C.c_uint32, C.c_uint32,
+                               C.POINTER(AutotuneResult)]),
     "crx_camera_projection": (None, [C.c_float, C.c_float, C.c_float, C.POINTER(Mat4)]),
     "crx_camera_look_at": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   C.POINTER(Mat4)]),
@@ -240,6 +282,8 @@ _GEN_SIGS = {
     "crx_gen_sphere": (None, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint64, _VP, _VP, _VP]),
     "crx_gen_heightfield": (None, [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint64,
                                    _VP, _VP, _VP, C.c_int]),
+    "crx_gen_heightfield_tile": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                        C.c_float, C.c_float, C.c_uint32, C.c_uint64, _VP, _VP, _VP, C.c_int]),
     "crx_gen_heightfield_indices": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                                  C.c_uint32, _VP]),
     "crx_gen_instances": (None, [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint64, C.POINTER(Mat4)]),
@@ -461,6 +505,16 @@ class Context:
                                                  g_arr, len(g_arr) if g_arr is not None else 0,
                                                  int(attribute), stream))
 
+    def autotune(self, scene: "Scene", views, fbs, flags: int = TUNE_SCHEDULE, reps: int = 5) -> AutotuneResult:
+        """Coordinate-descent search of the tuning on this scene and these
+        views (crx_autotune); the context keeps the best tuning."""
+        v_arr = _as_array(View, views)
+        n = len(v_arr)
+        fb_arr = (C.c_void_p * n)(*[b.ptr for b in fbs])
+        res = AutotuneResult()
+        _check(lib().crx_autotune(self.handle, scene.handle, v_arr, n, fb_arr, flags, reps, C.byref(res)))
+        return res
+
     def clear_views(self, targets, colorbuffers: bool = False, stream=None) -> None:
         g_arr = _as_array(RenderTarget, targets)
         _check(lib().crx_clear_views(self.handle, g_arr, len(g_arr) if g_arr is not None else 0,
@@ -624,6 +678,101 @@ class Scene:
     def close(self) -> None:
         if self.handle:
             lib().crx_scene_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+class PinnedBuffer:
+    """Page-locked host memory (crx_host_alloc) the server streams node geometry from."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        p = C.c_void_p()
+        _check(lib().crx_host_alloc(ctx.handle, nbytes, C.byref(p)))
+        self.ctx = ctx
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def array(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        """numpy view of `count` elements at byte `offset` (no copy)."""
+        dt = np.dtype(dtype)
+        assert offset + count * dt.itemsize <= self.nbytes
+        buf = (C.c_char * (count * dt.itemsize)).from_address(self.ptr + offset)
+        return np.frombuffer(buf, dtype=dt, count=count)
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().crx_host_free(self.ctx.handle, C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def server_config(**kw) -> ServerConfig:
+    c = ServerConfig()
+    lib().crx_server_config_default(C.byref(c))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching render server over an out-of-core node store (crx_server_*)."""
+
+    def __init__(self, ctx: "Context", config: ServerConfig | None = None):
+        h = C.c_void_p()
+        cfg = config if config is not None else server_config()
+        _check(lib().crx_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self.ctx = ctx
+        self.handle = h.value
+        self.config = cfg
+
+    def add_node(self, positions_ptr: int, num_vertices: int, indices_ptr, num_triangles: int, colors_ptr,
+                 aabb: Box3, world: Mat4 | None = None, resident: bool = False) -> int:
+        nid = C.c_uint32()
+        w = identity() if world is None else world
+        _check(lib().crx_server_add_node(self.handle, C.c_void_p(positions_ptr), num_vertices,
+                                         C.c_void_p(indices_ptr) if indices_ptr else None, num_triangles,
+                                         C.c_void_p(colors_ptr) if colors_ptr else None, C.byref(aabb),
+                                         C.byref(w), int(bool(resident)), C.byref(nid)))
+        return nid.value
+
+    def start(self) -> None:
+        _check(lib().crx_server_start(self.handle))
+
+    def submit(self, view: View) -> int:
+        t = C.c_uint64()
+        _check(lib().crx_server_submit(self.handle, C.byref(view), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> RequestResult:
+        r = RequestResult()
+        _check(lib().crx_server_wait(self.handle, ticket, C.byref(r)))
+        return r
+
+    def release(self, ticket: int) -> None:
+        _check(lib().crx_server_release(self.handle, ticket))
+
+    def clock(self) -> float:
+        return float(lib().crx_server_clock(self.handle))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().crx_server_get_stats(self.handle, C.byref(st)))
+        return st
+
+    def view_nodes(self, view: View) -> list[int]:
+        n = C.c_uint32()
+        _check(lib().crx_server_view_nodes(self.handle, C.byref(view), None, 0, C.byref(n)))
+        ids = (C.c_uint32 * max(1, n.value))()
+        _check(lib().crx_server_view_nodes(self.handle, C.byref(view), ids, n.value, C.byref(n)))
+        return list(ids[:n.value])
+
+    def download(self, ptr: int, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=np.uint64)
+        _check(lib().crx_memcpy_dtoh(self.ctx.handle, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
+                                     8 * count, None))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crx_server_destroy(C.c_void_p(self.handle))
             self.handle = None
 
 

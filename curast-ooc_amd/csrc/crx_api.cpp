@@ -19,158 +19,25 @@
 #include <vector>
 
 #include "crx_device.h"
+#include "crx_host.h"
 #include "crx_internal.h"
 
 using namespace crx;
 
-namespace {
+namespace crx {
 
+namespace {
 thread_local std::string g_error;
+}  // namespace
 
 crx_status fail(crx_status s, const std::string& msg) {
     g_error = msg;
     return s;
 }
 
-#define CRX_HIP(expr)                                                                         \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return fail(CRX_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+}  // namespace crx
 
-#define CRX_TRY(expr)                      \
-    do {                                   \
-        crx_status s_ = (expr);            \
-        if (s_ != CRX_OK) return s_;       \
-    } while (0)
-
-inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
-
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t capacity = 0;
-    crx_status reserve(size_t bytes) {
-        if (bytes <= capacity) return CRX_OK;
-        if (ptr) {
-            hipError_t e = hipFree(ptr);
-            ptr = nullptr;
-            capacity = 0;
-            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipFree: ") + hipGetErrorString(e));
-        }
-        size_t want = std::max<size_t>(bytes, 4096);
-        hipError_t e = hipMalloc(&ptr, want);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            return fail(CRX_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
-        }
-        capacity = want;
-        return CRX_OK;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        capacity = 0;
-    }
-};
-
-// Pinned host staging with an event guarding reuse: the buffer is rewritten only
-// after the copy that last read it has completed.
-struct Staging {
-    void* ptr = nullptr;
-    size_t capacity = 0;
-    hipEvent_t done = nullptr;
-    bool pending = false;
-    crx_status acquire(size_t bytes) {
-        if (pending) {
-            hipError_t e = hipEventSynchronize(done);
-            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
-            pending = false;
-        }
-        if (!done) {
-            hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
-            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventCreate: ") + hipGetErrorString(e));
-        }
-        if (bytes <= capacity) return CRX_OK;
-        if (ptr) (void)hipHostFree(ptr);
-        ptr = nullptr;
-        capacity = 0;
-        size_t want = std::max<size_t>(bytes * 3 / 2, 65536);
-        hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            return fail(CRX_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-        }
-        capacity = want;
-        return CRX_OK;
-    }
-    crx_status mark(hipStream_t stream) {
-        hipError_t e = hipEventRecord(done, stream);
-        if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventRecord: ") + hipGetErrorString(e));
-        pending = true;
-        return CRX_OK;
-    }
-    void release() {
-        if (pending && done) (void)hipEventSynchronize(done);
-        if (ptr) (void)hipHostFree(ptr);
-        if (done) (void)hipEventDestroy(done);
-        ptr = nullptr;
-        done = nullptr;
-        capacity = 0;
-        pending = false;
-    }
-};
-
-inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-enum TimerSlot { T_CLEAR = 0, T_S1, T_S2, T_S3, T_S3_END, T_RESOLVE, T_RESOLVE_END, T_COUNT };
-
-}  // namespace
-
-struct crx_context {
-    int device = 0;
-    int numCUs = 0;
-    crx_tuning tuning{};
-    uint32_t* ctr = nullptr;          // device counters (CTR_COUNT)
-    uint32_t* ctrHost = nullptr;      // pinned readback
-    DeviceBuffer nontrivial;          // uint64 entries
-    DeviceBuffer huge;                // int4 entries
-    uint32_t ntCap = 0, hugeCap = 0;
-    DeviceBuffer drawMeta;            // items | instances | transforms
-    DeviceBuffer resolveMeta;         // instances | prefix | per-view records
-    DeviceBuffer clearMeta;           // per-buffer clear records
-    Staging drawStaging, resolveStaging, clearStaging;
-    bool timing = false;
-    hipEvent_t ev[T_COUNT] = {};
-    bool recorded[T_COUNT] = {};
-    uint64_t meshSeq = 0;
-};
-
-struct crx_mesh {
-    crx_context* ctx = nullptr;
-    float* positions = nullptr;
-    uint32_t* indices = nullptr;
-    uint32_t* colors = nullptr;
-    bool owned = false;
-    uint32_t numVertices = 0;
-    uint32_t numTriangles = 0;
-    crx_box3 aabb{};
-    uint64_t seq = 0;
-};
-
-struct crx_scene {
-    crx_context* ctx = nullptr;
-    struct Node {
-        crx_mesh* mesh;
-        crx_mat4 world;
-    };
-    std::vector<Node> nodes;
-    // frame-preparation scratch, kept across frames like the reference's statics
-    std::vector<crx_cmesh> unique, instances;
-    std::vector<crx_mat4> transforms;
-    std::vector<uint64_t> prefix;
-};
-
-namespace {
+namespace crx {
 
 // ---- host math in GLM 0.9.9 order --------------------------------------------
 
@@ -349,7 +216,8 @@ ViewParams make_view_params(const crx_render_target& t, const Transform* transfo
 // run the batched stage 1, which fetches each triangle once for all views.
 crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                       uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
-                      uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats) {
+                      uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
+                      const uint8_t* skip) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
     if (numViews == 0 || numViews > kMaxViews)
         return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
@@ -362,6 +230,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
     if (instanced && (!meshes || numMeshes == 0)) return fail(CRX_INVALID_ARG, "instanced draw without meshes");
     if (numViews > 1 && numInstances > kInstanceMask + 1u)
         return fail(CRX_INVALID_ARG, "a multi-view draw takes at most 2^24 instances");
+    if (skip && numViews < 2) return fail(CRX_INVALID_ARG, "per-view instance culling needs a multi-view draw");
     CRX_HIP(hipSetDevice(ctx->device));
     const crx_tuning& tu = ctx->tuning;
     const uint32_t block = tu.stage1_block;
@@ -406,7 +275,8 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
     for (size_t i = 0; i < numXf; i++) {
         std::memcpy(xf[i].m, transforms[i].m, sizeof(xf[i].m));
         xf[i].flip = mirror_flag(transforms[i]) ? 1u : 0u;
-        xf[i].pad[0] = xf[i].pad[1] = xf[i].pad[2] = 0u;
+        xf[i].skip = (skip && skip[i]) ? 1u : 0u;
+        xf[i].pad[0] = xf[i].pad[1] = 0u;
     }
     if (stats) stats->num_triangles = tris * numViews;
     if (batches == 0) return CRX_OK;
@@ -611,6 +481,52 @@ crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint3
     return CRX_OK;
 }
 
+// SNTriangles::update (SNTriangles.h:29-105): the four side planes of
+// proj * view, normalised, as (nx, ny, nz, d) x 4.
+void frustum_planes(const crx_view& view, float out[16]) {
+    const crx_mat4 viewProj = mat_mul(view.proj, view.view);
+    for (int p = 0; p < 4; p++) {
+        const int r = p < 2 ? 0 : 1;
+        const float sg = (p & 1) ? -1.0f : 1.0f;
+        float q[4];
+        for (int c = 0; c < 4; c++) {
+            const float r3 = viewProj.m[4 * c + 3], rr = viewProj.m[4 * c + r];
+            q[c] = sg > 0.0f ? r3 + rr : r3 - rr;
+        }
+        const f3 n{q[0], q[1], q[2]};
+        const float len = sqrtf(dot3(n, n));
+        out[4 * p + 0] = n.x / len;
+        out[4 * p + 1] = n.y / len;
+        out[4 * p + 2] = n.z / len;
+        out[4 * p + 3] = q[3] / len;
+    }
+}
+
+// Box3::transform: world-space AABB of the 8 transformed corners.
+crx_box3 world_box(const crx_box3& b, const crx_mat4& world) {
+    const float cx[2] = {b.min.x, b.max.x}, cy[2] = {b.min.y, b.max.y}, cz[2] = {b.min.z, b.max.z};
+    f3 wmin{INFINITY, INFINITY, INFINITY}, wmax{-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 8; k++) {
+        const f3 w = xform(world.m, f3{cx[k & 1], cy[(k >> 1) & 1], cz[(k >> 2) & 1]}, 1.0f);
+        wmin.x = std::fmin(wmin.x, w.x); wmin.y = std::fmin(wmin.y, w.y); wmin.z = std::fmin(wmin.z, w.z);
+        wmax.x = std::fmax(wmax.x, w.x); wmax.y = std::fmax(wmax.y, w.y); wmax.z = std::fmax(wmax.z, w.z);
+    }
+    return crx_box3{{wmin.x, wmin.y, wmin.z}, {wmax.x, wmax.y, wmax.z}};
+}
+
+// Node frustum test of SNTriangles::update: visible unless the box lies
+// entirely on the negative side of one plane (positive-vertex test).
+bool box_in_frustum(const float planes[16], const crx_box3& wb) {
+    for (int p = 0; p < 4; p++) {
+        const float* pl = planes + 4 * p;
+        const f3 n{pl[0], pl[1], pl[2]};
+        const f3 c{n.x >= 0.0f ? wb.max.x : wb.min.x, n.y >= 0.0f ? wb.max.y : wb.min.y,
+                   n.z >= 0.0f ? wb.max.z : wb.min.z};
+        if (dot3(n, c) + pl[3] < 0.0f) return false;
+    }
+    return true;
+}
+
 // CuRast::render / CuRast::draw (CuRast_render.h:266-519, :873-904) for
 // numViews >= 1 views: node culling (SNTriangles::update, SNTriangles.h:29-105)
 // keeps a node visible in any view, the visible nodes are sorted and grouped
@@ -631,51 +547,17 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
         if (settings.resolve && !colorbuffers[v]) return fail(CRX_INVALID_ARG, "resolve needs a colorbuffer");
     }
 
-    // SNTriangles::update (SNTriangles.h:29-105): frustum side planes of proj*view
     std::vector<float> planes((size_t)numViews * 16);
-    for (uint32_t v = 0; v < numViews; v++) {
-        const crx_mat4 viewProj = mat_mul(views[v].proj, views[v].view);
-        for (int p = 0; p < 4; p++) {
-            const int r = p < 2 ? 0 : 1;
-            const float sg = (p & 1) ? -1.0f : 1.0f;
-            float q[4];
-            for (int c = 0; c < 4; c++) {
-                const float r3 = viewProj.m[4 * c + 3], rr = viewProj.m[4 * c + r];
-                q[c] = sg > 0.0f ? r3 + rr : r3 - rr;
-            }
-            const f3 n{q[0], q[1], q[2]};
-            const float len = sqrtf(dot3(n, n));
-            float* out = &planes[(size_t)v * 16 + 4 * p];
-            out[0] = n.x / len; out[1] = n.y / len; out[2] = n.z / len;
-            out[3] = q[3] / len;
-        }
-    }
+    for (uint32_t v = 0; v < numViews; v++) frustum_planes(views[v], &planes[(size_t)v * 16]);
     std::vector<uint32_t> visible;
     visible.reserve(scene->nodes.size());
     for (uint32_t i = 0; i < scene->nodes.size(); i++) {
         const crx_scene::Node& nd = scene->nodes[i];
         bool keep = true;
         if (settings.frustum_culling) {
-            const crx_box3& b = nd.mesh->aabb;
-            const float cx[2] = {b.min.x, b.max.x}, cy[2] = {b.min.y, b.max.y}, cz[2] = {b.min.z, b.max.z};
-            f3 wmin{INFINITY, INFINITY, INFINITY}, wmax{-INFINITY, -INFINITY, -INFINITY};
-            for (int k = 0; k < 8; k++) {   // Box3::transform corner order
-                const f3 w = xform(nd.world.m, f3{cx[k & 1], cy[(k >> 1) & 1], cz[(k >> 2) & 1]}, 1.0f);
-                wmin.x = std::fmin(wmin.x, w.x); wmin.y = std::fmin(wmin.y, w.y); wmin.z = std::fmin(wmin.z, w.z);
-                wmax.x = std::fmax(wmax.x, w.x); wmax.y = std::fmax(wmax.y, w.y); wmax.z = std::fmax(wmax.z, w.z);
-            }
+            const crx_box3 wb = world_box(nd.mesh->aabb, nd.world);
             keep = false;
-            for (uint32_t v = 0; v < numViews && !keep; v++) {
-                bool inside = true;
-                for (int p = 0; p < 4 && inside; p++) {
-                    const float* pl = &planes[(size_t)v * 16 + 4 * p];
-                    const f3 n{pl[0], pl[1], pl[2]};
-                    const f3 c{n.x >= 0.0f ? wmax.x : wmin.x, n.y >= 0.0f ? wmax.y : wmin.y,
-                               n.z >= 0.0f ? wmax.z : wmin.z};
-                    if (dot3(n, c) + pl[3] < 0.0f) inside = false;
-                }
-                keep = inside;
-            }
+            for (uint32_t v = 0; v < numViews && !keep; v++) keep = box_in_frustum(&planes[(size_t)v * 16], wb);
         }
         if (keep) visible.push_back(i);
     }
@@ -761,7 +643,7 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
     return CRX_OK;
 }
 
-}  // namespace
+}  // namespace crx
 
 extern "C" {
 
@@ -1191,6 +1073,111 @@ crx_status crx_render_views(crx_context* ctx, crx_scene* scene, const crx_view* 
                             const crx_render_settings* settings, uint64_t* const* framebuffers,
                             uint64_t* const* colorbuffers, void* stream, crx_draw_stats* stats) {
     return render_views(ctx, scene, views, numViews, settings, framebuffers, colorbuffers, as_stream(stream), stats);
+}
+
+crx_status crx_autotune(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
+                        uint64_t* const* framebuffers, uint32_t flags, uint32_t reps, crx_autotune_result* out) {
+    if (!ctx || !scene || !views || !framebuffers || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (reps == 0 || reps > 1000) return fail(CRX_INVALID_ARG, "reps must be in [1, 1000]");
+    if ((flags & ~(CRX_TUNE_SCHEDULE | CRX_TUNE_ROUTING | CRX_TUNE_QUEUES)) != 0)
+        return fail(CRX_INVALID_ARG, "unknown autotune flags");
+    std::memset(out, 0, sizeof(*out));
+    crx_render_settings settings;
+    crx_render_settings_default(&settings);
+    settings.resolve = 0;
+    const crx_tuning start = ctx->tuning;
+    const bool wasTiming = ctx->timing;
+    ctx->timing = true;
+    uint32_t trials = 0;
+    crx_draw_stats lastStats{};
+
+    // median device time of stages 1-3 over `reps` draws with tuning t
+    auto score = [&](const crx_tuning& t, float* ms) -> crx_status {
+        CRX_TRY(check_tuning(t));
+        ctx->tuning = t;
+        CRX_TRY(render_views(ctx, scene, views, numViews, &settings, framebuffers, nullptr, nullptr, &lastStats));
+        std::vector<float> v;
+        for (uint32_t r = 0; r < reps; r++) {
+            CRX_TRY(render_views(ctx, scene, views, numViews, &settings, framebuffers, nullptr, nullptr, &lastStats));
+            crx_timings tm;
+            CRX_TRY(crx_context_timings(ctx, &tm));
+            v.push_back(tm.pipeline_ms);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        trials++;
+        return CRX_OK;
+    };
+
+    // knob table: field and candidate values
+    struct Knob {
+        uint32_t crx_tuning::*field;
+        std::vector<uint32_t> values;
+        uint32_t cls;
+    };
+    const std::vector<Knob> knobs = {
+        {&crx_tuning::stage1_block, {128, 256, 512}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_waves_per_cu, {8, 16, 24, 32, 48, 64, 0}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_cull_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_fast_tiny, {0, 1}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage2_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage3_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::small_threshold, {32, 64, 128, 256, 512}, CRX_TUNE_ROUTING},
+        {&crx_tuning::tile_size, {32, 64, 128}, CRX_TUNE_ROUTING},
+    };
+
+    crx_status st = CRX_OK;
+    crx_tuning best = start;
+    float bestMs = 0.0f;
+    do {
+        st = score(best, &bestMs);
+        if (st != CRX_OK) break;
+        out->start_ms = bestMs;
+        for (int pass = 0; pass < 2 && st == CRX_OK; pass++) {
+            bool improved = false;
+            for (const Knob& k : knobs) {
+                if (!(flags & k.cls)) continue;
+                for (uint32_t val : k.values) {
+                    if (best.*(k.field) == val) continue;
+                    crx_tuning cand = best;
+                    cand.*(k.field) = val;
+                    float ms = 0.0f;
+                    st = score(cand, &ms);
+                    if (st != CRX_OK) break;
+                    if (ms < bestMs * 0.98f) {   // keep a change only for a clear gain
+                        best = cand;
+                        bestMs = ms;
+                        improved = true;
+                    }
+                }
+                if (st != CRX_OK) break;
+            }
+            if (!improved) break;
+        }
+        if (st != CRX_OK) break;
+        if (flags & CRX_TUNE_QUEUES) {
+            // one more draw with the best tuning gives the queue counts; keep 2x headroom
+            float ms = 0.0f;
+            st = score(best, &ms);
+            if (st != CRX_OK) break;
+            best.max_nontrivial = std::max<uint32_t>(4096u, (uint32_t)std::min<uint64_t>(
+                0xffffffffull, 2ull * lastStats.num_nontrivial));
+            best.max_huge = std::max<uint32_t>(4096u, (uint32_t)std::min<uint64_t>(
+                0xffffffffull, 2ull * lastStats.num_huge));
+        }
+    } while (false);
+    ctx->timing = wasTiming;
+    for (bool& r : ctx->recorded) r = false;
+    if (st != CRX_OK) {
+        ctx->tuning = start;
+        return st;
+    }
+    ctx->tuning = best;
+    out->tuning = best;
+    out->best_ms = bestMs;
+    out->trials = trials;
+    return CRX_OK;
 }
 
 }  // extern "C"

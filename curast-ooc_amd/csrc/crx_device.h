@@ -45,10 +45,13 @@ struct InstanceRec {
 
 // world-to-view transform of one instance plus its mirror flag
 // (det(worldView[0..2]) < 0, computed on the host in the same float order).
+// skip != 0 drops the instance from this view (multi-view draws only: the
+// host-side node cull of that view).
 struct Transform {
     float m[16];
     uint32_t flip;
-    uint32_t pad[3];
+    uint32_t skip;
+    uint32_t pad[2];
 };
 
 // Counter block, one per context, cleared by the clear kernel of each draw.

@@ -86,22 +86,24 @@ void crx_gen_sphere(uint32_t rings, uint32_t segments, float radius, float noise
     }
 }
 
-void crx_gen_heightfield(uint32_t n, float spacing, float amplitude, float scale, uint32_t octaves,
-                         uint64_t seed, float* positions, uint32_t* indices, uint32_t* colors,
-                         int nthreads) {
+void crx_gen_heightfield_tile(uint32_t n, uint32_t r0, uint32_t c0, uint32_t rows, uint32_t cols, float spacing,
+                              float amplitude, float scale, uint32_t octaves, uint64_t seed, float* positions,
+                              uint32_t* indices, uint32_t* colors, int nthreads) {
 #ifdef _OPENMP
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #else
     (void)nthreads;
     const int nt = 1;
 #endif
+    if (rows == 0 || cols == 0 || r0 >= n || c0 >= n || rows > n - r0 || cols > n - c0) return;
     if (octaves > 16) octaves = 16;
     uint64_t octSeed[16];
     for (uint32_t o = 0; o < octaves; o++) octSeed[o] = splitmix_at(seed, o);
     const uint64_t colorSeed = seed ^ 0xC0C0C0C0C0C0C0C0ull;
     if (positions || colors) {
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
-        for (int64_t r = 0; r < (int64_t)n; r++) {
+        for (int64_t lr = 0; lr < (int64_t)rows; lr++) {
+            const uint64_t r = r0 + (uint64_t)lr;
             // per octave: the two lattice rows bracketing this grid row
             double fy[16];
             int64_t iy[16];
@@ -115,7 +117,8 @@ void crx_gen_heightfield(uint32_t n, float spacing, float amplitude, float scale
             int64_t curIx[16];
             double l00[16], l10[16], l01[16], l11[16];
             for (uint32_t o = 0; o < octaves; o++) curIx[o] = INT64_MIN;
-            for (uint32_t c = 0; c < n; c++) {
+            for (uint32_t lc = 0; lc < cols; lc++) {
+                const uint64_t c = c0 + lc;
                 double h = 0.0, amp = 1.0;
                 for (uint32_t o = 0; o < octaves; o++) {
                     const double x = (double)c * freq[o];
@@ -135,29 +138,38 @@ void crx_gen_heightfield(uint32_t n, float spacing, float amplitude, float scale
                     amp *= 0.5;
                 }
                 const double height = (double)amplitude * h;
-                const uint64_t v = (uint64_t)r * n + c;
+                const uint64_t gv = r * n + c;                 // global vertex id (colour stream)
+                const uint64_t lv = (uint64_t)lr * cols + lc;  // local vertex id
                 if (positions) {
-                    positions[3 * v + 0] = (float)((double)c * (double)spacing);
-                    positions[3 * v + 1] = (float)height;
-                    positions[3 * v + 2] = (float)((double)r * (double)spacing);
+                    positions[3 * lv + 0] = (float)((double)c * (double)spacing);
+                    positions[3 * lv + 1] = (float)height;
+                    positions[3 * lv + 2] = (float)((double)r * (double)spacing);
                 }
                 if (colors) {
                     double t = 0.5 + h;   // h in about [-1, 1]
                     t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
                     const uint32_t rr = (uint32_t)(40.0 + 180.0 * t);
                     const uint32_t gg = (uint32_t)(200.0 - 120.0 * t);
-                    const uint32_t bb = (uint32_t)(splitmix_at(colorSeed, v) & 0x3f) + 40u;
-                    colors[v] = 0xff000000u | (bb << 16) | (gg << 8) | rr;
+                    const uint32_t bb = (uint32_t)(splitmix_at(colorSeed, gv) & 0x3f) + 40u;
+                    colors[lv] = 0xff000000u | (bb << 16) | (gg << 8) | rr;
                 }
             }
         }
     }
+    if (indices && rows >= 2 && cols >= 2) crx_gen_heightfield_indices(cols, 0, rows - 1, 0, cols - 1, indices);
+}
+
+void crx_gen_heightfield(uint32_t n, float spacing, float amplitude, float scale, uint32_t octaves,
+                         uint64_t seed, float* positions, uint32_t* indices, uint32_t* colors,
+                         int nthreads) {
+    crx_gen_heightfield_tile(n, 0, 0, n, n, spacing, amplitude, scale, octaves, seed, positions, nullptr, colors,
+                             nthreads);
     if (indices) crx_gen_heightfield_indices(n, 0, n - 1, 0, n - 1, indices);
 }
 
 uint64_t crx_gen_heightfield_indices(uint32_t n, uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1,
                                      uint32_t* indices) {
-    if (n < 2 || r1 <= r0 || c1 <= c0 || r1 > n - 1 || c1 > n - 1) return 0;
+    if (n < 2 || r1 <= r0 || c1 <= c0 || c1 > n - 1) return 0;
     const uint64_t w = c1 - c0;
     const uint64_t quads = (uint64_t)(r1 - r0) * w;
 #pragma omp parallel for schedule(static)

@@ -1,0 +1,184 @@
+// crx_host.h — host-side internals of the C-ABI shared by crx_api.cpp and
+// crx_server.cpp: device / pinned buffers, the context, mesh and scene
+// objects, and the draw / resolve / clear drivers behind the entry points.
+#pragma once
+
+#include "../../include/crx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "crx_device.h"
+
+namespace crx {
+
+// Records the message of a failing call (crx_last_error) and returns s.
+crx_status fail(crx_status s, const std::string& msg);
+
+#define CRX_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(CRX_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CRX_TRY(expr)                      \
+    do {                                   \
+        crx_status s_ = (expr);            \
+        if (s_ != CRX_OK) return s_;       \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t capacity = 0;
+    crx_status reserve(size_t bytes) {
+        if (bytes <= capacity) return CRX_OK;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            ptr = nullptr;
+            capacity = 0;
+            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipFree: ") + hipGetErrorString(e));
+        }
+        size_t want = std::max<size_t>(bytes, 4096);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return fail(CRX_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
+        }
+        capacity = want;
+        return CRX_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        capacity = 0;
+    }
+};
+
+// Pinned host staging with an event guarding reuse: the buffer is rewritten only
+// after the copy that last read it has completed.
+struct Staging {
+    void* ptr = nullptr;
+    size_t capacity = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    crx_status acquire(size_t bytes) {
+        if (pending) {
+            hipError_t e = hipEventSynchronize(done);
+            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+            pending = false;
+        }
+        if (!done) {
+            hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+            if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+        if (bytes <= capacity) return CRX_OK;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        capacity = 0;
+        size_t want = std::max<size_t>(bytes * 3 / 2, 65536);
+        hipError_t e = hipHostMalloc(&ptr, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return fail(CRX_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        capacity = want;
+        return CRX_OK;
+    }
+    crx_status mark(hipStream_t stream) {
+        hipError_t e = hipEventRecord(done, stream);
+        if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipEventRecord: ") + hipGetErrorString(e));
+        pending = true;
+        return CRX_OK;
+    }
+    void release() {
+        if (pending && done) (void)hipEventSynchronize(done);
+        if (ptr) (void)hipHostFree(ptr);
+        if (done) (void)hipEventDestroy(done);
+        ptr = nullptr;
+        done = nullptr;
+        capacity = 0;
+        pending = false;
+    }
+};
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+enum TimerSlot { T_CLEAR = 0, T_S1, T_S2, T_S3, T_S3_END, T_RESOLVE, T_RESOLVE_END, T_COUNT };
+
+}  // namespace crx
+
+struct crx_context {
+    int device = 0;
+    int numCUs = 0;
+    crx_tuning tuning{};
+    uint32_t* ctr = nullptr;          // device counters (CTR_COUNT)
+    uint32_t* ctrHost = nullptr;      // pinned readback
+    crx::DeviceBuffer nontrivial;          // uint64 entries
+    crx::DeviceBuffer huge;                // int4 entries
+    uint32_t ntCap = 0, hugeCap = 0;
+    crx::DeviceBuffer drawMeta;            // items | instances | transforms
+    crx::DeviceBuffer resolveMeta;         // instances | prefix | per-view records
+    crx::DeviceBuffer clearMeta;           // per-buffer clear records
+    crx::Staging drawStaging, resolveStaging, clearStaging;
+    bool timing = false;
+    hipEvent_t ev[crx::T_COUNT] = {};
+    bool recorded[crx::T_COUNT] = {};
+    uint64_t meshSeq = 0;
+};
+
+struct crx_mesh {
+    crx_context* ctx = nullptr;
+    float* positions = nullptr;
+    uint32_t* indices = nullptr;
+    uint32_t* colors = nullptr;
+    bool owned = false;
+    uint32_t numVertices = 0;
+    uint32_t numTriangles = 0;
+    crx_box3 aabb{};
+    uint64_t seq = 0;
+};
+
+struct crx_scene {
+    crx_context* ctx = nullptr;
+    struct Node {
+        crx_mesh* mesh;
+        crx_mat4 world;
+    };
+    std::vector<Node> nodes;
+    // frame-preparation scratch, kept across frames like the reference's statics
+    std::vector<crx_cmesh> unique, instances;
+    std::vector<crx_mat4> transforms;
+    std::vector<uint64_t> prefix;
+};
+
+
+namespace crx {
+
+crx_mat4 mat_mul(const crx_mat4& A, const crx_mat4& B);
+crx_mat4 mat_inverse(const crx_mat4& M);
+bool mirror_flag(const crx_mat4& m);
+crx_status check_tuning(const crx_tuning& t);
+crx_status check_target(const crx_render_target* t);
+void frustum_planes(const crx_view& view, float out[16]);
+crx_box3 world_box(const crx_box3& b, const crx_mat4& world);
+bool box_in_frustum(const float planes[16], const crx_box3& worldBox);
+
+// drawTrianglesVisbuffer for numViews >= 1 targets sharing one instance list
+// (transforms view-major).  skip (nullable, numViews * numInstances bytes,
+// view-major): a non-zero byte drops instance i from view v, as the host node
+// cull of that view would; it needs numViews > 1.
+crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
+                      uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
+                      uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
+                      const uint8_t* skip = nullptr);
+crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                         const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                         hipStream_t stream);
+crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,
+                         hipStream_t stream);
+
+}  // namespace crx

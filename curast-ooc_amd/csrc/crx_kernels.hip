@@ -400,33 +400,52 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
 
 namespace {
 
-// DPP move of x (lanes outside rowMask keep `old`).
+// Order-preserving integer key of a float: signed-integer order of the keys
+// is the numeric order of the non-NaN floats (-0 sorts just below +0).  A
+// positive NaN sorts above +inf, a negative NaN below -inf.
+__device__ __forceinline__ int32_t fkey(float f) {
+    const int32_t b = __builtin_bit_cast(int32_t, f);
+    return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+
+__device__ __forceinline__ float fkey_inv(int32_t k) {
+    return __builtin_bit_cast(float, k ^ ((k >> 31) & 0x7FFFFFFF));
+}
+
+constexpr int32_t kKeyPosInf = 0x7F800000;                  // fkey(+inf)
+constexpr int32_t kKeyNegInf = (int32_t)0xFF800000 ^ 0x7FFFFFFF;   // fkey(-inf)
+
+// Integer minimum / maximum over the 64 lanes (all active), as a scalar.
+// Each step is one v_min_i32 / v_max_i32 with a DPP operand: quad exchanges
+// and row rotations reduce each row of 16 lanes, row broadcasts fold rows
+// 0-1 into 1 and 2-3 into 3, then rows 0-1 into 2-3; lane 63 holds the
+// result.  Lanes a DPP source does not feed read the identity, which lets
+// the compiler fold each move into the min / max.
 template <int CTRL, int ROWMASK>
-__device__ __forceinline__ float dpp_f(float old, float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, x),
-                                                                 CTRL, ROWMASK, 0xF, false));
+__device__ __forceinline__ int32_t dpp_or(int32_t identity, int32_t x) {
+    return __builtin_amdgcn_update_dpp(identity, x, CTRL, ROWMASK, 0xF, false);
 }
 
-// Minimum / maximum over the 64 lanes (all active), as a scalar: quad
-// exchanges, row rotations, then row broadcasts leave the result in lane 63.
-__device__ __forceinline__ float wave_min(float x) {
-    x = fminf(x, dpp_f<0xB1, 0xF>(x, x));    // quad_perm [1,0,3,2]
-    x = fminf(x, dpp_f<0x4E, 0xF>(x, x));    // quad_perm [2,3,0,1]
-    x = fminf(x, dpp_f<0x124, 0xF>(x, x));   // row_ror:4
-    x = fminf(x, dpp_f<0x128, 0xF>(x, x));   // row_ror:8
-    x = fminf(x, dpp_f<0x142, 0xA>(x, x));   // row_bcast:15 into rows 1, 3
-    x = fminf(x, dpp_f<0x143, 0xC>(x, x));   // row_bcast:31 into rows 2, 3
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+__device__ __forceinline__ int32_t wave_min_i(int32_t x) {
+    constexpr int32_t I = 0x7FFFFFFF;
+    x = min(x, dpp_or<0xB1, 0xF>(I, x));    // quad_perm [1,0,3,2]
+    x = min(x, dpp_or<0x4E, 0xF>(I, x));    // quad_perm [2,3,0,1]
+    x = min(x, dpp_or<0x124, 0xF>(I, x));   // row_ror:4
+    x = min(x, dpp_or<0x128, 0xF>(I, x));   // row_ror:8
+    x = min(x, dpp_or<0x142, 0xA>(I, x));   // row_bcast:15 into rows 1, 3
+    x = min(x, dpp_or<0x143, 0xC>(I, x));   // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(x, 63);
 }
 
-__device__ __forceinline__ float wave_max(float x) {
-    x = fmaxf(x, dpp_f<0xB1, 0xF>(x, x));
-    x = fmaxf(x, dpp_f<0x4E, 0xF>(x, x));
-    x = fmaxf(x, dpp_f<0x124, 0xF>(x, x));
-    x = fmaxf(x, dpp_f<0x128, 0xF>(x, x));
-    x = fmaxf(x, dpp_f<0x142, 0xA>(x, x));
-    x = fmaxf(x, dpp_f<0x143, 0xC>(x, x));
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+__device__ __forceinline__ int32_t wave_max_i(int32_t x) {
+    constexpr int32_t I = (int32_t)0x80000000;
+    x = max(x, dpp_or<0xB1, 0xF>(I, x));
+    x = max(x, dpp_or<0x4E, 0xF>(I, x));
+    x = max(x, dpp_or<0x124, 0xF>(I, x));
+    x = max(x, dpp_or<0x128, 0xF>(I, x));
+    x = max(x, dpp_or<0x142, 0xA>(I, x));
+    x = max(x, dpp_or<0x143, 0xC>(I, x));
+    return __builtin_amdgcn_readlane(x, 63);
 }
 
 // Object-space box of a wave's 64 triangles.
@@ -446,10 +465,12 @@ struct WaveBox {
 struct ViewCull {
     float px[4], nx[4], py[4], ny[4], z[4];
     float slopeX, cnstX, slopeY, cnstY, slopeZ, cnstZ;
+    bool skip;   // the instance is dropped from this view
 };
 
 __device__ __forceinline__ ViewCull view_cull(const float* m, float fa, float f) {
     ViewCull c;
+    c.skip = false;
     #pragma unroll
     for (int k = 0; k < 4; k++) {
         const float x = m[4 * k], y = m[4 * k + 1], z = m[4 * k + 2];
@@ -504,11 +525,13 @@ __device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B
 
 __device__ __forceinline__ ViewCull lane_view_cull(const RasterParams& P, uint32_t view, uint32_t inst) {
     const ViewParams* V = P.views + view;
-    const float* m = V->transforms[inst].m;
+    const Transform& T = V->transforms[inst];
     float mm[16];
     #pragma unroll
-    for (int e = 0; e < 16; e++) mm[e] = m[e];
-    return view_cull(mm, V->fa, V->f);
+    for (int e = 0; e < 16; e++) mm[e] = T.m[e];
+    ViewCull c = view_cull(mm, V->fa, V->f);
+    c.skip = T.skip != 0u;
+    return c;
 }
 
 }  // namespace
@@ -559,17 +582,28 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
         if (more) s1_fetch(P, next, nxt, true);
 
         // object-space box of the wave's triangles (lanes past the item's end
-        // hold its last triangle, which is inside the mesh anyway)
+        // hold its last triangle, which is inside the mesh anyway), reduced on
+        // order-preserving integer keys
         WaveBox box;
-        box.lo = f3{wave_min(fminf(A.x, fminf(B.x, C.x))), wave_min(fminf(A.y, fminf(B.y, C.y))),
-                    wave_min(fminf(A.z, fminf(B.z, C.z)))};
-        box.hi = f3{wave_max(fmaxf(A.x, fmaxf(B.x, C.x))), wave_max(fmaxf(A.y, fmaxf(B.y, C.y))),
-                    wave_max(fmaxf(A.z, fmaxf(B.z, C.z)))};
+        int32_t klo[3], khi[3];
+        {
+            const int32_t ax = fkey(A.x), ay = fkey(A.y), az = fkey(A.z);
+            const int32_t bx = fkey(B.x), by = fkey(B.y), bz = fkey(B.z);
+            const int32_t cx = fkey(C.x), cy = fkey(C.y), cz = fkey(C.z);
+            klo[0] = wave_min_i(min(ax, min(bx, cx)));
+            klo[1] = wave_min_i(min(ay, min(by, cy)));
+            klo[2] = wave_min_i(min(az, min(bz, cz)));
+            khi[0] = wave_max_i(max(ax, max(bx, cx)));
+            khi[1] = wave_max_i(max(ay, max(by, cy)));
+            khi[2] = wave_max_i(max(az, max(bz, cz)));
+        }
+        // a NaN or infinite coordinate anywhere: no box, every view draws the wave
+        const bool unbounded = min(klo[0], min(klo[1], klo[2])) <= kKeyNegInf ||
+                               max(khi[0], max(khi[1], khi[2])) >= kKeyPosInf;
+        box.lo = f3{fkey_inv(klo[0]), fkey_inv(klo[1]), fkey_inv(klo[2])};
+        box.hi = f3{fkey_inv(khi[0]), fkey_inv(khi[1]), fkey_inv(khi[2])};
         box.M = fmaxf(fmaxf(fmaxf(fabsf(box.lo.x), fabsf(box.hi.x)), fmaxf(fabsf(box.lo.y), fabsf(box.hi.y))),
                       fmaxf(fabsf(box.lo.z), fabsf(box.hi.z)));
-        // a NaN or infinite coordinate anywhere: no box, every view draws the wave
-        const float s = (((A.x + A.y) + (A.z + B.x)) + ((B.y + B.z) + (C.x + C.y))) + C.z;
-        const bool unbounded = __ballot(!(s - s == 0.0f)) != 0ull;
 
         const bool valid = cur.tri < cur.it.numTriangles;
         const uint32_t tri = cur.tri;
@@ -581,7 +615,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
                 bool maybe = false;
                 if (v < numViews) {
                     if (!cacheCull || inst != cullInst) cull = lane_view_cull(P, v, inst);
-                    maybe = unbounded || view_may_see(cull, box);
+                    maybe = !cull.skip && (unbounded || view_may_see(cull, box));
                 }
                 if (cacheCull) cullInst = inst;
                 uint64_t mask = __ballot(maybe);

@@ -153,3 +153,105 @@ def multiview_views(n: int, positions: np.ndarray, count: int = 64, closeup_frac
         view = crx.look_at(eye, (eye[0] + d[0], eye[1] + d[1], eye[2] + d[2]), (0.0, 1.0, 0.0))
         out.append((view, projection(W, H), W, H, bool(close[k])))
     return out
+
+
+SEED_SERVING = 0xC0FFEE05
+
+
+def tiled_heightfield(ctx, n: int, tile: int, amplitude: float = 120.0, scale: float = 2048.0, octaves: int = 6,
+                      seed: int = SEED_HEIGHTFIELD, chunk_bytes: int = 1 << 30, threads: int = 0,
+                      colors: bool = True):
+    """The n x n heightfield of heightfield_mesh cut into square nodes of
+    `tile` x `tile` cells ((tile+1)^2 vertices; neighbours share their border
+    vertices), generated directly into page-locked host memory.
+
+    Returns (nodes, buffers): nodes is a list of dicts with the pinned
+    pointers (pos, idx, col), counts (nv, nt), the object-space aabb and the
+    tile origin (r0, c0); buffers are the crx.PinnedBuffer chunks that own
+    the memory."""
+    al = lambda x: (x + 255) // 256 * 256
+    starts = list(range(0, n - 1, tile))
+    specs = []
+    for r0 in starts:
+        for c0 in starts:
+            rows = min(tile, n - 1 - r0) + 1
+            cols = min(tile, n - 1 - c0) + 1
+            nv = rows * cols
+            nt = 2 * (rows - 1) * (cols - 1)
+            size = al(12 * nv) + al(12 * nt) + (al(4 * nv) if colors else 0)
+            specs.append((r0, c0, rows, cols, nv, nt, size))
+    buffers, nodes = [], []
+    cur, used = None, 0
+    for (r0, c0, rows, cols, nv, nt, size) in specs:
+        if cur is None or used + size > cur.nbytes:
+            cur = crx.PinnedBuffer(ctx, max(chunk_bytes, size))
+            buffers.append(cur)
+            used = 0
+        pos = cur.array(np.float32, 3 * nv, used)
+        idx = cur.array(np.uint32, 3 * nt, used + al(12 * nv))
+        col = cur.array(np.uint32, nv, used + al(12 * nv) + al(12 * nt)) if colors else None
+        crx.gen().crx_gen_heightfield_tile(n, r0, c0, rows, cols, 1.0, amplitude, scale, octaves, seed,
+                                           _p(pos), _p(idx), _p(col), threads)
+        p3 = pos.reshape(-1, 3)
+        lo, hi = p3.min(axis=0), p3.max(axis=0)
+        box = crx.Box3(crx.Vec3(*map(float, lo)), crx.Vec3(*map(float, hi)))
+        nodes.append({"pos": pos.ctypes.data, "idx": idx.ctypes.data, "col": col.ctypes.data if colors else None,
+                      "nv": nv, "nt": nt, "aabb": box, "r0": r0, "c0": c0, "rows": rows, "cols": cols,
+                      "heights": p3[:, 1].reshape(rows, cols)})
+        used += size
+    return nodes, buffers
+
+
+def ground_lookup(nodes, tile: int):
+    """Height of the tiled terrain at the grid vertex nearest to (x, z)."""
+    by_origin = {(nd["r0"], nd["c0"]): nd for nd in nodes}
+    last = max(nd["r0"] for nd in nodes)
+
+    def ground(x: float, z: float) -> float:
+        r, c = int(round(z)), int(round(x))
+        r0 = min((max(r, 0) // tile) * tile, last)
+        c0 = min((max(c, 0) // tile) * tile, last)
+        nd = by_origin[(r0, c0)]
+        lr = min(max(r - r0, 0), nd["rows"] - 1)
+        lc = min(max(c - c0, 0), nd["cols"] - 1)
+        return float(nd["heights"][lr, lc])
+    return ground
+
+
+def serving_requests(n: int, ground, duration: float = 10.0, rate: float = 200.0, sessions: int = 8,
+                     sizes=MULTIVIEW_SIZES, seed: int = SEED_SERVING, speed: float = 40.0,
+                     altitude=(80.0, 250.0)):
+    """Seeded request stream for the serving workload: Poisson arrivals at
+    `rate` per second for `duration` seconds, each from one of `sessions`
+    cameras flying over the terrain (straight paths at `speed` units/s that
+    reflect at the terrain border, `altitude` (80-250) units above the ground,
+    looking down 35-70 degrees), resolutions uniform over `sizes`.
+    Returns a list of (arrival_seconds, view, proj, width, height, session)."""
+    rng = np.random.default_rng(seed)
+    span = float(n - 1)
+    sess = []
+    for _ in range(sessions):
+        sess.append({
+            "x": float(rng.uniform(0.1, 0.9) * span), "z": float(rng.uniform(0.1, 0.9) * span),
+            "heading": float(rng.uniform(0.0, 2.0 * math.pi)), "alt": float(rng.uniform(*altitude)),
+            "pitch": math.radians(float(rng.uniform(35.0, 70.0)))})
+
+    def reflect(p: float) -> float:
+        p = p % (2.0 * span)
+        return p if p <= span else 2.0 * span - p
+
+    out, t = [], 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            break
+        k = int(rng.integers(sessions))
+        s = sess[k]
+        x = reflect(s["x"] + speed * t * math.sin(s["heading"]))
+        z = reflect(s["z"] + speed * t * math.cos(s["heading"]))
+        eye = (x, ground(x, z) + s["alt"], z)
+        d = _look_dir(s["heading"], s["pitch"])
+        view = crx.look_at(eye, (eye[0] + d[0], eye[1] + d[1], eye[2] + d[2]), (0.0, 1.0, 0.0))
+        W, H = sizes[int(rng.integers(len(sizes)))]
+        out.append((t, view, projection(W, H), W, H, k))
+    return out

@@ -391,6 +391,131 @@ crx_status crx_render_views(crx_context* ctx, crx_scene* scene, const crx_view* 
                             uint64_t* const* framebuffers, uint64_t* const* colorbuffers, void* stream,
                             crx_draw_stats* stats /* nullable */);
 
+/* ------------------------------------------------------------------ */
+/* Serving with continuous batching over an out-of-core node store.    */
+/* No single reference counterpart: it combines CuRast::render's       */
+/* per-frame draw (CuRast_render.h:873-904), the node culling of        */
+/* SNTriangles::update (SNTriangles.h:29-105) and the residency /       */
+/* async-upload scheme of the point-cloud octree (globals.h:433-501,    */
+/* structureUpdate.cpp:256, SNCOctree.h:32,102-119) applied to          */
+/* triangle nodes.                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct crx_server crx_server;
+
+typedef struct crx_server_config {
+    uint64_t node_cache_bytes;   /* device memory for resident node geometry (LRU cache of slots) */
+    uint64_t frame_bytes;        /* device memory for in-flight result buffers: requests are
+                                    admitted while their W*H*8 (x2 with resolve) bytes fit */
+    uint32_t max_batch_views;    /* views per batched draw, 1..CRX_MAX_VIEWS */
+    uint32_t lanes;              /* batches in flight at once (each its own stream, queues and host
+                                    thread), 1..8 */
+    int32_t resolve;             /* also resolve every request to RGBA8 vertex colours */
+    uint32_t reserved;
+} crx_server_config;
+
+void crx_server_config_default(crx_server_config* out);
+
+/* Page-locked host memory (hipHostMalloc) for node geometry the server
+ * streams from. */
+crx_status crx_host_alloc(crx_context* ctx, size_t bytes, void** out);
+crx_status crx_host_free(crx_context* ctx, void* ptr);
+
+crx_status crx_server_create(crx_context* ctx, const crx_server_config* config, crx_server** out);
+/* Stops the lanes after the submitted requests are drawn and frees the
+ * server's device memory (result buffers included). */
+crx_status crx_server_destroy(crx_server* server);
+
+/* A scene node: a triangle mesh in HOST memory (page-locked, from
+ * crx_host_alloc, valid while the server lives; colors nullable), its
+ * object-space bounding box and world transform.  resident != 0 places it in
+ * the device cache at crx_server_start (while slots remain); the others stay
+ * host-resident until a request needs them.  Node ids are consecutive from 0;
+ * node i's triangle ids start at the sum of the triangle counts of nodes
+ * 0..i-1, whatever else a request draws. */
+crx_status crx_server_add_node(crx_server* server, const float* positions, uint32_t numVertices,
+                               const uint32_t* indices, uint32_t numTriangles, const uint32_t* colors,
+                               const crx_box3* aabb, const crx_mat4* world, int resident, uint32_t* node_id);
+
+/* Uploads the resident nodes and starts the lanes.  No nodes can be added
+ * afterwards. */
+crx_status crx_server_start(crx_server* server);
+
+/* Queues a render request for one view; returns at once with a ticket.
+ * Each lane, when free, takes the oldest queued requests whose buffers fit
+ * the frame budget (up to max_batch_views), streams in the nodes they see
+ * that are not resident, and draws them in one multi-view pass on its own
+ * stream.  A request's visibility buffer depends only on its view and the
+ * scene: it is bit-identical whether drawn alone or batched with others. */
+crx_status crx_server_submit(crx_server* server, const crx_view* view, uint64_t* ticket);
+
+typedef struct crx_request_result {
+    const uint64_t* framebuffer;   /* device, width*height; valid until crx_server_release */
+    const uint64_t* colorbuffer;   /* device, or NULL without resolve */
+    double submit_time;            /* seconds on the server's clock */
+    double start_time;             /* its batch was taken by a lane */
+    double done_time;              /* its batch completed on the device */
+    uint32_t batch_views;          /* views in that batch */
+    uint32_t uploads;              /* nodes streamed in for that batch */
+    crx_status status;             /* CRX_OK, or why the request failed */
+    uint32_t reserved;
+} crx_request_result;
+
+/* Blocks until the request is drawn. */
+crx_status crx_server_wait(crx_server* server, uint64_t ticket, crx_request_result* out);
+/* Returns the request's buffers to the frame budget. */
+crx_status crx_server_release(crx_server* server, uint64_t ticket);
+/* Seconds on the server's clock (the time base of crx_request_result). */
+double crx_server_clock(const crx_server* server);
+
+typedef struct crx_server_stats {
+    uint64_t requests;         /* completed */
+    uint64_t batches;
+    uint64_t uploads;          /* nodes streamed host -> device */
+    uint64_t upload_bytes;
+    uint64_t evictions;
+    uint32_t resident_nodes;
+    uint32_t cache_slots;
+    uint32_t max_batch;        /* largest batch drawn */
+    uint32_t reserved;
+} crx_server_stats;
+
+crx_status crx_server_get_stats(const crx_server* server, crx_server_stats* out);
+
+/* Nodes a view draws (the node cull of that view), ascending ids; returns
+ * their number in *count (at most `capacity` ids are written). */
+crx_status crx_server_view_nodes(const crx_server* server, const crx_view* view, uint32_t* ids, uint32_t capacity,
+                                 uint32_t* count);
+
+/* ------------------------------------------------------------------ */
+/* Autotuner (no reference counterpart: the reference fixes these as   */
+/* compile-time constants, HostDeviceInterface.h:261-266, and a        */
+/* blockDim-coupled tile size, triangles_visbuffer.cu:683-684).        */
+/* ------------------------------------------------------------------ */
+
+#define CRX_TUNE_SCHEDULE 1u   /* grid sizes, block size, batch order, cull order, fast tiny cull:
+                                  the result is bit-identical under every choice */
+#define CRX_TUNE_ROUTING 2u    /* small_threshold and tile_size: they move triangles between stages
+                                  and so change result bits (the oracle follows the same values) */
+#define CRX_TUNE_QUEUES 4u     /* size max_nontrivial / max_huge from the observed queue counts */
+
+typedef struct crx_autotune_result {
+    crx_tuning tuning;      /* best tuning found (also installed on the context) */
+    float start_ms;         /* median device time of stages 1-3 with the starting tuning */
+    float best_ms;          /* the same with the best tuning */
+    uint32_t trials;        /* candidate configurations measured */
+    uint32_t reserved;
+} crx_autotune_result;
+
+/* Coordinate descent over the knobs selected by `flags`, starting from the
+ * context's tuning: each candidate draws the scene into all views `reps`
+ * times (after one warm-up draw) and is scored by the median device time of
+ * stages 1-3.  The buffers are overwritten.  On return the context uses the
+ * best tuning. */
+crx_status crx_autotune(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
+                        uint64_t* const* framebuffers, uint32_t flags, uint32_t reps,
+                        crx_autotune_result* out);
+
 #ifdef __cplusplus
 }
 #endif

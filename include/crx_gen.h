@@ -42,9 +42,20 @@ void crx_gen_heightfield(uint32_t n, float spacing, float amplitude, float scale
                          uint64_t seed, float* positions, uint32_t* indices, uint32_t* colors,
                          int nthreads);
 
-/* Index buffer of a sub-grid of an n x n heightfield: quads with rows in
- * [r0, r1) and columns in [c0, c1), 2 triangles each, indices into the full
- * vertex array.  Returns the number of triangles written. */
+/* Tile of the same heightfield: the rows x cols vertices starting at grid
+ * row r0, column c0, bit-identical to those of crx_gen_heightfield(n, ...),
+ * stored tile-locally (vertex (lr, lc) at lr*cols + lc), and the
+ * 2*(rows-1)*(cols-1) triangles of the tile's quads with tile-local indices
+ * (same orientation).  Neighbouring tiles that share a row or column of
+ * vertices stitch without cracks.  Any output may be NULL. */
+void crx_gen_heightfield_tile(uint32_t n, uint32_t r0, uint32_t c0, uint32_t rows, uint32_t cols, float spacing,
+                              float amplitude, float scale, uint32_t octaves, uint64_t seed, float* positions,
+                              uint32_t* indices, uint32_t* colors, int nthreads);
+
+/* Index buffer of a sub-grid of a heightfield whose vertex rows hold n
+ * vertices: quads with rows in [r0, r1) and columns in [c0, c1), c1 <= n-1,
+ * 2 triangles each, indices r*n + c into the vertex array.  Returns the
+ * number of triangles written. */
 uint64_t crx_gen_heightfield_indices(uint32_t n, uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1,
                                      uint32_t* indices);
 

@@ -169,3 +169,44 @@ def test_multiview_rejects_bad_counts(ctx):
     with pytest.raises(crx.CrxError) as e:
         ctx.draw_triangles_visbuffer_views([], [crx.CMesh()], [crx.identity()] * 257, [t] * 257, False)
     assert e.value.status == 1
+
+
+def test_autotuner(ctx):
+    """crx_autotune: schedule knobs never change a bit; routing knobs change
+    the routing and the oracle, given the tuned values, still agrees."""
+    n = 257
+    pos, spec = _terrain(n)
+    sizes = ((256, 256), (512, 512), (640, 360))
+    vs = wl.multiview_views(n, pos, count=6, sizes=sizes, seed=99)
+    fresh = crx.Context(0)
+    try:
+        scene_mesh = crx.Mesh(fresh, pos, spec.meshes[0].indices, spec.meshes[0].colors)
+        scene = crx.Scene(fresh)
+        scene.add(scene_mesh)
+        views = [crx.make_view(v, p, W, H) for (v, p, W, H, _) in vs]
+        fbs = [fresh.alloc(8 * v.width * v.height) for v in views]
+        settings = crx.render_settings(resolve=0)
+
+        def frame():
+            scene.render_views(views, fbs, None, settings)
+            return [b.download(np.uint64, v.width * v.height) for b, v in zip(fbs, views)]
+
+        base = frame()
+        res = fresh.autotune(scene, views, fbs, crx.TUNE_SCHEDULE | crx.TUNE_QUEUES, reps=2)
+        print(f"[autotune] schedule: {res.start_ms:.3f} -> {res.best_ms:.3f} ms in {res.trials} trials", flush=True)
+        assert res.trials >= 20 and res.best_ms <= res.start_ms
+        assert bytes(fresh.tuning) == bytes(res.tuning)
+        assert res.tuning.small_threshold == 128 and res.tuning.tile_size == 64
+        after = frame()
+        for k in range(len(views)):
+            assert np.array_equal(after[k], base[k]), f"view {k}: schedule tuning changed the result"
+
+        res2 = fresh.autotune(scene, views, fbs, crx.TUNE_ROUTING, reps=2)
+        t = res2.tuning
+        print(f"[autotune] routing: small_threshold={t.small_threshold} tile={t.tile_size}", flush=True)
+        got = frame()
+        for k, (v, p, W, H, _) in enumerate(vs):
+            want, _, _ = oracle_render(spec, W, H, v, p, tuning=t)
+            assert np.array_equal(got[k], want), f"view {k}\n" + describe_mismatch(got[k], want, W)
+    finally:
+        fresh.close()

@@ -1,0 +1,146 @@
+"""The continuous-batching server over an out-of-core node store.
+
+A tiled terrain lives in page-locked host memory with a device cache smaller
+than the scene, so requests stream nodes in and evict others.  Every result
+must be bit-identical whether its request is drawn alone or batched with the
+others (tolerance: none), and equal to the oracle's draw of the nodes that
+view sees, numbered with the scene-wide triangle prefix.
+"""
+import numpy as np
+import pytest
+
+import crx
+import oracle as orc
+import workloads as wl
+from crxtest import describe_mismatch
+
+pytestmark = pytest.mark.gpu
+
+N, TILE = 513, 64
+SIZES = ((128, 128), (256, 144), (200, 200), (96, 160))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = crx.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def terrain(ctx):
+    nodes, bufs = wl.tiled_heightfield(ctx, N, TILE, amplitude=30.0, scale=128.0, chunk_bytes=1 << 24)
+    yield nodes
+    for b in bufs:
+        b.free()
+
+
+def _server(ctx, nodes, slots, frame_mb, batch, lanes):
+    slot = max((n["nv"] * 12 + 255) // 256 * 256 + (n["nt"] * 12 + 255) // 256 * 256 + (n["nv"] * 4 + 255) // 256 * 256
+               for n in nodes)
+    cfg = crx.server_config(node_cache_bytes=slot * slots, frame_bytes=frame_mb << 20, max_batch_views=batch,
+                            lanes=lanes, resolve=1)
+    srv = crx.Server(ctx, cfg)
+    for k, nd in enumerate(nodes):
+        srv.add_node(nd["pos"], nd["nv"], nd["idx"], nd["nt"], nd["col"], nd["aabb"], resident=(k % 2 == 0))
+    srv.start()
+    return srv
+
+
+def _requests(nodes, count):
+    ground = wl.ground_lookup(nodes, TILE)
+    reqs = wl.serving_requests(N, ground, duration=count / 200.0 + 1.0, rate=200.0, sessions=5, sizes=SIZES,
+                               seed=0x5E4E, speed=60.0, altitude=(15.0, 40.0))[:count]
+    return [crx.make_view(v, p, W, H) for (_, v, p, W, H, _) in reqs]
+
+
+def _fetch(srv, ticket, view):
+    r = srv.wait(ticket)
+    assert r.status == 0, crx.lib().crx_last_error()
+    n = view.width * view.height
+    fb = srv.download(r.framebuffer, n)
+    cb = srv.download(r.colorbuffer, n)
+    srv.release(ticket)
+    return fb, cb, r
+
+
+def test_alone_vs_mix_and_oracle(ctx, terrain):
+    views = _requests(terrain, 24)
+    srv = _server(ctx, terrain, slots=20, frame_mb=2, batch=8, lanes=2)
+    try:
+        alone = []
+        for v in views:
+            alone.append(_fetch(srv, srv.submit(v), v)[:2])
+        tickets = [srv.submit(v) for v in views]      # all at once: continuous batching
+        mix, batches = [], []
+        for t, v in zip(tickets, views):
+            fb, cb, r = _fetch(srv, t, v)
+            mix.append((fb, cb))
+            batches.append(r.batch_views)
+        st = srv.stats()
+        print(f"[server] batches {st.batches} max_batch {st.max_batch} uploads {st.uploads} "
+              f"evictions {st.evictions} resident {st.resident_nodes}/{st.cache_slots}", flush=True)
+        assert max(batches) > 1, "requests submitted together should share batches"
+        assert st.uploads > 0 and st.evictions > 0, "the cache is smaller than the scene"
+        for k, v in enumerate(views):
+            assert np.array_equal(mix[k][0], alone[k][0]), f"request {k}\n" + describe_mismatch(mix[k][0], alone[k][0], v.width)
+            assert np.array_equal(mix[k][1], alone[k][1]), f"request {k} colours"
+        assert sum(int((fb != crx.CLEAR_PIXEL).sum()) for fb, _ in alone) > 50_000
+        # oracle: the nodes the view sees, numbered with the scene-wide prefix
+        prefix = np.concatenate([[0], np.cumsum([nd["nt"] for nd in terrain])[:-1]]).astype(np.uint64)
+        for k in range(0, len(views), 4):
+            v = views[k]
+            ids = srv.view_nodes(v)
+            inst = []
+            for j, i in enumerate(ids):
+                nd = terrain[i]
+                m = crx.CMesh()
+                m.numTriangles = nd["nt"]
+                m.positions = nd["pos"]
+                m.indices = nd["idx"]
+                m.colors = nd["col"]
+                m.cummulativeTriangleCount = int(prefix[i])
+                m.instances.offset = j
+                m.instances.count = 1
+                m.world = crx.identity()
+                m.id = i
+                m.numVertices = nd["nv"]
+                m.isLoaded = 1
+                inst.append(m)
+            xf = [crx.mat4_mul(v.view, crx.identity()) for _ in ids]
+            want_fb, _ = orc.draw(crx, inst, inst, xf, v.width, v.height, v.view, v.proj, False)
+            assert np.array_equal(alone[k][0], want_fb), f"request {k} vs oracle\n" + describe_mismatch(alone[k][0], want_fb, v.width)
+            want_cb = orc.resolve(crx, inst, prefix[ids] if ids else np.zeros(0, np.uint64), want_fb, v.width,
+                                  v.height, v.view, v.proj, crx.DISPLAY_VERTEX_COLORS)
+            assert np.array_equal(alone[k][1], want_cb), f"request {k} colours vs oracle"
+    finally:
+        srv.close()
+
+
+def test_admission_waits_for_memory(ctx, terrain):
+    """A frame budget that holds only a few results: requests are admitted as
+    earlier results are released, and all complete."""
+    views = _requests(terrain, 16)
+    srv = _server(ctx, terrain, slots=24, frame_mb=1, batch=16, lanes=2)
+    try:
+        tickets = [srv.submit(v) for v in views]
+        for t, v in zip(tickets, views):
+            fb, _, r = _fetch(srv, t, v)
+            assert r.done_time >= r.start_time >= r.submit_time
+        st = srv.stats()
+        assert st.requests == len(views)
+        assert st.max_batch < len(views)
+    finally:
+        srv.close()
+
+
+def test_view_seeing_too_many_nodes_is_rejected(ctx, terrain):
+    srv = _server(ctx, terrain, slots=2, frame_mb=4, batch=4, lanes=1)
+    try:
+        far = crx.make_view(crx.look_at((256.0, 400.0, -200.0), (256.0, 0.0, 256.0)), wl.projection(128, 128), 128, 128)
+        t = srv.submit(far)
+        r = srv.wait(t)
+        assert r.status == 7       # CRX_REJECTED
+        srv.release(t)
+    finally:
+        srv.close()
