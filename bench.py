@@ -228,14 +228,15 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
     return out
 
 
-def cpu_baseline(pos, idx, col, worlds, view, proj, threads: int):
+def cpu_baseline(pos, idx, col, worlds, view, proj, threads: int, compressed: bool = False):
     """The CPU oracle rendering the same frame (stages + resolve), repeated for >= 10 s."""
     sys.path.insert(0, str(ROOT / "oracle"))
     sys.path.insert(0, str(ROOT / "tests"))
     import crx
     import oracle as orc
-    from crxtest import SceneSpec, oracle_render
-    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), worlds)
+    from crxtest import CompressedMesh, SceneSpec, oracle_render
+    mesh = CompressedMesh(pos, idx, col) if compressed else orc.HostMesh(crx, pos, idx, col)
+    spec = SceneSpec().add(mesh, worlds)
     T = len(idx) * len(worlds)
     reps, t0 = 0, time.perf_counter()
     while True:
@@ -260,6 +261,8 @@ def main() -> int:
                     help="skip the 64-view batched vs plain measurement on the heightfield")
     ap.add_argument("--multiview-only", action="store_true",
                     help="only the 64-view batched vs plain measurement (heightfield workload)")
+    ap.add_argument("--compressed", action="store_true",
+                    help="store the mesh IX_PU16-compressed (bit-packed indices, 3 x uint16 positions)")
     ap.add_argument("--serve", action="store_true",
                     help="only the out-of-core serving workload (Poisson requests, p50/p99 latency)")
     ap.add_argument("--serve-n", type=int, default=31623, help="serving terrain grid size (vertices per side)")
@@ -303,7 +306,11 @@ def main() -> int:
             setattr(tu, k.strip(), int(v))
         ctx.tuning = tu
         log(f"[bench] tuning overrides: {args.tuning}")
-    mesh = crx.Mesh(ctx, pos, idx, col)
+    t1 = time.perf_counter()
+    mesh = crx.Mesh(ctx, pos, idx, col, compressed=args.compressed)
+    if args.compressed:
+        desc = dict(desc, workload=desc["workload"] + "-ix_pu16")
+        log(f"[bench] IX_PU16 compression + upload in {time.perf_counter() - t1:.1f} s")
     scene = crx.Scene(ctx)
     for w in worlds:
         scene.add(mesh, w)
@@ -370,12 +377,18 @@ def main() -> int:
     value = T * args.steps / elapsed
     log(f"[bench] {args.steps} frames in {elapsed:.3f} s: {ms_per_step:.3f} ms/frame; stages {med}")
 
-    algo = stage1_bytes(len(idx), V) * (len(worlds) if args.workload != "instances" else 1)
+    if args.compressed:
+        bits = int(np.ceil(np.log2(np.float32(int(idx.max()) - int(idx.min()) + 1))))
+        mesh_bytes = (3 * len(idx) * bits + 7) // 8 + 6 * V
+    else:
+        mesh_bytes = stage1_bytes(len(idx), V)
+    algo = mesh_bytes * (len(worlds) if args.workload != "instances" else 1)
     if args.workload == "instances":
-        algo = stage1_bytes(len(idx), V) + 64 * len(worlds)
+        algo = mesh_bytes + 64 * len(worlds)
     achieved = algo / (mean_s1 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.workload),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if args.compressed else load_traffic(args.workload),
                 "kernel": "k_stage1", "algorithmic_bytes_per_launch": algo,
                 "stage1_ms_mean": round(mean_s1, 4)}
 
@@ -389,7 +402,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
         log(f"[bench] CPU oracle baseline on {threads} threads ...")
-        cpu = cpu_baseline(pos, idx, col, worlds, view, proj, threads)
+        cpu = cpu_baseline(pos, idx, col, worlds, view, proj, threads, compressed=args.compressed)
         log(f"[bench] CPU baseline {cpu['value']:.3e} triangles/s ({cpu['sample']})")
 
     out = {

@@ -261,6 +261,9 @@ _SIGS = {
     "crx_mat4_inverse": (None, [C.POINTER(Mat4), C.POINTER(Mat4)]),
     "crx_mat4_mul": (None, [C.POINTER(Mat4), C.POINTER(Mat4), C.POINTER(Mat4)]),
     "crx_mesh_create": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, C.POINTER(_VP)]),
+    "crx_compress_mesh": (C.c_int, [_VP, C.c_uint32, _VP, C.c_uint32, _VP, _VP, C.POINTER(C.c_uint64),
+                                    C.POINTER(CMesh)]),
+    "crx_mesh_create_compressed": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, C.POINTER(_VP)]),
     "crx_mesh_create_device": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP, C.POINTER(Box3),
                                          C.POINTER(_VP)]),
     "crx_mesh_destroy": (C.c_int, [_VP]),
@@ -596,13 +599,14 @@ class Mesh:
     colors (V,) uint32 RGBA8 or None; copied to device memory owned by the mesh."""
 
     def __init__(self, ctx: Context, positions: np.ndarray, indices: np.ndarray | None,
-                 colors: np.ndarray | None = None, num_triangles: int | None = None):
+                 colors: np.ndarray | None = None, num_triangles: int | None = None, compressed: bool = False):
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         idx = None if indices is None else np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
         col = None if colors is None else np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
         nt = (len(idx) // 3 if idx is not None else len(pos) // 3) if num_triangles is None else num_triangles
         h = C.c_void_p()
-        _check(lib().crx_mesh_create(ctx.handle, _ptr(pos), len(pos), _ptr(idx), nt, _ptr(col), C.byref(h)))
+        create = lib().crx_mesh_create_compressed if compressed else lib().crx_mesh_create
+        _check(create(ctx.handle, _ptr(pos), len(pos), _ptr(idx), nt, _ptr(col), C.byref(h)))
         self.ctx = ctx
         self.handle = h.value
         self.num_triangles = nt
@@ -679,6 +683,20 @@ class Scene:
         if self.handle:
             lib().crx_scene_destroy(C.c_void_p(self.handle))
             self.handle = None
+
+
+def compress_mesh(positions: np.ndarray, indices: np.ndarray):
+    """IX_PU16 compression (crx_compress_mesh): returns (qpositions uint16 [V,3],
+    packed index words uint32, CMesh with the compression fields set)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
+    q = np.empty((len(pos), 3), dtype=np.uint16)
+    packed = np.zeros(3 * len(idx) + 1, dtype=np.uint32)
+    words = C.c_uint64()
+    meta = CMesh()
+    _check(lib().crx_compress_mesh(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(q), _ptr(packed),
+                                   C.byref(words), C.byref(meta)))
+    return q, packed[:max(1, words.value)].copy(), meta
 
 
 class PinnedBuffer:

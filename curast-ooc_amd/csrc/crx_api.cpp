@@ -141,11 +141,14 @@ crx_status check_target(const crx_render_target* t) {
 }
 
 crx_status check_mesh(const crx_cmesh& m, const char* what, uint32_t k) {
-    if (m.compressed) return fail(CRX_UNSUPPORTED, std::string(what) + "[" + std::to_string(k) + "] is compressed (IX_PU16 is not built)");
-    if (m.numTriangles > 1431655765u)
-        return fail(CRX_INVALID_ARG, std::string(what) + "[" + std::to_string(k) + "] has more than 1431655765 triangles");
-    if (m.numTriangles > 0 && !m.positions)
-        return fail(CRX_INVALID_ARG, std::string(what) + "[" + std::to_string(k) + "] has no positions");
+    const std::string name = std::string(what) + "[" + std::to_string(k) + "]";
+    if (m.numTriangles > 1431655765u) return fail(CRX_INVALID_ARG, name + " has more than 1431655765 triangles");
+    if (m.numTriangles > 0 && !m.positions) return fail(CRX_INVALID_ARG, name + " has no positions");
+    if (m.compressed) {
+        // IX_PU16 is an index-buffer format (the _compressed kernels are _indexbuffer ones)
+        if (m.numTriangles > 0 && !m.indices) return fail(CRX_INVALID_ARG, name + " is compressed without indices");
+        if (m.bitsPerIndex > 32) return fail(CRX_INVALID_ARG, name + " has bitsPerIndex > 32");
+    }
     return CRX_OK;
 }
 
@@ -162,6 +165,12 @@ InstanceRec make_instance(const crx_cmesh& m) {
     r.id = m.id;
     r.isLoaded = m.isLoaded ? 1u : 0u;
     std::memcpy(r.world, m.world.m, sizeof(r.world));
+    r.compressed = m.compressed ? 1u : 0u;
+    r.bitsPerIndex = m.bitsPerIndex;
+    r.indexMin = m.index_min;
+    r.cf[0] = m.compressionFactor.x; r.cf[1] = m.compressionFactor.y; r.cf[2] = m.compressionFactor.z;
+    r.amin[0] = m.aabb.min.x; r.amin[1] = m.aabb.min.y; r.amin[2] = m.aabb.min.z;
+    r.amax[0] = m.aabb.max.x; r.amax[1] = m.aabb.max.y; r.amax[2] = m.aabb.max.z;
     return r;
 }
 
@@ -243,9 +252,14 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
     const uint32_t numItems = instanced ? numMeshes : numInstances;
     std::vector<DrawItem> items(numItems);
     uint64_t batches = 0, tris = 0;
+    // the draw's stage-1 kernel is chosen by its first mesh, as the reference
+    // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
+    const bool pu16 = numItems > 0 && list[0].compressed;
     for (uint32_t k = 0; k < numItems; k++) {
         const crx_cmesh& m = list[k];
         CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));
+        if ((m.compressed != 0) != pu16)
+            return fail(CRX_UNSUPPORTED, "a draw mixes compressed and uncompressed meshes");
         const int64_t off = m.instances.offset;
         const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
         if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
@@ -258,6 +272,10 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         it.instOffset = (uint32_t)off;
         it.instCount = (uint32_t)cnt;
         it.firstBatch = (uint32_t)batches;
+        it.bitsPerIndex = m.bitsPerIndex;
+        it.indexMin = m.index_min;
+        it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
+        it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
         batches += ((uint64_t)m.numTriangles + batchTris - 1) >> batchShift;
         tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
         if (batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
@@ -338,8 +356,8 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         P.hugeCap = ctx->hugeCap;
         CRX_HIP(launch_reset_counters(ctx->ctr, stream));
         record(ctx, T_S1, stream);
-        if (numViews == 1) CRX_HIP(launch_stage1(P, block, g1, stream));
-        else CRX_HIP(launch_stage1_views(P, block, g1, stream));
+        if (numViews == 1) CRX_HIP(launch_stage1(P, block, g1, pu16, stream));
+        else CRX_HIP(launch_stage1_views(P, block, g1, pu16, stream));
         record(ctx, T_S2, stream);
         CRX_HIP(launch_stage2(P, g2, stream));
         record(ctx, T_S3, stream);
@@ -591,6 +609,11 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
         cm.address = nd.mesh->seq;
         cm.isLoaded = 1;
         cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
+        cm.compressed = nd.mesh->compressed ? 1 : 0;
+        cm.bitsPerIndex = nd.mesh->bitsPerIndex;
+        cm.index_min = nd.mesh->indexMin;
+        cm.index_max = nd.mesh->indexMax;
+        cm.compressionFactor = nd.mesh->compressionFactor;
         cm.cummulativeTriangleCount = sum;
         cm.instances.offset = (int32_t)i;
         cm.instances.count = 0;
@@ -966,6 +989,117 @@ crx_status crx_mesh_create(crx_context* ctx, const float* positions, uint32_t nu
     if (s != CRX_OK) {
         crx_mesh_destroy(m);
         return s;
+    }
+    *out = m;
+    return CRX_OK;
+}
+
+crx_status crx_compress_mesh(const float* positions, uint32_t numVertices, const uint32_t* indices,
+                             uint32_t numTriangles, uint16_t* qpositions, uint32_t* packedIndices,
+                             uint64_t* packedWords, crx_cmesh* meta) {
+    if (!meta || !packedWords) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (numVertices > 0 && (!positions || !qpositions)) return fail(CRX_INVALID_ARG, "positions is NULL");
+    if (numTriangles > 0 && (!indices || !packedIndices)) return fail(CRX_INVALID_ARG, "indices is NULL");
+    if (numTriangles > 1431655765u) return fail(CRX_INVALID_ARG, "more than 1431655765 triangles");
+    const uint64_t numIndices = 3ull * numTriangles;
+    // LargeGlbLoader.h:482-504
+    uint32_t imin = 0xffffffffu, imax = 0;
+    for (uint64_t i = 0; i < numIndices; i++) {
+        if (indices[i] >= numVertices) return fail(CRX_INVALID_ARG, "index " + std::to_string(i) + " out of range");
+        imin = std::min(imin, indices[i]);
+        imax = std::max(imax, indices[i]);
+    }
+    if (numIndices == 0) imin = imax = 0;
+    const uint64_t range = (uint64_t)imax - imin;
+    const uint64_t bits = (uint64_t)std::ceil(std::log2f((float)(range + 1)));
+    const uint64_t words = ((numIndices * bits + 7) / 8 + 3) / 4;
+    std::memset(packedIndices ? packedIndices : nullptr, 0, packedIndices ? (size_t)(numIndices + 1) * 4 : 0);
+    for (uint64_t i = 0; i < numIndices; i++) {
+        // BitEdit::writeU32 (BitEdit.h:28-49)
+        const uint64_t off = i * bits;
+        if (bits == 0) break;
+        const uint64_t w = off / 32;
+        const uint32_t sh = (uint32_t)(off % 32);
+        const uint32_t mask = bits == 32 ? 0xffffffffu : ((1u << bits) - 1u);
+        const uint32_t value = (indices[i] - imin) & mask;
+        packedIndices[w] = (packedIndices[w] & ~(mask << sh)) | (value << sh);
+        if (sh + bits > 32) {
+            const uint32_t written = 32 - sh;
+            packedIndices[w + 1] = (packedIndices[w + 1] & ~(mask >> written)) | (value >> written);
+        }
+    }
+    // LargeGlbLoader.h:640-680: accessor min / max, then 16-bit quantisation
+    crx_box3 box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+    for (uint32_t v = 0; v < numVertices; v++) {
+        const float* q = positions + 3ull * v;
+        box.min.x = std::fmin(box.min.x, q[0]); box.max.x = std::fmax(box.max.x, q[0]);
+        box.min.y = std::fmin(box.min.y, q[1]); box.max.y = std::fmax(box.max.y, q[1]);
+        box.min.z = std::fmin(box.min.z, q[2]); box.max.z = std::fmax(box.max.z, q[2]);
+    }
+    if (numVertices == 0) box = crx_box3{{0, 0, 0}, {0, 0, 0}};
+    const float size[3] = {box.max.x - box.min.x, box.max.y - box.min.y, box.max.z - box.min.z};
+    const float lo[3] = {box.min.x, box.min.y, box.min.z};
+    for (uint32_t v = 0; v < numVertices; v++) {
+        for (int a = 0; a < 3; a++) {
+            const float d = positions[3ull * v + a] - lo[a];
+            double x = size[a] > 0.0f ? 65536.0 * (double)d / (double)size[a] : 0.0;
+            x = x < 0.0 ? 0.0 : (x > 65535.0 ? 65535.0 : x);
+            qpositions[3ull * v + a] = (uint16_t)x;
+        }
+    }
+    std::memset(meta, 0, sizeof(*meta));
+    meta->numTriangles = numTriangles;
+    meta->numVertices = numVertices;
+    meta->aabb = box;
+    // CuRast_render.h:358-378
+    meta->compressionFactor = crx_vec3{size[0] / 65536.0f, size[1] / 65536.0f, size[2] / 65536.0f};
+    meta->index_min = imin;
+    meta->index_max = imax;
+    meta->bitsPerIndex = (uint32_t)bits;
+    meta->compressed = 1;
+    meta->isLoaded = 1;
+    *packedWords = words;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_create_compressed(crx_context* ctx, const float* positions, uint32_t numVertices,
+                                      const uint32_t* indices, uint32_t numTriangles, const uint32_t* colors,
+                                      crx_mesh** out) {
+    if (!ctx || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    if (numTriangles > 0 && !indices) return fail(CRX_INVALID_ARG, "a compressed mesh needs indices");
+    std::vector<uint16_t> q(3ull * numVertices);
+    std::vector<uint32_t> packed(3ull * numTriangles + 1);
+    uint64_t words = 0;
+    crx_cmesh meta;
+    CRX_TRY(crx_compress_mesh(positions, numVertices, indices, numTriangles, q.data(), packed.data(), &words, &meta));
+    CRX_HIP(hipSetDevice(ctx->device));
+    crx_mesh* m = new crx_mesh();
+    m->ctx = ctx;
+    m->owned = true;
+    m->numVertices = numVertices;
+    m->numTriangles = numTriangles;
+    m->seq = ++ctx->meshSeq;
+    m->aabb = meta.aabb;
+    m->compressed = true;
+    m->bitsPerIndex = meta.bitsPerIndex;
+    m->indexMin = meta.index_min;
+    m->indexMax = meta.index_max;
+    m->compressionFactor = meta.compressionFactor;
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> crx_status {
+        if (!src || bytes == 0) return CRX_OK;
+        hipError_t e = hipMalloc(dst, bytes);
+        if (e != hipSuccess) { *dst = nullptr; return fail(CRX_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+        e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        return CRX_OK;
+    };
+    crx_status st = upload((void**)&m->positions, q.data(), 6ull * numVertices);
+    if (st == CRX_OK) st = upload((void**)&m->indices, packed.data(), 4ull * std::max<uint64_t>(words, 1));
+    if (st == CRX_OK && colors) st = upload((void**)&m->colors, colors, 4ull * numVertices);
+    if (st != CRX_OK) {
+        crx_mesh_destroy(m);
+        return st;
     }
     *out = m;
     return CRX_OK;

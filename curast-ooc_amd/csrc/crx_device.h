@@ -19,13 +19,18 @@ namespace crx {
 // over instCount consecutive instances.  Non-instanced draws: one unit per
 // instance (instCount == 1).
 struct DrawItem {
-    const uint32_t* indices;    // nullptr = non-indexed
-    const float* positions;     // xyz, 12-byte stride
+    const uint32_t* indices;    // nullptr = non-indexed; IX_PU16: bit-packed (BitEdit layout)
+    const float* positions;     // xyz, 12-byte stride; IX_PU16: 3 x uint16, 6-byte stride
     uint64_t cumm;              // cummulativeTriangleCount of the first instance
     uint32_t numTriangles;
     uint32_t instOffset;        // first instance (index into transforms / instances)
     uint32_t instCount;
     uint32_t firstBatch;        // exclusive prefix sum of batches over items
+    // IX_PU16 (CMesh::bitsPerIndex, index_min, compressionFactor, aabb.min)
+    uint32_t bitsPerIndex;
+    uint32_t indexMin;
+    float cf[3];
+    float amin[3];
 };
 
 // Per-instance record for stage 2, stage 3 and resolve.
@@ -41,6 +46,13 @@ struct InstanceRec {
     int32_t id;
     uint32_t isLoaded;
     float world[16];
+    // IX_PU16 (CMesh::compressed, bitsPerIndex, index_min, compressionFactor, aabb)
+    uint32_t compressed;
+    uint32_t bitsPerIndex;
+    uint32_t indexMin;
+    float cf[3];
+    float amin[3];
+    float amax[3];
 };
 
 // world-to-view transform of one instance plus its mirror flag

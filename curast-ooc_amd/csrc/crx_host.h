@@ -132,14 +132,18 @@ struct crx_context {
 
 struct crx_mesh {
     crx_context* ctx = nullptr;
-    float* positions = nullptr;
-    uint32_t* indices = nullptr;
+    float* positions = nullptr;       // IX_PU16: 3 x uint16 per vertex
+    uint32_t* indices = nullptr;      // IX_PU16: bit-packed
     uint32_t* colors = nullptr;
     bool owned = false;
     uint32_t numVertices = 0;
     uint32_t numTriangles = 0;
     crx_box3 aabb{};
     uint64_t seq = 0;
+    // IX_PU16 description (crx_compress_mesh)
+    bool compressed = false;
+    uint32_t bitsPerIndex = 0, indexMin = 0, indexMax = 0;
+    crx_vec3 compressionFactor{};
 };
 
 struct crx_scene {

@@ -303,9 +303,25 @@ struct S1Batch {
     uint32_t i0, i1, i2; // its vertex indices
 };
 
+// BitEdit::readU32 (BitEdit.h:8-26): `bits` bits (1..32) at bit `off`, the
+// second word read only when the value straddles it.
+__device__ __forceinline__ uint32_t bit_read(const uint32_t* data, uint64_t off, uint32_t bits) {
+    if (bits == 0u || bits > 32u) return 0u;
+    const auto d = kglobal(data);
+    const uint64_t w = off >> 5;
+    const uint32_t sh = (uint32_t)off & 31u;
+    uint64_t combined = d[w];
+    if (sh + bits > 32u) combined |= (uint64_t)d[w + 1] << 32;
+    const uint32_t mask = bits == 32u ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    return (uint32_t)(combined >> sh) & mask;
+}
+
 // Locate `batch`'s item (advancing from `cur` when possible) and issue the
 // index loads of this lane's triangle.  Lanes past the item's last triangle
 // load the last triangle instead (a valid address) and are masked later.
+// PU16: the index buffer is bit-packed (getVertex<INDEXBUFFER, IX_PU16>,
+// triangles_visbuffer.cu:48-49).
+template <bool PU16>
 __device__ __forceinline__ void s1_fetch(const RasterParams& P, uint32_t batch, S1Batch& b, bool haveItem) {
     if (!haveItem || batch >= b.itemEnd) {
         b.item = find_item(P, batch);
@@ -315,7 +331,13 @@ __device__ __forceinline__ void s1_fetch(const RasterParams& P, uint32_t batch, 
     }
     b.tri = ((batch - b.it.firstBatch) << P.batchShift) + threadIdx.x;
     const uint32_t t = min(b.tri, b.it.numTriangles - 1u);
-    if (b.it.indices) {
+    if (PU16) {
+        const uint32_t bits = b.it.bitsPerIndex;
+        const uint64_t off = (uint64_t)bits * (3ull * t);
+        b.i0 = bit_read(b.it.indices, off, bits) + b.it.indexMin;
+        b.i1 = bit_read(b.it.indices, off + bits, bits) + b.it.indexMin;
+        b.i2 = bit_read(b.it.indices, off + 2u * bits, bits) + b.it.indexMin;
+    } else if (b.it.indices) {
         const auto ip = kglobal(b.it.indices) + 3ull * t;
         b.i0 = ip[0];
         b.i1 = ip[1];
@@ -332,6 +354,19 @@ __device__ __forceinline__ f3 load_pos_g(const float* positions, uint32_t i) {
     return f3{p[0], p[1], p[2]};
 }
 
+// IX_PU16 position: float(X) * compressionFactor + aabb.min per axis
+// (triangles_visbuffer.cu:51-59; product then sum, no contraction).
+__device__ __forceinline__ f3 load_pos_pu16(const float* positions, uint32_t i, const float* cf, const float* amin) {
+    const auto q = kglobal(reinterpret_cast<const uint16_t*>(positions)) + 3ull * i;
+    return f3{(float)q[0] * cf[0] + amin[0], (float)q[1] * cf[1] + amin[1], (float)q[2] * cf[2] + amin[2]};
+}
+
+template <bool PU16>
+__device__ __forceinline__ f3 s1_vertex(const S1Batch& b, uint32_t i) {
+    if (PU16) return load_pos_pu16(b.it.positions, i, b.it.cf, b.it.amin);
+    return load_pos_g(b.it.positions, i);
+}
+
 }  // namespace
 
 // Stage 1.  A batch is BLOCK consecutive triangles of one draw item, lane t of
@@ -344,7 +379,7 @@ __device__ __forceinline__ f3 load_pos_g(const float* positions, uint32_t i) {
 // (workgroup b is dispatched to XCD b % 8, so blocks b, b+8, ... split the
 // share), keeping the vertices neighbouring batches share in one L2; 1 is a
 // plain grid-stride loop; 2 gives each workgroup one contiguous range.
-template <int BLOCK>
+template <int BLOCK, bool PU16>
 __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
     const uint32_t numBatches = P.numBatches;
     const uint32_t G = gridDim.x;
@@ -368,17 +403,17 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
     const auto transforms = kconst(P.view0.transforms);
 
     S1Batch cur;
-    s1_fetch(P, begin, cur, false);
+    s1_fetch<PU16>(P, begin, cur, false);
     for (uint32_t batch = begin;;) {
         // vertex loads of this batch
-        const f3 A = load_pos_g(cur.it.positions, cur.i0);
-        const f3 B = load_pos_g(cur.it.positions, cur.i1);
-        const f3 C = load_pos_g(cur.it.positions, cur.i2);
+        const f3 A = s1_vertex<PU16>(cur, cur.i0);
+        const f3 B = s1_vertex<PU16>(cur, cur.i1);
+        const f3 C = s1_vertex<PU16>(cur, cur.i2);
         // index loads of the next batch
         const uint32_t next = batch + step;
         const bool more = next < end;
         S1Batch nxt = cur;
-        if (more) s1_fetch(P, next, nxt, true);
+        if (more) s1_fetch<PU16>(P, next, nxt, true);
 
         if (cur.tri < cur.it.numTriangles) {
             const uint32_t tri = cur.tri;
@@ -543,7 +578,7 @@ __device__ __forceinline__ ViewCull lane_view_cull(const RasterParams& P, uint32
 // draw), and the wave runs the exact per-triangle code of the single-view
 // stage 1 for the views left in the ballot.  Queue entries carry the view in
 // the top 8 bits of their instance field.
-template <int BLOCK>
+template <int BLOCK, bool PU16>
 __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
     const uint32_t numBatches = P.numBatches;
     const uint32_t G = gridDim.x;
@@ -571,15 +606,15 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
     uint32_t cullInst = 0xffffffffu;
 
     S1Batch cur;
-    s1_fetch(P, begin, cur, false);
+    s1_fetch<PU16>(P, begin, cur, false);
     for (uint32_t batch = begin;;) {
-        const f3 A = load_pos_g(cur.it.positions, cur.i0);
-        const f3 B = load_pos_g(cur.it.positions, cur.i1);
-        const f3 C = load_pos_g(cur.it.positions, cur.i2);
+        const f3 A = s1_vertex<PU16>(cur, cur.i0);
+        const f3 B = s1_vertex<PU16>(cur, cur.i1);
+        const f3 C = s1_vertex<PU16>(cur, cur.i2);
         const uint32_t next = batch + step;
         const bool more = next < end;
         S1Batch nxt = cur;
-        if (more) s1_fetch(P, next, nxt, true);
+        if (more) s1_fetch<PU16>(P, next, nxt, true);
 
         // object-space box of the wave's triangles (lanes past the item's end
         // hold its last triangle, which is inside the mesh anyway), reduced on
@@ -765,6 +800,41 @@ __device__ __forceinline__ uint32_t entry_view(const RasterParams& P, uint32_t f
     return P.numViews == 1u ? 0u : (field >> kViewShift);
 }
 
+// Vertex indices of triangle `tri` of an instance (plain, non-indexed or
+// IX_PU16 bit-packed).
+__device__ __forceinline__ void triangle_indices(const InstanceRec& mesh, uint32_t tri, uint32_t& i0, uint32_t& i1,
+                                                 uint32_t& i2) {
+    if (mesh.compressed) {
+        const uint32_t bits = mesh.bitsPerIndex;
+        const uint64_t off = (uint64_t)bits * (3ull * tri);
+        i0 = bit_read(mesh.indices, off, bits) + mesh.indexMin;
+        i1 = bit_read(mesh.indices, off + bits, bits) + mesh.indexMin;
+        i2 = bit_read(mesh.indices, off + 2u * bits, bits) + mesh.indexMin;
+    } else if (mesh.indices) {
+        i0 = mesh.indices[3u * tri + 0u];
+        i1 = mesh.indices[3u * tri + 1u];
+        i2 = mesh.indices[3u * tri + 2u];
+    } else {
+        i0 = 3u * tri; i1 = i0 + 1u; i2 = i0 + 2u;
+    }
+}
+
+// Object-space vertex as the raster stages decode it (getVertex).
+__device__ __forceinline__ f3 raster_vertex(const InstanceRec& mesh, uint32_t i) {
+    if (mesh.compressed) return load_pos_pu16(mesh.positions, i, mesh.cf, mesh.amin);
+    return load_pos(mesh.positions, i);
+}
+
+// Object-space vertex as the resolve decodes it (getVertex_resolved,
+// resolve.cu:69-89): (X / 65536) * (aabb.max - aabb.min) + aabb.min.
+__device__ __forceinline__ f3 resolve_vertex(const InstanceRec& mesh, uint32_t i) {
+    if (!mesh.compressed) return load_pos(mesh.positions, i);
+    const uint16_t* q = reinterpret_cast<const uint16_t*>(mesh.positions) + 3ull * i;
+    return f3{((float)q[0] / 65536.0f) * (mesh.amax[0] - mesh.amin[0]) + mesh.amin[0],
+              ((float)q[1] / 65536.0f) * (mesh.amax[1] - mesh.amin[1]) + mesh.amin[1],
+              ((float)q[2] / 65536.0f) * (mesh.amax[2] - mesh.amin[2]) + mesh.amin[2]};
+}
+
 }  // namespace
 
 // Stage 2.  A persistent grid of waves; each wave claims up to 64 queue
@@ -809,16 +879,10 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
             const ViewParams& V = P.views[entry_view(P, meshIndex)];
             const float* m = V.transforms[mesh.transformIndex].m;
             uint32_t i0, i1, i2;
-            if (mesh.indices) {
-                i0 = mesh.indices[3u * triangleIndex + 0u];
-                i1 = mesh.indices[3u * triangleIndex + 1u];
-                i2 = mesh.indices[3u * triangleIndex + 2u];
-            } else {
-                i0 = 3u * triangleIndex; i1 = i0 + 1u; i2 = i0 + 2u;
-            }
-            const f3 av = xform_point(m, load_pos(mesh.positions, i0));
-            const f3 bv = xform_point(m, load_pos(mesh.positions, i1));
-            const f3 cv = xform_point(m, load_pos(mesh.positions, i2));
+            triangle_indices(mesh, triangleIndex, i0, i1, i2);
+            const f3 av = xform_point(m, raster_vertex(mesh, i0));
+            const f3 bv = xform_point(m, raster_vertex(mesh, i1));
+            const f3 cv = xform_point(m, raster_vertex(mesh, i2));
             const float f = V.f, aspect = V.aspect;
             const float da = -av.z, db = -bv.z, dc = -cv.z;
             const f2 an = view_to_ndc_xy(av, f, aspect);
@@ -991,16 +1055,10 @@ __global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
         const float* m = V.transforms[mesh.transformIndex].m;
         const uint32_t tri = (uint32_t)e.y;
         uint32_t i0, i1, i2;
-        if (mesh.indices) {
-            i0 = mesh.indices[3u * tri + 0u];
-            i1 = mesh.indices[3u * tri + 1u];
-            i2 = mesh.indices[3u * tri + 2u];
-        } else {
-            i0 = 3u * tri; i1 = i0 + 1u; i2 = i0 + 2u;
-        }
-        const f3 av = xform_point(m, load_pos(mesh.positions, i0));
-        const f3 bv = xform_point(m, load_pos(mesh.positions, i1));
-        const f3 cv = xform_point(m, load_pos(mesh.positions, i2));
+        triangle_indices(mesh, tri, i0, i1, i2);
+        const f3 av = xform_point(m, raster_vertex(mesh, i0));
+        const f3 bv = xform_point(m, raster_vertex(mesh, i1));
+        const f3 cv = xform_point(m, raster_vertex(mesh, i2));
         MTTri T;
         T.e1 = sub3(bv, av);
         T.e2 = sub3(cv, av);
@@ -1075,13 +1133,7 @@ __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int
             color = 0xffff00ffu;
         } else {
             uint32_t i0, i1, i2;
-            if (mesh.indices) {
-                i0 = mesh.indices[3u * tri + 0u];
-                i1 = mesh.indices[3u * tri + 1u];
-                i2 = mesh.indices[3u * tri + 2u];
-            } else {
-                i0 = 3u * tri; i1 = i0 + 1u; i2 = i0 + 2u;
-            }
+            triangle_indices(mesh, tri, i0, i1, i2);   // resolve.cu:744-760
             // ray through the pixel centre (computeRayDir, resolve.cu:662-680)
             const f3 origin{R.origin[0], R.origin[1], R.origin[2]};
             const float u = 2.0f * ((float)x + 0.5f) / R.Wf - 1.0f;
@@ -1089,9 +1141,9 @@ __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int
             const f3 rview = normalize3(f3{R.p00I * u, R.p11I * v, -1.0f});
             const f3 rayDir = normalize3(sub3(xform_point(R.viewI, rview), origin));
 
-            const f3 aw = xform_point(mesh.world, load_pos(mesh.positions, i0));
-            const f3 bw = xform_point(mesh.world, load_pos(mesh.positions, i1));
-            const f3 cw = xform_point(mesh.world, load_pos(mesh.positions, i2));
+            const f3 aw = xform_point(mesh.world, resolve_vertex(mesh, i0));
+            const f3 bw = xform_point(mesh.world, resolve_vertex(mesh, i1));
+            const f3 cw = xform_point(mesh.world, resolve_vertex(mesh, i2));
             const f3 v0 = sub3(bw, aw);
             const f3 v1 = sub3(cw, aw);
             float t;
@@ -1247,21 +1299,27 @@ hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
-    switch (block) {
-        case 128: hipLaunchKernelGGL(k_stage1<128>, dim3(grid), dim3(128), 0, stream, P); break;
-        case 256: hipLaunchKernelGGL(k_stage1<256>, dim3(grid), dim3(256), 0, stream, P); break;
-        case 512: hipLaunchKernelGGL(k_stage1<512>, dim3(grid), dim3(512), 0, stream, P); break;
+hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream) {
+    switch (block * 2u + (pu16 ? 1u : 0u)) {
+        case 256: hipLaunchKernelGGL((k_stage1<128, false>), dim3(grid), dim3(128), 0, stream, P); break;
+        case 257: hipLaunchKernelGGL((k_stage1<128, true>), dim3(grid), dim3(128), 0, stream, P); break;
+        case 512: hipLaunchKernelGGL((k_stage1<256, false>), dim3(grid), dim3(256), 0, stream, P); break;
+        case 513: hipLaunchKernelGGL((k_stage1<256, true>), dim3(grid), dim3(256), 0, stream, P); break;
+        case 1024: hipLaunchKernelGGL((k_stage1<512, false>), dim3(grid), dim3(512), 0, stream, P); break;
+        case 1025: hipLaunchKernelGGL((k_stage1<512, true>), dim3(grid), dim3(512), 0, stream, P); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
-    switch (block) {
-        case 128: hipLaunchKernelGGL(k_stage1_views<128>, dim3(grid), dim3(128), 0, stream, P); break;
-        case 256: hipLaunchKernelGGL(k_stage1_views<256>, dim3(grid), dim3(256), 0, stream, P); break;
-        case 512: hipLaunchKernelGGL(k_stage1_views<512>, dim3(grid), dim3(512), 0, stream, P); break;
+hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream) {
+    switch (block * 2u + (pu16 ? 1u : 0u)) {
+        case 256: hipLaunchKernelGGL((k_stage1_views<128, false>), dim3(grid), dim3(128), 0, stream, P); break;
+        case 257: hipLaunchKernelGGL((k_stage1_views<128, true>), dim3(grid), dim3(128), 0, stream, P); break;
+        case 512: hipLaunchKernelGGL((k_stage1_views<256, false>), dim3(grid), dim3(256), 0, stream, P); break;
+        case 513: hipLaunchKernelGGL((k_stage1_views<256, true>), dim3(grid), dim3(256), 0, stream, P); break;
+        case 1024: hipLaunchKernelGGL((k_stage1_views<512, false>), dim3(grid), dim3(512), 0, stream, P); break;
+        case 1025: hipLaunchKernelGGL((k_stage1_views<512, true>), dim3(grid), dim3(512), 0, stream, P); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

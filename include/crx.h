@@ -43,7 +43,12 @@
  *     (triangles_visbuffer.cu:548-549 and :694-731);
  *   - queues are bounded; on overflow the draw grows its queues and redraws the
  *     frame, so no triangle is ever dropped (triangles_visbuffer.cu:192-197, :503-511);
- *   - a mesh may hold at most 1431655765 triangles (3*T must fit in uint32).
+ *   - a mesh may hold at most 1431655765 triangles (3*T must fit in uint32);
+ *   - IX_PU16: a draw's meshes are all compressed or all uncompressed (the
+ *     reference picks the kernel by meshes[0] and would misread the others),
+ *     compressed meshes need an index buffer, and the resolve takes the bit
+ *     width from CMesh::bitsPerIndex (the reference recomputes it with the
+ *     same formula the host used, CuRast_render.h:358-359).
  * Kept quirks: stage 1 never samples column W-1 or row H-1; the instanced
  * stage-1 triangle ID adds (instance*numTriangles) mod 2^32; stage 2 forms its
  * triangle ID in 32-bit arithmetic; stage 1 uses strict edge tests, stage 2
@@ -99,13 +104,13 @@ typedef struct crx_texture {
  * device pointers.  Only the fields named in the comments are read. */
 typedef struct crx_cmesh {
     uint32_t numTriangles;              /* read */
-    uint32_t* indices;                  /* read; NULL = non-indexed (vertex 3t+k) */
-    crx_vec3* positions;                /* read; xyz float, 12-byte stride */
+    uint32_t* indices;                  /* read; NULL = non-indexed (vertex 3t+k); IX_PU16: bit-packed */
+    crx_vec3* positions;                /* read; xyz float, 12-byte stride; IX_PU16: 3 x uint16 */
     uint64_t cummulativeTriangleCount;  /* read: ID of triangle 0 */
-    crx_box3 aabb;                      /* read by the host frustum cull */
-    crx_vec3 compressionFactor;
-    uint32_t index_min;
-    uint32_t bitsPerIndex;
+    crx_box3 aabb;                      /* read by the host frustum cull; IX_PU16 decode */
+    crx_vec3 compressionFactor;         /* read (IX_PU16) */
+    uint32_t index_min;                 /* read (IX_PU16) */
+    uint32_t bitsPerIndex;              /* read (IX_PU16), 0..32 */
     struct { int32_t offset; int32_t count; } instances;  /* read */
     crx_mat4 world;                     /* read by resolve */
     int32_t id;                         /* read by resolve (MESH_ID) */
@@ -119,7 +124,7 @@ typedef struct crx_cmesh {
     crx_texture texture;
     uint8_t isLoaded;                   /* read by resolve */
     uint8_t flipTriangles;
-    uint8_t compressed;                 /* must be 0: the IX_PU16 variant is not built */
+    uint8_t compressed;                 /* read: 1 = IX_PU16 (crx_compress_mesh) */
     struct { crx_vec3* positions; uint32_t* colors; uint32_t numPoints; } impostor;
 } crx_cmesh;
 
@@ -337,6 +342,28 @@ typedef struct crx_scene crx_scene;
 crx_status crx_mesh_create(crx_context* ctx, const float* positions, uint32_t numVertices,
                            const uint32_t* indices, uint32_t numTriangles,
                            const uint32_t* colors, crx_mesh** out);
+/* IX_PU16 compression of an indexed mesh, as CuRast's loader does it
+ * (LargeGlbLoader.h:482-504 for indices, :640-680 for positions):
+ * indices become index - index_min packed at bitsPerIndex =
+ * ceil(log2f(index_max - index_min + 1)) bits each (BitEdit layout, word
+ * count rounded up to whole 32-bit words); each position becomes 3 x uint16
+ * clamp(65536 * (p - aabb.min) / (aabb.max - aabb.min), 0, 65535) evaluated in
+ * double (0 on a flat axis).  qpositions holds 3*numVertices uint16,
+ * packedIndices at least 3*numTriangles + 1 words; *packedWords receives the
+ * words used.  meta receives numTriangles, numVertices, aabb,
+ * compressionFactor = (aabb.max - aabb.min) / 65536, index_min, index_max,
+ * bitsPerIndex and compressed = 1 (CuRast_render.h:358-378); its pointers are
+ * left to the caller. */
+crx_status crx_compress_mesh(const float* positions, uint32_t numVertices, const uint32_t* indices,
+                             uint32_t numTriangles, uint16_t* qpositions, uint32_t* packedIndices,
+                             uint64_t* packedWords, crx_cmesh* meta);
+
+/* Mesh stored IX_PU16-compressed in device memory (crx_compress_mesh of the
+ * host arrays, then upload); draws through the _compressed path. */
+crx_status crx_mesh_create_compressed(crx_context* ctx, const float* positions, uint32_t numVertices,
+                                      const uint32_t* indices, uint32_t numTriangles, const uint32_t* colors,
+                                      crx_mesh** out);
+
 /* Same, but wraps caller-owned device buffers (not freed by the mesh). */
 crx_status crx_mesh_create_device(crx_context* ctx, const float* d_positions, uint32_t numVertices,
                                   const uint32_t* d_indices, uint32_t numTriangles,

@@ -215,11 +215,34 @@ void bitedit_write(uint32_t* data, uint64_t bitOffset, uint64_t bitCount, uint32
     }
 }
 
-// getVertex<INDEXBUFFER, UNCOMPRESSED>, triangles_visbuffer.cu:37-47
+// getVertex<INDEXBUFFER, UNCOMPRESSED>, triangles_visbuffer.cu:37-47, and
+// getVertex<INDEXBUFFER, IX_PU16>, :48-62: bit-packed index + index_min,
+// 3 x uint16 position scaled by compressionFactor plus aabb.min (a product
+// then a sum: the contract has no contraction).
 inline V4 getVertex(const crx_cmesh& mesh, uint32_t vertexIndex) {
+    if (mesh.compressed) {
+        const uint32_t resolvedIndex =
+            bitedit_read(mesh.indices, (uint64_t)mesh.bitsPerIndex * vertexIndex, mesh.bitsPerIndex) + mesh.index_min;
+        const uint16_t* q = reinterpret_cast<const uint16_t*>(mesh.positions) + 3ull * resolvedIndex;
+        return V4{(float)q[0] * mesh.compressionFactor.x + mesh.aabb.min.x,
+                  (float)q[1] * mesh.compressionFactor.y + mesh.aabb.min.y,
+                  (float)q[2] * mesh.compressionFactor.z + mesh.aabb.min.z, 1.0f};
+    }
     uint32_t resolvedIndex = mesh.indices ? mesh.indices[vertexIndex] : vertexIndex;
     const crx_vec3 p = mesh.positions[resolvedIndex];
     return V4{p.x, p.y, p.z, 1.0f};
+}
+
+// getVertex_resolved, resolve.cu:69-89: the resolve decodes compressed
+// positions with its own formula, (X / 65536) * (aabb.max - aabb.min) + aabb.min.
+inline crx_vec3 getVertex_resolved(const crx_cmesh& mesh, uint32_t resolvedIndex) {
+    if (!mesh.compressed) return mesh.positions[resolvedIndex];
+    const uint16_t* q = reinterpret_cast<const uint16_t*>(mesh.positions) + 3ull * resolvedIndex;
+    const float sx = mesh.aabb.max.x - mesh.aabb.min.x;
+    const float sy = mesh.aabb.max.y - mesh.aabb.min.y;
+    const float sz = mesh.aabb.max.z - mesh.aabb.min.z;
+    return crx_vec3{((float)q[0] / 65536.0f) * sx + mesh.aabb.min.x, ((float)q[1] / 65536.0f) * sy + mesh.aabb.min.y,
+                    ((float)q[2] / 65536.0f) * sz + mesh.aabb.min.z};
 }
 
 struct Target {
@@ -845,7 +868,14 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
                     color = 0xffff00ff;
                 } else {
                     uint32_t i0, i1, i2;
-                    if (mesh.indices) {
+                    if (mesh.compressed) {
+                        // resolve.cu:755-760 (bit width taken from the CMesh: the
+                        // reference recomputes it with the formula the host used)
+                        const uint64_t b = mesh.bitsPerIndex;
+                        i0 = bitedit_read(mesh.indices, b * (3ull * triangleIndex + 0), b) + mesh.index_min;
+                        i1 = bitedit_read(mesh.indices, b * (3ull * triangleIndex + 1), b) + mesh.index_min;
+                        i2 = bitedit_read(mesh.indices, b * (3ull * triangleIndex + 2), b) + mesh.index_min;
+                    } else if (mesh.indices) {
                         i0 = mesh.indices[3 * triangleIndex + 0];
                         i1 = mesh.indices[3 * triangleIndex + 1];
                         i2 = mesh.indices[3 * triangleIndex + 2];
@@ -854,7 +884,8 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
                         i1 = 3 * triangleIndex + 1;
                         i2 = 3 * triangleIndex + 2;
                     }
-                    crx_vec3 pa = mesh.positions[i0], pb = mesh.positions[i1], pc = mesh.positions[i2];
+                    crx_vec3 pa = getVertex_resolved(mesh, i0), pb = getVertex_resolved(mesh, i1),
+                             pc = getVertex_resolved(mesh, i2);
                     V3 a_world = xyz(mulmv(mesh.world, V4{pa.x, pa.y, pa.z, 1.0f}));
                     V3 b_world = xyz(mulmv(mesh.world, V4{pb.x, pb.y, pb.z, 1.0f}));
                     V3 c_world = xyz(mulmv(mesh.world, V4{pc.x, pc.y, pc.z, 1.0f}));

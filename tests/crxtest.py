@@ -11,6 +11,36 @@ import crx
 import oracle as orc
 
 
+class CompressedMesh:
+    """IX_PU16 copy of a mesh: 3 x uint16 positions, bit-packed indices and
+    the CMesh compression fields (crx_compress_mesh)."""
+
+    def __init__(self, positions, indices, colors=None):
+        self.positions, self.indices, self.meta = crx.compress_mesh(positions, indices)
+        self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+        self.num_triangles = len(np.asarray(indices).reshape(-1, 3))
+
+    def decoded_positions(self):
+        """Float positions as the raster stages decode them: X * factor + min
+        per axis in float32, a product then a sum."""
+        cf = np.array([self.meta.compressionFactor.x, self.meta.compressionFactor.y,
+                       self.meta.compressionFactor.z], dtype=np.float32)
+        lo = np.array([self.meta.aabb.min.x, self.meta.aabb.min.y, self.meta.aabb.min.z], dtype=np.float32)
+        return (self.positions.astype(np.float32) * cf) + lo
+
+
+def _apply_compression(m, mesh):
+    meta = getattr(mesh, "meta", None)
+    if meta is None:
+        return
+    m.compressed = 1
+    m.bitsPerIndex = meta.bitsPerIndex
+    m.index_min = meta.index_min
+    m.index_max = meta.index_max
+    m.compressionFactor = meta.compressionFactor
+    m.aabb = meta.aabb
+
+
 @dataclass
 class SceneSpec:
     """Meshes and, per mesh, the world matrices of its instances."""
@@ -52,6 +82,7 @@ def prepare(spec: SceneSpec, view: crx.Mat4, pointers, instanced_ids=True) -> Fr
             m.id = len(instances)
             m.numVertices = len(mesh.positions)
             m.isLoaded = 1
+            _apply_compression(m, mesh)
             prefix.append(total)
             total += mesh.num_triangles
             instances.append(m)
