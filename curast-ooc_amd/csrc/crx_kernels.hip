@@ -24,6 +24,26 @@ namespace {
 
 constexpr int kWave = 64;
 
+// Loads through the constant address space are scalar (s_load) when the index
+// is uniform; the draw records are read-only for the whole kernel.  Loads
+// through the global address space avoid flat instructions.  (The host pass
+// of the compiler sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* kconst(const T* p) {
+    return (const __attribute__((address_space(4))) T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* kglobal(const T* p) {
+    return (const __attribute__((address_space(1))) T*)(p);
+}
+#else
+template <typename T>
+__device__ __forceinline__ const T* kconst(const T* p) { return p; }
+template <typename T>
+__device__ __forceinline__ const T* kglobal(const T* p) { return p; }
+#endif
+
 // Orders LDS traffic between the lanes of one wave (no hardware barrier: the
 // waves of a workgroup run independent loops here).
 __device__ __forceinline__ void wave_sync() {
@@ -37,7 +57,7 @@ __device__ __forceinline__ void atomic_min_fb(uint64_t* fb, uint32_t pixel, uint
 }
 
 __device__ __forceinline__ f3 load_pos(const float* __restrict__ positions, uint32_t i) {
-    const float* p = positions + 3ull * i;
+    const auto p = kglobal(positions) + 3ull * i;
     return f3{p[0], p[1], p[2]};
 }
 
@@ -341,25 +361,6 @@ __global__ __launch_bounds__(64) void k_reset_counters(uint32_t* __restrict__ ct
 
 namespace {
 
-// Loads through the constant address space are scalar (s_load) when the index
-// is uniform; the draw records are read-only for the whole kernel.  Loads
-// through the global address space avoid flat instructions.  (The host pass
-// of the compiler sees plain pointers.)
-#if defined(__HIP_DEVICE_COMPILE__)
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(4))) T* kconst(const T* p) {
-    return (const __attribute__((address_space(4))) T*)(p);
-}
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* kglobal(const T* p) {
-    return (const __attribute__((address_space(1))) T*)(p);
-}
-#else
-template <typename T>
-__device__ __forceinline__ const T* kconst(const T* p) { return p; }
-template <typename T>
-__device__ __forceinline__ const T* kglobal(const T* p) { return p; }
-#endif
 
 // last draw item whose firstBatch <= batch
 __device__ __forceinline__ uint32_t find_item(const RasterParams& P, uint32_t batch) {
@@ -889,9 +890,10 @@ __device__ __forceinline__ void triangle_indices(const InstanceRec& mesh, uint32
         i1 = bit_read(mesh.indices, off + bits, bits) + mesh.indexMin;
         i2 = bit_read(mesh.indices, off + 2u * bits, bits) + mesh.indexMin;
     } else if (mesh.indices) {
-        i0 = mesh.indices[3u * tri + 0u];
-        i1 = mesh.indices[3u * tri + 1u];
-        i2 = mesh.indices[3u * tri + 2u];
+        const auto ip = kglobal(mesh.indices) + 3ull * tri;
+        i0 = ip[0];
+        i1 = ip[1];
+        i2 = ip[2];
     } else {
         i0 = 3u * tri; i1 = i0 + 1u; i2 = i0 + 2u;
     }
@@ -907,7 +909,7 @@ __device__ __forceinline__ f3 raster_vertex(const InstanceRec& mesh, uint32_t i)
 // resolve.cu:69-89): (X / 65536) * (aabb.max - aabb.min) + aabb.min.
 __device__ __forceinline__ f3 resolve_vertex(const InstanceRec& mesh, uint32_t i) {
     if (!mesh.compressed) return load_pos(mesh.positions, i);
-    const uint16_t* q = reinterpret_cast<const uint16_t*>(mesh.positions) + 3ull * i;
+    const auto q = kglobal(reinterpret_cast<const uint16_t*>(mesh.positions)) + 3ull * i;
     return f3{((float)q[0] / 65536.0f) * (mesh.amax[0] - mesh.amin[0]) + mesh.amin[0],
               ((float)q[1] / 65536.0f) * (mesh.amax[1] - mesh.amin[1]) + mesh.amin[1],
               ((float)q[2] / 65536.0f) * (mesh.amax[2] - mesh.amin[2]) + mesh.amin[2]};
@@ -1178,12 +1180,112 @@ __device__ __forceinline__ float intersect_plane(f3 origin, f3 dir, f3 normal, f
     return (t >= 0.0f) ? t : __builtin_inff();
 }
 
+// Colour of a covered pixel whose triangle id `total` lies in `mesh`
+// (resolve.cu:736-1035).  `mesh` is a register copy of the instance record.
+__device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const InstanceRec& mesh, uint64_t total,
+                                                int x, int y) {
+    uint32_t color = 0;
+    uint32_t tri = (uint32_t)(total - mesh.cumm);
+    tri = min(tri, mesh.numTriangles - 1u);
+    // an empty mesh has no triangle to fetch: shade it like an unloaded one
+    if (!mesh.isLoaded || mesh.numTriangles == 0) {
+        color = 0xffff00ffu;
+    } else {
+        uint32_t i0, i1, i2;
+        triangle_indices(mesh, tri, i0, i1, i2);   // resolve.cu:744-760
+        // ray through the pixel centre (computeRayDir, resolve.cu:662-680)
+        const f3 origin{R.origin[0], R.origin[1], R.origin[2]};
+        const float u = 2.0f * ((float)x + 0.5f) / R.Wf - 1.0f;
+        const float v = 2.0f * ((float)y + 0.5f) / R.Hf - 1.0f;
+        const f3 rview = normalize3(f3{R.p00I * u, R.p11I * v, -1.0f});
+        const f3 rayDir = normalize3(sub3(xform_point(R.viewI, rview), origin));
+
+        const f3 aw = xform_point(mesh.world, resolve_vertex(mesh, i0));
+        const f3 bw = xform_point(mesh.world, resolve_vertex(mesh, i1));
+        const f3 cw = xform_point(mesh.world, resolve_vertex(mesh, i2));
+        const f3 v0 = sub3(bw, aw);
+        const f3 v1 = sub3(cw, aw);
+        float t;
+        {   // resolve.cu:790-801
+            const f3 normal = normalize3(cross3(v0, v1));
+            const float d = dot3(aw, normal);
+            t = intersect_plane(origin, rayDir, normal, -d);
+        }
+        float stx, sty;
+        {   // resolve.cu:805-834
+            const float d00 = dot3(v0, v0);
+            const float d01 = dot3(v0, v1);
+            const float d11 = dot3(v1, v1);
+            const float denom = d00 * d11 - d01 * d01;
+            const float denomI = 1.0f / denom;
+            const f3 p = add3(origin, f3{t * rayDir.x, t * rayDir.y, t * rayDir.z});
+            const f3 v2 = sub3(p, aw);
+            const float d20 = dot3(v2, v0);
+            const float d21 = dot3(v2, v1);
+            stx = (d11 * d20 - d01 * d21) * denomI;
+            sty = (d00 * d21 - d01 * d20) * denomI;
+        }
+        const float w = 1.0f - stx - sty;
+        color = 0xffff00ffu;
+        const int attr = R.attribute;
+        if (attr == 0) {                       // NONE
+            color = 0xffffffffu;
+        } else if (attr == 5) {                // TRIANGLE_ID
+            color = tri * 12345678u;
+        } else if (attr == 6) {                // MESH_ID
+            color = ((uint32_t)mesh.id + 1u) * 12345678u;
+        } else if (attr == 2 && mesh.uvs) {    // UV
+            const float* ua = mesh.uvs + 2ull * i0;
+            const float* ub = mesh.uvs + 2ull * i1;
+            const float* uc = mesh.uvs + 2ull * i2;
+            float ux = (ua[0] * w + ub[0] * stx) + uc[0] * sty;
+            float uy = (ua[1] * w + ub[1] * stx) + uc[1] * sty;
+            ux = ux - floorf(ux);
+            uy = uy - floorf(uy);
+            const uint32_t r = f2u8(fmodf(ux - floorf(ux), 1.0f) * 256.0f);
+            const uint32_t g = f2u8(fmodf(uy - floorf(uy), 1.0f) * 256.0f);
+            color = (color & 0xff000000u) | r | (g << 8);
+        } else if (attr == 3) {                // NORMAL
+            f3 N;
+            if (mesh.normals) {
+                const f3 n0 = xform(mesh.world, load_pos(mesh.normals, i0), 0.0f);
+                const f3 n1 = xform(mesh.world, load_pos(mesh.normals, i1), 0.0f);
+                const f3 n2 = xform(mesh.world, load_pos(mesh.normals, i2), 0.0f);
+                N = normalize3(add3(add3(scale3(n0, w), scale3(n1, stx)), scale3(n2, sty)));
+            } else {
+                N = normalize3(cross3(v0, v1));
+            }
+            const uint32_t r = f2u8(clampf(N.x * 255.0f, 0.0f, 255.0f));
+            const uint32_t g = f2u8(clampf(N.y * 255.0f, 0.0f, 255.0f));
+            const uint32_t b = f2u8(clampf(N.z * 255.0f, 0.0f, 255.0f));
+            color = (color & 0xff000000u) | r | (g << 8) | (b << 16);
+        } else if (attr == 4 && mesh.colors) { // VERTEX_COLORS (resolve.cu:952-973)
+            const auto colors = kglobal(mesh.colors);
+        const uint32_t Ca = colors[i0], Cb = colors[i1], Cc = colors[i2];
+            uint32_t rgb = 0;
+            #pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float ca = (float)((Ca >> (8 * k)) & 0xffu);
+                const float cb = (float)((Cb >> (8 * k)) & 0xffu);
+                const float cc = (float)((Cc >> (8 * k)) & 0xffu);
+                const float ch = (ca * w + cb * stx) + cc * sty;
+                rgb |= f2u8(clampf(ch, 0.0f, 255.0f)) << (8 * k);
+            }
+            color = (Ca & 0xff000000u) | rgb;
+        } else {
+            color = 0xff0000ffu;
+        }
+        color |= 0xff000000u;
+    }
+    return color;
+}
+
 // Resolve of one pixel (x, y) < (W, H).
 __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int y) {
     const uint32_t pixelID = (uint32_t)x + (uint32_t)R.W * (uint32_t)y;
 
-    const uint64_t pixel = R.fb[pixelID];
-    const uint64_t pixel_cb = R.cb[pixelID];
+    const uint64_t pixel = kglobal(R.fb)[pixelID];
+    const uint64_t pixel_cb = kglobal((const uint64_t*)R.cb)[pixelID];
     const float depth = unpack_depth(pixel);
     const uint64_t total = pixel & 0xFFFFFFFFFull;
     const float depth_cb = __builtin_bit_cast(float, (uint32_t)(pixel_cb >> 32));
@@ -1193,107 +1295,26 @@ __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int
     if (!__builtin_isinf(depth) && R.numInstances > 0) {
         // binary search of the instance holding `total` (resolve.cu:697-721)
         const uint32_t n = R.numInstances;
+        const auto prefix = kglobal(R.prefix);
         int left = 0, right = (int)(n - 1u), found = 0;
         while (left <= right) {
             const int mid = left + (right - left) / 2;
-            const int64_t before = (int64_t)R.prefix[mid];
-            const int64_t after = ((uint32_t)mid == n - 1u) ? 0xffffffffffll : (int64_t)R.prefix[mid + 1];
+            const int64_t before = (int64_t)prefix[mid];
+            const int64_t after = ((uint32_t)mid == n - 1u) ? 0xffffffffffll : (int64_t)prefix[mid + 1];
             if (total >= (uint64_t)before && total < (uint64_t)after) { found = mid; break; }
             else if ((uint64_t)after <= total) left = mid + 1;
             else right = mid - 1;
             if (mid == left) break;
         }
-        const InstanceRec& mesh = R.inst[found];
-        uint32_t tri = (uint32_t)(total - mesh.cumm);
-        tri = min(tri, mesh.numTriangles - 1u);
-        // an empty mesh has no triangle to fetch: shade it like an unloaded one
-        if (!mesh.isLoaded || mesh.numTriangles == 0) {
-            color = 0xffff00ffu;
+        // a wave whose covered pixels share one instance reads its record
+        // with scalar loads; otherwise every lane loads its own
+        const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane(found);
+        if (__ballot((uint32_t)found != f0) == 0ull) {
+            const InstanceRec mesh = kconst(R.inst)[f0];
+            color = shade_pixel(R, mesh, total, x, y);
         } else {
-            uint32_t i0, i1, i2;
-            triangle_indices(mesh, tri, i0, i1, i2);   // resolve.cu:744-760
-            // ray through the pixel centre (computeRayDir, resolve.cu:662-680)
-            const f3 origin{R.origin[0], R.origin[1], R.origin[2]};
-            const float u = 2.0f * ((float)x + 0.5f) / R.Wf - 1.0f;
-            const float v = 2.0f * ((float)y + 0.5f) / R.Hf - 1.0f;
-            const f3 rview = normalize3(f3{R.p00I * u, R.p11I * v, -1.0f});
-            const f3 rayDir = normalize3(sub3(xform_point(R.viewI, rview), origin));
-
-            const f3 aw = xform_point(mesh.world, resolve_vertex(mesh, i0));
-            const f3 bw = xform_point(mesh.world, resolve_vertex(mesh, i1));
-            const f3 cw = xform_point(mesh.world, resolve_vertex(mesh, i2));
-            const f3 v0 = sub3(bw, aw);
-            const f3 v1 = sub3(cw, aw);
-            float t;
-            {   // resolve.cu:790-801
-                const f3 normal = normalize3(cross3(v0, v1));
-                const float d = dot3(aw, normal);
-                t = intersect_plane(origin, rayDir, normal, -d);
-            }
-            float stx, sty;
-            {   // resolve.cu:805-834
-                const float d00 = dot3(v0, v0);
-                const float d01 = dot3(v0, v1);
-                const float d11 = dot3(v1, v1);
-                const float denom = d00 * d11 - d01 * d01;
-                const float denomI = 1.0f / denom;
-                const f3 p = add3(origin, f3{t * rayDir.x, t * rayDir.y, t * rayDir.z});
-                const f3 v2 = sub3(p, aw);
-                const float d20 = dot3(v2, v0);
-                const float d21 = dot3(v2, v1);
-                stx = (d11 * d20 - d01 * d21) * denomI;
-                sty = (d00 * d21 - d01 * d20) * denomI;
-            }
-            const float w = 1.0f - stx - sty;
-            color = 0xffff00ffu;
-            const int attr = R.attribute;
-            if (attr == 0) {                       // NONE
-                color = 0xffffffffu;
-            } else if (attr == 5) {                // TRIANGLE_ID
-                color = tri * 12345678u;
-            } else if (attr == 6) {                // MESH_ID
-                color = ((uint32_t)mesh.id + 1u) * 12345678u;
-            } else if (attr == 2 && mesh.uvs) {    // UV
-                const float* ua = mesh.uvs + 2ull * i0;
-                const float* ub = mesh.uvs + 2ull * i1;
-                const float* uc = mesh.uvs + 2ull * i2;
-                float ux = (ua[0] * w + ub[0] * stx) + uc[0] * sty;
-                float uy = (ua[1] * w + ub[1] * stx) + uc[1] * sty;
-                ux = ux - floorf(ux);
-                uy = uy - floorf(uy);
-                const uint32_t r = f2u8(fmodf(ux - floorf(ux), 1.0f) * 256.0f);
-                const uint32_t g = f2u8(fmodf(uy - floorf(uy), 1.0f) * 256.0f);
-                color = (color & 0xff000000u) | r | (g << 8);
-            } else if (attr == 3) {                // NORMAL
-                f3 N;
-                if (mesh.normals) {
-                    const f3 n0 = xform(mesh.world, load_pos(mesh.normals, i0), 0.0f);
-                    const f3 n1 = xform(mesh.world, load_pos(mesh.normals, i1), 0.0f);
-                    const f3 n2 = xform(mesh.world, load_pos(mesh.normals, i2), 0.0f);
-                    N = normalize3(add3(add3(scale3(n0, w), scale3(n1, stx)), scale3(n2, sty)));
-                } else {
-                    N = normalize3(cross3(v0, v1));
-                }
-                const uint32_t r = f2u8(clampf(N.x * 255.0f, 0.0f, 255.0f));
-                const uint32_t g = f2u8(clampf(N.y * 255.0f, 0.0f, 255.0f));
-                const uint32_t b = f2u8(clampf(N.z * 255.0f, 0.0f, 255.0f));
-                color = (color & 0xff000000u) | r | (g << 8) | (b << 16);
-            } else if (attr == 4 && mesh.colors) { // VERTEX_COLORS (resolve.cu:952-973)
-                const uint32_t Ca = mesh.colors[i0], Cb = mesh.colors[i1], Cc = mesh.colors[i2];
-                uint32_t rgb = 0;
-                #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const float ca = (float)((Ca >> (8 * k)) & 0xffu);
-                    const float cb = (float)((Cb >> (8 * k)) & 0xffu);
-                    const float cc = (float)((Cc >> (8 * k)) & 0xffu);
-                    const float ch = (ca * w + cb * stx) + cc * sty;
-                    rgb |= f2u8(clampf(ch, 0.0f, 255.0f)) << (8 * k);
-                }
-                color = (Ca & 0xff000000u) | rgb;
-            } else {
-                color = 0xff0000ffu;
-            }
-            color |= 0xff000000u;
+            const InstanceRec mesh = R.inst[found];
+            color = shade_pixel(R, mesh, total, x, y);
         }
     }
     if (depth != __builtin_inff()) {
