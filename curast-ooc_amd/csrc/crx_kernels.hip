@@ -625,7 +625,17 @@ __device__ __forceinline__ float box_min(const float* a, const WaveBox& B) {
 // the margin is 2^-16 = 256u times that bound, plus 2^-90 for underflow.
 // The side planes are only used when every depth is above that margin too,
 // so each vertex has d >= 2^-102 and the division-free frustum test applies.
-__device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B) {
+// max over the box of a.p + b.
+__device__ __forceinline__ float box_max(const float* a, const WaveBox& B) {
+    return ((fmaxf(a[0] * B.lo.x, a[0] * B.hi.x) + fmaxf(a[1] * B.lo.y, a[1] * B.hi.y)) +
+            fmaxf(a[2] * B.lo.z, a[2] * B.hi.z)) + a[3];
+}
+
+// *inside is set when the box is also well inside all four side planes: a
+// hint that lets the filtered early-out skip its side-plane tests (skipping
+// an optional cull test cannot change a result, so the hint needs no proof).
+__device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B, bool* inside) {
+    *inside = false;
     const float mz = (c.slopeZ * B.M + c.cnstZ) * 0x1p-16f + 0x1p-90f;
     if (box_min(c.z, B) > mz) return false;                 // behind the camera
     const float negz[4] = {-c.z[0], -c.z[1], -c.z[2], -c.z[3]};
@@ -634,6 +644,8 @@ __device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B
     const float my = (c.slopeY * B.M + c.cnstY) * 0x1p-16f + 0x1p-90f;
     if (box_min(c.px, B) > mx || box_min(c.nx, B) > mx) return false;
     if (box_min(c.py, B) > my || box_min(c.ny, B) > my) return false;
+    *inside = box_max(c.px, B) < -mx && box_max(c.nx, B) < -mx && box_max(c.py, B) < -my &&
+              box_max(c.ny, B) < -my;
     return true;
 }
 
@@ -726,13 +738,15 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
             const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
             for (uint32_t vb = 0; vb < numViews; vb += kWave) {
                 const uint32_t v = vb + lane;
-                bool maybe = false;
+                bool maybe = false, inside = false;
                 if (v < numViews) {
                     if (!cacheCull || inst != cullInst) cull = lane_view_cull(P, v, inst);
-                    maybe = !cull.skip && (unbounded || view_may_see(cull, box));
+                    if (!cull.skip) maybe = unbounded || view_may_see(cull, box, &inside);
+                    inside = inside && !unbounded;
                 }
                 if (cacheCull) cullInst = inst;
                 uint64_t mask = __ballot(maybe);
+                const uint64_t insideMask = __ballot(inside);
                 while (mask) {
                     const uint32_t vi = vb + (uint32_t)__builtin_ctzll(mask);
                     mask &= mask - 1ull;
@@ -743,7 +757,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
                     for (int e = 0; e < 16; e++) m[e] = T->m[e];
                     if (valid) {
                         const uint64_t entry = ((uint64_t)((vi << kViewShift) | inst) << 32) | (uint64_t)tri;
-                        stage1_triangle(P, V, m, T->amax, T->bmax, T->flip != 0u, A, B, C, id, entry);
+                        const bool frustum = ((insideMask >> (vi - vb)) & 1ull) == 0ull;
+                        stage1_triangle(P, V, m, T->amax, T->bmax, T->flip != 0u, A, B, C, id, entry, frustum);
                     }
                 }
             }
