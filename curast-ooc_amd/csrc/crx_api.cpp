@@ -226,6 +226,163 @@ ViewParams make_view_params(const crx_render_target& t, const Transform* transfo
     return V;
 }
 
+// Host records and metadata layout of a draw of numViews >= 1 views sharing
+// one instance list (transforms view-major).  No device work.
+crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
+                     uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
+                     uint32_t numViews, int instanced, const uint8_t* skip, DrawPlan* plan) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    if (numViews == 0 || numViews > kMaxViews)
+        return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
+    if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
+    for (uint32_t v = 0; v < numViews; v++) CRX_TRY(check_target(targets + v));
+    DrawPlan& D = *plan;
+    D = DrawPlan{};
+    D.numViews = numViews;
+    D.numInstances = numInstances;
+    D.targets.assign(targets, targets + numViews);
+    // drawTrianglesVisbuffer returns early for an empty instance list (CuRast_render.h:231)
+    if (numInstances == 0) return CRX_OK;
+    if (!instances || !transforms) return fail(CRX_INVALID_ARG, "instances / transforms is NULL");
+    if (instanced && (!meshes || numMeshes == 0)) return fail(CRX_INVALID_ARG, "instanced draw without meshes");
+    if (numViews > 1 && numInstances > kInstanceMask + 1u)
+        return fail(CRX_INVALID_ARG, "a multi-view draw takes at most 2^24 instances");
+    const crx_tuning& tu = ctx->tuning;
+    D.block = tu.stage1_block;
+    D.batchShift = 0;   // log2(triangles per batch)
+    while ((1u << D.batchShift) < D.block) D.batchShift++;
+    const uint64_t batchTris = 1ull << D.batchShift;
+
+    const crx_cmesh* list = instanced ? meshes : instances;
+    D.numItems = instanced ? numMeshes : numInstances;
+    D.items.resize(D.numItems);
+    // the draw's stage-1 kernel is chosen by its first mesh, as the reference
+    // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
+    D.pu16 = D.numItems > 0 && list[0].compressed;
+    for (uint32_t k = 0; k < D.numItems; k++) {
+        const crx_cmesh& m = list[k];
+        CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));
+        if ((m.compressed != 0) != D.pu16)
+            return fail(CRX_UNSUPPORTED, "a draw mixes compressed and uncompressed meshes");
+        const int64_t off = m.instances.offset;
+        const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
+        if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
+            return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
+        DrawItem& it = D.items[k];
+        it.indices = m.indices;
+        it.positions = reinterpret_cast<const float*>(m.positions);
+        it.cumm = m.cummulativeTriangleCount;
+        it.numTriangles = m.numTriangles;
+        it.instOffset = (uint32_t)off;
+        it.instCount = (uint32_t)cnt;
+        it.firstBatch = (uint32_t)D.batches;
+        it.bitsPerIndex = m.bitsPerIndex;
+        it.indexMin = m.index_min;
+        it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
+        it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
+        D.batches += ((uint64_t)m.numTriangles + batchTris - 1) >> D.batchShift;
+        D.tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
+        if (D.batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
+    }
+    D.inst.resize(numInstances);
+    for (uint32_t i = 0; i < numInstances; i++) {
+        CRX_TRY(check_mesh(instances[i], "instances", i));
+        const int64_t ti = instances[i].instances.offset;
+        if (ti < 0 || ti >= (int64_t)numInstances)
+            return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
+        D.inst[i] = make_instance(instances[i]);
+    }
+    const size_t numXf = (size_t)numViews * numInstances;
+    D.xf.resize(numXf);
+    for (size_t i = 0; i < numXf; i++) {
+        Transform& x = D.xf[i];
+        std::memcpy(x.m, transforms[i].m, sizeof(x.m));
+        x.flip = mirror_flag(transforms[i]) ? 1u : 0u;
+        x.skip = (skip && skip[i]) ? 1u : 0u;
+        // row bounds of the filtered early-out (rows 0..2), rounded up
+        double a = 0.0, b = 0.0;
+        for (int r = 0; r < 3; r++) {
+            const float* m = transforms[i].m;
+            a = std::max(a, (double)std::fabs(m[r]) + std::fabs(m[4 + r]) + std::fabs(m[8 + r]));
+            b = std::max(b, (double)std::fabs(m[12 + r]));
+        }
+        x.amax = (float)(a * (1.0 + 0x1p-20));
+        x.bmax = (float)(b * (1.0 + 0x1p-20));
+    }
+    D.offItems = 0;
+    D.offInst = align_up(D.offItems + D.items.size() * sizeof(DrawItem), 256);
+    D.offViews = align_up(D.offInst + D.inst.size() * sizeof(InstanceRec), 256);
+    D.offXf = align_up(D.offViews + (size_t)numViews * sizeof(ViewParams), 256);
+    D.metaBytes = D.offXf + D.xf.size() * sizeof(Transform);
+    if (tu.stage1_waves_per_cu == 0) {
+        D.g1 = (uint32_t)std::min<uint64_t>(D.batches, 1ull << 31);
+    } else {
+        D.g1 = grid_for(ctx, tu.stage1_waves_per_cu, D.block / 64);
+        if (D.g1 > D.batches) D.g1 = (uint32_t)D.batches;
+    }
+    D.g1 = (D.g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
+    D.g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4);
+    D.g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
+    return CRX_OK;
+}
+
+// Writes the plan's metadata image (to be copied to device address `dev`)
+// into `host` and returns the kernel parameters; the queues are the
+// context's.
+RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* dev) {
+    const Transform* xfDev = reinterpret_cast<const Transform*>(dev + D.offXf);
+    std::vector<ViewParams> views(D.numViews);
+    for (uint32_t v = 0; v < D.numViews; v++)
+        views[v] = make_view_params(D.targets[v], xfDev + (size_t)v * D.numInstances);
+    std::memcpy(host + D.offItems, D.items.data(), D.items.size() * sizeof(DrawItem));
+    std::memcpy(host + D.offInst, D.inst.data(), D.inst.size() * sizeof(InstanceRec));
+    std::memcpy(host + D.offViews, views.data(), views.size() * sizeof(ViewParams));
+    std::memcpy(host + D.offXf, D.xf.data(), D.xf.size() * sizeof(Transform));
+    const crx_tuning& tu = ctx->tuning;
+    RasterParams P{};
+    P.items = reinterpret_cast<const DrawItem*>(dev + D.offItems);
+    P.inst = reinterpret_cast<const InstanceRec*>(dev + D.offInst);
+    P.views = reinterpret_cast<const ViewParams*>(dev + D.offViews);
+    P.view0 = views[0];
+    P.numViews = D.numViews;
+    P.ctr = ctx->ctr;
+    P.numItems = D.numItems;
+    P.numBatches = (uint32_t)D.batches;
+    P.smallThreshold = tu.small_threshold;
+    P.tile = (int32_t)tu.tile_size;
+    P.batchShift = D.batchShift;
+    P.order = tu.stage1_order;
+    P.cullOrder = tu.stage1_cull_order;
+    P.fastTiny = tu.stage1_fast_tiny;
+    P.numWaves2 = D.g2 * 4u;
+    P.numWaves3 = D.g3 * 4u;
+    P.nontrivial = static_cast<uint64_t*>(ctx->nontrivial.ptr);
+    P.huge = static_cast<int4*>(ctx->huge.ptr);
+    P.ntCap = ctx->ntCap;
+    P.hugeCap = ctx->hugeCap;
+    return P;
+}
+
+// Counter reset and the three stages.  viewsKernel forces the multi-view
+// stage 1 (which reads every view record from memory) for a single view.
+hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParams& P, bool viewsKernel,
+                          hipStream_t stream) {
+    hipError_t e = launch_reset_counters(ctx->ctr, stream);
+    if (e != hipSuccess) return e;
+    record(ctx, T_S1, stream);
+    if (D.numViews == 1 && !viewsKernel) e = launch_stage1(P, D.block, D.g1, D.pu16, stream);
+    else e = launch_stage1_views(P, D.block, D.g1, D.pu16, stream);
+    if (e != hipSuccess) return e;
+    record(ctx, T_S2, stream);
+    e = launch_stage2(P, D.g2, stream);
+    if (e != hipSuccess) return e;
+    record(ctx, T_S3, stream);
+    e = launch_stage3(P, D.g3, stream);
+    if (e != hipSuccess) return e;
+    record(ctx, T_S3_END, stream);
+    return hipSuccess;
+}
+
 // drawTrianglesVisbuffer (CuRast_render.h:211-264) for numViews >= 1 views
 // sharing one instance list: transforms[v * numInstances + i] is view v's
 // view * world of instance i.  One view runs the plain stage 1; several views
@@ -234,134 +391,21 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
                       uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
                       uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
                       const uint8_t* skip) {
-    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
-    if (numViews == 0 || numViews > kMaxViews)
-        return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
-    if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
-    for (uint32_t v = 0; v < numViews; v++) CRX_TRY(check_target(targets + v));
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    // drawTrianglesVisbuffer returns early for an empty instance list (CuRast_render.h:231)
-    if (numInstances == 0) return CRX_OK;
-    if (!instances || !transforms) return fail(CRX_INVALID_ARG, "instances / transforms is NULL");
-    if (instanced && (!meshes || numMeshes == 0)) return fail(CRX_INVALID_ARG, "instanced draw without meshes");
-    if (numViews > 1 && numInstances > kInstanceMask + 1u)
-        return fail(CRX_INVALID_ARG, "a multi-view draw takes at most 2^24 instances");
     if (skip && numViews < 2) return fail(CRX_INVALID_ARG, "per-view instance culling needs a multi-view draw");
+    DrawPlan D;
+    CRX_TRY(plan_draw(ctx, meshes, numMeshes, instances, numInstances, transforms, targets, numViews, instanced,
+                      skip, &D));
+    if (stats) stats->num_triangles = D.tris * numViews;
+    if (D.batches == 0) return CRX_OK;
     CRX_HIP(hipSetDevice(ctx->device));
-    const crx_tuning& tu = ctx->tuning;
-    const uint32_t block = tu.stage1_block;
-    uint32_t batchShift = 0;   // log2(triangles per batch)
-    while ((1u << batchShift) < block) batchShift++;
-    const uint64_t batchTris = 1ull << batchShift;
-
-    // ---- host records -----------------------------------------------------
-    const crx_cmesh* list = instanced ? meshes : instances;
-    const uint32_t numItems = instanced ? numMeshes : numInstances;
-    std::vector<DrawItem> items(numItems);
-    uint64_t batches = 0, tris = 0;
-    // the draw's stage-1 kernel is chosen by its first mesh, as the reference
-    // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
-    const bool pu16 = numItems > 0 && list[0].compressed;
-    for (uint32_t k = 0; k < numItems; k++) {
-        const crx_cmesh& m = list[k];
-        CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));
-        if ((m.compressed != 0) != pu16)
-            return fail(CRX_UNSUPPORTED, "a draw mixes compressed and uncompressed meshes");
-        const int64_t off = m.instances.offset;
-        const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
-        if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
-            return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
-        DrawItem& it = items[k];
-        it.indices = m.indices;
-        it.positions = reinterpret_cast<const float*>(m.positions);
-        it.cumm = m.cummulativeTriangleCount;
-        it.numTriangles = m.numTriangles;
-        it.instOffset = (uint32_t)off;
-        it.instCount = (uint32_t)cnt;
-        it.firstBatch = (uint32_t)batches;
-        it.bitsPerIndex = m.bitsPerIndex;
-        it.indexMin = m.index_min;
-        it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
-        it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
-        batches += ((uint64_t)m.numTriangles + batchTris - 1) >> batchShift;
-        tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
-        if (batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
-    }
-    std::vector<InstanceRec> inst(numInstances);
-    for (uint32_t i = 0; i < numInstances; i++) {
-        CRX_TRY(check_mesh(instances[i], "instances", i));
-        const int64_t ti = instances[i].instances.offset;
-        if (ti < 0 || ti >= (int64_t)numInstances)
-            return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
-        inst[i] = make_instance(instances[i]);
-    }
-    const size_t numXf = (size_t)numViews * numInstances;
-    std::vector<Transform> xf(numXf);
-    for (size_t i = 0; i < numXf; i++) {
-        std::memcpy(xf[i].m, transforms[i].m, sizeof(xf[i].m));
-        xf[i].flip = mirror_flag(transforms[i]) ? 1u : 0u;
-        xf[i].skip = (skip && skip[i]) ? 1u : 0u;
-        // row bounds of the filtered early-out (rows 0..2), rounded up
-        double a = 0.0, b = 0.0;
-        for (int r = 0; r < 3; r++) {
-            const float* m = transforms[i].m;
-            a = std::max(a, (double)std::fabs(m[r]) + std::fabs(m[4 + r]) + std::fabs(m[8 + r]));
-            b = std::max(b, (double)std::fabs(m[12 + r]));
-        }
-        xf[i].amax = (float)(a * (1.0 + 0x1p-20));
-        xf[i].bmax = (float)(b * (1.0 + 0x1p-20));
-    }
-    if (stats) stats->num_triangles = tris * numViews;
-    if (batches == 0) return CRX_OK;
-
-    const size_t offItems = 0;
-    const size_t offInst = align_up(offItems + items.size() * sizeof(DrawItem), 256);
-    const size_t offViews = align_up(offInst + inst.size() * sizeof(InstanceRec), 256);
-    const size_t offXf = align_up(offViews + (size_t)numViews * sizeof(ViewParams), 256);
-    const size_t metaBytes = offXf + xf.size() * sizeof(Transform);
-    CRX_TRY(ctx->drawMeta.reserve(metaBytes));
+    CRX_TRY(ctx->drawMeta.reserve(D.metaBytes));
+    CRX_TRY(ctx->drawStaging.acquire(D.metaBytes));
+    CRX_TRY(ensure_queues(ctx, ctx->tuning.max_nontrivial, ctx->tuning.max_huge));
     char* meta = static_cast<char*>(ctx->drawMeta.ptr);
-    const Transform* xfDev = reinterpret_cast<const Transform*>(meta + offXf);
-    std::vector<ViewParams> views(numViews);
-    for (uint32_t v = 0; v < numViews; v++) views[v] = make_view_params(targets[v], xfDev + (size_t)v * numInstances);
-    CRX_TRY(ctx->drawStaging.acquire(metaBytes));
-    char* stage = static_cast<char*>(ctx->drawStaging.ptr);
-    std::memcpy(stage + offItems, items.data(), items.size() * sizeof(DrawItem));
-    std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
-    std::memcpy(stage + offViews, views.data(), views.size() * sizeof(ViewParams));
-    std::memcpy(stage + offXf, xf.data(), xf.size() * sizeof(Transform));
-    CRX_HIP(hipMemcpyAsync(ctx->drawMeta.ptr, stage, metaBytes, hipMemcpyHostToDevice, stream));
+    RasterParams P = plan_write(ctx, D, static_cast<char*>(ctx->drawStaging.ptr), meta);
+    CRX_HIP(hipMemcpyAsync(meta, ctx->drawStaging.ptr, D.metaBytes, hipMemcpyHostToDevice, stream));
     CRX_TRY(ctx->drawStaging.mark(stream));
-    CRX_TRY(ensure_queues(ctx, tu.max_nontrivial, tu.max_huge));
-
-    RasterParams P{};
-    P.items = reinterpret_cast<const DrawItem*>(meta + offItems);
-    P.inst = reinterpret_cast<const InstanceRec*>(meta + offInst);
-    P.views = reinterpret_cast<const ViewParams*>(meta + offViews);
-    P.view0 = views[0];
-    P.numViews = numViews;
-    P.ctr = ctx->ctr;
-    P.numItems = numItems;
-    P.numBatches = (uint32_t)batches;
-    P.smallThreshold = tu.small_threshold;
-    P.tile = (int32_t)tu.tile_size;
-    P.batchShift = batchShift;
-    P.order = tu.stage1_order;
-    P.cullOrder = tu.stage1_cull_order;
-    P.fastTiny = tu.stage1_fast_tiny;
-
-    uint32_t g1;
-    if (tu.stage1_waves_per_cu == 0) {
-        g1 = (uint32_t)std::min<uint64_t>(batches, 1ull << 31);
-    } else {
-        g1 = grid_for(ctx, tu.stage1_waves_per_cu, block / 64);
-        if (g1 > batches) g1 = (uint32_t)batches;
-    }
-    g1 = (g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
-    const uint32_t g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4);
-    const uint32_t g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
-    P.numWaves2 = g2 * 4u;
-    P.numWaves3 = g3 * 4u;
 
     uint32_t redraws = 0;
     for (;;) {
@@ -369,15 +413,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         P.huge = static_cast<int4*>(ctx->huge.ptr);
         P.ntCap = ctx->ntCap;
         P.hugeCap = ctx->hugeCap;
-        CRX_HIP(launch_reset_counters(ctx->ctr, stream));
-        record(ctx, T_S1, stream);
-        if (numViews == 1) CRX_HIP(launch_stage1(P, block, g1, pu16, stream));
-        else CRX_HIP(launch_stage1_views(P, block, g1, pu16, stream));
-        record(ctx, T_S2, stream);
-        CRX_HIP(launch_stage2(P, g2, stream));
-        record(ctx, T_S3, stream);
-        CRX_HIP(launch_stage3(P, g3, stream));
-        record(ctx, T_S3_END, stream);
+        CRX_HIP(enqueue_stages(ctx, D, P, false, stream));
         CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         CRX_HIP(hipStreamSynchronize(stream));
         const uint32_t nNt = ctx->ctrHost[CTR_NONTRIVIAL];
@@ -428,10 +464,11 @@ ResolveParams make_resolve_params(const crx_render_target& t, const InstanceRec*
 
 // kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141) for
 // numViews >= 1 targets sharing one instance list, in one launch.
-crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+// Metadata image of a resolve: InstanceRec[n] | prefix[n] | ResolveParams[numViews],
+// with device pointers for base address `dev`.
+crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
-                         hipStream_t stream) {
-    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+                         char* dev, std::vector<char>* image, size_t* offViews, uint32_t* blocks) {
     if (numViews == 0 || numViews > kMaxViews)
         return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
     if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
@@ -443,7 +480,6 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
         return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
     if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
     if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
-    CRX_HIP(hipSetDevice(ctx->device));
     std::vector<InstanceRec> inst(numInstances);
     for (uint32_t i = 0; i < numInstances; i++) {
         CRX_TRY(check_mesh(instances[i], "instances", i));
@@ -451,37 +487,76 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
     }
     const size_t offInst = 0;
     const size_t offPrefix = align_up(inst.size() * sizeof(InstanceRec), 256);
-    const size_t offViews = align_up(offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8, 256);
-    const size_t bytes = offViews + (numViews > 1 ? (size_t)numViews * sizeof(ResolveParams) : 0);
-    CRX_TRY(ctx->resolveMeta.reserve(bytes));
-    char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
-    const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(meta + offInst);
-    const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(meta + offPrefix);
+    *offViews = align_up(offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8, 256);
+    const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(dev + offInst);
+    const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(dev + offPrefix);
     std::vector<ResolveParams> rv(numViews);
-    uint64_t blocks = 0;
+    uint64_t nb = 0;
     for (uint32_t v = 0; v < numViews; v++) {
         rv[v] = make_resolve_params(targets[v], instDev, prefixDev, numInstances, attribute);
-        rv[v].firstBlock = (uint32_t)blocks;
-        blocks += (uint64_t)((targets[v].width + 63) / 64) * (uint64_t)((targets[v].height + 3) / 4);
+        rv[v].firstBlock = (uint32_t)nb;
+        nb += (uint64_t)((targets[v].width + 63) / 64) * (uint64_t)((targets[v].height + 3) / 4);
     }
-    if (blocks >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
+    if (nb >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
+    *blocks = (uint32_t)nb;
+    image->assign(*offViews + (size_t)numViews * sizeof(ResolveParams), 0);
+    std::memcpy(image->data() + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
+    if (numInstances) std::memcpy(image->data() + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
+    std::memcpy(image->data() + *offViews, rv.data(), rv.size() * sizeof(ResolveParams));
+    return CRX_OK;
+}
+
+// kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141) for
+// numViews >= 1 targets sharing one instance list, in one launch.
+crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                         const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                         hipStream_t stream) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    CRX_HIP(hipSetDevice(ctx->device));
+    // first pass sizes the image, second pass fills it for the reserved buffer
+    std::vector<char> image;
+    size_t offViews = 0;
+    uint32_t blocks = 0;
+    CRX_TRY(resolve_image(instances, numInstances, prefix, targets, numViews, attribute, nullptr, &image, &offViews,
+                          &blocks));
+    CRX_TRY(ctx->resolveMeta.reserve(image.size()));
+    char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
+    CRX_TRY(resolve_image(instances, numInstances, prefix, targets, numViews, attribute, meta, &image, &offViews,
+                          &blocks));
+    const ResolveParams* rv = reinterpret_cast<const ResolveParams*>(image.data() + offViews);
     if (numInstances || numViews > 1) {
-        CRX_TRY(ctx->resolveStaging.acquire(bytes));
-        char* stage = static_cast<char*>(ctx->resolveStaging.ptr);
-        std::memcpy(stage + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
-        std::memcpy(stage + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
-        if (numViews > 1) std::memcpy(stage + offViews, rv.data(), rv.size() * sizeof(ResolveParams));
-        CRX_HIP(hipMemcpyAsync(ctx->resolveMeta.ptr, stage, bytes, hipMemcpyHostToDevice, stream));
+        CRX_TRY(ctx->resolveStaging.acquire(image.size()));
+        std::memcpy(ctx->resolveStaging.ptr, image.data(), image.size());
+        CRX_HIP(hipMemcpyAsync(meta, ctx->resolveStaging.ptr, image.size(), hipMemcpyHostToDevice, stream));
         CRX_TRY(ctx->resolveStaging.mark(stream));
     }
     record(ctx, T_RESOLVE, stream);
     if (numViews == 1) {
         CRX_HIP(launch_resolve(rv[0], stream));
     } else {
-        CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews,
-                                     (uint32_t)blocks, stream));
+        CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews, blocks, stream));
     }
     record(ctx, T_RESOLVE_END, stream);
+    return CRX_OK;
+}
+
+// Records of a multi-buffer clear (empty buffers skipped) and its block count.
+crx_status clear_records(uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n, std::vector<ClearView>* cv,
+                         uint32_t* blocks) {
+    cv->clear();
+    uint64_t nb = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        if (numPixels[k] == 0) continue;
+        if (!buffers[k]) return fail(CRX_INVALID_ARG, "buffer to clear is NULL");
+        ClearView c;
+        c.fb = buffers[k];
+        c.numPixels = numPixels[k];
+        c.firstBlock = (uint32_t)nb;
+        cv->push_back(c);
+        nb += (numPixels[k] + kClearChunk - 1) / kClearChunk;
+    }
+    if (nb >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one clear");
+    *blocks = (uint32_t)nb;
     return CRX_OK;
 }
 
@@ -489,28 +564,17 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
 crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,
                          hipStream_t stream) {
     std::vector<ClearView> cv;
-    cv.reserve(n);
-    uint64_t blocks = 0;
-    for (uint32_t k = 0; k < n; k++) {
-        if (numPixels[k] == 0) continue;
-        if (!buffers[k]) return fail(CRX_INVALID_ARG, "buffer to clear is NULL");
-        ClearView c;
-        c.fb = buffers[k];
-        c.numPixels = numPixels[k];
-        c.firstBlock = (uint32_t)blocks;
-        cv.push_back(c);
-        blocks += (numPixels[k] + kClearChunk - 1) / kClearChunk;
-    }
+    uint32_t blocks = 0;
+    CRX_TRY(clear_records(buffers, numPixels, n, &cv, &blocks));
     if (cv.empty()) return CRX_OK;
-    if (blocks >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one clear");
     const size_t bytes = cv.size() * sizeof(ClearView);
     CRX_TRY(ctx->clearMeta.reserve(bytes));
     CRX_TRY(ctx->clearStaging.acquire(bytes));
     std::memcpy(ctx->clearStaging.ptr, cv.data(), bytes);
     CRX_HIP(hipMemcpyAsync(ctx->clearMeta.ptr, ctx->clearStaging.ptr, bytes, hipMemcpyHostToDevice, stream));
     CRX_TRY(ctx->clearStaging.mark(stream));
-    CRX_HIP(launch_clear_views(static_cast<const ClearView*>(ctx->clearMeta.ptr), (uint32_t)cv.size(),
-                               (uint32_t)blocks, nullptr, stream));
+    CRX_HIP(launch_clear_views(static_cast<const ClearView*>(ctx->clearMeta.ptr), (uint32_t)cv.size(), blocks, nullptr,
+                               stream));
     return CRX_OK;
 }
 
@@ -560,6 +624,125 @@ bool box_in_frustum(const float planes[16], const crx_box3& wb) {
     return true;
 }
 
+// CuRast::draw's frame preparation (CuRast_render.h:338-435) for numViews
+// views: the node cull of each view (SNTriangles::update, SNTriangles.h:29-105),
+// the nodes kept (visible in any view, or all with keepAll), sorted by
+// triangle count then mesh with node order kept inside a mesh, grouped into
+// unique meshes, with triangle prefix, view-major transforms and per-view
+// skip flags for the nodes a view's own cull dropped.  Fills the scene's
+// scratch vectors; returns the number of unique meshes.
+uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews, const crx_render_settings& settings,
+                     bool keepAll) {
+    std::vector<float> planes((size_t)numViews * 16);
+    for (uint32_t v = 0; v < numViews; v++) frustum_planes(views[v], &planes[(size_t)v * 16]);
+    const uint32_t N = (uint32_t)scene->nodes.size();
+    std::vector<uint8_t> vis((size_t)N * numViews, 1);
+    std::vector<uint32_t> kept;
+    kept.reserve(N);
+    for (uint32_t i = 0; i < N; i++) {
+        const crx_scene::Node& nd = scene->nodes[i];
+        bool any = true;
+        if (settings.frustum_culling) {
+            const crx_box3 wb = world_box(nd.mesh->aabb, nd.world);
+            any = false;
+            for (uint32_t v = 0; v < numViews; v++) {
+                const bool in = box_in_frustum(&planes[(size_t)v * 16], wb);
+                vis[(size_t)i * numViews + v] = in ? 1 : 0;
+                any = any || in;
+            }
+        }
+        if (any || keepAll) kept.push_back(i);
+    }
+    // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
+    // instances of one mesh keep node order.
+    std::stable_sort(kept.begin(), kept.end(), [&](uint32_t a, uint32_t b) {
+        const crx_mesh* A = scene->nodes[a].mesh;
+        const crx_mesh* B = scene->nodes[b].mesh;
+        if (A->numTriangles == B->numTriangles) return A->seq < B->seq;
+        return A->numTriangles > B->numTriangles;
+    });
+    const uint32_t n = (uint32_t)kept.size();
+    scene->unique.resize(n);
+    scene->instances.resize(n);
+    scene->transforms.resize((size_t)n * numViews);
+    scene->skip.assign((size_t)n * numViews, 0);
+    scene->prefix.resize(n);
+    scene->nodeOf.resize(n);
+    uint64_t sum = 0;
+    uint32_t u = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const crx_scene::Node& nd = scene->nodes[kept[i]];
+        crx_cmesh cm;
+        std::memset(&cm, 0, sizeof(cm));
+        cm.numTriangles = nd.mesh->numTriangles;
+        cm.indices = nd.mesh->indices;
+        cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
+        cm.colors = nd.mesh->colors;
+        cm.aabb = nd.mesh->aabb;
+        cm.world = nd.world;
+        cm.id = (int32_t)kept[i];
+        cm.numVertices = nd.mesh->numVertices;
+        cm.address = nd.mesh->seq;
+        cm.isLoaded = 1;
+        cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
+        cm.compressed = nd.mesh->compressed ? 1 : 0;
+        cm.bitsPerIndex = nd.mesh->bitsPerIndex;
+        cm.index_min = nd.mesh->indexMin;
+        cm.index_max = nd.mesh->indexMax;
+        cm.compressionFactor = nd.mesh->compressionFactor;
+        cm.cummulativeTriangleCount = sum;
+        cm.instances.offset = (int32_t)i;
+        cm.instances.count = 0;
+        for (uint32_t v = 0; v < numViews; v++) {
+            scene->transforms[(size_t)v * n + i] = mat_mul(views[v].view, nd.world);
+            scene->skip[(size_t)v * n + i] = vis[(size_t)kept[i] * numViews + v] ? 0 : 1;
+        }
+        scene->prefix[i] = sum;
+        scene->nodeOf[i] = kept[i];
+        sum += cm.numTriangles;
+        scene->instances[i] = cm;
+        if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
+            scene->unique[u] = cm;
+            u++;
+        }
+        scene->unique[u - 1].instances.count++;
+    }
+    scene->unique.resize(u);
+    return u;
+}
+
+// Render targets for views (framebuffers / colourbuffers per view) and the
+// list of buffers a frame clears.
+void view_targets(const crx_view* views, uint32_t numViews, uint64_t* const* framebuffers,
+                  uint64_t* const* colorbuffers, bool resolve, std::vector<crx_render_target>* targets,
+                  std::vector<uint64_t*>* clearBufs, std::vector<uint32_t>* clearPx) {
+    targets->resize(numViews);
+    clearBufs->clear();
+    clearPx->clear();
+    for (uint32_t v = 0; v < numViews; v++) {
+        crx_render_target& t = (*targets)[v];
+        std::memset(&t, 0, sizeof(t));
+        t.framebuffer = framebuffers[v];
+        t.colorbuffer = resolve ? colorbuffers[v] : nullptr;
+        t.width = views[v].width;
+        t.height = views[v].height;
+        t.view = views[v].view;
+        t.viewI = mat_inverse(views[v].view);
+        t.proj = views[v].proj;
+        const f3 cam = xform(t.viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
+        t.cameraPos = crx_vec3{cam.x, cam.y, cam.z};
+        t.f = views[v].proj.m[5];
+        t.aspect = (float)views[v].width / (float)views[v].height;
+        const uint32_t px = (uint32_t)views[v].width * (uint32_t)views[v].height;
+        clearBufs->push_back(t.framebuffer);
+        clearPx->push_back(px);
+        if (resolve) {
+            clearBufs->push_back(t.colorbuffer);
+            clearPx->push_back(px);
+        }
+    }
+}
+
 // CuRast::render / CuRast::draw (CuRast_render.h:266-519, :873-904) for
 // numViews >= 1 views: node culling (SNTriangles::update, SNTriangles.h:29-105)
 // keeps a node visible in any view, the visible nodes are sorted and grouped
@@ -580,100 +763,22 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
         if (settings.resolve && !colorbuffers[v]) return fail(CRX_INVALID_ARG, "resolve needs a colorbuffer");
     }
 
-    std::vector<float> planes((size_t)numViews * 16);
-    for (uint32_t v = 0; v < numViews; v++) frustum_planes(views[v], &planes[(size_t)v * 16]);
-    std::vector<uint32_t> visible;
-    visible.reserve(scene->nodes.size());
-    for (uint32_t i = 0; i < scene->nodes.size(); i++) {
-        const crx_scene::Node& nd = scene->nodes[i];
-        bool keep = true;
-        if (settings.frustum_culling) {
-            const crx_box3 wb = world_box(nd.mesh->aabb, nd.world);
-            keep = false;
-            for (uint32_t v = 0; v < numViews && !keep; v++) keep = box_in_frustum(&planes[(size_t)v * 16], wb);
-        }
-        if (keep) visible.push_back(i);
-    }
-    // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
-    // instances of one mesh keep node order.
-    std::stable_sort(visible.begin(), visible.end(), [&](uint32_t a, uint32_t b) {
-        const crx_mesh* A = scene->nodes[a].mesh;
-        const crx_mesh* B = scene->nodes[b].mesh;
-        if (A->numTriangles == B->numTriangles) return A->seq < B->seq;
-        return A->numTriangles > B->numTriangles;
-    });
-    const uint32_t n = (uint32_t)visible.size();
-    scene->unique.resize(n);
-    scene->instances.resize(n);
-    scene->transforms.resize((size_t)n * numViews);
-    scene->prefix.resize(n);
-    uint64_t sum = 0;
-    uint32_t u = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const crx_scene::Node& nd = scene->nodes[visible[i]];
-        crx_cmesh cm;
-        std::memset(&cm, 0, sizeof(cm));
-        cm.numTriangles = nd.mesh->numTriangles;
-        cm.indices = nd.mesh->indices;
-        cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
-        cm.colors = nd.mesh->colors;
-        cm.aabb = nd.mesh->aabb;
-        cm.world = nd.world;
-        cm.id = (int32_t)visible[i];
-        cm.numVertices = nd.mesh->numVertices;
-        cm.address = nd.mesh->seq;
-        cm.isLoaded = 1;
-        cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
-        cm.compressed = nd.mesh->compressed ? 1 : 0;
-        cm.bitsPerIndex = nd.mesh->bitsPerIndex;
-        cm.index_min = nd.mesh->indexMin;
-        cm.index_max = nd.mesh->indexMax;
-        cm.compressionFactor = nd.mesh->compressionFactor;
-        cm.cummulativeTriangleCount = sum;
-        cm.instances.offset = (int32_t)i;
-        cm.instances.count = 0;
-        for (uint32_t v = 0; v < numViews; v++) scene->transforms[(size_t)v * n + i] = mat_mul(views[v].view, nd.world);
-        scene->prefix[i] = sum;
-        sum += cm.numTriangles;
-        scene->instances[i] = cm;
-        if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
-            scene->unique[u] = cm;
-            u++;
-        }
-        scene->unique[u - 1].instances.count++;
-    }
+    const uint32_t u = scene_lists(scene, views, numViews, settings, false);
+    const uint32_t n = (uint32_t)scene->instances.size();
 
-    std::vector<crx_render_target> targets(numViews);
+    std::vector<crx_render_target> targets;
     std::vector<uint64_t*> clearBufs;
     std::vector<uint32_t> clearPx;
-    for (uint32_t v = 0; v < numViews; v++) {
-        crx_render_target& t = targets[v];
-        std::memset(&t, 0, sizeof(t));
-        t.framebuffer = framebuffers[v];
-        t.colorbuffer = settings.resolve ? colorbuffers[v] : nullptr;
-        t.width = views[v].width;
-        t.height = views[v].height;
-        t.view = views[v].view;
-        t.viewI = mat_inverse(views[v].view);
-        t.proj = views[v].proj;
-        const f3 cam = xform(t.viewI.m, f3{0.0f, 0.0f, 0.0f}, 1.0f);
-        t.cameraPos = crx_vec3{cam.x, cam.y, cam.z};
-        t.f = views[v].proj.m[5];
-        t.aspect = (float)views[v].width / (float)views[v].height;
-        const uint32_t px = (uint32_t)views[v].width * (uint32_t)views[v].height;
-        clearBufs.push_back(t.framebuffer);
-        clearPx.push_back(px);
-        if (settings.resolve) {
-            clearBufs.push_back(t.colorbuffer);
-            clearPx.push_back(px);
-        }
-    }
+    view_targets(views, numViews, framebuffers, colorbuffers, settings.resolve != 0, &targets, &clearBufs, &clearPx);
 
     CRX_HIP(hipSetDevice(ctx->device));
     record(ctx, T_CLEAR, stream);
     CRX_TRY(clear_buffers(ctx, clearBufs.data(), clearPx.data(), (uint32_t)clearBufs.size(), stream));
+    bool anySkip = false;
+    for (uint8_t k : scene->skip) anySkip = anySkip || k != 0;
     CRX_TRY(draw_views(ctx, scene->unique.data(), u, scene->instances.data(), n, scene->transforms.data(),
-                       targets.data(), numViews, settings.instanced, stream, stats));
+                       targets.data(), numViews, settings.instanced, stream, stats,
+                       (anySkip && numViews > 1) ? scene->skip.data() : nullptr));
     if (settings.resolve) {
         CRX_TRY(resolve_views(ctx, scene->instances.data(), n, scene->prefix.data(), targets.data(), numViews,
                               (crx_display_attribute)settings.attribute, stream));

@@ -157,6 +157,8 @@ struct crx_scene {
     std::vector<crx_cmesh> unique, instances;
     std::vector<crx_mat4> transforms;
     std::vector<uint64_t> prefix;
+    std::vector<uint8_t> skip;        // [view][instance]: dropped by that view's node cull
+    std::vector<uint32_t> nodeOf;     // node of each listed instance
 };
 
 
@@ -171,6 +173,27 @@ void frustum_planes(const crx_view& view, float out[16]);
 crx_box3 world_box(const crx_box3& b, const crx_mat4& world);
 bool box_in_frustum(const float planes[16], const crx_box3& worldBox);
 
+// Host side of one draw: records, metadata layout and grid sizes.
+struct DrawPlan {
+    std::vector<DrawItem> items;
+    std::vector<InstanceRec> inst;
+    std::vector<Transform> xf;                 // view-major
+    std::vector<crx_render_target> targets;
+    uint32_t numViews = 0, numInstances = 0, numItems = 0;
+    uint64_t batches = 0, tris = 0;
+    bool pu16 = false;
+    uint32_t block = 256, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
+    size_t offItems = 0, offInst = 0, offViews = 0, offXf = 0, metaBytes = 0;
+};
+
+crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
+                     uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
+                     uint32_t numViews, int instanced, const uint8_t* skip, DrawPlan* plan);
+RasterParams plan_write(crx_context* ctx, const DrawPlan& plan, char* host, char* dev);
+hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& plan, const RasterParams& P, bool viewsKernel,
+                          hipStream_t stream);
+crx_status ensure_queues(crx_context* ctx, uint32_t nt, uint32_t hg);
+
 // drawTrianglesVisbuffer for numViews >= 1 targets sharing one instance list
 // (transforms view-major).  skip (nullable, numViews * numInstances bytes,
 // view-major): a non-zero byte drops instance i from view v, as the host node
@@ -184,5 +207,15 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
                          hipStream_t stream);
 crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,
                          hipStream_t stream);
+uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews, const crx_render_settings& settings,
+                     bool keepAll);
+void view_targets(const crx_view* views, uint32_t numViews, uint64_t* const* framebuffers,
+                  uint64_t* const* colorbuffers, bool resolve, std::vector<crx_render_target>* targets,
+                  std::vector<uint64_t*>* clearBufs, std::vector<uint32_t>* clearPx);
+crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                         const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                         char* dev, std::vector<char>* image, size_t* offViews, uint32_t* blocks);
+crx_status clear_records(uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n, std::vector<ClearView>* cv,
+                         uint32_t* blocks);
 
 }  // namespace crx
