@@ -238,6 +238,12 @@ _SIGS = {
     "crx_clear_views": (C.c_int, [_VP, C.POINTER(RenderTarget), C.c_uint32, C.c_int, _VP]),
     "crx_render_views": (C.c_int, [_VP, _VP, C.POINTER(View), C.c_uint32, C.POINTER(RenderSettings),
                                    C.POINTER(_VP), C.POINTER(_VP), _VP, C.POINTER(DrawStats)]),
+    "crx_frame_create": (C.c_int, [_VP, _VP, C.POINTER(View), C.c_uint32, C.POINTER(RenderSettings),
+                                   C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
+    "crx_frame_set_views": (C.c_int, [_VP, C.POINTER(View)]),
+    "crx_frame_launch": (C.c_int, [_VP, _VP]),
+    "crx_frame_wait": (C.c_int, [_VP, C.POINTER(DrawStats)]),
+    "crx_frame_destroy": (C.c_int, [_VP]),
     "crx_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "crx_host_alloc": (C.c_int, [_VP, C.c_size_t, C.POINTER(_VP)]),
     "crx_host_free": (C.c_int, [_VP, _VP]),
@@ -791,6 +797,41 @@ class Server:
     def close(self) -> None:
         if self.handle:
             lib().crx_server_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+
+class Frame:
+    """A scene frame recorded once into a hipGraph (crx_frame_*): clear,
+    stages 1-3 and resolve for fixed view sizes and buffers."""
+
+    def __init__(self, scene: "Scene", views, fbs, cbs=None, settings: RenderSettings | None = None):
+        v_arr = _as_array(View, views)
+        n = len(v_arr)
+        self.n = n
+        fb_arr = (C.c_void_p * n)(*[b.ptr for b in fbs])
+        cb_arr = (C.c_void_p * n)(*[b.ptr for b in cbs]) if cbs is not None else None
+        s = settings if settings is not None else render_settings()
+        h = C.c_void_p()
+        _check(lib().crx_frame_create(scene.ctx.handle, scene.handle, v_arr, n, C.byref(s), fb_arr, cb_arr,
+                                      C.byref(h)))
+        self.handle = h.value
+
+    def set_views(self, views) -> None:
+        v_arr = _as_array(View, views)
+        assert len(v_arr) == self.n
+        _check(lib().crx_frame_set_views(self.handle, v_arr))
+
+    def launch(self, stream=None) -> None:
+        _check(lib().crx_frame_launch(self.handle, stream))
+
+    def wait(self) -> DrawStats:
+        st = DrawStats()
+        _check(lib().crx_frame_wait(self.handle, C.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crx_frame_destroy(C.c_void_p(self.handle))
             self.handle = None
 
 

@@ -367,7 +367,7 @@ RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* d
 // stage 1 (which reads every view record from memory) for a single view.
 hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParams& P, bool viewsKernel,
                           hipStream_t stream) {
-    hipError_t e = launch_reset_counters(ctx->ctr, stream);
+    hipError_t e = launch_reset_counters(P.ctr, stream);
     if (e != hipSuccess) return e;
     record(ctx, T_S1, stream);
     if (D.numViews == 1 && !viewsKernel) e = launch_stage1(P, D.block, D.g1, D.pu16, stream);

@@ -419,6 +419,34 @@ crx_status crx_render_views(crx_context* ctx, crx_scene* scene, const crx_view* 
                             crx_draw_stats* stats /* nullable */);
 
 /* ------------------------------------------------------------------ */
+/* Frames captured in a hipGraph (no reference counterpart: CuRast      */
+/* launches each kernel from the host every frame, CuRast_render.h:     */
+/* 491-592).                                                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct crx_frame crx_frame;
+
+/* Records clear -> stages 1-3 -> (resolve) of the scene for numViews views of
+ * fixed sizes and buffers into one hipGraph, once.  The frame draws every
+ * node of the scene (triangle ids numbered over all nodes in CuRast::draw's
+ * sort order); each view skips the nodes its own cull drops.  The frame owns
+ * its metadata, queues (sized by the context's tuning) and counters.  The
+ * scene's node set must not change while the frame lives. */
+crx_status crx_frame_create(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
+                            const crx_render_settings* settings, uint64_t* const* framebuffers,
+                            uint64_t* const* colorbuffers, crx_frame** out);
+/* New cameras (same sizes) and the scene's current node transforms for the
+ * next launch; waits for the previous launch to finish reading them. */
+crx_status crx_frame_set_views(crx_frame* frame, const crx_view* views);
+/* Enqueues the captured graph on `stream`. */
+crx_status crx_frame_launch(crx_frame* frame, void* stream);
+/* Waits for the last launch; CRX_QUEUE_OVERFLOW if a stage queue overflowed
+ * (the result is then incomplete: recreate the frame with larger
+ * crx_tuning queue capacities). */
+crx_status crx_frame_wait(crx_frame* frame, crx_draw_stats* stats);
+crx_status crx_frame_destroy(crx_frame* frame);
+
+/* ------------------------------------------------------------------ */
 /* Serving with continuous batching over an out-of-core node store.    */
 /* No single reference counterpart: it combines CuRast::render's       */
 /* per-frame draw (CuRast_render.h:873-904), the node culling of        */

@@ -210,3 +210,49 @@ def test_autotuner(ctx):
             assert np.array_equal(got[k], want), f"view {k}\n" + describe_mismatch(got[k], want, W)
     finally:
         fresh.close()
+
+
+def test_captured_frame_replays(ctx):
+    """crx_frame: the frame recorded into a hipGraph, replayed with new
+    cameras and with eager draws and synchronisations between the replays;
+    every replay equals the eager crx_render_views of the same views."""
+    pos, idx, col = wl.sphere_mesh(rings=32, segments=64, seed=51)
+    pos2, idx2, col2 = wl.sphere_mesh(rings=16, segments=32, radius=2.0, seed=52)
+    meshes = [crx.Mesh(ctx, pos, idx, col), crx.Mesh(ctx, pos2, idx2, col2)]
+    scene = crx.Scene(ctx)
+    for k, w in enumerate(wl.instance_transforms(10, extent=12.0, seed=53)):
+        scene.add(meshes[k % 2], w)
+    sizes = ((320, 200), (256, 256))
+
+    def views_at(t):
+        out = []
+        for k, (W, H) in enumerate(sizes):
+            a = 0.7 * t + 2.5 * k
+            eye = (6.0 + 28.0 * math.cos(a), 14.0 + 2.0 * t, 6.0 + 28.0 * math.sin(a))
+            out.append(crx.make_view(crx.look_at(eye, (6.0, 6.0, 6.0)), wl.projection(W, H), W, H))
+        return out
+
+    settings = crx.render_settings()
+    fbs = [ctx.alloc(8 * W * H) for (W, H) in sizes]
+    cbs = [ctx.alloc(8 * W * H) for (W, H) in sizes]
+    efbs = [ctx.alloc(8 * W * H) for (W, H) in sizes]
+    ecbs = [ctx.alloc(8 * W * H) for (W, H) in sizes]
+    frame = crx.Frame(scene, views_at(0), fbs, cbs, settings)
+    try:
+        for t in range(4):
+            views = views_at(t)
+            if t:
+                frame.set_views(views)
+            frame.launch()
+            st = frame.wait()
+            est = scene.render_views(views, efbs, ecbs, settings)      # eager work between replays
+            ctx.synchronize()
+            assert st.num_nontrivial == est.num_nontrivial and st.num_huge == est.num_huge
+            for k, (W, H) in enumerate(sizes):
+                got = fbs[k].download(np.uint64, W * H)
+                want = efbs[k].download(np.uint64, W * H)
+                assert np.array_equal(got, want), f"replay {t} view {k}\n" + describe_mismatch(got, want, W)
+                assert np.array_equal(cbs[k].download(np.uint64, W * H), ecbs[k].download(np.uint64, W * H))
+                assert int((got != crx.CLEAR_PIXEL).sum()) > 1000
+    finally:
+        frame.close()
