@@ -216,13 +216,6 @@ ViewParams make_view_params(const crx_render_target& t, const Transform* transfo
     // |v_rcp_f32 screen coordinate - IEEE one| <= 0.75*max(W,H)*2^-20 for |ndc| <= 2;
     // the margin is 5x that plus an absolute 2^-6
     V.tinyMargin = (float)std::max(t.width, t.height) * 0x1p-18f + 0x1p-6f;
-    // constants of the filtered early-out, evaluated in double and rounded up
-    const double up = 1.0 + 0x1p-20;
-    const double fa = std::fabs((double)V.fa), f = std::fabs((double)V.f);
-    V.tfx = (float)((fa + 1.0) * 0x1p-19 * up);
-    V.tfy = (float)((f + 1.0) * 0x1p-19 * up);
-    V.kx = (float)(0.5 * t.width * (fa + 2.0) * 1.01 * up);
-    V.ky = (float)(0.5 * t.height * (f + 2.0) * 1.01 * up);
     return V;
 }
 
@@ -299,15 +292,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         std::memcpy(x.m, transforms[i].m, sizeof(x.m));
         x.flip = mirror_flag(transforms[i]) ? 1u : 0u;
         x.skip = (skip && skip[i]) ? 1u : 0u;
-        // row bounds of the filtered early-out (rows 0..2), rounded up
-        double a = 0.0, b = 0.0;
-        for (int r = 0; r < 3; r++) {
-            const float* m = transforms[i].m;
-            a = std::max(a, (double)std::fabs(m[r]) + std::fabs(m[4 + r]) + std::fabs(m[8 + r]));
-            b = std::max(b, (double)std::fabs(m[12 + r]));
-        }
-        x.amax = (float)(a * (1.0 + 0x1p-20));
-        x.bmax = (float)(b * (1.0 + 0x1p-20));
+        x.pad[0] = x.pad[1] = 0u;
     }
     D.offItems = 0;
     D.offInst = align_up(D.offItems + D.items.size() * sizeof(DrawItem), 256);

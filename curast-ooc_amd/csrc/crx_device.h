@@ -63,9 +63,7 @@ struct Transform {
     float m[16];
     uint32_t flip;
     uint32_t skip;
-    // bounds for the filtered stage-1 early-out, rounded up on the host:
-    // amax >= max_r (|m_r0| + |m_r1| + |m_r2|), bmax >= max_r |m_r3| (rows 0..2)
-    float amax, bmax;
+    uint32_t pad[2];
 };
 
 // Counter block, one per context, cleared by the clear kernel of each draw.
@@ -91,10 +89,7 @@ struct ViewParams {
     float fa;                // f / aspect
     float faI, fI;           // 1/(f/aspect), 1/f (stage 3)
     float tinyMargin;        // bound on |approximate - IEEE| screen coordinates, with slack
-    // filtered stage-1 early-out constants (rounded up on the host):
-    float tfx, tfy;          // (|f/aspect| + 1) * 2^-19, (|f| + 1) * 2^-19
-    float kx, ky;            // (W/2) * (|f/aspect| + 2) * 1.01, (H/2) * (|f| + 2) * 1.01
-    uint32_t pad[4];
+    uint32_t pad[2];
 };
 
 constexpr uint32_t kViewShift = 24;                 // instance field: view << 24 | instance

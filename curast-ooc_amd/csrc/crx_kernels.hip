@@ -96,82 +96,6 @@ __device__ __forceinline__ bool surely_culled_axis(float lo, float hi, float m) 
     return (lo + m < k) && (hi + m < k);
 }
 
-// One view-space row of M * (x, y, z, 1) with fused multiply-adds:
-// m_r0*x + (m_r1*y + (m_r2*z + m_r3)), three roundings.
-__device__ __forceinline__ float fma_row(const float* m, int r, f3 p) {
-    return fmaf(m[r], p.x, fmaf(m[4 + r], p.y, fmaf(m[8 + r], p.z, m[12 + r])));
-}
-
-__device__ __forceinline__ float max3abs(f3 p) { return fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z)); }
-
-// Filtered early-out of stage 1: true only when the exact per-triangle code
-// below would drop the triangle (all behind the camera, outside one side
-// plane, or tiny with no sample), decided from view-space coordinates
-// evaluated with fused multiply-adds.
-//
-// Error bound.  With u = 2^-24 and S(p) = sum_k |m_rk||p_k| + |m_r3|, both the
-// FMA row and the reference-order row are within 3.01u*S of the exact value,
-// so they differ by at most 6.02u*S.  Per lane S <= amax*Mv + bmax with Mv the
-// largest |coordinate| of the triangle, and E = S*2^-21 (8u) + 2^-140 bounds
-// the difference of every row.  Then:
-//  * behind: d' < -E at all three vertices gives d <= 0 there;
-//  * a vertex is surely in front with small relative error when
-//    d' > 1024*E + 2^-90 (so d >= 2^-90 and |d - d'| <= d'/1024);
-//  * side plane: fma(fa, x', -d') > (|fa| + 1)*S*2^-19 implies fa*x/d >=
-//    1 + 2^-23 for the reference values (the bound covers |fa|*E + E, one
-//    rounding each of fa*x, the fma and the 2^-23 relative gap, about
-//    11u*(|fa| + 1)*S), hence RN(fa*x/d) > 1; likewise for -1 and for y;
-//  * tiny: q' = RN(fa*x')*rcp(d') is within (|fa| + 2)*E/d*1.003 + 8u of the
-//    reference NDC value when |q'| <= 2, so the screen coordinate
-//    fma(q', W/2, W/2 - 0.5) is within (W/2)*(|fa| + 2)*E*1.01*rcp(d') plus
-//    the rcp-path margin tinyMargin of the reference one; with that margin
-//    surely_culled_axis decides the reference's tiny cull.
-// `frustum` = false skips the side-plane tests (the caller knows they cannot
-// fire).
-__device__ __forceinline__ bool stage1_filtered_cull(const ViewParams& V, const float* __restrict__ m, float amax,
-                                                     float bmax, f3 A, f3 B, f3 C, bool frustum) {
-    const float da = -fma_row(m, 2, A), db = -fma_row(m, 2, B), dc = -fma_row(m, 2, C);
-    const float Mv = fmaxf(fmaxf(max3abs(A), max3abs(B)), max3abs(C));
-    const float S = fmaf(amax, Mv, bmax);
-    const float E = fmaf(S, 0x1p-21f, 0x1p-140f);
-    if (da < -E && db < -E && dc < -E) return true;
-    const float lowD = fmaf(E, 1024.0f, 0x1p-90f);
-    if (!(da > lowD && db > lowD && dc > lowD && da < 0x1p99f && db < 0x1p99f && dc < 0x1p99f)) return false;
-    const float fa = V.fa, f = V.f;
-    const float ya = fma_row(m, 1, A), yb = fma_row(m, 1, B), yc = fma_row(m, 1, C);
-    if (frustum) {
-        const float T = fmaf(V.tfy, S, 0x1p-100f);
-        if ((fmaf(f, ya, -da) > T && fmaf(f, yb, -db) > T && fmaf(f, yc, -dc) > T) ||
-            (fmaf(-f, ya, -da) > T && fmaf(-f, yb, -db) > T && fmaf(-f, yc, -dc) > T))
-            return true;
-    }
-    const float xa = fma_row(m, 0, A), xb = fma_row(m, 0, B), xc = fma_row(m, 0, C);
-    if (frustum) {
-        const float T = fmaf(V.tfx, S, 0x1p-100f);
-        if ((fmaf(fa, xa, -da) > T && fmaf(fa, xb, -db) > T && fmaf(fa, xc, -dc) > T) ||
-            (fmaf(-fa, xa, -da) > T && fmaf(-fa, xb, -db) > T && fmaf(-fa, xc, -dc) > T))
-            return true;
-    }
-    const float ra = __builtin_amdgcn_rcpf(da), rb = __builtin_amdgcn_rcpf(db), rc = __builtin_amdgcn_rcpf(dc);
-    const float qxa = (fa * xa) * ra, qxb = (fa * xb) * rb, qxc = (fa * xc) * rc;
-    const float qya = (f * ya) * ra, qyb = (f * yb) * rb, qyc = (f * yc) * rc;
-    const float qmax = fmaxf(fmaxf(fmaxf(fabsf(qxa), fabsf(qxb)), fmaxf(fabsf(qxc), fabsf(qya))),
-                             fmaxf(fabsf(qyb), fabsf(qyc)));
-    if (!(qmax <= 2.0f)) return false;
-    const float hw = 0.5f * V.Wf, hh = 0.5f * V.Hf;
-    const float ox = hw - 0.5f, oy = hh - 0.5f;
-    const float sxa = fmaf(qxa, hw, ox), sxb = fmaf(qxb, hw, ox), sxc = fmaf(qxc, hw, ox);
-    const float sya = fmaf(qya, hh, oy), syb = fmaf(qyb, hh, oy), syc = fmaf(qyc, hh, oy);
-    const float rmax = fmaxf(fmaxf(ra, rb), rc);
-    const float mx = fmaf(E * V.kx, rmax, V.tinyMargin);
-    const float my = fmaf(E * V.ky, rmax, V.tinyMargin);
-    const float lox = fmaxf(fminf(sxa, fminf(sxb, sxc)), 0.0f);
-    const float hix = fminf(fmaxf(sxa, fmaxf(sxb, sxc)), V.Wm1);
-    const float loy = fmaxf(fminf(sya, fminf(syb, syc)), 0.0f);
-    const float hiy = fminf(fmaxf(sya, fmaxf(syb, syc)), V.Hm1);
-    return surely_culled_axis(lox, hix, mx) || surely_culled_axis(loy, hiy, my);
-}
-
 // rasterize<INDEXBUFFER, UNCOMPRESSED>, triangles_visbuffer.cu:83-271, for one
 // triangle-instance.  `id` is the packed triangle ID the reference forms at
 // :253 and `queueEntry` the stage-2 entry it forms at :193-195.
@@ -187,10 +111,9 @@ __device__ __forceinline__ bool stage1_filtered_cull(const ViewParams& V, const 
 //    screen transform, with slack); only triangles that are culled for sure
 //    leave early, all others take the exact path.
 __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const ViewParams& V,
-                                                const float* __restrict__ m, float amax, float bmax,
+                                                const float* __restrict__ m,
                                                 bool flip, f3 A, f3 B, f3 C,
-                                                uint64_t id, uint64_t queueEntry, bool frustum = true) {
-    if (P.fastTiny && stage1_filtered_cull(V, m, amax, bmax, A, B, C, frustum)) return;
+                                                uint64_t id, uint64_t queueEntry) {
     // depth first: ndc.z = depth = -view.z
     const float da = -xform_row(m, 2, A), db = -xform_row(m, 2, B), dc = -xform_row(m, 2, C);
     // all vertices behind the camera (:119)
@@ -502,8 +425,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                 #pragma unroll
                 for (int e = 0; e < 16; e++) m[e] = T->m[e];
                 const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
-                stage1_triangle(P, P.view0, m, T->amax, T->bmax, T->flip != 0u, A, B, C, id,
-                                ((uint64_t)inst << 32) | (uint64_t)tri);
+                stage1_triangle(P, P.view0, m, T->flip != 0u, A, B, C, id, ((uint64_t)inst << 32) | (uint64_t)tri);
             }
         }
         if (!more) break;
@@ -625,17 +547,7 @@ __device__ __forceinline__ float box_min(const float* a, const WaveBox& B) {
 // the margin is 2^-16 = 256u times that bound, plus 2^-90 for underflow.
 // The side planes are only used when every depth is above that margin too,
 // so each vertex has d >= 2^-102 and the division-free frustum test applies.
-// max over the box of a.p + b.
-__device__ __forceinline__ float box_max(const float* a, const WaveBox& B) {
-    return ((fmaxf(a[0] * B.lo.x, a[0] * B.hi.x) + fmaxf(a[1] * B.lo.y, a[1] * B.hi.y)) +
-            fmaxf(a[2] * B.lo.z, a[2] * B.hi.z)) + a[3];
-}
-
-// *inside is set when the box is also well inside all four side planes: a
-// hint that lets the filtered early-out skip its side-plane tests (skipping
-// an optional cull test cannot change a result, so the hint needs no proof).
-__device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B, bool* inside) {
-    *inside = false;
+__device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B) {
     const float mz = (c.slopeZ * B.M + c.cnstZ) * 0x1p-16f + 0x1p-90f;
     if (box_min(c.z, B) > mz) return false;                 // behind the camera
     const float negz[4] = {-c.z[0], -c.z[1], -c.z[2], -c.z[3]};
@@ -644,8 +556,6 @@ __device__ __forceinline__ bool view_may_see(const ViewCull& c, const WaveBox& B
     const float my = (c.slopeY * B.M + c.cnstY) * 0x1p-16f + 0x1p-90f;
     if (box_min(c.px, B) > mx || box_min(c.nx, B) > mx) return false;
     if (box_min(c.py, B) > my || box_min(c.ny, B) > my) return false;
-    *inside = box_max(c.px, B) < -mx && box_max(c.nx, B) < -mx && box_max(c.py, B) < -my &&
-              box_max(c.ny, B) < -my;
     return true;
 }
 
@@ -738,15 +648,13 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
             const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
             for (uint32_t vb = 0; vb < numViews; vb += kWave) {
                 const uint32_t v = vb + lane;
-                bool maybe = false, inside = false;
+                bool maybe = false;
                 if (v < numViews) {
                     if (!cacheCull || inst != cullInst) cull = lane_view_cull(P, v, inst);
-                    if (!cull.skip) maybe = unbounded || view_may_see(cull, box, &inside);
-                    inside = inside && !unbounded;
+                    maybe = !cull.skip && (unbounded || view_may_see(cull, box));
                 }
                 if (cacheCull) cullInst = inst;
                 uint64_t mask = __ballot(maybe);
-                const uint64_t insideMask = __ballot(inside);
                 while (mask) {
                     const uint32_t vi = vb + (uint32_t)__builtin_ctzll(mask);
                     mask &= mask - 1ull;
@@ -757,8 +665,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
                     for (int e = 0; e < 16; e++) m[e] = T->m[e];
                     if (valid) {
                         const uint64_t entry = ((uint64_t)((vi << kViewShift) | inst) << 32) | (uint64_t)tri;
-                        const bool frustum = ((insideMask >> (vi - vb)) & 1ull) == 0ull;
-                        stage1_triangle(P, V, m, T->amax, T->bmax, T->flip != 0u, A, B, C, id, entry, frustum);
+                        stage1_triangle(P, V, m, T->flip != 0u, A, B, C, id, entry);
                     }
                 }
             }
