@@ -132,7 +132,7 @@ class Tuning(C.Structure):
                 ("stage1_block", C.c_uint32), ("stage1_waves_per_cu", C.c_uint32),
                 ("stage2_waves_per_cu", C.c_uint32), ("stage3_waves_per_cu", C.c_uint32),
                 ("stage1_cull_order", C.c_uint32), ("stage1_order", C.c_uint32),
-                ("stage1_fast_tiny", C.c_uint32), ("reserved", C.c_uint32)]
+                ("stage1_fast_tiny", C.c_uint32), ("stage1_tris_per_lane", C.c_uint32)]
 
 
 class DrawStats(C.Structure):

@@ -7,7 +7,7 @@
 // feeds the kernels (view*world, inverse(view), mirror flags) follows GLM's
 // evaluation order and is compiled without contraction, so it produces the
 // same bits as the device or the reference's GLM would under IEEE rules.
-#include "../../include/crx.h"
+#include "crx.h"
 
 #include <hip/hip_runtime.h>
 
@@ -124,6 +124,8 @@ crx_status check_tuning(const crx_tuning& t) {
     if (t.stage1_cull_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0, 1 or 2");
     if (t.stage1_fast_tiny > 1) return fail(CRX_INVALID_ARG, "tuning.stage1_fast_tiny must be 0 or 1");
     if (t.stage1_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_order must be 0, 1 or 2");
+    if (t.stage1_tris_per_lane != 1 && t.stage1_tris_per_lane != 2 && t.stage1_tris_per_lane != 4)
+        return fail(CRX_INVALID_ARG, "tuning.stage1_tris_per_lane must be 1, 2 or 4");
     if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
     if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
         t.stage3_waves_per_cu > 64 || t.stage1_waves_per_cu > 64)
@@ -242,8 +244,9 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         return fail(CRX_INVALID_ARG, "a multi-view draw takes at most 2^24 instances");
     const crx_tuning& tu = ctx->tuning;
     D.block = tu.stage1_block;
+    D.trisPerLane = tu.stage1_tris_per_lane;
     D.batchShift = 0;   // log2(triangles per batch)
-    while ((1u << D.batchShift) < D.block) D.batchShift++;
+    while ((1u << D.batchShift) < D.block * D.trisPerLane) D.batchShift++;
     const uint64_t batchTris = 1ull << D.batchShift;
 
     const crx_cmesh* list = instanced ? meshes : instances;
@@ -355,8 +358,8 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
     hipError_t e = launch_reset_counters(P.ctr, stream);
     if (e != hipSuccess) return e;
     record(ctx, T_S1, stream);
-    if (D.numViews == 1 && !viewsKernel) e = launch_stage1(P, D.block, D.g1, D.pu16, stream);
-    else e = launch_stage1_views(P, D.block, D.g1, D.pu16, stream);
+    if (D.numViews == 1 && !viewsKernel) e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
+    else e = launch_stage1_views(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
     if (e != hipSuccess) return e;
     record(ctx, T_S2, stream);
     e = launch_stage2(P, D.g2, stream);
@@ -806,7 +809,7 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage1_cull_order = 1;
     out->stage1_order = 1;
     out->stage1_fast_tiny = 1;
-    out->reserved = 0;
+    out->stage1_tris_per_lane = 2;
 }
 
 crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {
@@ -1356,6 +1359,7 @@ crx_status crx_autotune(crx_context* ctx, crx_scene* scene, const crx_view* view
     };
     const std::vector<Knob> knobs = {
         {&crx_tuning::stage1_block, {128, 256, 512}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_tris_per_lane, {1, 2, 4}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_waves_per_cu, {8, 16, 24, 32, 48, 64, 0}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_cull_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},

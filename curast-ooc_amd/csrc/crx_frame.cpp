@@ -8,7 +8,7 @@
 // of the counters back to page-locked memory and the resolve.  Launching the
 // frame is one hipGraphLaunch; new cameras rewrite the staging buffers.  The
 // clear and the counter reset are kernels: the graph records no memset node.
-#include "../../include/crx.h"
+#include "crx.h"
 
 #include <hip/hip_runtime.h>
 

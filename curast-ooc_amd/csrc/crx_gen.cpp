@@ -1,5 +1,5 @@
 // crx_gen.cpp — seeded synthetic workloads (include/crx_gen.h).
-#include "../../include/crx_gen.h"
+#include "crx_gen.h"
 
 #include <cmath>
 #include <cstring>

@@ -3,7 +3,7 @@
 // objects, and the draw / resolve / clear drivers behind the entry points.
 #pragma once
 
-#include "../../include/crx.h"
+#include "crx.h"
 
 #include <hip/hip_runtime.h>
 
@@ -182,7 +182,7 @@ struct DrawPlan {
     uint32_t numViews = 0, numInstances = 0, numItems = 0;
     uint64_t batches = 0, tris = 0;
     bool pu16 = false;
-    uint32_t block = 256, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
+    uint32_t block = 256, trisPerLane = 1, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
     size_t offItems = 0, offInst = 0, offViews = 0, offXf = 0, metaBytes = 0;
 };
 

@@ -9,9 +9,12 @@ namespace crx {
 
 hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream);
 hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream);
+// trisPerLane: 1, 2 or 4 (P.batchShift = log2(block * trisPerLane));
 // pu16: the draw's meshes are IX_PU16-compressed
-hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream);
-hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream);
+hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
+                         hipStream_t stream);
+hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
+                               hipStream_t stream);
 hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t stream);
 hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream);
 hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream);

@@ -10,7 +10,7 @@
 // (IEEE binary32, GLM evaluation order, no contraction); what changes is the
 // schedule.  Cull predicates are independent early-outs, so they are evaluated
 // cheapest first; the frustum test is done without division by an exact
-// rewrite of "RN(p/d) > 1" (see frustum_outcode).
+// rewrite of "RN(p/d) > 1" (see frustum_culled_axis).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -61,17 +61,21 @@ __device__ __forceinline__ f3 load_pos(const float* __restrict__ positions, uint
     return f3{p[0], p[1], p[2]};
 }
 
-// Frustum outcode of one view-space vertex with depth d > 0 (d >= 2^-102 or
-// +inf), without dividing.  With p = RN(fa*x) the reference tests
-// RN(p/d) > 1.  Round-to-nearest-even gives RN(q) > 1 <=> q > 1 + 2^-24, i.e.
-// p - d > d*2^-24.  d*2^-24 is exact (no underflow for d >= 2^-102) and so is
-// p - d wherever it matters: Sterbenz makes it exact for p in [d/2, 2d], and
-// outside that range the rounded difference cannot cross the threshold.  The
-// same argument gives RN(p/d) < -1 <=> p + d < -d*2^-24.  Infinite and NaN p
-// compare exactly like the quotient they stand for.
-__device__ __forceinline__ uint32_t outcode_axis(float p, float d) {
-    const float thr = d * 0x1p-24f;
-    return ((p - d > thr) ? 1u : 0u) | ((p + d < -thr) ? 2u : 0u);
+// Frustum test of one axis for a triangle whose depths d are all >= 2^-102
+// (or +inf), without dividing.  With p = RN(fa*x) the reference culls when
+// RN(p/d) > 1 holds for all three vertices, or RN(p/d) < -1 for all three
+// (triangles_visbuffer.cu:113-118).  Round-to-nearest-even gives
+// RN(q) > 1 <=> q > 1 + 2^-24, i.e. p - d > d*2^-24.  d*2^-24 is exact (no
+// underflow for d >= 2^-102) and so is p - d wherever it matters: Sterbenz
+// makes it exact for p in [d/2, 2d], and outside that range the rounded
+// difference cannot cross the threshold.  The same argument gives
+// RN(p/d) < -1 <=> p + d < -d*2^-24.  Infinite and NaN p compare exactly like
+// the quotient they stand for.  The comparisons combine as lane masks.
+__device__ __forceinline__ bool frustum_culled_axis(float pa, float pb, float pc, float da, float db, float dc) {
+    const float ta = da * 0x1p-24f, tb = db * 0x1p-24f, tc = dc * 0x1p-24f;
+    const bool above = (pa - da > ta) & (pb - db > tb) & (pc - dc > tc);
+    const bool below = (pa + da < -ta) & (pb + db < -tb) & (pc + dc < -tc);
+    return above | below;
 }
 
 // One view-space row of M * (x, y, z, 1): (r0*x + r1*y) + (r2*z + r3).
@@ -104,7 +108,7 @@ __device__ __forceinline__ bool surely_culled_axis(float lo, float hi, float m) 
 // predicates, so any order gives the same result; coordinates are computed
 // lazily in the order the cheapest test needs them.  Two of the tests are
 // decided without IEEE divisions when that is provably exact:
-//  * frustum: outcode_axis rewrites RN(p/d) > 1 as p - d > d*2^-24;
+//  * frustum: frustum_culled_axis rewrites RN(p/d) > 1 as p - d > d*2^-24;
 //  * tiny cull (P.fastTiny): screen coordinates from v_rcp_f32 (1 ulp) are
 //    within P.tinyMargin of the IEEE ones for |ndc| <= 2 (the margin covers
 //    the quotient error 2^-21*|ndc| scaled by W/2 plus the roundings of the
@@ -131,17 +135,17 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
         if (P.cullOrder == 1) {
             vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
             haveY = true;
-            if (outcode_axis(f * vya, da) & outcode_axis(f * vyb, db) & outcode_axis(f * vyc, dc)) return;
+            if (frustum_culled_axis(f * vya, f * vyb, f * vyc, da, db, dc)) return;
             vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C);
             haveX = true;
-            if (outcode_axis(fa * vxa, da) & outcode_axis(fa * vxb, db) & outcode_axis(fa * vxc, dc)) return;
+            if (frustum_culled_axis(fa * vxa, fa * vxb, fa * vxc, da, db, dc)) return;
         } else {
             vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C);
             haveX = true;
-            if (outcode_axis(fa * vxa, da) & outcode_axis(fa * vxb, db) & outcode_axis(fa * vxc, dc)) return;
+            if (frustum_culled_axis(fa * vxa, fa * vxb, fa * vxc, da, db, dc)) return;
             vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
             haveY = true;
-            if (outcode_axis(f * vya, da) & outcode_axis(f * vyb, db) & outcode_axis(f * vyc, dc)) return;
+            if (frustum_culled_axis(f * vya, f * vyb, f * vyc, da, db, dc)) return;
         }
     }
     if (!haveX) { vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C); }
@@ -160,8 +164,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     }
     if (!frustumFirst && !nontrivial) {
         if (safeDepth) {
-            if ((outcode_axis(pxa, da) & outcode_axis(pxb, db) & outcode_axis(pxc, dc)) ||
-                (outcode_axis(pya, da) & outcode_axis(pyb, db) & outcode_axis(pyc, dc)))
+            if (frustum_culled_axis(pxa, pxb, pxc, da, db, dc) || frustum_culled_axis(pya, pyb, pyc, da, db, dc))
                 return;
         }
     }
@@ -296,13 +299,39 @@ __device__ __forceinline__ uint32_t find_item(const RasterParams& P, uint32_t ba
     return lo;
 }
 
-struct S1Batch {
-    DrawItem it;         // uniform
-    uint32_t itemEnd;    // first batch past this item (uniform)
-    uint32_t item;       // uniform
-    uint32_t tri;        // this lane's triangle
-    uint32_t i0, i1, i2; // its vertex indices
+// Uniform fields of the draw item a batch belongs to, held in scalar
+// registers (plain scalars, so the compiler never stages them in memory).
+struct S1Item {
+    const uint32_t* indices;    // nullptr = non-indexed
+    const float* positions;
+    uint64_t cumm;
+    uint32_t numTriangles, instOffset, instCount;
+    uint32_t firstBatch, endBatch;
+    uint32_t bits, indexMin;    // IX_PU16
+    float cfx, cfy, cfz, mnx, mny, mnz;
 };
+
+__device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item) {
+    const auto items = kconst(P.items);
+    S1Item r;
+    r.indices = items[item].indices;
+    r.positions = items[item].positions;
+    r.cumm = items[item].cumm;
+    r.numTriangles = items[item].numTriangles;
+    r.instOffset = items[item].instOffset;
+    r.instCount = items[item].instCount;
+    r.firstBatch = items[item].firstBatch;
+    r.endBatch = item + 1u < P.numItems ? items[item + 1u].firstBatch : P.numBatches;
+    r.bits = items[item].bitsPerIndex;
+    r.indexMin = items[item].indexMin;
+    r.cfx = items[item].cf[0];
+    r.cfy = items[item].cf[1];
+    r.cfz = items[item].cf[2];
+    r.mnx = items[item].amin[0];
+    r.mny = items[item].amin[1];
+    r.mnz = items[item].amin[2];
+    return r;
+}
 
 // BitEdit::readU32 (BitEdit.h:8-26): `bits` bits (1..32) at bit `off`, the
 // second word read only when the value straddles it.
@@ -317,36 +346,29 @@ __device__ __forceinline__ uint32_t bit_read(const uint32_t* data, uint64_t off,
     return (uint32_t)(combined >> sh) & mask;
 }
 
-// Locate `batch`'s item (advancing from `cur` when possible) and issue the
-// index loads of this lane's triangle.  Lanes past the item's last triangle
-// load the last triangle instead (a valid address) and are masked later.
-// PU16: the index buffer is bit-packed (getVertex<INDEXBUFFER, IX_PU16>,
-// triangles_visbuffer.cu:48-49).
+// Issue the index loads of triangle `tri` of the item.  Triangles past the
+// item's end load its last triangle instead (a valid address) and are masked
+// by the caller.  PU16: the index buffer is bit-packed
+// (getVertex<INDEXBUFFER, IX_PU16>, triangles_visbuffer.cu:48-49).
 template <bool PU16>
-__device__ __forceinline__ void s1_fetch(const RasterParams& P, uint32_t batch, S1Batch& b, bool haveItem) {
-    if (!haveItem || batch >= b.itemEnd) {
-        b.item = find_item(P, batch);
-        const auto items = kconst(P.items);
-        b.it = items[b.item];
-        b.itemEnd = (b.item + 1u < P.numItems) ? items[b.item + 1u].firstBatch : P.numBatches;
-    }
-    b.tri = ((batch - b.it.firstBatch) << P.batchShift) + threadIdx.x;
-    const uint32_t t = min(b.tri, b.it.numTriangles - 1u);
+__device__ __forceinline__ void s1_indices(const S1Item& it, uint32_t tri, uint32_t& i0, uint32_t& i1,
+                                           uint32_t& i2) {
+    const uint32_t t = min(tri, it.numTriangles - 1u);
     if (PU16) {
-        const uint32_t bits = b.it.bitsPerIndex;
-        const uint64_t off = (uint64_t)bits * (3ull * t);
-        b.i0 = bit_read(b.it.indices, off, bits) + b.it.indexMin;
-        b.i1 = bit_read(b.it.indices, off + bits, bits) + b.it.indexMin;
-        b.i2 = bit_read(b.it.indices, off + 2u * bits, bits) + b.it.indexMin;
-    } else if (b.it.indices) {
-        const auto ip = kglobal(b.it.indices) + 3ull * t;
-        b.i0 = ip[0];
-        b.i1 = ip[1];
-        b.i2 = ip[2];
+        const uint64_t off = (uint64_t)it.bits * (3ull * t);
+        i0 = bit_read(it.indices, off, it.bits) + it.indexMin;
+        i1 = bit_read(it.indices, off + it.bits, it.bits) + it.indexMin;
+        i2 = bit_read(it.indices, off + 2u * it.bits, it.bits) + it.indexMin;
     } else {
-        b.i0 = 3u * t;
-        b.i1 = b.i0 + 1u;
-        b.i2 = b.i0 + 2u;
+        // A non-indexed item (vertices 3t..3t+2) loads from its positions
+        // instead, a valid address whose value is discarded: the load is then
+        // issued on every path and the vertex waits can leave it in flight.
+        const bool indexed = it.indices != nullptr;
+        const auto ip = kglobal(indexed ? it.indices : reinterpret_cast<const uint32_t*>(it.positions)) + 3ull * t;
+        const uint32_t r0 = ip[0], r1 = ip[1], r2 = ip[2];
+        i0 = indexed ? r0 : 3u * t;
+        i1 = indexed ? r1 : 3u * t + 1u;
+        i2 = indexed ? r2 : 3u * t + 2u;
     }
 }
 
@@ -362,29 +384,25 @@ __device__ __forceinline__ f3 load_pos_pu16(const float* positions, uint32_t i, 
     return f3{(float)q[0] * cf[0] + amin[0], (float)q[1] * cf[1] + amin[1], (float)q[2] * cf[2] + amin[2]};
 }
 
+// Vertex i of the item (IX_PU16: decoded as load_pos_pu16 does, from the
+// item's scalar fields).
 template <bool PU16>
-__device__ __forceinline__ f3 s1_vertex(const S1Batch& b, uint32_t i) {
-    if (PU16) return load_pos_pu16(b.it.positions, i, b.it.cf, b.it.amin);
-    return load_pos_g(b.it.positions, i);
+__device__ __forceinline__ f3 s1_vertex(const S1Item& it, uint32_t i) {
+    if (PU16) {
+        const auto q = kglobal(reinterpret_cast<const uint16_t*>(it.positions)) + 3ull * i;
+        return f3{(float)q[0] * it.cfx + it.mnx, (float)q[1] * it.cfy + it.mny, (float)q[2] * it.cfz + it.mnz};
+    }
+    return load_pos_g(it.positions, i);
 }
 
-}  // namespace
-
-// Stage 1.  A batch is BLOCK consecutive triangles of one draw item, lane t of
-// the workgroup takes triangle t (a wave's index load reads 768 contiguous
-// bytes).  The loop is software-pipelined: the index loads of the next batch
-// are issued before the vertex loads of the current one complete, so one
-// index-fetch latency is hidden behind every batch.
-//
-// Batch order (P.order): 0 gives each XCD a contiguous share of the batch list
-// (workgroup b is dispatched to XCD b % 8, so blocks b, b+8, ... split the
-// share), keeping the vertices neighbouring batches share in one L2; 1 is a
-// plain grid-stride loop; 2 gives each workgroup one contiguous range.
-template <int BLOCK, bool PU16>
-__global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
+// The batches of this workgroup: [begin, end) with stride `step`.
+// Workgroup b is dispatched to XCD b % 8, and each XCD has its own L2.
+// P.order 0 gives each XCD a contiguous share of the batch list (blocks
+// b, b+8, ... split the share); 1 is a plain grid-stride loop; 2 gives each
+// workgroup one contiguous range.
+__device__ __forceinline__ void s1_range(const RasterParams& P, uint32_t& begin, uint32_t& end, uint32_t& step) {
     const uint32_t numBatches = P.numBatches;
     const uint32_t G = gridDim.x;
-    uint32_t begin, end, step;
     if (P.order == 0) {
         const uint32_t perXcd = (numBatches + 7u) >> 3;
         begin = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
@@ -400,41 +418,53 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
         end = min(numBatches, begin + per);
         step = 1;
     }
-    if (begin >= end) return;
-    const auto transforms = kconst(P.view0.transforms);
-
-    S1Batch cur;
-    s1_fetch<PU16>(P, begin, cur, false);
-    for (uint32_t batch = begin;;) {
-        // vertex loads of this batch
-        const f3 A = s1_vertex<PU16>(cur, cur.i0);
-        const f3 B = s1_vertex<PU16>(cur, cur.i1);
-        const f3 C = s1_vertex<PU16>(cur, cur.i2);
-        // index loads of the next batch
-        const uint32_t next = batch + step;
-        const bool more = next < end;
-        S1Batch nxt = cur;
-        if (more) s1_fetch<PU16>(P, next, nxt, true);
-
-        if (cur.tri < cur.it.numTriangles) {
-            const uint32_t tri = cur.tri;
-            for (uint32_t j = 0; j < cur.it.instCount; j++) {
-                const uint32_t inst = cur.it.instOffset + j;
-                const auto T = transforms + inst;
-                float m[16];
-                #pragma unroll
-                for (int e = 0; e < 16; e++) m[e] = T->m[e];
-                const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
-                stage1_triangle(P, P.view0, m, T->flip != 0u, A, B, C, id, ((uint64_t)inst << 32) | (uint64_t)tri);
-            }
-        }
-        if (!more) break;
-        batch = next;
-        cur = nxt;
-    }
 }
 
-namespace {
+// The batch loop of both stage-1 kernels.  A batch is BLOCK*K consecutive
+// triangles of one draw item; lane t of the workgroup takes triangles
+// t, t + BLOCK, ..., t + (K-1)*BLOCK (each of a wave's index loads reads 768
+// contiguous bytes).  The loop is software-pipelined: the index loads of the
+// next batch are issued right after the vertex loads of the current one and
+// complete behind its arithmetic.  They are issued on every iteration (the
+// last one reloads its own batch), so the waits for the vertices never have
+// to cover them.  body(it, base, A, B, C) draws the batch whose first
+// triangle is `base` within item `it`.
+template <int BLOCK, int K, bool PU16, typename Body>
+__device__ __forceinline__ void s1_sweep(const RasterParams& P, Body&& body) {
+    uint32_t begin, end, step;
+    s1_range(P, begin, end, step);
+    uint32_t batch = begin;
+    if (batch >= end) return;
+    S1Item it = s1_item(P, find_item(P, batch));
+    uint32_t i0[K], i1[K], i2[K];
+    {
+        const uint32_t base = (batch - it.firstBatch) << P.batchShift;
+        #pragma unroll
+        for (int k = 0; k < K; k++) s1_indices<PU16>(it, base + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
+    }
+    for (;;) {
+        f3 A[K], B[K], C[K];
+        #pragma unroll
+        for (int k = 0; k < K; k++) {
+            A[k] = s1_vertex<PU16>(it, i0[k]);
+            B[k] = s1_vertex<PU16>(it, i1[k]);
+            C[k] = s1_vertex<PU16>(it, i2[k]);
+        }
+        const uint32_t next = batch + step;
+        const bool more = next < end && next > batch;
+        const uint32_t nb = more ? next : batch;
+        const S1Item nit = nb < it.endBatch ? it : s1_item(P, find_item(P, nb));
+        {
+            const uint32_t nbase = (nb - nit.firstBatch) << P.batchShift;
+            #pragma unroll
+            for (int k = 0; k < K; k++) s1_indices<PU16>(nit, nbase + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
+        }
+        body(it, (batch - it.firstBatch) << P.batchShift, A, B, C);
+        if (!more) break;
+        batch = next;
+        it = nit;
+    }
+}
 
 // Order-preserving integer key of a float: signed-integer order of the keys
 // is the numeric order of the non-NaN floats (-0 sorts just below +0).  A
@@ -570,7 +600,57 @@ __device__ __forceinline__ ViewCull lane_view_cull(const RasterParams& P, uint32
     return c;
 }
 
+// Object-space box of the 64 triangles (a, b, c) of the wave, reduced on
+// order-preserving integer keys.  Returns false when a coordinate is NaN or
+// infinite: then there is no box and no box test may drop the wave.
+__device__ __forceinline__ bool wave_box(const f3& A, const f3& B, const f3& C, WaveBox& box) {
+    int32_t klo[3], khi[3];
+    const int32_t ax = fkey(A.x), ay = fkey(A.y), az = fkey(A.z);
+    const int32_t bx = fkey(B.x), by = fkey(B.y), bz = fkey(B.z);
+    const int32_t cx = fkey(C.x), cy = fkey(C.y), cz = fkey(C.z);
+    klo[0] = wave_min_i(min(ax, min(bx, cx)));
+    klo[1] = wave_min_i(min(ay, min(by, cy)));
+    klo[2] = wave_min_i(min(az, min(bz, cz)));
+    khi[0] = wave_max_i(max(ax, max(bx, cx)));
+    khi[1] = wave_max_i(max(ay, max(by, cy)));
+    khi[2] = wave_max_i(max(az, max(bz, cz)));
+    const bool unbounded = min(klo[0], min(klo[1], klo[2])) <= kKeyNegInf ||
+                           max(khi[0], max(khi[1], khi[2])) >= kKeyPosInf;
+    box.lo = f3{fkey_inv(klo[0]), fkey_inv(klo[1]), fkey_inv(klo[2])};
+    box.hi = f3{fkey_inv(khi[0]), fkey_inv(khi[1]), fkey_inv(khi[2])};
+    box.M = fmaxf(fmaxf(fmaxf(fabsf(box.lo.x), fabsf(box.hi.x)), fmaxf(fabsf(box.lo.y), fabsf(box.hi.y))),
+                  fmaxf(fabsf(box.lo.z), fabsf(box.hi.z)));
+    return !unbounded;
+}
+
 }  // namespace
+
+// Stage 1 (stage1_drawSmallTriangles, triangles_visbuffer.cu:274-384): every
+// lane draws its K triangles into the single view, once per instance of the
+// item.
+template <int BLOCK, int K, bool PU16>
+__global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
+    const auto transforms = kconst(P.view0.transforms);
+    s1_sweep<BLOCK, K, PU16>(P, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
+                                    const f3 (&C)[K]) __attribute__((always_inline)) {
+        for (uint32_t j = 0; j < it.instCount; j++) {
+            const uint32_t inst = it.instOffset + j;
+            const auto T = transforms + inst;
+            float m[16];
+            #pragma unroll
+            for (int e = 0; e < 16; e++) m[e] = T->m[e];
+            const bool flip = T->flip != 0u;
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
+                const uint32_t tri = base + k * BLOCK + threadIdx.x;
+                if (tri < it.numTriangles) {
+                    const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(j * it.numTriangles);
+                    stage1_triangle(P, P.view0, m, flip, A[k], B[k], C[k], id, ((uint64_t)inst << 32) | (uint64_t)tri);
+                }
+            }
+        }
+    });
+}
 
 // Stage 1 of a multi-view draw.  Each batch of triangles is fetched once and
 // drawn into every view that may see it: a wave reduces the object-space box
@@ -579,101 +659,55 @@ __device__ __forceinline__ ViewCull lane_view_cull(const RasterParams& P, uint32
 // draw), and the wave runs the exact per-triangle code of the single-view
 // stage 1 for the views left in the ballot.  Queue entries carry the view in
 // the top 8 bits of their instance field.
-template <int BLOCK, bool PU16>
+template <int BLOCK, int K, bool PU16>
 __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
-    const uint32_t numBatches = P.numBatches;
-    const uint32_t G = gridDim.x;
-    uint32_t begin, end, step;
-    if (P.order == 0) {
-        const uint32_t perXcd = (numBatches + 7u) >> 3;
-        begin = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
-        end = min(numBatches, (blockIdx.x & 7u) * perXcd + perXcd);
-        step = G >> 3;
-    } else if (P.order == 1) {
-        begin = blockIdx.x;
-        end = numBatches;
-        step = G;
-    } else {
-        const uint32_t per = (numBatches + G - 1u) / G;
-        begin = blockIdx.x * per;
-        end = min(numBatches, begin + per);
-        step = 1;
-    }
-    if (begin >= end) return;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t numViews = P.numViews;
     const bool cacheCull = numViews <= (uint32_t)kWave;   // lane v keeps view v's planes
     ViewCull cull{};
     uint32_t cullInst = 0xffffffffu;
 
-    S1Batch cur;
-    s1_fetch<PU16>(P, begin, cur, false);
-    for (uint32_t batch = begin;;) {
-        const f3 A = s1_vertex<PU16>(cur, cur.i0);
-        const f3 B = s1_vertex<PU16>(cur, cur.i1);
-        const f3 C = s1_vertex<PU16>(cur, cur.i2);
-        const uint32_t next = batch + step;
-        const bool more = next < end;
-        S1Batch nxt = cur;
-        if (more) s1_fetch<PU16>(P, next, nxt, true);
+    s1_sweep<BLOCK, K, PU16>(P, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
+                                    const f3 (&C)[K]) __attribute__((always_inline)) {
+        #pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t tri = base + k * BLOCK + threadIdx.x;
+            const bool valid = tri < it.numTriangles;
+            if (__ballot(valid) == 0ull) continue;
+            // object-space box of the wave's triangles (lanes past the item's
+            // end hold its last triangle, which is inside the mesh anyway)
+            WaveBox box;
+            const bool unbounded = !wave_box(A[k], B[k], C[k], box);
 
-        // object-space box of the wave's triangles (lanes past the item's end
-        // hold its last triangle, which is inside the mesh anyway), reduced on
-        // order-preserving integer keys
-        WaveBox box;
-        int32_t klo[3], khi[3];
-        {
-            const int32_t ax = fkey(A.x), ay = fkey(A.y), az = fkey(A.z);
-            const int32_t bx = fkey(B.x), by = fkey(B.y), bz = fkey(B.z);
-            const int32_t cx = fkey(C.x), cy = fkey(C.y), cz = fkey(C.z);
-            klo[0] = wave_min_i(min(ax, min(bx, cx)));
-            klo[1] = wave_min_i(min(ay, min(by, cy)));
-            klo[2] = wave_min_i(min(az, min(bz, cz)));
-            khi[0] = wave_max_i(max(ax, max(bx, cx)));
-            khi[1] = wave_max_i(max(ay, max(by, cy)));
-            khi[2] = wave_max_i(max(az, max(bz, cz)));
-        }
-        // a NaN or infinite coordinate anywhere: no box, every view draws the wave
-        const bool unbounded = min(klo[0], min(klo[1], klo[2])) <= kKeyNegInf ||
-                               max(khi[0], max(khi[1], khi[2])) >= kKeyPosInf;
-        box.lo = f3{fkey_inv(klo[0]), fkey_inv(klo[1]), fkey_inv(klo[2])};
-        box.hi = f3{fkey_inv(khi[0]), fkey_inv(khi[1]), fkey_inv(khi[2])};
-        box.M = fmaxf(fmaxf(fmaxf(fabsf(box.lo.x), fabsf(box.hi.x)), fmaxf(fabsf(box.lo.y), fabsf(box.hi.y))),
-                      fmaxf(fabsf(box.lo.z), fabsf(box.hi.z)));
-
-        const bool valid = cur.tri < cur.it.numTriangles;
-        const uint32_t tri = cur.tri;
-        for (uint32_t j = 0; j < cur.it.instCount; j++) {
-            const uint32_t inst = cur.it.instOffset + j;
-            const uint64_t id = cur.it.cumm + (uint64_t)tri + (uint64_t)(j * cur.it.numTriangles);
-            for (uint32_t vb = 0; vb < numViews; vb += kWave) {
-                const uint32_t v = vb + lane;
-                bool maybe = false;
-                if (v < numViews) {
-                    if (!cacheCull || inst != cullInst) cull = lane_view_cull(P, v, inst);
-                    maybe = !cull.skip && (unbounded || view_may_see(cull, box));
-                }
-                if (cacheCull) cullInst = inst;
-                uint64_t mask = __ballot(maybe);
-                while (mask) {
-                    const uint32_t vi = vb + (uint32_t)__builtin_ctzll(mask);
-                    mask &= mask - 1ull;
-                    const ViewParams V = kconst(P.views)[vi];
-                    const auto T = kconst(V.transforms) + inst;
-                    float m[16];
-                    #pragma unroll
-                    for (int e = 0; e < 16; e++) m[e] = T->m[e];
-                    if (valid) {
-                        const uint64_t entry = ((uint64_t)((vi << kViewShift) | inst) << 32) | (uint64_t)tri;
-                        stage1_triangle(P, V, m, T->flip != 0u, A, B, C, id, entry);
+            for (uint32_t j = 0; j < it.instCount; j++) {
+                const uint32_t inst = it.instOffset + j;
+                const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(j * it.numTriangles);
+                for (uint32_t vb = 0; vb < numViews; vb += kWave) {
+                    const uint32_t v = vb + lane;
+                    bool maybe = false;
+                    if (v < numViews) {
+                        if (!cacheCull || inst != cullInst) cull = lane_view_cull(P, v, inst);
+                        maybe = !cull.skip && (unbounded || view_may_see(cull, box));
+                    }
+                    if (cacheCull) cullInst = inst;
+                    uint64_t mask = __ballot(maybe);
+                    while (mask) {
+                        const uint32_t vi = vb + (uint32_t)__builtin_ctzll(mask);
+                        mask &= mask - 1ull;
+                        const ViewParams V = kconst(P.views)[vi];
+                        const auto T = kconst(V.transforms) + inst;
+                        float m[16];
+                        #pragma unroll
+                        for (int e = 0; e < 16; e++) m[e] = T->m[e];
+                        if (valid) {
+                            const uint64_t entry = ((uint64_t)((vi << kViewShift) | inst) << 32) | (uint64_t)tri;
+                            stage1_triangle(P, V, m, T->flip != 0u, A[k], B[k], C[k], id, entry);
+                        }
                     }
                 }
             }
         }
-        if (!more) break;
-        batch = next;
-        cur = nxt;
-    }
+    });
 }
 
 namespace {
@@ -1320,30 +1354,50 @@ hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream) {
-    switch (block * 2u + (pu16 ? 1u : 0u)) {
-        case 256: hipLaunchKernelGGL((k_stage1<128, false>), dim3(grid), dim3(128), 0, stream, P); break;
-        case 257: hipLaunchKernelGGL((k_stage1<128, true>), dim3(grid), dim3(128), 0, stream, P); break;
-        case 512: hipLaunchKernelGGL((k_stage1<256, false>), dim3(grid), dim3(256), 0, stream, P); break;
-        case 513: hipLaunchKernelGGL((k_stage1<256, true>), dim3(grid), dim3(256), 0, stream, P); break;
-        case 1024: hipLaunchKernelGGL((k_stage1<512, false>), dim3(grid), dim3(512), 0, stream, P); break;
-        case 1025: hipLaunchKernelGGL((k_stage1<512, true>), dim3(grid), dim3(512), 0, stream, P); break;
-        default: return hipErrorInvalidValue;
-    }
+namespace {
+
+template <int BLOCK, int K, bool PU16>
+hipError_t launch_s1(const RasterParams& P, uint32_t grid, bool views, hipStream_t stream) {
+    if (views) hipLaunchKernelGGL((k_stage1_views<BLOCK, K, PU16>), dim3(grid), dim3(BLOCK), 0, stream, P);
+    else hipLaunchKernelGGL((k_stage1<BLOCK, K, PU16>), dim3(grid), dim3(BLOCK), 0, stream, P);
     return hipGetLastError();
 }
 
-hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream) {
-    switch (block * 2u + (pu16 ? 1u : 0u)) {
-        case 256: hipLaunchKernelGGL((k_stage1_views<128, false>), dim3(grid), dim3(128), 0, stream, P); break;
-        case 257: hipLaunchKernelGGL((k_stage1_views<128, true>), dim3(grid), dim3(128), 0, stream, P); break;
-        case 512: hipLaunchKernelGGL((k_stage1_views<256, false>), dim3(grid), dim3(256), 0, stream, P); break;
-        case 513: hipLaunchKernelGGL((k_stage1_views<256, true>), dim3(grid), dim3(256), 0, stream, P); break;
-        case 1024: hipLaunchKernelGGL((k_stage1_views<512, false>), dim3(grid), dim3(512), 0, stream, P); break;
-        case 1025: hipLaunchKernelGGL((k_stage1_views<512, true>), dim3(grid), dim3(512), 0, stream, P); break;
+template <int BLOCK, int K>
+hipError_t launch_s1_fmt(const RasterParams& P, uint32_t grid, bool pu16, bool views, hipStream_t stream) {
+    return pu16 ? launch_s1<BLOCK, K, true>(P, grid, views, stream) : launch_s1<BLOCK, K, false>(P, grid, views, stream);
+}
+
+template <int BLOCK>
+hipError_t launch_s1_k(const RasterParams& P, uint32_t k, uint32_t grid, bool pu16, bool views, hipStream_t stream) {
+    switch (k) {
+        case 1: return launch_s1_fmt<BLOCK, 1>(P, grid, pu16, views, stream);
+        case 2: return launch_s1_fmt<BLOCK, 2>(P, grid, pu16, views, stream);
+        case 4: return launch_s1_fmt<BLOCK, 4>(P, grid, pu16, views, stream);
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+}
+
+hipError_t launch_s1_any(const RasterParams& P, uint32_t block, uint32_t k, uint32_t grid, bool pu16, bool views,
+                         hipStream_t stream) {
+    switch (block) {
+        case 128: return launch_s1_k<128>(P, k, grid, pu16, views, stream);
+        case 256: return launch_s1_k<256>(P, k, grid, pu16, views, stream);
+        case 512: return launch_s1_k<512>(P, k, grid, pu16, views, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
+                         hipStream_t stream) {
+    return launch_s1_any(P, block, trisPerLane, grid, pu16, false, stream);
+}
+
+hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
+                               hipStream_t stream) {
+    return launch_s1_any(P, block, trisPerLane, grid, pu16, true, stream);
 }
 
 hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, hipStream_t stream) {

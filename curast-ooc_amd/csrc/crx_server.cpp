@@ -19,7 +19,7 @@
 // nodes before it, and a view's per-node culling (SNTriangles.h:29-105) does
 // not depend on the other views in its batch, so a result does not depend
 // on how requests were batched.
-#include "../../include/crx.h"
+#include "crx.h"
 
 #include <hip/hip_runtime.h>
 

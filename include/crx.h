@@ -183,7 +183,8 @@ typedef struct crx_tuning {
                                      1 = grid-stride, 2 = contiguous share per workgroup */
     uint32_t stage1_fast_tiny;    /* 1 = decide sure tiny-triangle culls from v_rcp_f32 screen
                                      coordinates with a proven error margin (exact either way) */
-    uint32_t reserved;
+    uint32_t stage1_tris_per_lane; /* triangles each stage-1 lane takes per batch: 1, 2 or 4 (a batch
+                                     is stage1_block * this consecutive triangles of one item) */
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */

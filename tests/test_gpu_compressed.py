@@ -50,6 +50,22 @@ def test_instanced_mirrored_compressed(ctx):
         _check(ctx, spec, 800, 600, view, proj, instanced=instanced)
 
 
+def test_instanced_compressed_two_per_lane():
+    """IX_PU16 stage 1 with each lane taking two triangles per batch."""
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = 2
+    c = crx.Context(0, tuning=t)
+    try:
+        pos, idx, col = wl.sphere_mesh(rings=47, segments=95, seed=8)
+        worlds = list(wl.instance_transforms(24, extent=20.0, smin=0.5, smax=2.0, seed=12))
+        spec = SceneSpec().add(CompressedMesh(pos, idx, col), worlds)
+        view, proj = wl.instances_view(extent=20.0, width=800, height=600, distance=30.0)
+        for instanced in (True, False):
+            _check(c, spec, 800, 600, view, proj, instanced=instanced)
+    finally:
+        c.close()
+
+
 def test_closeup_compressed_stage2_stage3(ctx):
     pos, idx, col = wl.heightfield_mesh(129, spacing=1.0, amplitude=6.0, scale=32.0)
     spec = SceneSpec().add(CompressedMesh(pos, idx, col), [crx.identity()])

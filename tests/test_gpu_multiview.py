@@ -79,6 +79,23 @@ def test_more_views_than_lanes(ctx):
     _check_views(ctx, spec, views, oracle_every=7)
 
 
+def test_two_triangles_per_lane_views():
+    """Batches of 2 x 256 triangles (each lane takes two): the per-wave boxes
+    and view ballots run once per half-batch; 70 views, mixed sizes."""
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = 2
+    c = crx.Context(0, tuning=t)
+    try:
+        n = 129
+        pos, spec = _terrain(n, amplitude=8.0, scale=32.0)
+        sizes = ((64, 64), (96, 48), (128, 72), (40, 56), (320, 180))
+        vs = wl.multiview_views(n, pos, count=70, sizes=sizes, seed=78)
+        views = [(W, H, v, p) for (v, p, W, H, _) in vs]
+        _check_views(c, spec, views, oracle_every=5)
+    finally:
+        c.close()
+
+
 def test_instanced_mirrored_views(ctx):
     """Instanced and non-instanced walks with mirrored instances, 5 views
     around the instance cube (one of them looking away from everything)."""

@@ -215,15 +215,39 @@ def test_scene_api_matches_oracle_frame(ctx):
     scene.close()
 
 
-@pytest.mark.parametrize("cull_order,fast_tiny,order,block", [
-    (0, 0, 0, 256), (0, 1, 1, 512), (1, 0, 2, 128), (1, 1, 1, 512), (2, 1, 0, 256), (2, 0, 1, 512)])
-def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, block):
-    """Cull order, the filtered tiny cull, batch order and workgroup size only
-    change the schedule: every variant must reproduce the oracle exactly."""
+@pytest.mark.parametrize("cull_order,fast_tiny,order,block,per_lane", [
+    (0, 0, 0, 256, 1), (0, 1, 1, 512, 2), (1, 0, 2, 128, 2), (1, 1, 1, 512, 1), (2, 1, 0, 256, 2),
+    (2, 0, 1, 512, 1), (1, 1, 1, 256, 2), (1, 1, 0, 128, 1), (1, 1, 2, 256, 4), (0, 1, 1, 512, 4),
+    (2, 1, 0, 128, 4)])
+def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, block, per_lane):
+    """Cull order, the filtered tiny cull, batch order, workgroup size and
+    triangles per lane only change the schedule: every variant must
+    reproduce the oracle exactly."""
     pos, idx, col = wl.sphere_mesh(rings=256, segments=512)
     worlds = [crx.identity()] + list(wl.instance_transforms(6, extent=3.0, smin=0.2, smax=0.6, seed=17))
     spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), worlds)
     view, proj = wl.sphere_view(1280, 720, radius=1.3)
     t = crx.default_tuning()
     t.stage1_cull_order, t.stage1_fast_tiny, t.stage1_order, t.stage1_block = cull_order, fast_tiny, order, block
+    t.stage1_tris_per_lane = per_lane
     _check(ctx, spec, 1280, 720, view, proj, tuning=t)
+
+
+@pytest.mark.parametrize("per_lane", [1, 2])
+@pytest.mark.parametrize("instanced", [True, False])
+def test_non_indexed_meshes(ctx, per_lane, instanced):
+    """Non-indexed meshes (triangle t = vertices 3t..3t+2) drawn in one draw
+    with an indexed one, ragged triangle counts so batches end mid-wave."""
+    pos, idx, col = wl.sphere_mesh(rings=61, segments=83, seed=41)
+    flat_pos = pos[idx.reshape(-1)]
+    flat_col = col[idx.reshape(-1)]
+    pos2, idx2, col2 = wl.sphere_mesh(rings=17, segments=29, seed=42)
+    worlds = list(wl.instance_transforms(5, extent=4.0, smin=0.4, smax=1.0, seed=43))
+    spec = (SceneSpec()
+            .add(orc.HostMesh(crx, flat_pos, None, flat_col), worlds[:3])
+            .add(orc.HostMesh(crx, pos2, idx2, col2), worlds[3:])
+            .add(orc.HostMesh(crx, pos2[idx2[:997].reshape(-1)], None, col2[idx2[:997].reshape(-1)]), [crx.identity()]))
+    view = crx.look_at((2.0, 1.5, 9.0), (2.0, 2.0, 2.0))
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = per_lane
+    _check(ctx, spec, 800, 600, view, wl.projection(800, 600), instanced=instanced, tuning=t)
