@@ -265,7 +265,11 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
             return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
         DrawItem& it = D.items[k];
-        it.indices = m.indices;
+        // a non-indexed item reads its positions in place of an index buffer
+        // (a valid address of at least 36 bytes per triangle) and masks the
+        // words off, so stage 1 issues the same loads for both kinds
+        it.indices = m.indices ? m.indices : reinterpret_cast<const uint32_t*>(m.positions);
+        it.indexMask = m.indices ? 0xFFFFFFFFu : 0u;
         it.positions = reinterpret_cast<const float*>(m.positions);
         it.cumm = m.cummulativeTriangleCount;
         it.numTriangles = m.numTriangles;

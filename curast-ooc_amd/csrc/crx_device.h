@@ -19,7 +19,8 @@ namespace crx {
 // over instCount consecutive instances.  Non-instanced draws: one unit per
 // instance (instCount == 1).
 struct DrawItem {
-    const uint32_t* indices;    // nullptr = non-indexed; IX_PU16: bit-packed (BitEdit layout)
+    const uint32_t* indices;    // IX_PU16: bit-packed (BitEdit layout); non-indexed: the positions
+    uint32_t indexMask;         // ~0 = indexed, 0 = non-indexed (triangle t = vertices 3t..3t+2)
     const float* positions;     // xyz, 12-byte stride; IX_PU16: 3 x uint16, 6-byte stride
     uint64_t cumm;              // cummulativeTriangleCount of the first instance
     uint32_t numTriangles;

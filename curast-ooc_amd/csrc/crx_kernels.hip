@@ -302,9 +302,10 @@ __device__ __forceinline__ uint32_t find_item(const RasterParams& P, uint32_t ba
 // Uniform fields of the draw item a batch belongs to, held in scalar
 // registers (plain scalars, so the compiler never stages them in memory).
 struct S1Item {
-    const uint32_t* indices;    // nullptr = non-indexed
+    const uint32_t* indices;    // non-indexed: the positions
     const float* positions;
     uint64_t cumm;
+    uint32_t indexMask;         // ~0 = indexed, 0 = non-indexed
     uint32_t numTriangles, instOffset, instCount;
     uint32_t firstBatch, endBatch;
     uint32_t bits, indexMin;    // IX_PU16
@@ -315,6 +316,7 @@ __device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item) 
     const auto items = kconst(P.items);
     S1Item r;
     r.indices = items[item].indices;
+    r.indexMask = items[item].indexMask;
     r.positions = items[item].positions;
     r.cumm = items[item].cumm;
     r.numTriangles = items[item].numTriangles;
@@ -360,15 +362,16 @@ __device__ __forceinline__ void s1_indices(const S1Item& it, uint32_t tri, uint3
         i1 = bit_read(it.indices, off + it.bits, it.bits) + it.indexMin;
         i2 = bit_read(it.indices, off + 2u * it.bits, it.bits) + it.indexMin;
     } else {
-        // A non-indexed item (vertices 3t..3t+2) loads from its positions
-        // instead, a valid address whose value is discarded: the load is then
-        // issued on every path and the vertex waits can leave it in flight.
-        const bool indexed = it.indices != nullptr;
-        const auto ip = kglobal(indexed ? it.indices : reinterpret_cast<const uint32_t*>(it.positions)) + 3ull * t;
+        // A non-indexed item (vertices 3t..3t+2) points `indices` at its
+        // positions and has a zero mask: the words read there are discarded.
+        // The load is one dwordx3 on every path, so the vertex waits can
+        // leave it in flight (the mask is data, not a branch).
+        const auto ip = kglobal(it.indices) + 3ull * t;
         const uint32_t r0 = ip[0], r1 = ip[1], r2 = ip[2];
-        i0 = indexed ? r0 : 3u * t;
-        i1 = indexed ? r1 : 3u * t + 1u;
-        i2 = indexed ? r2 : 3u * t + 2u;
+        const uint32_t mk = it.indexMask;
+        i0 = (r0 & mk) | ((3u * t) & ~mk);
+        i1 = (r1 & mk) | ((3u * t + 1u) & ~mk);
+        i2 = (r2 & mk) | ((3u * t + 2u) & ~mk);
     }
 }
 
@@ -420,51 +423,85 @@ __device__ __forceinline__ void s1_range(const RasterParams& P, uint32_t& begin,
     }
 }
 
+// Index loads of the K triangles of `batch` (an item-relative triangle base
+// from the item's first batch) for this lane.
+template <int BLOCK, int K, bool PU16>
+__device__ __forceinline__ void s1_batch_indices(const RasterParams& P, const S1Item& it, uint32_t batch,
+                                                 uint32_t (&i0)[K], uint32_t (&i1)[K], uint32_t (&i2)[K]) {
+    const uint32_t base = (batch - it.firstBatch) << P.batchShift;
+    #pragma unroll
+    for (int k = 0; k < K; k++) s1_indices<PU16>(it, base + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
+}
+
+template <int K, bool PU16>
+__device__ __forceinline__ void s1_batch_vertices(const S1Item& it, const uint32_t (&i0)[K], const uint32_t (&i1)[K],
+                                                  const uint32_t (&i2)[K], f3 (&A)[K], f3 (&B)[K], f3 (&C)[K]) {
+    #pragma unroll
+    for (int k = 0; k < K; k++) {
+        A[k] = s1_vertex<PU16>(it, i0[k]);
+        B[k] = s1_vertex<PU16>(it, i1[k]);
+        C[k] = s1_vertex<PU16>(it, i2[k]);
+    }
+}
+
+// The next batch of the sweep after `batch`, or `batch` itself (reloaded,
+// a valid address) when there is none; *more tells which.
+__device__ __forceinline__ uint32_t s1_next(uint32_t batch, uint32_t step, uint32_t end, bool* more) {
+    const uint32_t next = batch + step;
+    *more = next < end && next > batch;
+    return *more ? next : batch;
+}
+
 // The batch loop of both stage-1 kernels.  A batch is BLOCK*K consecutive
 // triangles of one draw item; lane t of the workgroup takes triangles
 // t, t + BLOCK, ..., t + (K-1)*BLOCK (each of a wave's index loads reads 768
-// contiguous bytes).  The loop is software-pipelined: the index loads of the
-// next batch are issued right after the vertex loads of the current one and
-// complete behind its arithmetic.  They are issued on every iteration (the
-// last one reloads its own batch), so the waits for the vertices never have
-// to cover them.  body(it, base, A, B, C) draws the batch whose first
-// triangle is `base` within item `it`.
+// contiguous bytes).  The loop is software-pipelined two deep: while batch i
+// is drawn, the vertex loads of batch i+1 and the index loads of batch i+2
+// are in flight, so a wave's memory latency overlaps its own arithmetic.
+// All loads are issued on every iteration (past the last batch they reload
+// it), so the waits for batch i never have to cover the younger loads.
+// body(it, base, A, B, C) draws the batch whose first triangle is `base`
+// within item `it`.
 template <int BLOCK, int K, bool PU16, typename Body>
 __device__ __forceinline__ void s1_sweep(const RasterParams& P, Body&& body) {
     uint32_t begin, end, step;
     s1_range(P, begin, end, step);
+    if (begin >= end) return;
     uint32_t batch = begin;
-    if (batch >= end) return;
     S1Item it = s1_item(P, find_item(P, batch));
     uint32_t i0[K], i1[K], i2[K];
-    {
-        const uint32_t base = (batch - it.firstBatch) << P.batchShift;
-        #pragma unroll
-        for (int k = 0; k < K; k++) s1_indices<PU16>(it, base + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
-    }
+    s1_batch_indices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2);
+    bool more;
+    uint32_t b1 = s1_next(batch, step, end, &more);
+    S1Item it1 = b1 < it.endBatch ? it : s1_item(P, find_item(P, b1));
+    uint32_t j0[K], j1[K], j2[K];
+    s1_batch_indices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2);
+    f3 A[K], B[K], C[K];
+    s1_batch_vertices<K, PU16>(it, i0, i1, i2, A, B, C);
     for (;;) {
-        f3 A[K], B[K], C[K];
-        #pragma unroll
-        for (int k = 0; k < K; k++) {
-            A[k] = s1_vertex<PU16>(it, i0[k]);
-            B[k] = s1_vertex<PU16>(it, i1[k]);
-            C[k] = s1_vertex<PU16>(it, i2[k]);
-        }
-        const uint32_t next = batch + step;
-        const bool more = next < end && next > batch;
-        const uint32_t nb = more ? next : batch;
-        const S1Item nit = nb < it.endBatch ? it : s1_item(P, find_item(P, nb));
-        {
-            const uint32_t nbase = (nb - nit.firstBatch) << P.batchShift;
-            #pragma unroll
-            for (int k = 0; k < K; k++) s1_indices<PU16>(nit, nbase + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
-        }
+        f3 An[K], Bn[K], Cn[K];
+        s1_batch_vertices<K, PU16>(it1, j0, j1, j2, An, Bn, Cn);
+        bool more2;
+        const uint32_t b2 = more ? s1_next(b1, step, end, &more2) : b1;
+        if (!more) more2 = false;
+        const S1Item it2 = b2 < it1.endBatch ? it1 : s1_item(P, find_item(P, b2));
+        s1_batch_indices<BLOCK, K, PU16>(P, it2, b2, j0, j1, j2);
         body(it, (batch - it.firstBatch) << P.batchShift, A, B, C);
         if (!more) break;
-        batch = next;
-        it = nit;
+        batch = b1;
+        it = it1;
+        #pragma unroll
+        for (int k = 0; k < K; k++) {
+            A[k] = An[k];
+            B[k] = Bn[k];
+            C[k] = Cn[k];
+        }
+        b1 = b2;
+        it1 = it2;
+        more = more2;
     }
 }
+
 
 // Order-preserving integer key of a float: signed-integer order of the keys
 // is the numeric order of the non-NaN floats (-0 sorts just below +0).  A
