@@ -233,7 +233,7 @@ def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, bloc
     _check(ctx, spec, 1280, 720, view, proj, tuning=t)
 
 
-@pytest.mark.parametrize("per_lane", [1, 2])
+@pytest.mark.parametrize("per_lane", [1, 2, 4])
 @pytest.mark.parametrize("instanced", [True, False])
 def test_non_indexed_meshes(ctx, per_lane, instanced):
     """Non-indexed meshes (triangle t = vertices 3t..3t+2) drawn in one draw
@@ -251,3 +251,20 @@ def test_non_indexed_meshes(ctx, per_lane, instanced):
     t = crx.default_tuning()
     t.stage1_tris_per_lane = per_lane
     _check(ctx, spec, 800, 600, view, wl.projection(800, 600), instanced=instanced, tuning=t)
+
+
+@pytest.mark.parametrize("per_lane", [1, 2])
+def test_random_index_order(ctx, per_lane):
+    """Triangles in random order (no locality between neighbouring lanes)
+    plus degenerate triangles that repeat an index."""
+    pos, idx, col = wl.sphere_mesh(rings=96, segments=128, seed=44)
+    rng = np.random.default_rng(45)
+    perm = rng.permutation(len(idx))
+    idx_r = idx[perm].copy()
+    idx_r[::17, 1] = idx_r[::17, 0]                     # degenerate: a repeated index
+    worlds = [crx.identity()]
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx_r, col), worlds)
+    view, proj = wl.sphere_view(960, 540, radius=1.2)
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = per_lane
+    _check(ctx, spec, 960, 540, view, proj, tuning=t)
