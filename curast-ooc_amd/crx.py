@@ -126,6 +126,32 @@ class HugeTriangle(C.Structure):
                 ("tile_x", C.c_int32), ("tile_y", C.c_int32)]
 
 
+class RasterArgs(C.Structure):
+    """RasterArgs (HostDeviceInterface.h:309-323) plus the two queue
+    capacities; every pointer is a device address."""
+    _fields_ = [
+        ("meshes", C.c_void_p),
+        ("numMeshes", C.c_uint32),
+        ("instances", C.c_void_p),
+        ("numInstances", C.c_uint32),
+        ("transforms", C.c_void_p),
+        ("numProcessedBatches", C.c_void_p),
+        ("numProcessedBatches_nontrivial", C.c_void_p),
+        ("hugeTriangles", C.c_void_p),
+        ("hugeTrianglesCounter", C.c_void_p),
+        ("numProcessedHugeTriangles", C.c_void_p),
+        ("nontrivialTrianglesCounter", C.c_void_p),
+        ("nontrivialTrianglesList", C.c_void_p),
+        ("target", RenderTarget),
+        ("nontrivialCapacity", C.c_uint32),
+        ("hugeCapacity", C.c_uint32),
+    ]
+
+
+RASTER_INSTANCED = 1
+RASTER_COMPRESSED = 2
+
+
 class Tuning(C.Structure):
     _fields_ = [("small_threshold", C.c_uint32), ("tile_size", C.c_uint32),
                 ("max_nontrivial", C.c_uint32), ("max_huge", C.c_uint32),
@@ -225,6 +251,9 @@ _SIGS = {
     "crx_stream_synchronize": (C.c_int, [_VP, _VP]),
     "crx_copy_buffer": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
+    "crx_raster_stage1": (C.c_int, [_VP, C.POINTER(RasterArgs), C.c_uint32, _VP]),
+    "crx_raster_stage2": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
+    "crx_raster_stage3": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
     "crx_draw_triangles_visbuffer": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(CMesh),
                                                C.c_uint32, C.POINTER(Mat4), C.POINTER(RenderTarget),
                                                C.c_int, _VP, C.POINTER(DrawStats)]),
@@ -458,6 +487,16 @@ class Context:
     def clear_framebuffer(self, fb: DeviceBuffer | int, num_pixels: int, stream=None) -> None:
         ptr = fb.ptr if isinstance(fb, DeviceBuffer) else fb
         _check(lib().crx_clear_framebuffer(C.c_void_p(ptr), num_pixels, stream))
+
+    def raster_stage1(self, args: RasterArgs, flags: int, stream=None) -> None:
+        """kernel_stage1_drawSmallTriangles_* over device CMesh arrays (crx_raster_stage1)."""
+        _check(lib().crx_raster_stage1(self.handle, C.byref(args), int(flags), stream))
+
+    def raster_stage2(self, args: RasterArgs, stream=None) -> None:
+        _check(lib().crx_raster_stage2(self.handle, C.byref(args), stream))
+
+    def raster_stage3(self, args: RasterArgs, stream=None) -> None:
+        _check(lib().crx_raster_stage3(self.handle, C.byref(args), stream))
 
     def draw_triangles_visbuffer(self, meshes, instances, transforms, target: RenderTarget,
                                  instanced: bool, stream=None) -> DrawStats:

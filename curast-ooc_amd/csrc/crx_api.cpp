@@ -199,6 +199,14 @@ uint32_t grid_for(const crx_context* ctx, uint32_t wavesPerCU, uint32_t wavesPer
     return (uint32_t)g;
 }
 
+// Points the four stage counters of P into one CTR_COUNT block.
+void set_counter_block(RasterParams& P, uint32_t* block) {
+    P.ctrNontrivial = block + CTR_NONTRIVIAL;
+    P.ctrHuge = block + CTR_HUGE;
+    P.cursor2 = block + CTR_CURSOR2;
+    P.cursor3 = block + CTR_CURSOR3;
+}
+
 // Per-view record of a draw: size, projection and the view's transforms.
 ViewParams make_view_params(const crx_render_target& t, const Transform* transforms) {
     ViewParams V{};
@@ -337,7 +345,7 @@ RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* d
     P.views = reinterpret_cast<const ViewParams*>(dev + D.offViews);
     P.view0 = views[0];
     P.numViews = D.numViews;
-    P.ctr = ctx->ctr;
+    set_counter_block(P, ctx->ctr);
     P.numItems = D.numItems;
     P.numBatches = (uint32_t)D.batches;
     P.smallThreshold = tu.small_threshold;
@@ -359,7 +367,7 @@ RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* d
 // stage 1 (which reads every view record from memory) for a single view.
 hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParams& P, bool viewsKernel,
                           hipStream_t stream) {
-    hipError_t e = launch_reset_counters(P.ctr, stream);
+    hipError_t e = launch_reset_counters(P.ctrNontrivial - CTR_NONTRIVIAL, stream);   // the plan's counter block
     if (e != hipSuccess) return e;
     record(ctx, T_S1, stream);
     if (D.numViews == 1 && !viewsKernel) e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
@@ -857,6 +865,8 @@ crx_status crx_context_destroy(crx_context* ctx) {
     ctx->drawMeta.release();
     ctx->resolveMeta.release();
     ctx->clearMeta.release();
+    ctx->argsStaging.release();
+    ctx->argsMeta.release();
     if (ctx->ctr) (void)hipFree(ctx->ctr);
     if (ctx->ctrHost) (void)hipHostFree(ctx->ctrHost);
     for (int i = 0; i < T_COUNT; i++)

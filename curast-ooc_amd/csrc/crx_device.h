@@ -101,7 +101,11 @@ struct RasterParams {
     const DrawItem* items;
     const InstanceRec* inst;
     const ViewParams* views;
-    uint32_t* ctr;
+    uint32_t* ctrNontrivial;      // stage-2 entries produced by stage 1
+    uint32_t* ctrHuge;            // stage-3 entries produced by stage 2
+    uint32_t* cursor2;            // stage-2 work cursor
+    uint32_t* cursor3;            // stage-3 work cursor
+    const uint32_t* batchCount;   // non-null: numBatches is read from here (plans built on the device)
     uint64_t* nontrivial;
     int4* huge;
     uint32_t numItems;
@@ -134,6 +138,13 @@ struct ResolveParams {
     float viewI[16];           // inverse(view), GLM order, computed once per frame
     float origin[3];           // viewI * (0,0,0,1)
     uint32_t firstBlock;       // multi-view resolve: first 64x4 tile block of this view
+};
+
+// The five stage counters of the kernel-level entry points (RasterArgs:
+// numProcessedBatches, numProcessedBatches_nontrivial, hugeTrianglesCounter,
+// nontrivialTrianglesCounter, numProcessedHugeTriangles).
+struct CounterSet {
+    uint32_t* p[5];
 };
 
 // One framebuffer of a multi-view clear.

@@ -96,7 +96,7 @@ crx_status frame_fill(crx_frame* F, bool write) {
         return fail(CRX_INVALID_ARG, "the scene's draw layout changed: recreate the frame");
     F->plan = D;
     F->P = plan_write(F->ctx, D, static_cast<char*>(F->stageDraw), static_cast<char*>(F->drawMeta.ptr));
-    F->P.ctr = F->ctr;
+    set_counter_block(F->P, F->ctr);
     F->P.nontrivial = static_cast<uint64_t*>(F->nontrivial.ptr);
     F->P.huge = static_cast<int4*>(F->huge.ptr);
     F->P.ntCap = F->ntCap;

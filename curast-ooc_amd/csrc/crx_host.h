@@ -128,6 +128,19 @@ struct crx_context {
     hipEvent_t ev[crx::T_COUNT] = {};
     bool recorded[crx::T_COUNT] = {};
     uint64_t meshSeq = 0;
+    // kernel-level entry points: the device-built plan of the last
+    // crx_raster_stage1 call, read by crx_raster_stage2/3 of the same args
+    crx::DeviceBuffer argsMeta;
+    crx::Staging argsStaging;
+    struct {
+        bool valid = false;
+        const crx_cmesh* instances = nullptr;
+        const crx_mat4* transforms = nullptr;
+        uint32_t numInstances = 0;
+        uint64_t* framebuffer = nullptr;
+        crx::RasterParams P{};
+        uint32_t g2 = 0, g3 = 0;
+    } args;
 };
 
 struct crx_mesh {
@@ -167,6 +180,9 @@ namespace crx {
 crx_mat4 mat_mul(const crx_mat4& A, const crx_mat4& B);
 crx_mat4 mat_inverse(const crx_mat4& M);
 bool mirror_flag(const crx_mat4& m);
+ViewParams make_view_params(const crx_render_target& t, const Transform* transforms);
+uint32_t grid_for(const crx_context* ctx, uint32_t wavesPerCU, uint32_t wavesPerBlock);
+void set_counter_block(RasterParams& P, uint32_t* block);
 crx_status check_tuning(const crx_tuning& t);
 crx_status check_target(const crx_render_target* t);
 void frustum_planes(const crx_view& view, float out[16]);

@@ -3,12 +3,19 @@
 
 #include <hip/hip_runtime.h>
 
+#include "crx.h"
 #include "crx_device.h"
 
 namespace crx {
 
 hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream);
 hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream);
+hipError_t launch_reset_counter_set(const CounterSet& c, hipStream_t stream);
+// device-built draw plan from caller-owned CMesh arrays (kernel-level entry points)
+hipError_t launch_plan(const crx_cmesh* list, uint32_t numItems, const crx_cmesh* instances,
+                       const crx_mat4* transforms, uint32_t numInstances, int instanced, int pu16,
+                       uint32_t batchShift, DrawItem* items, InstanceRec* inst, Transform* xf, uint32_t* batchCount,
+                       hipStream_t stream);
 // trisPerLane: 1, 2 or 4 (P.batchShift = log2(block * trisPerLane));
 // pu16: the draw's meshes are IX_PU16-compressed
 hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 
+#include "crx.h"
 #include "crx_device.h"
 #include "crx_internal.h"
 
@@ -227,7 +228,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     if (numFragments > P.smallThreshold) nontrivial = true;   // unsigned compare, as the reference
 
     if (nontrivial) {
-        const uint32_t slot = atomicAdd(P.ctr + CTR_NONTRIVIAL, 1u);
+        const uint32_t slot = atomicAdd(P.ctrNontrivial, 1u);
         if (slot < P.ntCap) P.nontrivial[slot] = queueEntry;
         return;
     }
@@ -285,6 +286,108 @@ __global__ __launch_bounds__(64) void k_reset_counters(uint32_t* __restrict__ ct
     if (threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
 }
 
+// Counter reset of the kernel-level entry points: the five counters the
+// reference's stage 1 zeroes before its grid barrier
+// (triangles_visbuffer.cu:280-286), each at its own caller address.
+__global__ __launch_bounds__(64) void k_reset_counter_set(CounterSet c) {
+    if (threadIdx.x < 5u && c.p[threadIdx.x]) c.p[threadIdx.x][0] = 0u;
+}
+
+// Device-built plan of the kernel-level entry points, from caller-owned
+// CMesh arrays (the same records plan_draw builds on the host).  One
+// workgroup: each thread takes a contiguous run of items, sums their batch
+// counts, a workgroup scan turns the sums into each run's first batch, and
+// the thread writes its DrawItems; thread 0 writes the batch count.  Items
+// whose instance range lies outside the instance list, or whose format is
+// not the draw's, are given no triangles.
+__global__ __launch_bounds__(1024) void k_plan_items(const crx_cmesh* __restrict__ list, uint32_t numItems,
+                                                     uint32_t numInstances, int instanced, int pu16,
+                                                     uint32_t batchShift, DrawItem* __restrict__ items,
+                                                     uint32_t* __restrict__ batchCount) {
+    __shared__ uint64_t sums[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (numItems + 1023u) / 1024u;
+    const uint32_t lo = min(numItems, t * per), hi = min(numItems, lo + per);
+    auto triangles = [&](uint32_t k) -> uint32_t {
+        const crx_cmesh& m = list[k];
+        const int64_t off = instanced ? (int64_t)m.instances.offset : (int64_t)k;
+        const int64_t cnt = instanced ? (int64_t)m.instances.count : 1;
+        const bool ok = off >= 0 && cnt >= 0 && off + cnt <= (int64_t)numInstances && (m.compressed != 0) == (pu16 != 0) &&
+                        m.positions != nullptr && (!pu16 || (m.indices != nullptr && m.bitsPerIndex <= 32u)) &&
+                        m.numTriangles <= 1431655765u;
+        return ok ? m.numTriangles : 0u;
+    };
+    uint64_t sum = 0;
+    for (uint32_t k = lo; k < hi; k++) sum += ((uint64_t)triangles(k) + (1ull << batchShift) - 1u) >> batchShift;
+    sums[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint64_t add = t >= d ? sums[t - d] : 0ull;
+        __syncthreads();
+        sums[t] += add;
+        __syncthreads();
+    }
+    uint64_t first = sums[t] - sum;
+    for (uint32_t k = lo; k < hi; k++) {
+        const crx_cmesh& m = list[k];
+        const uint32_t n = triangles(k);
+        DrawItem it{};
+        it.indices = m.indices ? m.indices : reinterpret_cast<const uint32_t*>(m.positions);
+        it.indexMask = m.indices ? 0xFFFFFFFFu : 0u;
+        it.positions = reinterpret_cast<const float*>(m.positions);
+        it.cumm = m.cummulativeTriangleCount;
+        it.numTriangles = n;
+        it.instOffset = instanced ? (uint32_t)m.instances.offset : k;
+        it.instCount = n == 0u ? 0u : (instanced ? (uint32_t)m.instances.count : 1u);
+        it.firstBatch = (uint32_t)min(first, 0xFFFFFFFFull);
+        it.bitsPerIndex = m.bitsPerIndex;
+        it.indexMin = m.index_min;
+        it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
+        it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
+        items[k] = it;
+        first += ((uint64_t)n + (1ull << batchShift) - 1u) >> batchShift;
+    }
+    if (t == 1023u) *batchCount = (uint32_t)min(sums[1023], 0xFFFFFFFFull);
+}
+
+// One lane per instance: its InstanceRec (make_instance on the host) and its
+// Transform with the mirror flag det(worldView[0..2]) < 0 computed in the
+// same float operations (triangles_visbuffer.cu:179-183).
+__global__ __launch_bounds__(256) void k_plan_instances(const crx_cmesh* __restrict__ instances,
+                                                        const crx_mat4* __restrict__ transforms,
+                                                        uint32_t numInstances, InstanceRec* __restrict__ inst,
+                                                        Transform* __restrict__ xf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= numInstances) return;
+    const crx_cmesh& m = instances[i];
+    InstanceRec r{};
+    r.indices = m.indices;
+    r.positions = reinterpret_cast<const float*>(m.positions);
+    r.colors = m.colors;
+    r.normals = reinterpret_cast<const float*>(m.normals);
+    r.uvs = reinterpret_cast<const float*>(m.uvs);
+    r.cumm = m.cummulativeTriangleCount;
+    r.numTriangles = m.numTriangles;
+    const int32_t ti = m.instances.offset;
+    r.transformIndex = (ti >= 0 && (uint32_t)ti < numInstances) ? (uint32_t)ti : i;
+    r.id = m.id;
+    r.isLoaded = m.isLoaded ? 1u : 0u;
+    for (int e = 0; e < 16; e++) r.world[e] = m.world.m[e];
+    r.compressed = m.compressed ? 1u : 0u;
+    r.bitsPerIndex = m.bitsPerIndex;
+    r.indexMin = m.index_min;
+    r.cf[0] = m.compressionFactor.x; r.cf[1] = m.compressionFactor.y; r.cf[2] = m.compressionFactor.z;
+    r.amin[0] = m.aabb.min.x; r.amin[1] = m.aabb.min.y; r.amin[2] = m.aabb.min.z;
+    r.amax[0] = m.aabb.max.x; r.amax[1] = m.aabb.max.y; r.amax[2] = m.aabb.max.z;
+    inst[i] = r;
+    Transform x{};
+    for (int e = 0; e < 16; e++) x.m[e] = transforms[i].m[e];
+    const f3 c0{x.m[0], x.m[1], x.m[2]}, c1{x.m[4], x.m[5], x.m[6]}, c2{x.m[8], x.m[9], x.m[10]};
+    x.flip = dot3(cross3(c0, c1), c2) < 0.0f ? 1u : 0u;
+    x.skip = 0u;
+    xf[i] = x;
+}
+
 namespace {
 
 
@@ -312,7 +415,7 @@ struct S1Item {
     float cfx, cfy, cfz, mnx, mny, mnz;
 };
 
-__device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item) {
+__device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item, uint32_t numBatches) {
     const auto items = kconst(P.items);
     S1Item r;
     r.indices = items[item].indices;
@@ -323,7 +426,7 @@ __device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item) 
     r.instOffset = items[item].instOffset;
     r.instCount = items[item].instCount;
     r.firstBatch = items[item].firstBatch;
-    r.endBatch = item + 1u < P.numItems ? items[item + 1u].firstBatch : P.numBatches;
+    r.endBatch = item + 1u < P.numItems ? items[item + 1u].firstBatch : numBatches;
     r.bits = items[item].bitsPerIndex;
     r.indexMin = items[item].indexMin;
     r.cfx = items[item].cf[0];
@@ -403,15 +506,15 @@ __device__ __forceinline__ f3 s1_vertex(const S1Item& it, uint32_t i) {
 // P.order 0 gives each XCD a contiguous share of the batch list (blocks
 // b, b+8, ... split the share); 1 is a plain grid-stride loop; 2 gives each
 // workgroup one contiguous range.
-__device__ __forceinline__ void s1_range(const RasterParams& P, uint32_t& begin, uint32_t& end, uint32_t& step) {
-    const uint32_t numBatches = P.numBatches;
+__device__ __forceinline__ void s1_range(uint32_t numBatches, uint32_t order, uint32_t& begin, uint32_t& end,
+                                         uint32_t& step) {
     const uint32_t G = gridDim.x;
-    if (P.order == 0) {
+    if (order == 0) {
         const uint32_t perXcd = (numBatches + 7u) >> 3;
         begin = (blockIdx.x & 7u) * perXcd + (blockIdx.x >> 3);
         end = min(numBatches, (blockIdx.x & 7u) * perXcd + perXcd);
         step = G >> 3;                                  // G is a multiple of 8
-    } else if (P.order == 1) {
+    } else if (order == 1) {
         begin = blockIdx.x;
         end = numBatches;
         step = G;
@@ -464,16 +567,17 @@ __device__ __forceinline__ uint32_t s1_next(uint32_t batch, uint32_t step, uint3
 // within item `it`.
 template <int BLOCK, int K, bool PU16, typename Body>
 __device__ __forceinline__ void s1_sweep(const RasterParams& P, Body&& body) {
+    const uint32_t numBatches = P.batchCount ? *kconst(P.batchCount) : P.numBatches;
     uint32_t begin, end, step;
-    s1_range(P, begin, end, step);
+    s1_range(numBatches, P.order, begin, end, step);
     if (begin >= end) return;
     uint32_t batch = begin;
-    S1Item it = s1_item(P, find_item(P, batch));
+    S1Item it = s1_item(P, find_item(P, batch), numBatches);
     uint32_t i0[K], i1[K], i2[K];
     s1_batch_indices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2);
     bool more;
     uint32_t b1 = s1_next(batch, step, end, &more);
-    S1Item it1 = b1 < it.endBatch ? it : s1_item(P, find_item(P, b1));
+    S1Item it1 = b1 < it.endBatch ? it : s1_item(P, find_item(P, b1), numBatches);
     uint32_t j0[K], j1[K], j2[K];
     s1_batch_indices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2);
     f3 A[K], B[K], C[K];
@@ -484,7 +588,7 @@ __device__ __forceinline__ void s1_sweep(const RasterParams& P, Body&& body) {
         bool more2;
         const uint32_t b2 = more ? s1_next(b1, step, end, &more2) : b1;
         if (!more) more2 = false;
-        const S1Item it2 = b2 < it1.endBatch ? it1 : s1_item(P, find_item(P, b2));
+        const S1Item it2 = b2 < it1.endBatch ? it1 : s1_item(P, find_item(P, b2), numBatches);
         s1_batch_indices<BLOCK, K, PU16>(P, it2, b2, j0, j1, j2);
         body(it, (batch - it.firstBatch) << P.batchShift, A, B, C);
         if (!more) break;
@@ -921,7 +1025,7 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = threadIdx.x / kWave;
     S2Setup* setup = sh_setup[wid];
-    const uint32_t count = min(P.ctr[CTR_NONTRIVIAL], P.ntCap);
+    const uint32_t count = min(*P.ctrNontrivial, P.ntCap);
     const int TILE = P.tile;
     // first claim is static (wave w takes entries [64w, 64w+64)), later claims
     // come from the shared cursor, so waves beyond the queue exit without an atomic
@@ -935,7 +1039,7 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
             base = waveId * (uint32_t)kWave;
             first = false;
         } else {
-            if (lane == 0) base = P.numWaves2 * (uint32_t)kWave + atomicAdd(P.ctr + CTR_CURSOR2, (uint32_t)kWave);
+            if (lane == 0) base = P.numWaves2 * (uint32_t)kWave + atomicAdd(P.cursor2, (uint32_t)kWave);
             base = __shfl(base, 0);
         }
         if (base >= count) break;
@@ -1014,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
         const uint32_t total = __shfl(prefix, kWave - 1);
         if (total > 0) {
             uint32_t hbase = 0;
-            if (lane == kWave - 1) hbase = atomicAdd(P.ctr + CTR_HUGE, total);
+            if (lane == kWave - 1) hbase = atomicAdd(P.ctrHuge, total);
             hbase = __shfl(hbase, kWave - 1);
             sh_tiles[wid][lane][0] = (uint32_t)tile_x;
             sh_tiles[wid][lane][1] = (uint32_t)tile_y;
@@ -1102,7 +1206,7 @@ __device__ __forceinline__ float intersect_mt(const MTTri& T, f3 dir) {
 // lane covers TILE/64 columns.
 __global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t count = min(P.ctr[CTR_HUGE], P.hugeCap);
+    const uint32_t count = min(*P.ctrHuge, P.hugeCap);
     const int TILE = P.tile;
     const int lanesPerRow = TILE < kWave ? TILE : kWave;
     const int rowsPerStep = kWave / lanesPerRow;
@@ -1116,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_stage3(RasterParams P) {
             idx = waveId;
             first = false;
         } else {
-            if (lane == 0) idx = P.numWaves3 + atomicAdd(P.ctr + CTR_CURSOR3, 1u);
+            if (lane == 0) idx = P.numWaves3 + atomicAdd(P.cursor3, 1u);
             idx = __shfl(idx, 0);
         }
         if (idx >= count) break;
@@ -1388,6 +1492,24 @@ hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int num
 
 hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counter_set(const CounterSet& c, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset_counter_set, dim3(1), dim3(64), 0, stream, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan(const crx_cmesh* list, uint32_t numItems, const crx_cmesh* instances,
+                       const crx_mat4* transforms, uint32_t numInstances, int instanced, int pu16,
+                       uint32_t batchShift, DrawItem* items, InstanceRec* inst, Transform* xf, uint32_t* batchCount,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(k_plan_items, dim3(1), dim3(1024), 0, stream, list, numItems, numInstances, instanced, pu16,
+                       batchShift, items, batchCount);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_plan_instances, dim3((numInstances + 255u) / 256u), dim3(256), 0, stream, instances,
+                       transforms, numInstances, inst, xf);
     return hipGetLastError();
 }
 

@@ -18,6 +18,11 @@
  *   crx_draw_triangles_visbuffer
  *                          drawTrianglesVisbuffer(...), src/CuRast_render.h:211-264, which launches
  *                          kernel_stage{1,2,3}_* src/kernels/triangles_visbuffer.cu:755-835
+ *   crx_raster_args        RasterArgs     src/kernels/HostDeviceInterface.h:309-323 (same field order,
+ *                          + two queue capacities)
+ *   crx_raster_stage1/2/3  kernel_stage1_drawSmallTriangles_* / kernel_stage2_drawMediumTriangles_* /
+ *                          kernel_stage3_drawHugeTriangles_*, src/kernels/triangles_visbuffer.cu:755-835,
+ *                          as launched by name in src/CuRast_render.h:235-264
  *   crx_resolve_visbuffer  kernel_resolve_visbuffer_to_colorbuffer2D launch, src/CuRast_render.h:580-592,
  *                          kernel src/kernels/resolve.cu:640-1141
  *   crx_scene_* / crx_mesh_* / crx_render
@@ -240,6 +245,56 @@ crx_status crx_draw_triangles_visbuffer(crx_context* ctx,
                                         const crx_render_target* target,
                                         int instanced, void* stream,
                                         crx_draw_stats* stats /* nullable */);
+
+/* ---- Kernel-level entry points: the three stages as separate calls over
+ * device data, for callers that launch kernel_stage{1,2,3}_* themselves. ---- */
+
+/* RasterArgs (HostDeviceInterface.h:309-323), same fields in the same order;
+ * every pointer is a DEVICE address.  meshes / instances are device arrays of
+ * CMesh, transforms a device array of numInstances view * world matrices;
+ * the five counters and the two queues are caller-owned device memory (the
+ * queues hold nontrivialCapacity / hugeCapacity entries: the two fields
+ * after `target` have no counterpart in the reference, whose queues are sized
+ * by MAX_NONTRIVIAL_TRIANGLES / MAX_HUGE_TRIANGLES).  `target` itself is
+ * passed by value, as in RasterArgs. */
+typedef struct crx_raster_args {
+    const crx_cmesh* meshes;
+    uint32_t numMeshes;
+    const crx_cmesh* instances;
+    uint32_t numInstances;
+    const crx_mat4* transforms;
+    uint32_t* numProcessedBatches;
+    uint32_t* numProcessedBatches_nontrivial;
+    crx_huge_triangle* hugeTriangles;
+    uint32_t* hugeTrianglesCounter;
+    uint32_t* numProcessedHugeTriangles;
+    uint32_t* nontrivialTrianglesCounter;
+    uint64_t* nontrivialTrianglesList;
+    crx_render_target target;
+    uint32_t nontrivialCapacity;
+    uint32_t hugeCapacity;
+} crx_raster_args;
+
+#define CRX_RASTER_INSTANCED 1u    /* the _instanced stage-1 kernels: walk `meshes`, loop over instances{offset,count} */
+#define CRX_RASTER_COMPRESSED 2u   /* the _compressed kernels: IX_PU16 meshes */
+
+/* kernel_stage1_drawSmallTriangles_indexbuffer{_compressed,_uncompressed}{_instanced,}
+ * (the flags pick the variant as the kernel name does; non-indexed meshes,
+ * the _noindexbuffer kernels, are handled by every variant).  Zeroes the five
+ * counters, as the reference's stage 1 does before its grid barrier, builds
+ * the draw records from the device CMesh arrays on the device and launches
+ * stage 1 on `stream`.  Asynchronous: it reads no device memory.  Meshes whose
+ * instance range falls outside the instance list, or whose format differs
+ * from the flags, draw nothing.  Entries past a queue's capacity are dropped;
+ * a counter above its capacity after the stage reports that. */
+crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* args, uint32_t flags, void* stream);
+/* kernel_stage2_drawMediumTriangles_* and kernel_stage3_drawHugeTriangles_*:
+ * each consumes its queue on `stream`.  They use the draw records of the last
+ * crx_raster_stage1 on this context, which must have had the same args (same
+ * instances, transforms and framebuffer), with no other draw on the context
+ * in between. */
+crx_status crx_raster_stage2(crx_context* ctx, const crx_raster_args* args, void* stream);
+crx_status crx_raster_stage3(crx_context* ctx, const crx_raster_args* args, void* stream);
 
 /* kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141).
  * `instances` is a HOST array of CMesh, `prefix` a HOST array of the per-

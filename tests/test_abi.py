@@ -51,12 +51,18 @@ CMESH_OFFSETS = {"numTriangles": 0, "indices": 8, "positions": 16, "cummulativeT
                  "compressed": 250, "impostor": 256}
 TARGET_OFFSETS = {"framebuffer": 0, "colorbuffer": 8, "width": 16, "height": 20, "view": 24, "viewI": 88,
                   "proj": 152, "cameraPos": 216, "f": 228, "aspect": 232, "debug": 236}
+# RasterArgs (HostDeviceInterface.h:309-323, 336 B) followed by the two queue capacities.
+ARGS_OFFSETS = {"meshes": 0, "numMeshes": 8, "instances": 16, "numInstances": 24, "transforms": 32,
+                "numProcessedBatches": 40, "numProcessedBatches_nontrivial": 48, "hugeTriangles": 56,
+                "hugeTrianglesCounter": 64, "numProcessedHugeTriangles": 72, "nontrivialTrianglesCounter": 80,
+                "nontrivialTrianglesList": 88, "target": 96, "nontrivialCapacity": 336, "hugeCapacity": 340}
 
 
 def test_struct_layout_matches_reference(tmp_path):
-    fields = [("crx_cmesh", k) for k in CMESH_OFFSETS] + [("crx_render_target", k) for k in TARGET_OFFSETS]
+    fields = ([("crx_cmesh", k) for k in CMESH_OFFSETS] + [("crx_render_target", k) for k in TARGET_OFFSETS] +
+              [("crx_raster_args", k) for k in ARGS_OFFSETS])
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "crx.h"', "int main(void){"]
-    src.append('printf("%zu %zu\\n", sizeof(crx_cmesh), sizeof(crx_render_target));')
+    src.append('printf("%zu %zu %zu\\n", sizeof(crx_cmesh), sizeof(crx_render_target), sizeof(crx_raster_args));')
     for s, f in fields:
         src.append(f'printf("%zu\\n", offsetof({s}, {f}));')
     src.append("return 0;}")
@@ -64,15 +70,17 @@ def test_struct_layout_matches_reference(tmp_path):
     subprocess.run(["gcc", "-std=c99", f"-I{INCLUDE}", str(tmp_path / "probe.c"), "-o", str(tmp_path / "probe")],
                    check=True)
     out = subprocess.run([str(tmp_path / "probe")], stdout=subprocess.PIPE, text=True, check=True).stdout.split()
-    assert out[0] == "280" and out[1] == "240"
-    got = [int(x) for x in out[2:]]
-    want = list(CMESH_OFFSETS.values()) + list(TARGET_OFFSETS.values())
+    assert out[0] == "280" and out[1] == "240" and out[2] == "344"
+    got = [int(x) for x in out[3:]]
+    want = list(CMESH_OFFSETS.values()) + list(TARGET_OFFSETS.values()) + list(ARGS_OFFSETS.values())
     assert got == want
     # the Python binding mirrors the same layout
     for name, off in CMESH_OFFSETS.items():
         assert getattr(crx.CMesh, name).offset == off
     for name, off in TARGET_OFFSETS.items():
         assert getattr(crx.RenderTarget, name).offset == off
+    for name, off in ARGS_OFFSETS.items():
+        assert getattr(crx.RasterArgs, name).offset == off
 
 
 def test_tuning_defaults_are_the_reference_constants():
