@@ -288,6 +288,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         it.indexMin = m.index_min;
         it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
         it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
+        it.numVertices = m.numVertices;
         D.batches += ((uint64_t)m.numTriangles + batchTris - 1) >> D.batchShift;
         D.tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
         if (D.batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");

@@ -32,6 +32,7 @@ struct DrawItem {
     uint32_t indexMin;
     float cf[3];
     float amin[3];
+    uint32_t numVertices;       // CMesh::numVertices (IX_PU16: bounds the 8-byte vertex reads)
 };
 
 // Per-instance record for stage 2, stage 3 and resolve.

@@ -344,6 +344,7 @@ __global__ __launch_bounds__(1024) void k_plan_items(const crx_cmesh* __restrict
         it.indexMin = m.index_min;
         it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
         it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
+        it.numVertices = m.numVertices;
         items[k] = it;
         first += ((uint64_t)n + (1ull << batchShift) - 1u) >> batchShift;
     }
@@ -413,6 +414,7 @@ struct S1Item {
     uint32_t firstBatch, endBatch;
     uint32_t bits, indexMin;    // IX_PU16
     float cfx, cfy, cfz, mnx, mny, mnz;
+    uint32_t numVertices;
 };
 
 __device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item, uint32_t numBatches) {
@@ -435,6 +437,7 @@ __device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item, 
     r.mnx = items[item].amin[0];
     r.mny = items[item].amin[1];
     r.mnz = items[item].amin[2];
+    r.numVertices = items[item].numVertices;
     return r;
 }
 
@@ -451,31 +454,21 @@ __device__ __forceinline__ uint32_t bit_read(const uint32_t* data, uint64_t off,
     return (uint32_t)(combined >> sh) & mask;
 }
 
-// Issue the index loads of triangle `tri` of the item.  Triangles past the
-// item's end load its last triangle instead (a valid address) and are masked
-// by the caller.  PU16: the index buffer is bit-packed
-// (getVertex<INDEXBUFFER, IX_PU16>, triangles_visbuffer.cu:48-49).
-template <bool PU16>
-__device__ __forceinline__ void s1_indices(const S1Item& it, uint32_t tri, uint32_t& i0, uint32_t& i1,
-                                           uint32_t& i2) {
+// Issue the index loads of triangle `tri` of the item (uncompressed formats).
+// Triangles past the item's end load its last triangle instead (a valid
+// address) and are masked by the caller.
+__device__ __forceinline__ void s1_indices(const S1Item& it, uint32_t tri, uint32_t& i0, uint32_t& i1, uint32_t& i2) {
     const uint32_t t = min(tri, it.numTriangles - 1u);
-    if (PU16) {
-        const uint64_t off = (uint64_t)it.bits * (3ull * t);
-        i0 = bit_read(it.indices, off, it.bits) + it.indexMin;
-        i1 = bit_read(it.indices, off + it.bits, it.bits) + it.indexMin;
-        i2 = bit_read(it.indices, off + 2u * it.bits, it.bits) + it.indexMin;
-    } else {
-        // A non-indexed item (vertices 3t..3t+2) points `indices` at its
-        // positions and has a zero mask: the words read there are discarded.
-        // The load is one dwordx3 on every path, so the vertex waits can
-        // leave it in flight (the mask is data, not a branch).
-        const auto ip = kglobal(it.indices) + 3ull * t;
-        const uint32_t r0 = ip[0], r1 = ip[1], r2 = ip[2];
-        const uint32_t mk = it.indexMask;
-        i0 = (r0 & mk) | ((3u * t) & ~mk);
-        i1 = (r1 & mk) | ((3u * t + 1u) & ~mk);
-        i2 = (r2 & mk) | ((3u * t + 2u) & ~mk);
-    }
+    // A non-indexed item (vertices 3t..3t+2) points `indices` at its
+    // positions and has a zero mask: the words read there are discarded.
+    // The load is one dwordx3 on every path, so the vertex waits can
+    // leave it in flight (the mask is data, not a branch).
+    const auto ip = kglobal(it.indices) + 3ull * t;
+    const uint32_t r0 = ip[0], r1 = ip[1], r2 = ip[2];
+    const uint32_t mk = it.indexMask;
+    i0 = (r0 & mk) | ((3u * t) & ~mk);
+    i1 = (r1 & mk) | ((3u * t + 1u) & ~mk);
+    i2 = (r2 & mk) | ((3u * t + 2u) & ~mk);
 }
 
 __device__ __forceinline__ f3 load_pos_g(const float* positions, uint32_t i) {
@@ -488,17 +481,6 @@ __device__ __forceinline__ f3 load_pos_g(const float* positions, uint32_t i) {
 __device__ __forceinline__ f3 load_pos_pu16(const float* positions, uint32_t i, const float* cf, const float* amin) {
     const auto q = kglobal(reinterpret_cast<const uint16_t*>(positions)) + 3ull * i;
     return f3{(float)q[0] * cf[0] + amin[0], (float)q[1] * cf[1] + amin[1], (float)q[2] * cf[2] + amin[2]};
-}
-
-// Vertex i of the item (IX_PU16: decoded as load_pos_pu16 does, from the
-// item's scalar fields).
-template <bool PU16>
-__device__ __forceinline__ f3 s1_vertex(const S1Item& it, uint32_t i) {
-    if (PU16) {
-        const auto q = kglobal(reinterpret_cast<const uint16_t*>(it.positions)) + 3ull * i;
-        return f3{(float)q[0] * it.cfx + it.mnx, (float)q[1] * it.cfy + it.mny, (float)q[2] * it.cfz + it.mnz};
-    }
-    return load_pos_g(it.positions, i);
 }
 
 // The batches of this workgroup: [begin, end) with stride `step`.
@@ -526,24 +508,190 @@ __device__ __forceinline__ void s1_range(uint32_t numBatches, uint32_t order, ui
     }
 }
 
-// Index loads of the K triangles of `batch` (an item-relative triangle base
-// from the item's first batch) for this lane.
-template <int BLOCK, int K, bool PU16>
-__device__ __forceinline__ void s1_batch_indices(const RasterParams& P, const S1Item& it, uint32_t batch,
-                                                 uint32_t (&i0)[K], uint32_t (&i1)[K], uint32_t (&i2)[K]) {
-    const uint32_t base = (batch - it.firstBatch) << P.batchShift;
-    #pragma unroll
-    for (int k = 0; k < K; k++) s1_indices<PU16>(it, base + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
+// IX_PU16 in stage 1, per wave group of 64 consecutive triangles
+// [tw, tw + 64) (tw clamped into the item, as the lanes' triangles are): the
+// group's indices occupy bits [3*bits*tw, 3*bits*(tw + 64)) of the stream,
+// at most 6*bits + 1 words.  Every lane loads four consecutive words of that
+// range with one coalesced dwordx4 (lane l from word 4l, clamped so that the
+// four words stay inside the range and the stream), the wave stores them in
+// LDS at their own positions (overlapping lanes store the same words), and
+// each lane cuts its three indices out with BitEdit::readU32's arithmetic
+// (BitEdit.h:8-26) plus index_min.  A vertex is one 8-byte read at the
+// 4-byte boundary below its 6 bytes.  No load sits in a divergent branch, so
+// the loads of the next batches stay in flight while a batch is drawn.
+constexpr uint32_t kPu16Words = 256;     // LDS words per wave
+
+// Raw buffer loads (buffer_load_dwordx2 / dwordx4 at any dword offset from
+// a scalar base).  A dword at or past num_records bytes reads as zero without
+// touching memory.  Word 3 of the descriptor is the gfx9 raw-buffer format.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint2 buffer_load_u2(__amdgpu_buffer_rsrc_t r, uint32_t offset) {
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)offset, 0, 0));
+}
+__device__ __forceinline__ uint4 buffer_load_u4(__amdgpu_buffer_rsrc_t r, uint32_t offset) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)offset, 0, 0));
+}
+#endif
+
+// threadIdx.x of the wave's lane 0, as a scalar.
+__device__ __forceinline__ uint32_t wave_offset() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x & ~(kWave - 1u));
 }
 
-template <int K, bool PU16>
-__device__ __forceinline__ void s1_batch_vertices(const S1Item& it, const uint32_t (&i0)[K], const uint32_t (&i1)[K],
-                                                  const uint32_t (&i2)[K], f3 (&A)[K], f3 (&B)[K], f3 (&C)[K]) {
+// First word of a wave group's stream range, the bit offset of the group's
+// first index inside it and the number of the range's words inside the
+// stream (all uniform over the wave).
+__device__ __forceinline__ void pu16_group(const S1Item& it, uint32_t groupBase, uint32_t& tw, uint64_t& firstWord,
+                                           uint32_t& bit0, uint32_t& avail) {
+    tw = min(groupBase, it.numTriangles - 1u);
+    const uint64_t b = (uint64_t)it.bits * 3ull * tw;
+    firstWord = b >> 5;
+    bit0 = (uint32_t)b & 31u;
+    const uint64_t numWords = ((uint64_t)it.numTriangles * 3ull * it.bits + 31u) >> 5;
+    const uint32_t groupWords = (bit0 + 192u * it.bits + 31u) >> 5;
+    avail = (uint32_t)min((uint64_t)groupWords, numWords - firstWord);
+}
+
+__device__ __forceinline__ void pu16_load_words(const S1Item& it, uint32_t groupBase, uint32_t& w0, uint32_t& w1,
+                                                uint32_t& w2, uint32_t& w3) {
+    w0 = w1 = w2 = w3 = 0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (it.bits == 0u || it.bits > 32u) return;
+    uint32_t tw, bit0, avail;
+    uint64_t firstWord;
+    pu16_group(it, groupBase, tw, firstWord, bit0, avail);
+    // lane l reads words 4l .. 4l + 3 of the group's range; those past the
+    // range (or the stream) read as zero
+    const uint4 v = buffer_load_u4(buffer_rsrc(it.indices + firstWord, 4u * avail), 16u * (threadIdx.x & (kWave - 1)));
+    w0 = v.x;
+    w1 = v.y;
+    w2 = v.z;
+    w3 = v.w;
+#endif
+}
+
+__device__ __forceinline__ void pu16_decode(const S1Item& it, uint32_t groupBase, uint32_t* words, uint32_t w0,
+                                            uint32_t w1, uint32_t w2, uint32_t w3, uint32_t& i0, uint32_t& i1,
+                                            uint32_t& i2) {
+    const uint32_t bits = it.bits;
+    if (bits == 0u || bits > 32u) {                     // BitEdit::readU32 reads nothing
+        i0 = i1 = i2 = it.indexMin;
+        return;
+    }
+    uint32_t tw, bit0, avail;
+    uint64_t firstWord;
+    pu16_group(it, groupBase, tw, firstWord, bit0, avail);
+    *reinterpret_cast<uint4*>(words + 4u * (threadIdx.x & (kWave - 1))) = make_uint4(w0, w1, w2, w3);
+    wave_sync();
+    const uint32_t t = min(groupBase + (threadIdx.x & (kWave - 1)), it.numTriangles - 1u);
+    // bit offset of the lane's first index from the group's first word (< 32 * 194)
+    uint32_t off = bit0 + 3u * bits * (t - tw);
+    const uint32_t mask = bits >= 32u ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    uint32_t v[3];
+    #pragma unroll
+    for (int j = 0; j < 3; j++) {
+        // (hi:lo) >> sh of the two words holding the index; the second word is
+        // inside the LDS words even when the index does not reach into it
+        const uint32_t wd = off >> 5;
+        const uint32_t lo = words[wd], hi = words[wd + 1];
+        v[j] = __builtin_amdgcn_alignbit(hi, lo, off & 31u) & mask;
+        off += bits;
+    }
+    wave_sync();
+    i0 = v[0] + it.indexMin;
+    i1 = v[1] + it.indexMin;
+    i2 = v[2] + it.indexMin;
+}
+
+// IX_PU16 vertex i, issued: the two words of the 8-byte window at the 4-byte
+// boundary below its 6 bytes (byte 6i, or 6i - 2 for odd i), carried as the
+// bits of an f3 (window words, flag) until pu16_vertex_decode.  Every vertex
+// but an even last one has its window inside the numVertices * 6 bytes of
+// the buffer; that one reads a dummy in-range window here (flag 2) and its
+// three uint16 in pu16_vertex_decode, in a branch only waves holding it
+// enter.  Meshes of fewer than three vertices, or of more than 2^32 bytes
+// (byte offsets of the buffer load are 32-bit), read uint16s directly.
+__device__ __forceinline__ f3 pu16_vertex_issue(const S1Item& it, uint32_t i) {
+    uint32_t lo = 0u, hi = 0u, flag = 0u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const auto base = reinterpret_cast<const char*>(it.positions);
+    const uint64_t bytes = 6ull * it.numVertices;
+    if (it.numVertices >= 3u && bytes < 0xFFFFFFFFull) {
+        const uint32_t odd = i & 1u;
+        const bool fast = (i + 2u <= it.numVertices && i + 2u > i) || (i + 1u == it.numVertices && odd);
+        const uint32_t at = fast ? 6u * i - 2u * odd : 6u * i - 4u;
+        const uint2 w = buffer_load_u2(buffer_rsrc(base, (uint32_t)bytes), at);
+        lo = w.x;
+        hi = w.y;
+        flag = fast ? odd : 2u;
+    } else {
+        const auto h = kglobal(reinterpret_cast<const uint16_t*>(base)) + 3ull * i;
+        lo = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+        hi = h[2];
+    }
+#endif
+    return f3{__builtin_bit_cast(float, lo), __builtin_bit_cast(float, hi), __builtin_bit_cast(float, flag)};
+}
+
+// X * compressionFactor + aabb.min per axis (triangles_visbuffer.cu:51-59;
+// product then sum, no contraction) from an issued window.
+__device__ __forceinline__ f3 pu16_vertex_decode(const S1Item& it, const f3& raw) {
+    uint32_t lo = __builtin_bit_cast(uint32_t, raw.x), hi = __builtin_bit_cast(uint32_t, raw.y);
+    uint32_t flag = __builtin_bit_cast(uint32_t, raw.z);
+    if (__ballot(flag == 2u) != 0ull && flag == 2u) {  // the even last vertex
+        const auto h = kglobal(reinterpret_cast<const uint16_t*>(it.positions)) + 3ull * (it.numVertices - 1u);
+        lo = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+        hi = h[2];
+        flag = 0u;
+    }
+    // even window: lo = x | y << 16, hi = z | ..; odd window: lo = .. | x << 16, hi = y | z << 16
+    const uint32_t odd = flag;
+    const uint32_t mid = __builtin_amdgcn_alignbit(hi, lo, 16u);
+    const uint32_t x = (odd ? mid : lo) & 0xFFFFu;
+    const uint32_t y = (mid >> (odd << 4)) & 0xFFFFu;
+    const uint32_t z = (odd ? hi : mid) >> 16;
+    return f3{(float)x * it.cfx + it.mnx, (float)y * it.cfy + it.mny, (float)z * it.cfz + it.mnz};
+}
+
+// Index loads of the K triangles of `batch` for this lane (IX_PU16: the raw
+// stream words of the lane's share of its wave groups, in i0..i3).
+template <int BLOCK, int K, bool PU16>
+__device__ __forceinline__ void s1_batch_indices(const RasterParams& P, const S1Item& it, uint32_t batch,
+                                                 uint32_t (&i0)[K], uint32_t (&i1)[K], uint32_t (&i2)[K],
+                                                 uint32_t (&i3)[K]) {
+    const uint32_t base = (batch - it.firstBatch) << P.batchShift;
     #pragma unroll
     for (int k = 0; k < K; k++) {
-        A[k] = s1_vertex<PU16>(it, i0[k]);
-        B[k] = s1_vertex<PU16>(it, i1[k]);
-        C[k] = s1_vertex<PU16>(it, i2[k]);
+        if (PU16) pu16_load_words(it, base + k * BLOCK + wave_offset(), i0[k], i1[k], i2[k], i3[k]);
+        else s1_indices(it, base + k * BLOCK + threadIdx.x, i0[k], i1[k], i2[k]);
+    }
+}
+
+// Vertex loads of `batch` (IX_PU16: after cutting the indices out of the
+// stream words through the wave's LDS words; the vertices stay raw windows
+// until pu16_vertex_decode).
+template <int BLOCK, int K, bool PU16>
+__device__ __forceinline__ void s1_batch_vertices(const RasterParams& P, const S1Item& it, uint32_t batch,
+                                                  const uint32_t (&i0)[K], const uint32_t (&i1)[K],
+                                                  const uint32_t (&i2)[K], const uint32_t (&i3)[K], uint32_t* words,
+                                                  f3 (&A)[K], f3 (&B)[K], f3 (&C)[K]) {
+    const uint32_t base = (batch - it.firstBatch) << P.batchShift;
+    #pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (PU16) {
+            uint32_t a, b, c;
+            pu16_decode(it, base + k * BLOCK + wave_offset(), words, i0[k], i1[k], i2[k], i3[k], a, b, c);
+            A[k] = pu16_vertex_issue(it, a);
+            B[k] = pu16_vertex_issue(it, b);
+            C[k] = pu16_vertex_issue(it, c);
+        } else {
+            A[k] = load_pos_g(it.positions, i0[k]);
+            B[k] = load_pos_g(it.positions, i1[k]);
+            C[k] = load_pos_g(it.positions, i2[k]);
+        }
     }
 }
 
@@ -566,30 +714,38 @@ __device__ __forceinline__ uint32_t s1_next(uint32_t batch, uint32_t step, uint3
 // body(it, base, A, B, C) draws the batch whose first triangle is `base`
 // within item `it`.
 template <int BLOCK, int K, bool PU16, typename Body>
-__device__ __forceinline__ void s1_sweep(const RasterParams& P, Body&& body) {
+__device__ __forceinline__ void s1_sweep(const RasterParams& P, uint32_t* words, Body&& body) {
     const uint32_t numBatches = P.batchCount ? *kconst(P.batchCount) : P.numBatches;
     uint32_t begin, end, step;
     s1_range(numBatches, P.order, begin, end, step);
     if (begin >= end) return;
     uint32_t batch = begin;
     S1Item it = s1_item(P, find_item(P, batch), numBatches);
-    uint32_t i0[K], i1[K], i2[K];
-    s1_batch_indices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2);
+    uint32_t i0[K], i1[K], i2[K], i3[K];
+    s1_batch_indices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2, i3);
     bool more;
     uint32_t b1 = s1_next(batch, step, end, &more);
     S1Item it1 = b1 < it.endBatch ? it : s1_item(P, find_item(P, b1), numBatches);
-    uint32_t j0[K], j1[K], j2[K];
-    s1_batch_indices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2);
+    uint32_t j0[K], j1[K], j2[K], j3[K];
+    s1_batch_indices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2, j3);
     f3 A[K], B[K], C[K];
-    s1_batch_vertices<K, PU16>(it, i0, i1, i2, A, B, C);
+    s1_batch_vertices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2, i3, words, A, B, C);
     for (;;) {
         f3 An[K], Bn[K], Cn[K];
-        s1_batch_vertices<K, PU16>(it1, j0, j1, j2, An, Bn, Cn);
+        s1_batch_vertices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2, j3, words, An, Bn, Cn);
         bool more2;
         const uint32_t b2 = more ? s1_next(b1, step, end, &more2) : b1;
         if (!more) more2 = false;
         const S1Item it2 = b2 < it1.endBatch ? it1 : s1_item(P, find_item(P, b2), numBatches);
-        s1_batch_indices<BLOCK, K, PU16>(P, it2, b2, j0, j1, j2);
+        s1_batch_indices<BLOCK, K, PU16>(P, it2, b2, j0, j1, j2, j3);
+        if (PU16) {
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
+                A[k] = pu16_vertex_decode(it, A[k]);
+                B[k] = pu16_vertex_decode(it, B[k]);
+                C[k] = pu16_vertex_decode(it, C[k]);
+            }
+        }
         body(it, (batch - it.firstBatch) << P.batchShift, A, B, C);
         if (!more) break;
         batch = b1;
@@ -772,7 +928,12 @@ __device__ __forceinline__ bool wave_box(const f3& A, const f3& B, const f3& C, 
 template <int BLOCK, int K, bool PU16>
 __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
     const auto transforms = kconst(P.view0.transforms);
-    s1_sweep<BLOCK, K, PU16>(P, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
+    uint32_t* words = nullptr;
+    if constexpr (PU16) {
+        __shared__ uint32_t pu16Words[BLOCK / kWave][kPu16Words];
+        words = pu16Words[threadIdx.x / kWave];
+    }
+    s1_sweep<BLOCK, K, PU16>(P, words, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
                                     const f3 (&C)[K]) __attribute__((always_inline)) {
         for (uint32_t j = 0; j < it.instCount; j++) {
             const uint32_t inst = it.instOffset + j;
@@ -808,8 +969,13 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
     ViewCull cull{};
     uint32_t cullInst = 0xffffffffu;
 
-    s1_sweep<BLOCK, K, PU16>(P, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
-                                    const f3 (&C)[K]) __attribute__((always_inline)) {
+    uint32_t* words = nullptr;
+    if constexpr (PU16) {
+        __shared__ uint32_t pu16Words[BLOCK / kWave][kPu16Words];
+        words = pu16Words[threadIdx.x / kWave];
+    }
+    s1_sweep<BLOCK, K, PU16>(P, words, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
+                                           const f3 (&C)[K]) __attribute__((always_inline)) {
         #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint32_t tri = base + k * BLOCK + threadIdx.x;

@@ -66,6 +66,35 @@ def test_instanced_compressed_two_per_lane():
         c.close()
 
 
+@pytest.mark.parametrize("per_lane", [1, 2, 4])
+def test_small_compressed_meshes(ctx, per_lane):
+    """Edge cases of the stage-1 IX_PU16 reads: meshes of one and two
+    triangles (a stream shorter than four words, fewer than three vertices
+    read directly), an odd vertex count whose even last vertex is read as
+    three uint16, and 1-bit indices."""
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = per_lane
+    c = crx.Context(0, tuning=t)
+    try:
+        tri = (np.array([[-1, -1, -3], [1, -1, -3], [0, 1, -3]], dtype=np.float32), np.array([[0, 1, 2]], dtype=np.uint32))
+        quad = (np.array([[-2, -1, -4], [2, -1, -4], [2, 1, -4], [-2, 1, -4], [0, 2, -5]], dtype=np.float32),
+                np.array([[0, 1, 2], [0, 2, 3], [3, 2, 4]], dtype=np.uint32))
+        two = (np.array([[-1, -1, -2], [1, 1, -2]], dtype=np.float32), np.array([[0, 1, 1], [1, 0, 0]], dtype=np.uint32))
+        hp, hi, hc = wl.heightfield_mesh(33, amplitude=2.0, scale=8.0)          # 1089 vertices: odd
+        spec = SceneSpec()
+        for p_, i_ in (tri, quad, two):
+            col = np.arange(len(p_), dtype=np.uint32) * 0x01020304 + 0xFF000000
+            spec.add(CompressedMesh(p_, i_, col), [crx.identity()])
+        shift = np.array([[1, 0, 0, -16], [0, 1, 0, -3], [0, 0, 1, -40], [0, 0, 0, 1]], dtype=np.float32)
+        spec.add(CompressedMesh(hp, hi, hc), [crx.Mat4.from_numpy(shift)])
+        view = crx.look_at((0.0, 0.0, 2.0), (0.0, 0.0, -10.0))
+        for instanced in (True, False):
+            fb, _ = _check(c, spec, 640, 480, view, wl.projection(640, 480), instanced=instanced)
+            assert int((fb != crx.CLEAR_PIXEL).sum()) > 1000
+    finally:
+        c.close()
+
+
 def test_closeup_compressed_stage2_stage3(ctx):
     pos, idx, col = wl.heightfield_mesh(129, spacing=1.0, amplitude=6.0, scale=32.0)
     spec = SceneSpec().add(CompressedMesh(pos, idx, col), [crx.identity()])
