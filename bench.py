@@ -79,11 +79,14 @@ def load_traffic(workload: str):
         return None
 
 
-def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int = 64):
+def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int = 64, autotune: bool = True):
     """Batched multi-view workload: `count` seeded views of the heightfield
     (mixed resolutions, a quarter close-ups), each a full frame (clear, three
-    stages, resolve).  Batched = one crx_render_views call per step; plain =
-    crx_render once per view.  Returns views/s of both."""
+    stages, resolve).  With `autotune` the context's tuning (stage-1 schedule,
+    routing thresholds, tile size, queue depths) is first searched by
+    crx_autotune on the batched frame; both paths then use it.  Batched = one
+    crx_render_views call per step; plain = crx_render once per view.
+    Returns views/s of both."""
     import crx
     import workloads as wl
     vs = wl.multiview_views(n, pos, count=count)
@@ -92,6 +95,14 @@ def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int
     cbs = [ctx.alloc(8 * v.width * v.height) for v in views]
     settings = crx.render_settings(instanced=1, frustum_culling=1, resolve=1, attribute=crx.DISPLAY_VERTEX_COLORS)
     pixels = sum(v.width * v.height for v in views)
+    tuned = None
+    if autotune:
+        t0 = time.perf_counter()
+        res = ctx.autotune(scene, views, fbs, flags=crx.TUNE_SCHEDULE | crx.TUNE_ROUTING | crx.TUNE_QUEUES, reps=3)
+        t = res.tuning
+        tuned = {"start_ms": round(res.start_ms, 3), "best_ms": round(res.best_ms, 3), "trials": res.trials,
+                 "search_s": round(time.perf_counter() - t0, 2),
+                 "tuning": {f: getattr(t, f) for f, _ in t._fields_ if f != "reserved"}}
 
     def batched():
         return scene.render_views(views, fbs, cbs, settings)
@@ -135,9 +146,32 @@ def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int
         "bit_identical_to_plain": bool(same),
         "batched_stage_ms": {k: round(x, 4) for k, x in stage.items()},
         "batched_nontrivial": st.num_nontrivial, "batched_huge": st.num_huge,
+        "autotune": tuned,
     }
     for b in fbs + cbs:
         b.free()
+    ctx.tuning = crx.default_tuning()
+    return out
+
+
+def bench_atomics(ctx, log_entries: int = 21, per_lane: int = 64, reps: int = 5):
+    """Measured peak of 64-bit atomicMin on a 16 MiB (1080p-sized) buffer,
+    scattered and coalesced (crx_probe_atomic_min), in atomics/s."""
+    buf = ctx.alloc(8 << log_entries)
+    ctx.clear_framebuffer(buf, 1 << log_entries)
+    out = {}
+    for scattered in (True, False):
+        ctx.probe_atomic_min(buf, log_entries, per_lane, scattered)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        n = 0
+        for _ in range(reps):
+            n += ctx.probe_atomic_min(buf, log_entries, per_lane, scattered)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        out["scattered" if scattered else "coalesced"] = {"atomics": n, "seconds": round(dt, 5),
+                                                          "atomics_per_s": n / dt}
+    buf.free()
     return out
 
 
@@ -271,6 +305,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run the 1 GiB copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="multi-view: keep the default tuning instead of running crx_autotune first")
     ap.add_argument("--tuning", default="",
                     help="comma-separated crx_tuning overrides, e.g. stage1_order=0,stage1_block=256")
     args = ap.parse_args()
@@ -326,7 +362,7 @@ def main() -> int:
     if args.multiview_only:
         if args.workload != "heightfield":
             raise SystemExit("--multiview-only needs the heightfield workload")
-        mv = bench_multiview(ctx, scene, pos, desc["grid"], args.steps, args.warmup)
+        mv = bench_multiview(ctx, scene, pos, desc["grid"], args.steps, args.warmup, autotune=not args.no_autotune)
         log(f"[bench] multi-view {mv}")
         print(json.dumps({
             "metric": "batched views/sec (64 mixed-resolution views, full frames)",
@@ -340,6 +376,9 @@ def main() -> int:
     for _ in range(args.warmup):
         scene.render(v, fb, cb, settings)
     ctx.synchronize()
+
+    atomics = bench_atomics(ctx)
+    log(f"[bench] atomicMin probe: {atomics}")
 
     if args.probe_copy:
         nbytes = 1 << 30
@@ -395,7 +434,8 @@ def main() -> int:
     multiview = None
     if args.workload == "heightfield" and not args.no_multiview:
         log("[bench] batched multi-view: 64 seeded views, batched vs plain ...")
-        multiview = bench_multiview(ctx, scene, pos, desc["grid"], args.steps, args.warmup)
+        multiview = bench_multiview(ctx, scene, pos, desc["grid"], args.steps, args.warmup,
+                                    autotune=not args.no_autotune)
         log(f"[bench] multi-view {multiview}")
 
     cpu = None
@@ -424,6 +464,7 @@ def main() -> int:
         "pipeline_triangles_per_s": T / (med["pipeline"] * 1e-3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "atomic_min_peak": atomics,
         "batched_multiview": multiview,
     }
     print(json.dumps(out), flush=True)

@@ -250,6 +250,7 @@ _SIGS = {
     "crx_memcpy_dtoh": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_stream_synchronize": (C.c_int, [_VP, _VP]),
     "crx_copy_buffer": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
+    "crx_probe_atomic_min": (C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), _VP]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
     "crx_raster_stage1": (C.c_int, [_VP, C.POINTER(RasterArgs), C.c_uint32, _VP]),
     "crx_raster_stage2": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
@@ -481,6 +482,14 @@ class Context:
 
     def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int, stream=None) -> None:
         _check(lib().crx_copy_buffer(self.handle, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, stream))
+
+    def probe_atomic_min(self, buf: DeviceBuffer, log_entries: int, per_lane: int, scattered: bool,
+                         stream=None) -> int:
+        """Enqueue the 64-bit atomicMin probe; returns the number of atomics."""
+        n = C.c_uint64()
+        _check(lib().crx_probe_atomic_min(self.handle, C.c_void_p(buf.ptr), log_entries, per_lane,
+                                          int(bool(scattered)), C.byref(n), stream))
+        return n.value
 
     # --- kernel-level entry points ------------------------------------------------
 

@@ -969,6 +969,17 @@ crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t 
     return CRX_OK;
 }
 
+crx_status crx_probe_atomic_min(crx_context* ctx, uint64_t* buffer, uint32_t logEntries, uint32_t atomicsPerLane,
+                                int scattered, uint64_t* issued, void* stream) {
+    if (!ctx || !buffer) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (logEntries < 6 || logEntries > 30) return fail(CRX_INVALID_ARG, "logEntries must be in [6, 30]");
+    CRX_HIP(hipSetDevice(ctx->device));
+    const uint32_t blocks = (uint32_t)ctx->numCUs * 8u;
+    CRX_HIP(launch_atomic_probe(buffer, logEntries, atomicsPerLane, scattered, blocks, as_stream(stream)));
+    if (issued) *issued = (uint64_t)blocks * 256ull * atomicsPerLane;
+    return CRX_OK;
+}
+
 crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void* stream) {
     if (!framebuffer && numPixels) return fail(CRX_INVALID_ARG, "framebuffer is NULL");
     if (numPixels == 0) return CRX_OK;

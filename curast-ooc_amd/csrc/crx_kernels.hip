@@ -1637,10 +1637,39 @@ __global__ __launch_bounds__(256) void k_clear_views(const ClearView* __restrict
     if (blockIdx.x == 0 && ctr && threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
 }
 
-// Plain streaming copy, 16 bytes per lane (calibrates the HBM traffic counters).
+// Plain streaming copy, 16 bytes per lane and four loads in flight per lane
+// (calibrates the HBM traffic counters and gives a measured copy bandwidth).
 __global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// 64-bit atomic minimum probe: every lane performs perLane atomicMin on a
+// buffer of 2^logEntries entries, at xorshift-hashed positions (scattered)
+// or at consecutive positions per wave (coalesced: a wave's 64 lanes hit
+// 512 contiguous bytes, as a stage-2/3 row of pixels does).  The reference
+// point for the stages' atomic throughput.
+__global__ __launch_bounds__(256) void k_atomic_probe(uint64_t* __restrict__ buf, uint32_t logEntries,
+                                                      uint32_t perLane, int scattered) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t mask = (1u << logEntries) - 1u;
+    uint32_t h = gid * 0x9E3779B9u + 0x7F4A7C15u;
+    for (uint32_t j = 0; j < perLane; j++) {
+        h ^= h << 13;
+        h ^= h >> 17;
+        h ^= h << 5;
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(h);
+        const uint32_t pos = scattered ? (h & mask) : ((wave & ~63u) + (threadIdx.x & 63u)) & mask;
+        atomicMin((unsigned long long*)(buf + pos), ((unsigned long long)h << 24) | j);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1745,6 +1774,12 @@ hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t strea
 
 hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream) {
     hipLaunchKernelGGL(k_stage3, dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_atomic_probe(uint64_t* buf, uint32_t logEntries, uint32_t perLane, int scattered, uint32_t blocks,
+                               hipStream_t stream) {
+    hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, stream, buf, logEntries, perLane, scattered);
     return hipGetLastError();
 }
 

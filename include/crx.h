@@ -370,6 +370,14 @@ crx_status crx_stream_synchronize(crx_context* ctx, void* stream);
  * the reference: it is the known-size kernel that calibrates HBM traffic
  * counters (FETCH_SIZE / WRITE_SIZE) for the roofline measurement. */
 crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* 64-bit atomic-minimum probe (no counterpart in the reference): every lane
+ * of a CUs * 8 x 256 grid performs atomicsPerLane atomicMin on the device
+ * buffer of 2^logEntries entries, scattered (hashed positions) or coalesced
+ * (a wave's lanes on 64 consecutive entries).  *issued receives the number
+ * of atomics enqueued; the caller times the stream.  The measured peak
+ * against which the stages' atomic rates are read. */
+crx_status crx_probe_atomic_min(crx_context* ctx, uint64_t* buffer, uint32_t logEntries, uint32_t atomicsPerLane,
+                                int scattered, uint64_t* issued, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Camera helpers                                                       */

@@ -152,3 +152,19 @@ def test_stages_queue_capacity_reports_overflow(ctx):
     fb, c = run.run(crx.RASTER_INSTANCED)
     _, _, st = g.render(W, H, view, wl.projection(W, H))
     assert c[4] == st.num_nontrivial and st.num_nontrivial > 4   # counts every entry, stores the first 4
+
+
+def test_atomic_min_probe(ctx):
+    """The atomicMin probe writes only inside its 2^logEntries entries and
+    leaves each entry at the minimum of the values aimed at it."""
+    n = 1 << 12
+    buf = ctx.alloc(8 * (n + 64))
+    ctx.clear_framebuffer(buf, n + 64)
+    for scattered in (True, False):
+        issued = ctx.probe_atomic_min(buf, 12, 4, scattered)
+        assert issued == ctx.num_cus * 8 * 256 * 4
+    out = buf.download(np.uint64, n + 64)
+    assert (out[n:] == crx.CLEAR_PIXEL).all()                 # nothing past the probed entries
+    assert (out[:n] != crx.CLEAR_PIXEL).sum() > n // 2
+    with pytest.raises(crx.CrxError):
+        ctx.probe_atomic_min(buf, 40, 1, True)
