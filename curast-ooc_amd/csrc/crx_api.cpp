@@ -121,7 +121,7 @@ crx_status check_tuning(const crx_tuning& t) {
     const uint32_t b = t.stage1_block;
     if (b != 128 && b != 256 && b != 512)
         return fail(CRX_INVALID_ARG, "tuning.stage1_block must be 128, 256 or 512");
-    if (t.stage1_cull_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0, 1 or 2");
+    if (t.stage1_cull_order > 3) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0, 1, 2 or 3");
     if (t.stage1_fast_tiny > 1) return fail(CRX_INVALID_ARG, "tuning.stage1_fast_tiny must be 0 or 1");
     if (t.stage1_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_order must be 0, 1 or 2");
     if (t.stage1_tris_per_lane != 1 && t.stage1_tris_per_lane != 2 && t.stage1_tris_per_lane != 4)
@@ -283,6 +283,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         it.numTriangles = m.numTriangles;
         it.instOffset = (uint32_t)off;
         it.instCount = (uint32_t)cnt;
+        if (cnt >= (int64_t)kBoxInstances) D.instanceBoxes = true;
         it.firstBatch = (uint32_t)D.batches;
         it.bitsPerIndex = m.bitsPerIndex;
         it.indexMin = m.index_min;
@@ -371,7 +372,8 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
     hipError_t e = launch_reset_counters(P.ctrNontrivial - CTR_NONTRIVIAL, stream);   // the plan's counter block
     if (e != hipSuccess) return e;
     record(ctx, T_S1, stream);
-    if (D.numViews == 1 && !viewsKernel) e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
+    if (D.numViews == 1 && !viewsKernel)
+        e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, D.instanceBoxes, stream);
     else e = launch_stage1_views(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
     if (e != hipSuccess) return e;
     record(ctx, T_S2, stream);
@@ -819,7 +821,7 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage1_waves_per_cu = 32;
     out->stage2_waves_per_cu = 16;
     out->stage3_waves_per_cu = 16;
-    out->stage1_cull_order = 1;
+    out->stage1_cull_order = 3;
     out->stage1_order = 1;
     out->stage1_fast_tiny = 1;
     out->stage1_tris_per_lane = 2;
@@ -1388,7 +1390,7 @@ crx_status crx_autotune(crx_context* ctx, crx_scene* scene, const crx_view* view
         {&crx_tuning::stage1_tris_per_lane, {1, 2, 4}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_waves_per_cu, {8, 16, 24, 32, 48, 64, 0}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},
-        {&crx_tuning::stage1_cull_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_cull_order, {0, 1, 2, 3}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_fast_tiny, {0, 1}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage2_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage3_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},

@@ -120,7 +120,7 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
                             a->numInstances, instanced ? 1 : 0, pu16 ? 1 : 0, batchShift,
                             reinterpret_cast<DrawItem*>(dev + offItems), reinterpret_cast<InstanceRec*>(dev + offInst),
                             reinterpret_cast<Transform*>(dev + offXf), reinterpret_cast<uint32_t*>(dev + offCount), s));
-        CRX_HIP(launch_stage1(P, tu.stage1_block, tu.stage1_tris_per_lane, g1, pu16, s));
+        CRX_HIP(launch_stage1(P, tu.stage1_block, tu.stage1_tris_per_lane, g1, pu16, instanced, s));
     }
     ctx->args.valid = true;
     ctx->args.instances = a->instances;

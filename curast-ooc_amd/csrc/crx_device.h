@@ -94,6 +94,10 @@ struct ViewParams {
     uint32_t pad[2];
 };
 
+// Instances per stage-1 item from which the single-view kernel reduces wave
+// boxes and tests them per instance (the reduction is paid once per batch).
+constexpr uint32_t kBoxInstances = 4;
+
 constexpr uint32_t kViewShift = 24;                 // instance field: view << 24 | instance
 constexpr uint32_t kInstanceMask = (1u << 24) - 1u;
 constexpr uint32_t kMaxViews = 256;

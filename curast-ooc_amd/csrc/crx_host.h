@@ -199,6 +199,7 @@ struct DrawPlan {
     uint64_t batches = 0, tris = 0;
     bool pu16 = false;
     uint32_t block = 256, trisPerLane = 1, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
+    bool instanceBoxes = false;   // some item loops over >= 4 instances
     size_t offItems = 0, offInst = 0, offViews = 0, offXf = 0, metaBytes = 0;
 };
 

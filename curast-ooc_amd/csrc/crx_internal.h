@@ -18,8 +18,10 @@ hipError_t launch_plan(const crx_cmesh* list, uint32_t numItems, const crx_cmesh
                        hipStream_t stream);
 // trisPerLane: 1, 2 or 4 (P.batchShift = log2(block * trisPerLane));
 // pu16: the draw's meshes are IX_PU16-compressed
+// instanceBoxes: the draw repeats meshes over many instances (wave boxes
+// let groups of triangles skip instances that cull them all as tiny)
 hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
-                         hipStream_t stream);
+                         bool instanceBoxes, hipStream_t stream);
 hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
                                hipStream_t stream);
 hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t stream);

@@ -101,6 +101,13 @@ __device__ __forceinline__ bool surely_culled_axis(float lo, float hi, float m) 
     return (lo + m < k) && (hi + m < k);
 }
 
+// Cull order of an item: tuning value 3 picks backface first for meshes
+// drawn over many instances (whose instances mostly sit inside the view, so
+// the frustum rarely culls) and frustum-y first for the rest.
+__device__ __forceinline__ uint32_t item_cull_order(const RasterParams& P, uint32_t instCount) {
+    return P.cullOrder == 3u ? (instCount >= kBoxInstances ? 0u : 1u) : P.cullOrder;
+}
+
 // rasterize<INDEXBUFFER, UNCOMPRESSED>, triangles_visbuffer.cu:83-271, for one
 // triangle-instance.  `id` is the packed triangle ID the reference forms at
 // :253 and `queueEntry` the stage-2 entry it forms at :193-195.
@@ -118,7 +125,7 @@ __device__ __forceinline__ bool surely_culled_axis(float lo, float hi, float m) 
 __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const ViewParams& V,
                                                 const float* __restrict__ m,
                                                 bool flip, f3 A, f3 B, f3 C,
-                                                uint64_t id, uint64_t queueEntry) {
+                                                uint64_t id, uint64_t queueEntry, uint32_t cullOrder) {
     // depth first: ndc.z = depth = -view.z
     const float da = -xform_row(m, 2, A), db = -xform_row(m, 2, B), dc = -xform_row(m, 2, C);
     // all vertices behind the camera (:119)
@@ -129,11 +136,11 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
 
     // frustum culling (:113-118), only when every depth is positive; the axis
     // the view culls most is tested before the other coordinate is computed
-    const bool frustumFirst = P.cullOrder != 0;
+    const bool frustumFirst = cullOrder != 0;
     float vya = 0.0f, vyb = 0.0f, vyc = 0.0f, vxa = 0.0f, vxb = 0.0f, vxc = 0.0f;
     bool haveX = false, haveY = false;
     if (frustumFirst && !nontrivial && safeDepth) {
-        if (P.cullOrder == 1) {
+        if (cullOrder == 1) {
             vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
             haveY = true;
             if (frustum_culled_axis(f * vya, f * vyb, f * vyc, da, db, dc)) return;
@@ -861,6 +868,61 @@ __device__ __forceinline__ float box_min(const float* a, const WaveBox& B) {
             fminf(a[2] * B.lo.z, a[2] * B.hi.z)) + a[3];
 }
 
+// max over the box of a.p + b.
+__device__ __forceinline__ float box_max(const float* a, const WaveBox& B) {
+    return ((fmaxf(a[0] * B.lo.x, a[0] * B.hi.x) + fmaxf(a[1] * B.lo.y, a[1] * B.hi.y)) +
+            fmaxf(a[2] * B.lo.z, a[2] * B.hi.z)) + a[3];
+}
+
+// Range of the computed view coordinate of row r of m over the box.  With
+// S = (|m_r0| + |m_r1| + |m_r2|) * M + |m_r3|, the per-triangle code's
+// xform_row and this bound are each within 8u*S of the exact affine value
+// (u = 2^-24: three products and three sums); the widening is 2^-18*S, 8x more.
+__device__ __forceinline__ void row_range(const float* m, int r, const WaveBox& B, float& lo, float& hi) {
+    const float a[4] = {m[r], m[4 + r], m[8 + r], m[12 + r]};
+    const float S = ((fabsf(a[0]) + fabsf(a[1])) + fabsf(a[2])) * B.M + fabsf(a[3]);
+    const float e = S * 0x1p-18f;
+    lo = box_min(a, B) - e;
+    hi = box_max(a, B) + e;
+}
+
+// True when every computed screen coordinate of one axis lies strictly
+// between the same two neighbouring sample centres inside the viewport.
+// [qlo, qhi] contains every view coordinate / depth quotient of the box;
+// the per-triangle code's RN(RN(g*v)/d) is within 2u of g*v/d, our products
+// with v_rcp_f32 within a few ulps, and the widening is 2^-16 of the
+// magnitude; the screen transform adds a few ulps of `size`, widened by
+// 2^-16*size.
+__device__ __forceinline__ bool axis_between_samples(float qlo, float qhi, float g, float size, float sizeM1) {
+    const float a = g * qlo, b = g * qhi;
+    const float w = 0x1p-16f * (fabsf(a) + fabsf(b)) + 0x1p-60f;
+    const float nlo = fminf(a, b) - w, nhi = fmaxf(a, b) + w;
+    const float ws = size * 0x1p-16f + 0x1p-20f;
+    const float slo = ((nlo * 0.5f + 0.5f) * size - 0.5f) - ws;
+    const float shi = ((nhi * 0.5f + 0.5f) * size - 0.5f) + ws;
+    const float k = floorf(slo);
+    return slo > k && shi < k + 1.0f && k >= 0.0f && k + 1.0f <= sizeM1;
+}
+
+// True only when stage1_triangle culls every triangle inside the box for the
+// transform m and view V by its tiny-triangle test (triangles_visbuffer.cu:
+// 153-158) or an earlier early-out: every depth is positive (so no triangle
+// is nontrivial), and in x or in y every computed screen coordinate lies
+// strictly between the same two sample centres k and k + 1 with 0 <= k and
+// k + 1 <= size - 1, so the clamp changes nothing and each triangle's box
+// holds no sample.
+__device__ __forceinline__ bool wave_tiny_culled(const float* m, const ViewParams& V, const WaveBox& B) {
+    float xlo, xhi, ylo, yhi, zlo, zhi;
+    row_range(m, 2, B, zlo, zhi);
+    const float dlo = -zhi, dhi = -zlo;                 // depth = -view z
+    if (!(dlo > 0x1p-60f) || !(dhi < 0x1p60f)) return false;
+    const float ilo = __builtin_amdgcn_rcpf(dhi), ihi = __builtin_amdgcn_rcpf(dlo);
+    row_range(m, 0, B, xlo, xhi);
+    if (axis_between_samples(fminf(xlo * ilo, xlo * ihi), fmaxf(xhi * ilo, xhi * ihi), V.fa, V.Wf, V.Wm1)) return true;
+    row_range(m, 1, B, ylo, yhi);
+    return axis_between_samples(fminf(ylo * ilo, ylo * ihi), fmaxf(yhi * ilo, yhi * ihi), V.f, V.Hf, V.Hm1);
+}
+
 // False only when every triangle inside the box is culled by the exact
 // per-triangle tests of stage1_triangle for this view: the box lies behind the
 // camera, or in front of it and beyond one side plane, by a margin.
@@ -925,7 +987,7 @@ __device__ __forceinline__ bool wave_box(const f3& A, const f3& B, const f3& C, 
 // Stage 1 (stage1_drawSmallTriangles, triangles_visbuffer.cu:274-384): every
 // lane draws its K triangles into the single view, once per instance of the
 // item.
-template <int BLOCK, int K, bool PU16>
+template <int BLOCK, int K, bool PU16, bool BOXES>
 __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
     const auto transforms = kconst(P.view0.transforms);
     uint32_t* words = nullptr;
@@ -934,7 +996,48 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
         words = pu16Words[threadIdx.x / kWave];
     }
     s1_sweep<BLOCK, K, PU16>(P, words, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
-                                    const f3 (&C)[K]) __attribute__((always_inline)) {
+                                           const f3 (&C)[K]) __attribute__((always_inline)) {
+        if (BOXES && it.instCount >= kBoxInstances) {
+            // A mesh drawn over many instances: per wave group, its
+            // object-space box is tested against 64 instances at a time, one
+            // per lane (wave_tiny_culled), and the group is drawn only for
+            // the instances left in the ballot.
+            const uint32_t lane = threadIdx.x & (kWave - 1);
+            #pragma unroll
+            for (int k = 0; k < K; k++) {
+                const uint32_t tri = base + k * BLOCK + threadIdx.x;
+                if (__ballot(tri < it.numTriangles) == 0ull) continue;
+                WaveBox box;
+                const bool bounded = wave_box(A[k], B[k], C[k], box);
+                for (uint32_t jb = 0; jb < it.instCount; jb += kWave) {
+                    const uint32_t j = jb + lane;
+                    bool draw = j < it.instCount;
+                    if (bounded && draw) {
+                        const auto Tj = kglobal(P.view0.transforms) + it.instOffset + j;
+                        float mm[16];
+                        #pragma unroll
+                        for (int e = 0; e < 16; e++) mm[e] = Tj->m[e];
+                        draw = !wave_tiny_culled(mm, P.view0, box);
+                    }
+                    uint64_t todo = __ballot(draw);
+                    while (todo) {
+                        const uint32_t jj = jb + (uint32_t)__builtin_ctzll(todo);
+                        todo &= todo - 1ull;
+                        const uint32_t inst = it.instOffset + jj;
+                        const auto T = transforms + inst;
+                        float m[16];
+                        #pragma unroll
+                        for (int e = 0; e < 16; e++) m[e] = T->m[e];
+                        if (tri < it.numTriangles) {
+                            const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(jj * it.numTriangles);
+                            stage1_triangle(P, P.view0, m, T->flip != 0u, A[k], B[k], C[k], id,
+                                            ((uint64_t)inst << 32) | (uint64_t)tri, item_cull_order(P, it.instCount));
+                        }
+                    }
+                }
+            }
+            return;
+        }
         for (uint32_t j = 0; j < it.instCount; j++) {
             const uint32_t inst = it.instOffset + j;
             const auto T = transforms + inst;
@@ -947,7 +1050,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                 const uint32_t tri = base + k * BLOCK + threadIdx.x;
                 if (tri < it.numTriangles) {
                     const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(j * it.numTriangles);
-                    stage1_triangle(P, P.view0, m, flip, A[k], B[k], C[k], id, ((uint64_t)inst << 32) | (uint64_t)tri);
+                    stage1_triangle(P, P.view0, m, flip, A[k], B[k], C[k], id, ((uint64_t)inst << 32) | (uint64_t)tri,
+                                    item_cull_order(P, it.instCount));
                 }
             }
         }
@@ -1008,7 +1112,8 @@ __global__ __launch_bounds__(BLOCK) void k_stage1_views(RasterParams P) {
                         for (int e = 0; e < 16; e++) m[e] = T->m[e];
                         if (valid) {
                             const uint64_t entry = ((uint64_t)((vi << kViewShift) | inst) << 32) | (uint64_t)tri;
-                            stage1_triangle(P, V, m, T->flip != 0u, A[k], B[k], C[k], id, entry);
+                            stage1_triangle(P, V, m, T->flip != 0u, A[k], B[k], C[k], id, entry,
+                                            item_cull_order(P, it.instCount));
                         }
                     }
                 }
@@ -1711,33 +1816,36 @@ hipError_t launch_plan(const crx_cmesh* list, uint32_t numItems, const crx_cmesh
 namespace {
 
 template <int BLOCK, int K, bool PU16>
-hipError_t launch_s1(const RasterParams& P, uint32_t grid, bool views, hipStream_t stream) {
+hipError_t launch_s1(const RasterParams& P, uint32_t grid, bool views, bool boxes, hipStream_t stream) {
     if (views) hipLaunchKernelGGL((k_stage1_views<BLOCK, K, PU16>), dim3(grid), dim3(BLOCK), 0, stream, P);
-    else hipLaunchKernelGGL((k_stage1<BLOCK, K, PU16>), dim3(grid), dim3(BLOCK), 0, stream, P);
+    else if (boxes) hipLaunchKernelGGL((k_stage1<BLOCK, K, PU16, true>), dim3(grid), dim3(BLOCK), 0, stream, P);
+    else hipLaunchKernelGGL((k_stage1<BLOCK, K, PU16, false>), dim3(grid), dim3(BLOCK), 0, stream, P);
     return hipGetLastError();
 }
 
 template <int BLOCK, int K>
-hipError_t launch_s1_fmt(const RasterParams& P, uint32_t grid, bool pu16, bool views, hipStream_t stream) {
-    return pu16 ? launch_s1<BLOCK, K, true>(P, grid, views, stream) : launch_s1<BLOCK, K, false>(P, grid, views, stream);
+hipError_t launch_s1_fmt(const RasterParams& P, uint32_t grid, bool pu16, bool views, bool boxes, hipStream_t stream) {
+    return pu16 ? launch_s1<BLOCK, K, true>(P, grid, views, boxes, stream)
+                : launch_s1<BLOCK, K, false>(P, grid, views, boxes, stream);
 }
 
 template <int BLOCK>
-hipError_t launch_s1_k(const RasterParams& P, uint32_t k, uint32_t grid, bool pu16, bool views, hipStream_t stream) {
+hipError_t launch_s1_k(const RasterParams& P, uint32_t k, uint32_t grid, bool pu16, bool views, bool boxes,
+                       hipStream_t stream) {
     switch (k) {
-        case 1: return launch_s1_fmt<BLOCK, 1>(P, grid, pu16, views, stream);
-        case 2: return launch_s1_fmt<BLOCK, 2>(P, grid, pu16, views, stream);
-        case 4: return launch_s1_fmt<BLOCK, 4>(P, grid, pu16, views, stream);
+        case 1: return launch_s1_fmt<BLOCK, 1>(P, grid, pu16, views, boxes, stream);
+        case 2: return launch_s1_fmt<BLOCK, 2>(P, grid, pu16, views, boxes, stream);
+        case 4: return launch_s1_fmt<BLOCK, 4>(P, grid, pu16, views, boxes, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_s1_any(const RasterParams& P, uint32_t block, uint32_t k, uint32_t grid, bool pu16, bool views,
-                         hipStream_t stream) {
+                         bool boxes, hipStream_t stream) {
     switch (block) {
-        case 128: return launch_s1_k<128>(P, k, grid, pu16, views, stream);
-        case 256: return launch_s1_k<256>(P, k, grid, pu16, views, stream);
-        case 512: return launch_s1_k<512>(P, k, grid, pu16, views, stream);
+        case 128: return launch_s1_k<128>(P, k, grid, pu16, views, boxes, stream);
+        case 256: return launch_s1_k<256>(P, k, grid, pu16, views, boxes, stream);
+        case 512: return launch_s1_k<512>(P, k, grid, pu16, views, boxes, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1745,13 +1853,13 @@ hipError_t launch_s1_any(const RasterParams& P, uint32_t block, uint32_t k, uint
 }  // namespace
 
 hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
-                         hipStream_t stream) {
-    return launch_s1_any(P, block, trisPerLane, grid, pu16, false, stream);
+                         bool instanceBoxes, hipStream_t stream) {
+    return launch_s1_any(P, block, trisPerLane, grid, pu16, false, instanceBoxes, stream);
 }
 
 hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
                                hipStream_t stream) {
-    return launch_s1_any(P, block, trisPerLane, grid, pu16, true, stream);
+    return launch_s1_any(P, block, trisPerLane, grid, pu16, true, false, stream);
 }
 
 hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, hipStream_t stream) {

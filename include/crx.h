@@ -182,7 +182,8 @@ typedef struct crx_tuning {
     uint32_t stage2_waves_per_cu; /* stage-2 persistent grid size in waves per CU */
     uint32_t stage3_waves_per_cu; /* stage-3 persistent grid size in waves per CU */
     uint32_t stage1_cull_order;   /* 0 = backface test before the frustum test, 1 = frustum first
-                                     testing y before x, 2 = frustum first testing x before y
+                                     testing y before x, 2 = frustum first testing x before y,
+                                     3 = per item: 0 for meshes drawn over >= 4 instances, else 1
                                      (all are exact early-outs; the order only changes cost) */
     uint32_t stage1_order;        /* batch-to-workgroup order: 0 = contiguous share per XCD,
                                      1 = grid-stride, 2 = contiguous share per workgroup */

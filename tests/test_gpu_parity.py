@@ -218,7 +218,7 @@ def test_scene_api_matches_oracle_frame(ctx):
 @pytest.mark.parametrize("cull_order,fast_tiny,order,block,per_lane", [
     (0, 0, 0, 256, 1), (0, 1, 1, 512, 2), (1, 0, 2, 128, 2), (1, 1, 1, 512, 1), (2, 1, 0, 256, 2),
     (2, 0, 1, 512, 1), (1, 1, 1, 256, 2), (1, 1, 0, 128, 1), (1, 1, 2, 256, 4), (0, 1, 1, 512, 4),
-    (2, 1, 0, 128, 4)])
+    (2, 1, 0, 128, 4), (3, 1, 1, 256, 2), (3, 0, 2, 128, 1)])
 def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, block, per_lane):
     """Cull order, the filtered tiny cull, batch order, workgroup size and
     triangles per lane only change the schedule: every variant must
@@ -231,6 +231,38 @@ def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, bloc
     t.stage1_cull_order, t.stage1_fast_tiny, t.stage1_order, t.stage1_block = cull_order, fast_tiny, order, block
     t.stage1_tris_per_lane = per_lane
     _check(ctx, spec, 1280, 720, view, proj, tuning=t)
+
+
+@pytest.mark.parametrize("per_lane", [1, 2])
+def test_many_instances_sub_pixel(ctx, per_lane):
+    """A dense mesh drawn as many small instances (config 3 scaled down):
+    most 64-triangle groups project inside one pixel and are skipped per
+    instance by the wave-box tiny test.  Extra instances cross the near plane,
+    sit behind the camera and hug the viewport's first and last columns and
+    rows; every bit must still be the oracle's."""
+    pos, idx, col = wl.sphere_mesh(rings=128, segments=256, seed=61)
+    worlds = list(wl.instance_transforms(300, extent=60.0, smin=0.2, smax=1.5, seed=62))
+    view, proj = wl.instances_view(extent=60.0, width=960, height=544, distance=110.0)
+    vi = crx.mat4_inverse(view)
+    def at(x, y, z, s):          # an instance at view-space (x, y, z), scale s
+        w = np.eye(4, dtype=np.float32) * s
+        w[3, 3] = 1.0
+        c = np.array(vi.m, dtype=np.float32).reshape(4, 4).T @ np.array([x, y, z, 1.0], dtype=np.float32)
+        w[:3, 3] = c[:3]
+        return crx.Mat4.from_numpy(w)
+    f = 1.0 / math.tan(wl.FOVY / 2)
+    a = 960.0 / 544.0
+    for z in (-30.0, -60.0, -95.0):
+        e = -z / f
+        for (x, y) in ((-e * a * 0.999, 0.0), (e * a * 0.999, 0.0), (0.0, -e * 0.999), (0.0, e * 0.999),
+                       (-e * a, -e), (e * a * 1.01, e * 1.01)):
+            worlds.append(at(x, y, z, 0.3))
+    worlds.append(at(0.0, 0.0, 0.5, 1.0))      # around the camera: crosses the near plane
+    worlds.append(at(0.0, 0.0, 20.0, 2.0))     # behind the camera
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), worlds)
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = per_lane
+    _check(ctx, spec, 960, 544, view, proj, instanced=True, tuning=t)
 
 
 @pytest.mark.parametrize("per_lane", [1, 2, 4])
