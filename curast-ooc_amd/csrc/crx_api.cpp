@@ -124,8 +124,8 @@ crx_status check_tuning(const crx_tuning& t) {
     if (t.stage1_cull_order > 3) return fail(CRX_INVALID_ARG, "tuning.stage1_cull_order must be 0, 1, 2 or 3");
     if (t.stage1_fast_tiny > 1) return fail(CRX_INVALID_ARG, "tuning.stage1_fast_tiny must be 0 or 1");
     if (t.stage1_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_order must be 0, 1 or 2");
-    if (t.stage1_tris_per_lane != 1 && t.stage1_tris_per_lane != 2 && t.stage1_tris_per_lane != 4)
-        return fail(CRX_INVALID_ARG, "tuning.stage1_tris_per_lane must be 1, 2 or 4");
+    if (t.stage1_tris_per_lane > 4 || t.stage1_tris_per_lane == 3)
+        return fail(CRX_INVALID_ARG, "tuning.stage1_tris_per_lane must be 0, 1, 2 or 4");
     if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
     if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
         t.stage3_waves_per_cu > 64 || t.stage1_waves_per_cu > 64)
@@ -251,14 +251,18 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     if (numViews > 1 && numInstances > kInstanceMask + 1u)
         return fail(CRX_INVALID_ARG, "a multi-view draw takes at most 2^24 instances");
     const crx_tuning& tu = ctx->tuning;
+    const crx_cmesh* list = instanced ? meshes : instances;
+    D.numItems = instanced ? numMeshes : numInstances;
+    bool manyInstances = false;
+    for (uint32_t k = 0; instanced && k < D.numItems; k++)
+        manyInstances = manyInstances || list[k].instances.count >= (int32_t)kBoxInstances;
+    const S1Schedule sched = s1_schedule(tu, manyInstances);
     D.block = tu.stage1_block;
-    D.trisPerLane = tu.stage1_tris_per_lane;
+    D.trisPerLane = sched.trisPerLane;
     D.batchShift = 0;   // log2(triangles per batch)
     while ((1u << D.batchShift) < D.block * D.trisPerLane) D.batchShift++;
     const uint64_t batchTris = 1ull << D.batchShift;
 
-    const crx_cmesh* list = instanced ? meshes : instances;
-    D.numItems = instanced ? numMeshes : numInstances;
     D.items.resize(D.numItems);
     // the draw's stage-1 kernel is chosen by its first mesh, as the reference
     // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
@@ -316,10 +320,10 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     D.offViews = align_up(D.offInst + D.inst.size() * sizeof(InstanceRec), 256);
     D.offXf = align_up(D.offViews + (size_t)numViews * sizeof(ViewParams), 256);
     D.metaBytes = D.offXf + D.xf.size() * sizeof(Transform);
-    if (tu.stage1_waves_per_cu == 0) {
+    if (sched.wavesPerCU == 0) {
         D.g1 = (uint32_t)std::min<uint64_t>(D.batches, 1ull << 31);
     } else {
-        D.g1 = grid_for(ctx, tu.stage1_waves_per_cu, D.block / 64);
+        D.g1 = grid_for(ctx, sched.wavesPerCU, D.block / 64);
         if (D.g1 > D.batches) D.g1 = (uint32_t)D.batches;
     }
     D.g1 = (D.g1 + 7u) & ~7u;   // the stage-1 XCD split assumes a multiple of 8 workgroups
@@ -824,7 +828,7 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage1_cull_order = 3;
     out->stage1_order = 1;
     out->stage1_fast_tiny = 1;
-    out->stage1_tris_per_lane = 2;
+    out->stage1_tris_per_lane = 0;
 }
 
 crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {

@@ -80,8 +80,12 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
     const crx_tuning& tu = ctx->tuning;
     RasterParams P{};
     bind_queues(P, a);
+    // instance counts live in device memory: an instanced draw takes the
+    // many-instances choice
+    const S1Schedule sched = s1_schedule(tu, instanced);
+    const uint32_t perLane = sched.trisPerLane;
     uint32_t batchShift = 0;
-    while ((1u << batchShift) < tu.stage1_block * tu.stage1_tris_per_lane) batchShift++;
+    while ((1u << batchShift) < tu.stage1_block * perLane) batchShift++;
     // device image: items | instances | transforms | view | batch count
     const size_t offItems = 0;
     const size_t offInst = align_up(offItems + (size_t)numItems * sizeof(DrawItem), 256);
@@ -105,7 +109,7 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
     P.order = tu.stage1_order;
     P.cullOrder = tu.stage1_cull_order;
     P.fastTiny = tu.stage1_fast_tiny;
-    const uint32_t g1 = (grid_for(ctx, tu.stage1_waves_per_cu ? tu.stage1_waves_per_cu : 32u, tu.stage1_block / 64) + 7u) & ~7u;
+    const uint32_t g1 = (grid_for(ctx, sched.wavesPerCU ? sched.wavesPerCU : 32u, tu.stage1_block / 64) + 7u) & ~7u;
     const uint32_t g2 = grid_for(ctx, tu.stage2_waves_per_cu, 4), g3 = grid_for(ctx, tu.stage3_waves_per_cu, 4);
     P.numWaves2 = g2 * 4u;
     P.numWaves3 = g3 * 4u;
@@ -120,7 +124,7 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
                             a->numInstances, instanced ? 1 : 0, pu16 ? 1 : 0, batchShift,
                             reinterpret_cast<DrawItem*>(dev + offItems), reinterpret_cast<InstanceRec*>(dev + offInst),
                             reinterpret_cast<Transform*>(dev + offXf), reinterpret_cast<uint32_t*>(dev + offCount), s));
-        CRX_HIP(launch_stage1(P, tu.stage1_block, tu.stage1_tris_per_lane, g1, pu16, instanced, s));
+        CRX_HIP(launch_stage1(P, tu.stage1_block, perLane, g1, pu16, instanced, s));
     }
     ctx->args.valid = true;
     ctx->args.instances = a->instances;

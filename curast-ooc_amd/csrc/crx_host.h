@@ -183,6 +183,21 @@ bool mirror_flag(const crx_mat4& m);
 ViewParams make_view_params(const crx_render_target& t, const Transform* transforms);
 uint32_t grid_for(const crx_context* ctx, uint32_t wavesPerCU, uint32_t wavesPerBlock);
 void set_counter_block(RasterParams& P, uint32_t* block);
+
+// Stage-1 schedule of a draw: triangles per lane and grid size in waves per
+// CU (0 = one workgroup per batch).  With stage1_tris_per_lane = 0 a draw
+// that repeats a mesh over many instances takes one triangle per lane (its
+// sweep keeps instance lists per wave and runs at higher occupancy) and 64
+// waves per CU (its batches differ widely in cost, so more workgroups than
+// fit at once let the dispatcher even out the load); other draws take two
+// triangles per lane and the tuning's grid.
+struct S1Schedule {
+    uint32_t trisPerLane, wavesPerCU;
+};
+inline S1Schedule s1_schedule(const crx_tuning& t, bool manyInstances) {
+    if (t.stage1_tris_per_lane) return {t.stage1_tris_per_lane, t.stage1_waves_per_cu};
+    return manyInstances ? S1Schedule{1u, 64u} : S1Schedule{2u, t.stage1_waves_per_cu};
+}
 crx_status check_tuning(const crx_tuning& t);
 crx_status check_target(const crx_render_target* t);
 void frustum_planes(const crx_view& view, float out[16]);

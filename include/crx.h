@@ -177,8 +177,9 @@ typedef struct crx_tuning {
     uint32_t tile_size;           /* TILE_SIZE = 64: stage-2 huge limit (tile^2) and stage-3 tile edge; 16..256, multiple of 16 */
     uint32_t max_nontrivial;      /* MAX_NONTRIVIAL_TRIANGLES = 5e6: initial stage-2 queue capacity */
     uint32_t max_huge;            /* MAX_HUGE_TRIANGLES = 5e6: initial stage-3 queue capacity */
-    uint32_t stage1_block;        /* threads per stage-1 workgroup: 64, 128, 256, 512 or 1024 */
-    uint32_t stage1_waves_per_cu; /* stage-1 grid = CUs * this / (stage1_block/64); 0 = one workgroup per batch */
+    uint32_t stage1_block;        /* threads per stage-1 workgroup: 128, 256 or 512 */
+    uint32_t stage1_waves_per_cu; /* stage-1 grid = CUs * this / (stage1_block/64), at most 64;
+                                     0 = one workgroup per batch */
     uint32_t stage2_waves_per_cu; /* stage-2 persistent grid size in waves per CU */
     uint32_t stage3_waves_per_cu; /* stage-3 persistent grid size in waves per CU */
     uint32_t stage1_cull_order;   /* 0 = backface test before the frustum test, 1 = frustum first
@@ -190,7 +191,10 @@ typedef struct crx_tuning {
     uint32_t stage1_fast_tiny;    /* 1 = decide sure tiny-triangle culls from v_rcp_f32 screen
                                      coordinates with a proven error margin (exact either way) */
     uint32_t stage1_tris_per_lane; /* triangles each stage-1 lane takes per batch: 1, 2 or 4 (a batch
-                                     is stage1_block * this consecutive triangles of one item) */
+                                     is stage1_block * this consecutive triangles of one item);
+                                     0 = per draw: a draw with a mesh drawn over >= 4 instances
+                                     (every instanced crx_raster_stage1 draw) takes 1 and a grid
+                                     of 64 waves per CU, others take 2 and stage1_waves_per_cu */
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */

@@ -114,6 +114,23 @@ def test_routing_tuning(ctx, small, tile):
     _check(ctx, spec, W, H, view, proj, tuning=t)
 
 
+def test_tuning_values_are_checked(ctx):
+    """crx_context_set_tuning takes 0 (per draw), 1, 2 and 4 triangles per
+    lane and cull orders 0-3, and rejects other values."""
+    for per_lane in (0, 1, 2, 4):
+        t = crx.default_tuning()
+        t.stage1_tris_per_lane = per_lane
+        ctx.tuning = t
+    for field, bad in (("stage1_tris_per_lane", 3), ("stage1_tris_per_lane", 8), ("stage1_cull_order", 4),
+                       ("stage1_block", 192)):
+        t = crx.default_tuning()
+        setattr(t, field, bad)
+        with pytest.raises(crx.CrxError) as e:
+            ctx.tuning = t
+        assert e.value.status == 1, field
+    ctx.tuning = crx.default_tuning()
+
+
 def test_queue_overflow_regrows(ctx):
     spec = _closeup_terrain()
     W, H = 1280, 720
@@ -233,13 +250,15 @@ def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, bloc
     _check(ctx, spec, 1280, 720, view, proj, tuning=t)
 
 
-@pytest.mark.parametrize("per_lane", [1, 2])
+@pytest.mark.parametrize("per_lane", [0, 1, 2, 4])
 def test_many_instances_sub_pixel(ctx, per_lane):
-    """A dense mesh drawn as many small instances (config 3 scaled down):
-    most 64-triangle groups project inside one pixel and are skipped per
-    instance by the wave-box tiny test.  Extra instances cross the near plane,
-    sit behind the camera and hug the viewport's first and last columns and
-    rows; every bit must still be the oracle's."""
+    """A dense mesh drawn as many small instances (the 3000-instance scene
+    scaled down): most 16-triangle row groups project inside one pixel and
+    are dropped from that row's instance list by the box tiny test, so the
+    rows of a wave draw different instances in the same step.  Extra
+    instances cross the near plane, sit behind the camera and hug the
+    viewport's first and last columns and rows; every bit must still be the
+    oracle's.  per_lane 0 is the per-draw choice."""
     pos, idx, col = wl.sphere_mesh(rings=128, segments=256, seed=61)
     worlds = list(wl.instance_transforms(300, extent=60.0, smin=0.2, smax=1.5, seed=62))
     view, proj = wl.instances_view(extent=60.0, width=960, height=544, distance=110.0)
