@@ -796,29 +796,43 @@ __device__ __forceinline__ int32_t dpp_or(int32_t identity, int32_t x) {
     return __builtin_amdgcn_update_dpp(identity, x, CTRL, ROWMASK, 0xF, false);
 }
 
-// Minimum / maximum over each row of 16 lanes; every lane of the row ends
-// with the row's value.
-__device__ __forceinline__ int32_t row_min_i(int32_t x) {
+// Minimum / maximum over each group of G = 8 or 16 consecutive lanes; every
+// lane of the group ends with the group's value.  Quad exchanges reduce each
+// quad; the half-row mirror folds quads 0-1 and 2-3 of a row (G = 8), row
+// rotations by 4 and 8 fold the whole row (G = 16).
+template <int G>
+__device__ __forceinline__ int32_t group_min_i(int32_t x) {
+    static_assert(G == 8 || G == 16, "lane groups of 8 or 16");
     constexpr int32_t I = 0x7FFFFFFF;
     x = min(x, dpp_or<0xB1, 0xF>(I, x));    // quad_perm [1,0,3,2]
     x = min(x, dpp_or<0x4E, 0xF>(I, x));    // quad_perm [2,3,0,1]
-    x = min(x, dpp_or<0x124, 0xF>(I, x));   // row_ror:4
-    x = min(x, dpp_or<0x128, 0xF>(I, x));   // row_ror:8
+    if constexpr (G == 8) {
+        x = min(x, dpp_or<0x141, 0xF>(I, x));   // row_half_mirror
+    } else {
+        x = min(x, dpp_or<0x124, 0xF>(I, x));   // row_ror:4
+        x = min(x, dpp_or<0x128, 0xF>(I, x));   // row_ror:8
+    }
     return x;
 }
 
-__device__ __forceinline__ int32_t row_max_i(int32_t x) {
+template <int G>
+__device__ __forceinline__ int32_t group_max_i(int32_t x) {
+    static_assert(G == 8 || G == 16, "lane groups of 8 or 16");
     constexpr int32_t I = (int32_t)0x80000000;
     x = max(x, dpp_or<0xB1, 0xF>(I, x));
     x = max(x, dpp_or<0x4E, 0xF>(I, x));
-    x = max(x, dpp_or<0x124, 0xF>(I, x));
-    x = max(x, dpp_or<0x128, 0xF>(I, x));
+    if constexpr (G == 8) {
+        x = max(x, dpp_or<0x141, 0xF>(I, x));
+    } else {
+        x = max(x, dpp_or<0x124, 0xF>(I, x));
+        x = max(x, dpp_or<0x128, 0xF>(I, x));
+    }
     return x;
 }
 
 __device__ __forceinline__ int32_t wave_min_i(int32_t x) {
     constexpr int32_t I = 0x7FFFFFFF;
-    x = row_min_i(x);
+    x = group_min_i<16>(x);
     x = min(x, dpp_or<0x142, 0xA>(I, x));   // row_bcast:15 into rows 1, 3
     x = min(x, dpp_or<0x143, 0xC>(I, x));   // row_bcast:31 into rows 2, 3
     return __builtin_amdgcn_readlane(x, 63);
@@ -826,7 +840,7 @@ __device__ __forceinline__ int32_t wave_min_i(int32_t x) {
 
 __device__ __forceinline__ int32_t wave_max_i(int32_t x) {
     constexpr int32_t I = (int32_t)0x80000000;
-    x = row_max_i(x);
+    x = group_max_i<16>(x);
     x = max(x, dpp_or<0x142, 0xA>(I, x));
     x = max(x, dpp_or<0x143, 0xC>(I, x));
     return __builtin_amdgcn_readlane(x, 63);
@@ -996,31 +1010,35 @@ __device__ __forceinline__ bool wave_box(const f3& A, const f3& B, const f3& C, 
     return !unbounded;
 }
 
-// Object-space boxes of the four rows of 16 triangles of the wave (row r is
-// lanes 16r .. 16r+15), as wave-uniform values.  bounded[r] is false when a
-// coordinate of row r is NaN or infinite.
-__device__ __forceinline__ void row_boxes(const f3& A, const f3& B, const f3& C, WaveBox (&box)[4],
-                                          bool (&bounded)[4]) {
-    const int32_t lo[3] = {row_min_i(min(fkey(A.x), min(fkey(B.x), fkey(C.x)))),
-                           row_min_i(min(fkey(A.y), min(fkey(B.y), fkey(C.y)))),
-                           row_min_i(min(fkey(A.z), min(fkey(B.z), fkey(C.z))))};
-    const int32_t hi[3] = {row_max_i(max(fkey(A.x), max(fkey(B.x), fkey(C.x)))),
-                           row_max_i(max(fkey(A.y), max(fkey(B.y), fkey(C.y)))),
-                           row_max_i(max(fkey(A.z), max(fkey(B.z), fkey(C.z))))};
+// Lanes per group of the per-group instance lists of k_stage1 (8 or 16).
+constexpr int kBoxGroup = 16;
+
+// Object-space boxes of the kWave / G lane groups of the wave (group g is
+// lanes G*g .. G*g + G - 1, one triangle each), as wave-uniform values.
+// bounded[g] is false when a coordinate of group g is NaN or infinite.
+template <int G>
+__device__ __forceinline__ void group_boxes(const f3& A, const f3& B, const f3& C, WaveBox (&box)[kWave / G],
+                                            bool (&bounded)[kWave / G]) {
+    const int32_t lo[3] = {group_min_i<G>(min(fkey(A.x), min(fkey(B.x), fkey(C.x)))),
+                           group_min_i<G>(min(fkey(A.y), min(fkey(B.y), fkey(C.y)))),
+                           group_min_i<G>(min(fkey(A.z), min(fkey(B.z), fkey(C.z))))};
+    const int32_t hi[3] = {group_max_i<G>(max(fkey(A.x), max(fkey(B.x), fkey(C.x)))),
+                           group_max_i<G>(max(fkey(A.y), max(fkey(B.y), fkey(C.y)))),
+                           group_max_i<G>(max(fkey(A.z), max(fkey(B.z), fkey(C.z))))};
     #pragma unroll
-    for (int r = 0; r < 4; r++) {
+    for (int g = 0; g < kWave / G; g++) {
         int32_t l[3], h[3];
         #pragma unroll
         for (int c = 0; c < 3; c++) {
-            l[c] = __builtin_amdgcn_readlane(lo[c], 16 * r);
-            h[c] = __builtin_amdgcn_readlane(hi[c], 16 * r);
+            l[c] = __builtin_amdgcn_readlane(lo[c], G * g);
+            h[c] = __builtin_amdgcn_readlane(hi[c], G * g);
         }
-        bounded[r] = min(l[0], min(l[1], l[2])) > kKeyNegInf && max(h[0], max(h[1], h[2])) < kKeyPosInf;
-        box[r].lo = f3{fkey_inv(l[0]), fkey_inv(l[1]), fkey_inv(l[2])};
-        box[r].hi = f3{fkey_inv(h[0]), fkey_inv(h[1]), fkey_inv(h[2])};
-        box[r].M = fmaxf(fmaxf(fmaxf(fabsf(box[r].lo.x), fabsf(box[r].hi.x)),
-                               fmaxf(fabsf(box[r].lo.y), fabsf(box[r].hi.y))),
-                         fmaxf(fabsf(box[r].lo.z), fabsf(box[r].hi.z)));
+        bounded[g] = min(l[0], min(l[1], l[2])) > kKeyNegInf && max(h[0], max(h[1], h[2])) < kKeyPosInf;
+        box[g].lo = f3{fkey_inv(l[0]), fkey_inv(l[1]), fkey_inv(l[2])};
+        box[g].hi = f3{fkey_inv(h[0]), fkey_inv(h[1]), fkey_inv(h[2])};
+        box[g].M = fmaxf(fmaxf(fmaxf(fabsf(box[g].lo.x), fabsf(box[g].hi.x)),
+                               fmaxf(fabsf(box[g].lo.y), fabsf(box[g].hi.y))),
+                         fmaxf(fabsf(box[g].lo.z), fabsf(box[g].hi.z)));
     }
 }
 
@@ -1037,68 +1055,102 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
         __shared__ uint32_t pu16Words[BLOCK / kWave][kPu16Words];
         words = pu16Words[threadIdx.x / kWave];
     }
+    uint32_t* queue = nullptr;
+    if constexpr (BOXES) {
+        __shared__ uint32_t instQueue[BLOCK / kWave][2 * kWave];
+        queue = instQueue[threadIdx.x / kWave];
+    }
     s1_sweep<BLOCK, K, PU16>(P, words, [&](const S1Item& it, uint32_t base, const f3 (&A)[K], const f3 (&B)[K],
                                            const f3 (&C)[K]) __attribute__((always_inline)) {
         if (BOXES && it.instCount >= kBoxInstances) {
-            // A mesh drawn over many instances.  Each row of 16 lanes has the
-            // object-space box of its 16 triangles (row_boxes).  64 instances
-            // at a time, one per lane, are tested against the four boxes
-            // (wave_tiny_culled), which leaves each row its own list of
-            // instances to draw.  Every step then draws, in each row, the
-            // next instance of that row's list, with the transform read from
-            // the lane that tested it: the wave takes as many steps as its
+            // A mesh drawn over many instances.  Per wave group: 64 instances
+            // at a time, one per lane, are tested against the box of the
+            // wave's 64 triangles (wave_tiny_culled), and the instances it
+            // keeps are queued in LDS.  Each 64 queued instances are tested
+            // against the boxes of the wave's lane groups of kBoxGroup
+            // triangles, which leaves each group its own list of instances
+            // to draw.  Every step then draws, in each group, the next
+            // instance of that group's list, with the transform read from the
+            // lane that tested it: the wave takes as many steps as its
             // longest list holds instances.
+            constexpr int NG = kWave / kBoxGroup;
             const uint32_t lane = threadIdx.x & (kWave - 1);
-            const uint32_t row = lane >> 4;
+            const uint32_t grp = lane / kBoxGroup;
             const uint32_t cullOrder = item_cull_order(P, it.instCount);
-            WaveBox box[K][4];
-            bool bounded[K][4], live[K];
+            const auto xf = kglobal(P.view0.transforms) + it.instOffset;
             #pragma unroll
             for (int k = 0; k < K; k++) {
-                live[k] = __ballot(base + k * BLOCK + threadIdx.x < it.numTriangles) != 0ull;
-                if (live[k]) row_boxes(A[k], B[k], C[k], box[k], bounded[k]);
-            }
-            for (uint32_t jb = 0; jb < it.instCount; jb += kWave) {
-                const bool valid = jb + lane < it.instCount;
-                const auto Tj = kglobal(P.view0.transforms) + it.instOffset + min(jb + lane, it.instCount - 1u);
-                float mj[16];
-                #pragma unroll
-                for (int e = 0; e < 16; e++) mj[e] = Tj->m[e];
-                const uint32_t fj = Tj->flip;
-                #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    if (!live[k]) continue;
-                    uint64_t q[4];
-                    #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        q[r] = __ballot(valid && !(bounded[k][r] && wave_tiny_culled(mj, P.view0, box[k][r])));
-                    const uint32_t tri = base + k * BLOCK + threadIdx.x;
-                    while ((q[0] | q[1]) | (q[2] | q[3])) {
-                        uint32_t src = 0;
-                        bool act = false;
+                const uint32_t tri = base + k * BLOCK + threadIdx.x;
+                if (__ballot(tri < it.numTriangles) == 0ull) continue;
+                WaveBox wbox, gbox[NG];
+                bool gbounded[NG];
+                const bool wbounded = wave_box(A[k], B[k], C[k], wbox);
+                group_boxes<kBoxGroup>(A[k], B[k], C[k], gbox, gbounded);
+                uint32_t queued = 0;
+                for (uint32_t jb = 0;; jb += kWave) {
+                    const bool more = jb < it.instCount;
+                    if (more) {
+                        const uint32_t j = jb + lane;
+                        const auto Tj = xf + min(j, it.instCount - 1u);
+                        float mm[16];
                         #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            if (q[r]) {
-                                const uint32_t sr = (uint32_t)__builtin_ctzll(q[r]);
-                                q[r] &= q[r] - 1ull;
-                                if (row == (uint32_t)r) {
-                                    src = sr;
-                                    act = true;
+                        for (int e = 0; e < 16; e++) mm[e] = Tj->m[e];
+                        const bool keep = j < it.instCount && !(wbounded && wave_tiny_culled(mm, P.view0, wbox));
+                        const uint64_t kept = __ballot(keep);
+                        const uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(kept >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)kept, 0u));
+                        if (keep) queue[queued + slot] = j;   // queued < 64: slot < 128
+                        queued += (uint32_t)__popcll(kept);
+                    }
+                    if (queued >= kWave || (!more && queued > 0u)) {
+                        wave_sync();
+                        const uint32_t n = min(queued, (uint32_t)kWave);
+                        const uint32_t qj = queue[min(lane, n - 1u)];
+                        const uint32_t rest = queue[kWave + lane];
+                        wave_sync();
+                        if (kWave + lane < queued) queue[lane] = rest;
+                        queued -= n;
+                        const auto Tq = xf + qj;
+                        float mq[16];
+                        #pragma unroll
+                        for (int e = 0; e < 16; e++) mq[e] = Tq->m[e];
+                        const uint32_t fq = Tq->flip;
+                        uint64_t q[NG];
+                        #pragma unroll
+                        for (int g = 0; g < NG; g++)
+                            q[g] = __ballot(lane < n && !(gbounded[g] && wave_tiny_culled(mq, P.view0, gbox[g])));
+                        for (;;) {
+                            uint64_t any = 0ull;
+                            #pragma unroll
+                            for (int g = 0; g < NG; g++) any |= q[g];
+                            if (!any) break;
+                            uint32_t src = 0;
+                            bool act = false;
+                            #pragma unroll
+                            for (int g = 0; g < NG; g++) {
+                                if (q[g]) {
+                                    const uint32_t sg = (uint32_t)__builtin_ctzll(q[g]);
+                                    q[g] &= q[g] - 1ull;
+                                    if (grp == (uint32_t)g) {
+                                        src = sg;
+                                        act = true;
+                                    }
                                 }
                             }
-                        }
-                        float m[16];
-                        #pragma unroll
-                        for (int e = 0; e < 16; e++) m[e] = __shfl(mj[e], (int)src);
-                        const uint32_t flip = (uint32_t)__shfl((int)fj, (int)src);
-                        if (act && tri < it.numTriangles) {
-                            const uint32_t jj = jb + src;
-                            const uint32_t inst = it.instOffset + jj;
-                            const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(jj * it.numTriangles);
-                            stage1_triangle(P, P.view0, m, flip != 0u, A[k], B[k], C[k], id,
-                                            ((uint64_t)inst << 32) | (uint64_t)tri, cullOrder);
+                            float m[16];
+                            #pragma unroll
+                            for (int e = 0; e < 16; e++) m[e] = __shfl(mq[e], (int)src);
+                            const uint32_t flip = (uint32_t)__shfl((int)fq, (int)src);
+                            const uint32_t jj = (uint32_t)__shfl((int)qj, (int)src);
+                            if (act && tri < it.numTriangles) {
+                                const uint32_t inst = it.instOffset + jj;
+                                const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(jj * it.numTriangles);
+                                stage1_triangle(P, P.view0, m, flip != 0u, A[k], B[k], C[k], id,
+                                                ((uint64_t)inst << 32) | (uint64_t)tri, cullOrder);
+                            }
                         }
                     }
+                    if (!more) break;
                 }
             }
             return;
