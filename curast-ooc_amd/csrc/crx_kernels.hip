@@ -424,6 +424,11 @@ struct S1Item {
     uint32_t numVertices;
 };
 
+// First batch past item `item`.
+__device__ __forceinline__ uint32_t item_end(const RasterParams& P, uint32_t item, uint32_t numBatches) {
+    return item + 1u < P.numItems ? kconst(P.items)[item + 1u].firstBatch : numBatches;
+}
+
 __device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item, uint32_t numBatches) {
     const auto items = kconst(P.items);
     S1Item r;
@@ -435,7 +440,7 @@ __device__ __forceinline__ S1Item s1_item(const RasterParams& P, uint32_t item, 
     r.instOffset = items[item].instOffset;
     r.instCount = items[item].instCount;
     r.firstBatch = items[item].firstBatch;
-    r.endBatch = item + 1u < P.numItems ? items[item + 1u].firstBatch : numBatches;
+    r.endBatch = item_end(P, item, numBatches);
     r.bits = items[item].bitsPerIndex;
     r.indexMin = items[item].indexMin;
     r.cfx = items[item].cf[0];
@@ -726,46 +731,51 @@ __device__ __forceinline__ void s1_sweep(const RasterParams& P, uint32_t* words,
     uint32_t begin, end, step;
     s1_range(numBatches, P.order, begin, end, step);
     if (begin >= end) return;
+    // One pass per item: within a pass the item's fields are loop-invariant
+    // scalars (holding the items of the batches in flight as well would spill
+    // scalar registers), and the pipeline drains at the item's end.
     uint32_t batch = begin;
-    S1Item it = s1_item(P, find_item(P, batch), numBatches);
-    uint32_t i0[K], i1[K], i2[K], i3[K];
-    s1_batch_indices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2, i3);
-    bool more;
-    uint32_t b1 = s1_next(batch, step, end, &more);
-    S1Item it1 = b1 < it.endBatch ? it : s1_item(P, find_item(P, b1), numBatches);
-    uint32_t j0[K], j1[K], j2[K], j3[K];
-    s1_batch_indices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2, j3);
-    f3 A[K], B[K], C[K];
-    s1_batch_vertices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2, i3, words, A, B, C);
     for (;;) {
-        f3 An[K], Bn[K], Cn[K];
-        s1_batch_vertices<BLOCK, K, PU16>(P, it1, b1, j0, j1, j2, j3, words, An, Bn, Cn);
-        bool more2;
-        const uint32_t b2 = more ? s1_next(b1, step, end, &more2) : b1;
-        if (!more) more2 = false;
-        const S1Item it2 = b2 < it1.endBatch ? it1 : s1_item(P, find_item(P, b2), numBatches);
-        s1_batch_indices<BLOCK, K, PU16>(P, it2, b2, j0, j1, j2, j3);
-        if (PU16) {
+        const S1Item it = s1_item(P, find_item(P, batch), numBatches);
+        const uint32_t iend = min(end, it.endBatch);
+        uint32_t i0[K], i1[K], i2[K], i3[K];
+        s1_batch_indices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2, i3);
+        bool more;
+        uint32_t b1 = s1_next(batch, step, iend, &more);
+        uint32_t j0[K], j1[K], j2[K], j3[K];
+        s1_batch_indices<BLOCK, K, PU16>(P, it, b1, j0, j1, j2, j3);
+        f3 A[K], B[K], C[K];
+        s1_batch_vertices<BLOCK, K, PU16>(P, it, batch, i0, i1, i2, i3, words, A, B, C);
+        for (;;) {
+            f3 An[K], Bn[K], Cn[K];
+            s1_batch_vertices<BLOCK, K, PU16>(P, it, b1, j0, j1, j2, j3, words, An, Bn, Cn);
+            bool more2;
+            const uint32_t b2 = more ? s1_next(b1, step, iend, &more2) : b1;
+            if (!more) more2 = false;
+            s1_batch_indices<BLOCK, K, PU16>(P, it, b2, j0, j1, j2, j3);
+            if (PU16) {
+                #pragma unroll
+                for (int k = 0; k < K; k++) {
+                    A[k] = pu16_vertex_decode(it, A[k]);
+                    B[k] = pu16_vertex_decode(it, B[k]);
+                    C[k] = pu16_vertex_decode(it, C[k]);
+                }
+            }
+            body(it, (batch - it.firstBatch) << P.batchShift, A, B, C);
+            if (!more) break;
+            batch = b1;
             #pragma unroll
             for (int k = 0; k < K; k++) {
-                A[k] = pu16_vertex_decode(it, A[k]);
-                B[k] = pu16_vertex_decode(it, B[k]);
-                C[k] = pu16_vertex_decode(it, C[k]);
+                A[k] = An[k];
+                B[k] = Bn[k];
+                C[k] = Cn[k];
             }
+            b1 = b2;
+            more = more2;
         }
-        body(it, (batch - it.firstBatch) << P.batchShift, A, B, C);
-        if (!more) break;
-        batch = b1;
-        it = it1;
-        #pragma unroll
-        for (int k = 0; k < K; k++) {
-            A[k] = An[k];
-            B[k] = Bn[k];
-            C[k] = Cn[k];
-        }
-        b1 = b2;
-        it1 = it2;
-        more = more2;
+        const uint32_t next = batch + step;
+        if (next >= end || next <= batch) break;
+        batch = next;
     }
 }
 
