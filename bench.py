@@ -175,6 +175,29 @@ def bench_atomics(ctx, log_entries: int = 21, per_lane: int = 64, reps: int = 5)
     return out
 
 
+def bench_hbm(ctx, nbytes: int = 1 << 30, reps: int = 20):
+    """Measured HBM bandwidth of plain streaming kernels over nbytes of
+    device memory: a read (crx_probe_read) and a copy (crx_copy_buffer, read
+    plus write), in GB/s of bytes moved, from wall-clock time over `reps`
+    launches between stream synchronisations."""
+    src, dst, sink = ctx.alloc(nbytes), ctx.alloc(nbytes), ctx.alloc(16)
+    out = {"bytes": nbytes}
+    for name, fn, moved in (("read", lambda: ctx.probe_read(src, nbytes, sink), nbytes),
+                            ("copy", lambda: ctx.copy(dst, src, nbytes), 2 * nbytes)):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[f"{name}_ms"] = round(dt * 1e3, 4)
+        out[f"{name}_GBps"] = round(moved / dt / 1e9, 1)
+    for b in (src, dst, sink):
+        b.free()
+    return out
+
+
 def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: int, batch: int, plain: bool = True):
     """Out-of-core serving workload: an n x n heightfield cut into tile x tile
     nodes in pinned host memory, half of them device-resident at start, the
@@ -380,15 +403,18 @@ def main() -> int:
     atomics = bench_atomics(ctx)
     log(f"[bench] atomicMin probe: {atomics}")
 
+    hbm = bench_hbm(ctx)
+    log(f"[bench] HBM probe: {hbm}")
     if args.probe_copy:
         nbytes = 1 << 30
-        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        src, dst, sink = ctx.alloc(nbytes), ctx.alloc(nbytes), ctx.alloc(16)
         for _ in range(3):
             ctx.copy(dst, src, nbytes)
+            ctx.probe_read(src, nbytes, sink)
         ctx.synchronize()
-        src.free()
-        dst.free()
-        log(f"[bench] copy probe: 3 x k_copy of {nbytes} bytes")
+        for b in (src, dst, sink):
+            b.free()
+        log(f"[bench] copy probe: 3 x k_copy and 3 x k_read of {nbytes} bytes")
 
     # per-stage device times (HIP events on the draw stream), separate from the wall-clock run
     ctx.enable_timing(True)
@@ -430,6 +456,9 @@ def main() -> int:
                 "traffic": None if args.compressed else load_traffic(args.workload),
                 "kernel": "k_stage1", "algorithmic_bytes_per_launch": algo,
                 "stage1_ms_mean": round(mean_s1, 4)}
+    if args.workload == "instances":
+        roofline["note"] = ("the mesh is read once for all 3000 instances: this stage 1 is bound by the "
+                            "per-instance arithmetic (VALU), not by HBM")
 
     multiview = None
     if args.workload == "heightfield" and not args.no_multiview:
@@ -465,6 +494,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "atomic_min_peak": atomics,
+        "hbm_probe": hbm,
         "batched_multiview": multiview,
     }
     print(json.dumps(out), flush=True)

@@ -251,6 +251,7 @@ _SIGS = {
     "crx_stream_synchronize": (C.c_int, [_VP, _VP]),
     "crx_copy_buffer": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_probe_atomic_min": (C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), _VP]),
+    "crx_probe_read": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _VP]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
     "crx_raster_stage1": (C.c_int, [_VP, C.POINTER(RasterArgs), C.c_uint32, _VP]),
     "crx_raster_stage2": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
@@ -482,6 +483,10 @@ class Context:
 
     def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int, stream=None) -> None:
         _check(lib().crx_copy_buffer(self.handle, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, stream))
+
+    def probe_read(self, src: DeviceBuffer, nbytes: int, sink: DeviceBuffer, stream=None) -> None:
+        """Enqueue the streaming-read bandwidth probe over nbytes of src."""
+        _check(lib().crx_probe_read(self.handle, C.c_void_p(src.ptr), nbytes, C.c_void_p(sink.ptr), stream))
 
     def probe_atomic_min(self, buf: DeviceBuffer, log_entries: int, per_lane: int, scattered: bool,
                          stream=None) -> int:

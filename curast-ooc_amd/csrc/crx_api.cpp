@@ -975,6 +975,15 @@ crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t 
     return CRX_OK;
 }
 
+crx_status crx_probe_read(crx_context* ctx, const void* src, size_t bytes, uint32_t* sink, void* stream) {
+    if (!ctx || !src || !sink) return fail(CRX_INVALID_ARG, "NULL argument");
+    if ((bytes % 16) != 0 || ((uintptr_t)src % 16) != 0)
+        return fail(CRX_INVALID_ARG, "crx_probe_read needs a 16-byte aligned pointer and size");
+    CRX_HIP(hipSetDevice(ctx->device));
+    CRX_HIP(launch_read(src, bytes, ctx->numCUs, sink, as_stream(stream)));
+    return CRX_OK;
+}
+
 crx_status crx_probe_atomic_min(crx_context* ctx, uint64_t* buffer, uint32_t logEntries, uint32_t atomicsPerLane,
                                 int scattered, uint64_t* issued, void* stream) {
     if (!ctx || !buffer) return fail(CRX_INVALID_ARG, "NULL argument");

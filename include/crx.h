@@ -371,10 +371,15 @@ crx_status crx_device_free(crx_context* ctx, void* ptr);
 crx_status crx_memcpy_htod(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 crx_status crx_memcpy_dtoh(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
 crx_status crx_stream_synchronize(crx_context* ctx, void* stream);
-/* Plain device-to-device copy kernel (16 bytes per lane).  No counterpart in
+/* Plain device-to-device copy kernel (16 bytes per lane, nontemporal).  No counterpart in
  * the reference: it is the known-size kernel that calibrates HBM traffic
  * counters (FETCH_SIZE / WRITE_SIZE) for the roofline measurement. */
 crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* Streaming read of `bytes` (16-byte aligned) of device memory with
+ * nontemporal loads (no counterpart in the reference): the measured read
+ * bandwidth against which stage 1's traffic is read.  sink is a device word
+ * the kernel writes only for one fixed XOR of the data. */
+crx_status crx_probe_read(crx_context* ctx, const void* src, size_t bytes, uint32_t* sink, void* stream);
 /* 64-bit atomic-minimum probe (no counterpart in the reference): every lane
  * of a CUs * 8 x 256 grid performs atomicsPerLane atomicMin on the device
  * buffer of 2^logEntries entries, scattered (hashed positions) or coalesced

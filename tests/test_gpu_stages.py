@@ -168,3 +168,18 @@ def test_atomic_min_probe(ctx):
     assert (out[:n] != crx.CLEAR_PIXEL).sum() > n // 2
     with pytest.raises(crx.CrxError):
         ctx.probe_atomic_min(buf, 40, 1, True)
+
+
+def test_read_probe(ctx):
+    """The streaming-read probe reads ragged sizes (vector loop and tail),
+    leaves its source alone, and rejects unaligned sizes."""
+    n = 4099 * 16
+    src = ctx.to_device(np.arange(n // 4, dtype=np.uint32))
+    sink = ctx.alloc(16)
+    sink.upload(np.full(4, 7, dtype=np.uint32))
+    for nbytes in (16, 4096, n):
+        ctx.probe_read(src, nbytes, sink)
+    ctx.synchronize()
+    assert np.array_equal(src.download(np.uint32, n // 4), np.arange(n // 4, dtype=np.uint32))
+    with pytest.raises(crx.CrxError):
+        ctx.probe_read(src, 20, sink)
