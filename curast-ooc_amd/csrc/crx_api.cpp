@@ -397,7 +397,7 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
 crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                       uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
                       uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
-                      const uint8_t* skip) {
+                      const uint8_t* skip, const std::function<crx_status()>* afterStages) {
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (skip && numViews < 2) return fail(CRX_INVALID_ARG, "per-view instance culling needs a multi-view draw");
     DrawPlan D;
@@ -422,6 +422,9 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         P.hugeCap = ctx->hugeCap;
         CRX_HIP(enqueue_stages(ctx, D, P, false, stream));
         CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        // work that follows the stages (the resolve) is prepared and enqueued
+        // while they run; after a redraw it is enqueued again
+        if (afterStages) CRX_TRY((*afterStages)());
         CRX_HIP(hipStreamSynchronize(stream));
         const uint32_t nNt = ctx->ctrHost[CTR_NONTRIVIAL];
         const uint32_t nHuge = ctx->ctrHost[CTR_HUGE];
@@ -473,6 +476,13 @@ ResolveParams make_resolve_params(const crx_render_target& t, const InstanceRec*
 // numViews >= 1 targets sharing one instance list, in one launch.
 // Metadata image of a resolve: InstanceRec[n] | prefix[n] | ResolveParams[numViews],
 // with device pointers for base address `dev`.
+// Layout of a resolve image: instance records | prefix | per-view records.
+size_t resolve_image_layout(uint32_t numInstances, uint32_t numViews, size_t* offPrefix, size_t* offViews) {
+    *offPrefix = align_up((size_t)numInstances * sizeof(InstanceRec), 256);
+    *offViews = align_up(*offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8, 256);
+    return *offViews + (size_t)numViews * sizeof(ResolveParams);
+}
+
 crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
                          char* dev, std::vector<char>* image, size_t* offViews, uint32_t* blocks) {
@@ -493,8 +503,8 @@ crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, cons
         inst[i] = make_instance(instances[i]);
     }
     const size_t offInst = 0;
-    const size_t offPrefix = align_up(inst.size() * sizeof(InstanceRec), 256);
-    *offViews = align_up(offPrefix + (size_t)numInstances * sizeof(uint64_t) + 8, 256);
+    size_t offPrefix = 0;
+    const size_t bytes = resolve_image_layout(numInstances, numViews, &offPrefix, offViews);
     const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(dev + offInst);
     const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(dev + offPrefix);
     std::vector<ResolveParams> rv(numViews);
@@ -506,7 +516,7 @@ crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, cons
     }
     if (nb >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
     *blocks = (uint32_t)nb;
-    image->assign(*offViews + (size_t)numViews * sizeof(ResolveParams), 0);
+    image->assign(bytes, 0);
     std::memcpy(image->data() + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
     if (numInstances) std::memcpy(image->data() + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
     std::memcpy(image->data() + *offViews, rv.data(), rv.size() * sizeof(ResolveParams));
@@ -520,13 +530,10 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
                          hipStream_t stream) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
     CRX_HIP(hipSetDevice(ctx->device));
-    // first pass sizes the image, second pass fills it for the reserved buffer
     std::vector<char> image;
-    size_t offViews = 0;
+    size_t offPrefix = 0, offViews = 0;
     uint32_t blocks = 0;
-    CRX_TRY(resolve_image(instances, numInstances, prefix, targets, numViews, attribute, nullptr, &image, &offViews,
-                          &blocks));
-    CRX_TRY(ctx->resolveMeta.reserve(image.size()));
+    CRX_TRY(ctx->resolveMeta.reserve(resolve_image_layout(numInstances, numViews, &offPrefix, &offViews)));
     char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
     CRX_TRY(resolve_image(instances, numInstances, prefix, targets, numViews, attribute, meta, &image, &offViews,
                           &blocks));
@@ -574,6 +581,12 @@ crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint3
     uint32_t blocks = 0;
     CRX_TRY(clear_records(buffers, numPixels, n, &cv, &blocks));
     if (cv.empty()) return CRX_OK;
+    if (cv.size() <= 2) {
+        // one or two buffers (a view's visibility and colour buffers): a
+        // launch each, with no metadata to stage and copy
+        for (const ClearView& c : cv) CRX_HIP(launch_clear(c.fb, c.numPixels, nullptr, ctx->numCUs, stream));
+        return CRX_OK;
+    }
     const size_t bytes = cv.size() * sizeof(ClearView);
     CRX_TRY(ctx->clearMeta.reserve(bytes));
     CRX_TRY(ctx->clearStaging.acquire(bytes));
@@ -783,13 +796,13 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
     CRX_TRY(clear_buffers(ctx, clearBufs.data(), clearPx.data(), (uint32_t)clearBufs.size(), stream));
     bool anySkip = false;
     for (uint8_t k : scene->skip) anySkip = anySkip || k != 0;
+    const std::function<crx_status()> resolve = [&]() {
+        return resolve_views(ctx, scene->instances.data(), n, scene->prefix.data(), targets.data(), numViews,
+                             (crx_display_attribute)settings.attribute, stream);
+    };
     CRX_TRY(draw_views(ctx, scene->unique.data(), u, scene->instances.data(), n, scene->transforms.data(),
                        targets.data(), numViews, settings.instanced, stream, stats,
-                       (anySkip && numViews > 1) ? scene->skip.data() : nullptr));
-    if (settings.resolve) {
-        CRX_TRY(resolve_views(ctx, scene->instances.data(), n, scene->prefix.data(), targets.data(), numViews,
-                              (crx_display_attribute)settings.attribute, stream));
-    }
+                       (anySkip && numViews > 1) ? scene->skip.data() : nullptr, settings.resolve ? &resolve : nullptr));
     return CRX_OK;
 }
 

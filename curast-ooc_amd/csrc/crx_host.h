@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -233,7 +234,7 @@ crx_status ensure_queues(crx_context* ctx, uint32_t nt, uint32_t hg);
 crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                       uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
                       uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
-                      const uint8_t* skip = nullptr);
+                      const uint8_t* skip = nullptr, const std::function<crx_status()>* afterStages = nullptr);
 crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
                          hipStream_t stream);
@@ -244,6 +245,7 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
 void view_targets(const crx_view* views, uint32_t numViews, uint64_t* const* framebuffers,
                   uint64_t* const* colorbuffers, bool resolve, std::vector<crx_render_target>* targets,
                   std::vector<uint64_t*>* clearBufs, std::vector<uint32_t>* clearPx);
+size_t resolve_image_layout(uint32_t numInstances, uint32_t numViews, size_t* offPrefix, size_t* offViews);
 crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
                          char* dev, std::vector<char>* image, size_t* offViews, uint32_t* blocks);

@@ -349,14 +349,13 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
         st = clear_buffers(lane.ctx, bufs.data(), px.data(), (uint32_t)bufs.size(), lane.stream);
         if (st != CRX_OK) break;
         crx_draw_stats ds;
+        const std::function<crx_status()> resolve = [&]() {
+            return resolve_views(lane.ctx, inst.data(), U, prefix.data(), targets.data(), V, CRX_DISPLAY_VERTEX_COLORS,
+                                 lane.stream);
+        };
         st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream, &ds,
-                        (anySkip && V > 1) ? skip.data() : nullptr);
+                        (anySkip && V > 1) ? skip.data() : nullptr, s->cfg.resolve ? &resolve : nullptr);
         if (st != CRX_OK) break;
-        if (s->cfg.resolve) {
-            st = resolve_views(lane.ctx, inst.data(), U, prefix.data(), targets.data(), V, CRX_DISPLAY_VERTEX_COLORS,
-                               lane.stream);
-            if (st != CRX_OK) break;
-        }
         hipError_t e = hipStreamSynchronize(lane.stream);
         if (e != hipSuccess) st = fail(CRX_HIP_ERROR, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
     } while (false);

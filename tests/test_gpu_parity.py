@@ -250,6 +250,29 @@ def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, bloc
     _check(ctx, spec, 1280, 720, view, proj, tuning=t)
 
 
+@pytest.mark.parametrize("order,per_lane,instanced", [
+    (0, 1, False), (1, 2, False), (2, 4, False), (2, 1, True), (0, 2, True), (1, 4, True)])
+def test_many_items_every_batch_order(ctx, order, per_lane, instanced):
+    """Thirty meshes of ragged sizes (from 7 triangles to several batches)
+    in one draw: every workgroup's batch sequence crosses item boundaries,
+    in the grid-stride order and in both contiguous orders, where the sweep
+    starts a new pass per item."""
+    rng = np.random.default_rng(71)
+    spec = SceneSpec()
+    for k in range(30):
+        rings, segs = int(rng.integers(2, 40)), int(rng.integers(3, 60))
+        pos, idx, col = wl.sphere_mesh(rings=rings, segments=segs, seed=100 + k)
+        if k % 7 == 3:
+            idx = idx[: 7 + k]                     # a few triangles only
+        cnt = 1 + k % 3
+        worlds = list(wl.instance_transforms(cnt, extent=6.0, smin=0.3, smax=0.9, seed=200 + k))
+        spec.add(orc.HostMesh(crx, pos, idx, col), worlds)
+    view = crx.look_at((3.0, 2.5, 12.0), (3.0, 3.0, 3.0))
+    t = crx.default_tuning()
+    t.stage1_order, t.stage1_tris_per_lane, t.stage1_block = order, per_lane, 128
+    _check(ctx, spec, 800, 600, view, wl.projection(800, 600), instanced=instanced, tuning=t)
+
+
 @pytest.mark.parametrize("per_lane", [0, 1, 2, 4])
 def test_many_instances_sub_pixel(ctx, per_lane):
     """A dense mesh drawn as many small instances (the 3000-instance scene
