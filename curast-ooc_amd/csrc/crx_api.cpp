@@ -349,7 +349,6 @@ RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* d
     P.items = reinterpret_cast<const DrawItem*>(dev + D.offItems);
     P.inst = reinterpret_cast<const InstanceRec*>(dev + D.offInst);
     P.views = reinterpret_cast<const ViewParams*>(dev + D.offViews);
-    P.view0 = views[0];
     P.numViews = D.numViews;
     set_counter_block(P, ctx->ctr);
     P.numItems = D.numItems;

@@ -98,7 +98,7 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
     P.items = reinterpret_cast<const DrawItem*>(dev + offItems);
     P.inst = reinterpret_cast<const InstanceRec*>(dev + offInst);
     P.views = reinterpret_cast<const ViewParams*>(dev + offView);
-    P.view0 = make_view_params(a->target, reinterpret_cast<const Transform*>(dev + offXf));
+    const ViewParams view = make_view_params(a->target, reinterpret_cast<const Transform*>(dev + offXf));
     P.numViews = 1;
     P.numItems = numItems;
     P.numBatches = 0;
@@ -117,7 +117,7 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
     if (numItems > 0 && a->numInstances > 0) {
         // the view record goes up through page-locked staging, the rest is built on the device
         CRX_TRY(ctx->argsStaging.acquire(sizeof(ViewParams)));
-        std::memcpy(ctx->argsStaging.ptr, &P.view0, sizeof(ViewParams));
+        std::memcpy(ctx->argsStaging.ptr, &view, sizeof(ViewParams));
         CRX_HIP(hipMemcpyAsync(dev + offView, ctx->argsStaging.ptr, sizeof(ViewParams), hipMemcpyHostToDevice, s));
         CRX_TRY(ctx->argsStaging.mark(s));
         CRX_HIP(launch_plan(instanced ? a->meshes : a->instances, numItems, a->instances, a->transforms,

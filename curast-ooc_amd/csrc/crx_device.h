@@ -127,7 +127,6 @@ struct RasterParams {
     uint32_t numWaves2;      // waves in the stage-2 grid
     uint32_t numWaves3;      // waves in the stage-3 grid
     uint32_t pad0;
-    ViewParams view0;        // copy of views[0] (kernel argument: lands in SGPRs)
 };
 
 struct ResolveParams {

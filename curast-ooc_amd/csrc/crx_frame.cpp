@@ -3,8 +3,8 @@
 //
 // The graph holds three host-to-device copies of the frame's metadata from
 // page-locked staging buffers, the clear of every target, the counter reset,
-// the three stages (always the multi-view stage 1, which reads each view's
-// record from memory, so new cameras only need new staging contents), a copy
+// the three stages (every stage reads the views' records from device memory,
+// so new cameras only need new staging contents), a copy
 // of the counters back to page-locked memory and the resolve.  Launching the
 // frame is one hipGraphLaunch; new cameras rewrite the staging buffers.  The
 // clear and the counter reset are kernels: the graph records no memset node.
@@ -122,7 +122,7 @@ crx_status frame_capture(crx_frame* F) {
     if (e == hipSuccess && F->numClear)
         e = launch_clear_views(static_cast<const ClearView*>(F->clearMeta.ptr), F->numClear, F->clearBlocks, nullptr, s);
     if (e == hipSuccess) e = launch_reset_counters(F->ctr, s);
-    if (e == hipSuccess && F->plan.batches) e = enqueue_stages(ctx, F->plan, F->P, true, s);
+    if (e == hipSuccess && F->plan.batches) e = enqueue_stages(ctx, F->plan, F->P, false, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(F->ctrHost, F->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && F->settings.resolve)

@@ -1059,7 +1059,10 @@ __device__ __forceinline__ void group_boxes(const f3& A, const f3& B, const f3& 
 // item.
 template <int BLOCK, int K, bool PU16, bool BOXES>
 __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
-    const auto transforms = kconst(P.view0.transforms);
+    // the view's record in device memory (a frame graph rewrites it between
+    // launches; the kernel arguments stay as captured)
+    const ViewParams V0 = kconst(P.views)[0];
+    const auto transforms = kconst(V0.transforms);
     uint32_t* words = nullptr;
     if constexpr (PU16) {
         __shared__ uint32_t pu16Words[BLOCK / kWave][kPu16Words];
@@ -1087,7 +1090,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
             const uint32_t lane = threadIdx.x & (kWave - 1);
             const uint32_t grp = lane / kBoxGroup;
             const uint32_t cullOrder = item_cull_order(P, it.instCount);
-            const auto xf = kglobal(P.view0.transforms) + it.instOffset;
+            const auto xf = kglobal(V0.transforms) + it.instOffset;
             #pragma unroll
             for (int k = 0; k < K; k++) {
                 const uint32_t tri = base + k * BLOCK + threadIdx.x;
@@ -1105,7 +1108,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                         float mm[16];
                         #pragma unroll
                         for (int e = 0; e < 16; e++) mm[e] = Tj->m[e];
-                        const bool keep = j < it.instCount && !(wbounded && wave_tiny_culled(mm, P.view0, wbox));
+                        const bool keep = j < it.instCount && !(wbounded && wave_tiny_culled(mm, V0, wbox));
                         const uint64_t kept = __ballot(keep);
                         const uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(kept >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)kept, 0u));
@@ -1128,7 +1131,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                         uint64_t q[NG];
                         #pragma unroll
                         for (int g = 0; g < NG; g++)
-                            q[g] = __ballot(lane < n && !(gbounded[g] && wave_tiny_culled(mq, P.view0, gbox[g])));
+                            q[g] = __ballot(lane < n && !(gbounded[g] && wave_tiny_culled(mq, V0, gbox[g])));
                         for (;;) {
                             uint64_t any = 0ull;
                             #pragma unroll
@@ -1155,7 +1158,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                             if (act && tri < it.numTriangles) {
                                 const uint32_t inst = it.instOffset + jj;
                                 const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(jj * it.numTriangles);
-                                stage1_triangle(P, P.view0, m, flip != 0u, A[k], B[k], C[k], id,
+                                stage1_triangle(P, V0, m, flip != 0u, A[k], B[k], C[k], id,
                                                 ((uint64_t)inst << 32) | (uint64_t)tri, cullOrder);
                             }
                         }
@@ -1177,7 +1180,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage1(RasterParams P) {
                 const uint32_t tri = base + k * BLOCK + threadIdx.x;
                 if (tri < it.numTriangles) {
                     const uint64_t id = it.cumm + (uint64_t)tri + (uint64_t)(j * it.numTriangles);
-                    stage1_triangle(P, P.view0, m, flip, A[k], B[k], C[k], id, ((uint64_t)inst << 32) | (uint64_t)tri,
+                    stage1_triangle(P, V0, m, flip, A[k], B[k], C[k], id, ((uint64_t)inst << 32) | (uint64_t)tri,
                                     item_cull_order(P, it.instCount));
                 }
             }

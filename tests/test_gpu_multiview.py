@@ -273,3 +273,43 @@ def test_captured_frame_replays(ctx):
                 assert int((got != crx.CLEAR_PIXEL).sum()) > 1000
     finally:
         frame.close()
+
+
+def test_captured_single_view_frame_replays(ctx):
+    """A one-view crx_frame runs the single-view stage 1 (here with a mesh
+    drawn over 40 instances: the per-wave instance lists), which reads its
+    view from device memory: replays with new cameras equal eager
+    crx_render of the same view."""
+    pos, idx, col = wl.sphere_mesh(rings=48, segments=96, seed=54)
+    pos2, idx2, col2 = wl.sphere_mesh(rings=12, segments=24, radius=1.5, seed=55)
+    meshes = [crx.Mesh(ctx, pos, idx, col), crx.Mesh(ctx, pos2, idx2, col2)]
+    scene = crx.Scene(ctx)
+    for w in wl.instance_transforms(40, extent=20.0, smin=0.3, smax=1.2, seed=56):
+        scene.add(meshes[0], w)
+    for w in wl.instance_transforms(2, extent=20.0, seed=57):
+        scene.add(meshes[1], w)
+    W, H = 320, 200
+
+    def view_at(t):
+        a = 0.6 * t
+        eye = (10.0 + 40.0 * math.cos(a), 12.0 + 3.0 * t, 10.0 + 40.0 * math.sin(a))
+        return crx.make_view(crx.look_at(eye, (10.0, 10.0, 10.0)), wl.projection(W, H), W, H)
+
+    settings = crx.render_settings()
+    fb, cb, efb, ecb = (ctx.alloc(8 * W * H) for _ in range(4))
+    frame = crx.Frame(scene, [view_at(0)], [fb], [cb], settings)
+    try:
+        for t in range(4):
+            if t:
+                frame.set_views([view_at(t)])
+            frame.launch()
+            st = frame.wait()
+            est = scene.render(view_at(t), efb, ecb, settings)
+            ctx.synchronize()
+            assert st.num_nontrivial == est.num_nontrivial and st.num_huge == est.num_huge
+            got, want = fb.download(np.uint64, W * H), efb.download(np.uint64, W * H)
+            assert np.array_equal(got, want), f"replay {t}\n" + describe_mismatch(got, want, W)
+            assert np.array_equal(cb.download(np.uint64, W * H), ecb.download(np.uint64, W * H))
+            assert int((got != crx.CLEAR_PIXEL).sum()) > 500
+    finally:
+        frame.close()

@@ -114,6 +114,42 @@ def test_routing_tuning(ctx, small, tile):
     _check(ctx, spec, W, H, view, proj, tuning=t)
 
 
+def test_scene_render_redraw_resolves_again(ctx):
+    """crx_render enqueues the resolve while the stages run; when the queues
+    overflow and the frame is drawn again, the resolve must run again on the
+    complete visibility buffer: a context with four-entry queues renders the
+    close-up terrain to the same visibility and colour buffers as a context
+    with default queues."""
+    pos, idx, col = wl.heightfield_mesh(129, spacing=1.0, amplitude=6.0, scale=32.0)
+    W, H = 1280, 720
+    view = crx.make_view(crx.look_at((64.0, 4.5, 20.0), (64.0, 2.0, 40.0), (0.0, 1.0, 0.0)),
+                         wl.projection(W, H), W, H)
+    settings = crx.render_settings(instanced=1, frustum_culling=1, resolve=1, attribute=crx.DISPLAY_VERTEX_COLORS)
+    out = []
+    for tiny in (False, True):
+        c = crx.Context(0)
+        try:
+            if tiny:
+                t = crx.default_tuning()
+                t.max_nontrivial = 4
+                t.max_huge = 4
+                c.tuning = t
+            scene = crx.Scene(c)
+            scene.add(crx.Mesh(c, pos, idx, col))
+            fb, cb = c.alloc(8 * W * H), c.alloc(8 * W * H)
+            st = scene.render(view, fb, cb, settings)
+            c.synchronize()
+            out.append((fb.download(np.uint64, W * H), cb.download(np.uint64, W * H), st))
+            scene.close()
+        finally:
+            c.close()
+    (fb0, cb0, st0), (fb1, cb1, st1) = out
+    assert st0.redraws == 0 and st1.redraws >= 1
+    assert st0.num_nontrivial > 8 and st0.num_huge > 8
+    assert np.array_equal(fb1, fb0), describe_mismatch(fb1, fb0, W)
+    assert np.array_equal(cb1, cb0), describe_mismatch(cb1, cb0, W)
+
+
 def test_tuning_values_are_checked(ctx):
     """crx_context_set_tuning takes 0 (per draw), 1, 2 and 4 triangles per
     lane and cull orders 0-3, and rejects other values."""
