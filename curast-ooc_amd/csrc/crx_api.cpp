@@ -143,13 +143,14 @@ crx_status check_target(const crx_render_target* t) {
 }
 
 crx_status check_mesh(const crx_cmesh& m, const char* what, uint32_t k) {
-    const std::string name = std::string(what) + "[" + std::to_string(k) + "]";
-    if (m.numTriangles > 1431655765u) return fail(CRX_INVALID_ARG, name + " has more than 1431655765 triangles");
-    if (m.numTriangles > 0 && !m.positions) return fail(CRX_INVALID_ARG, name + " has no positions");
+    // the message is built only on failure: draws check thousands of instances
+    const auto name = [&]() { return std::string(what) + "[" + std::to_string(k) + "]"; };
+    if (m.numTriangles > 1431655765u) return fail(CRX_INVALID_ARG, name() + " has more than 1431655765 triangles");
+    if (m.numTriangles > 0 && !m.positions) return fail(CRX_INVALID_ARG, name() + " has no positions");
     if (m.compressed) {
         // IX_PU16 is an index-buffer format (the _compressed kernels are _indexbuffer ones)
-        if (m.numTriangles > 0 && !m.indices) return fail(CRX_INVALID_ARG, name + " is compressed without indices");
-        if (m.bitsPerIndex > 32) return fail(CRX_INVALID_ARG, name + " has bitsPerIndex > 32");
+        if (m.numTriangles > 0 && !m.indices) return fail(CRX_INVALID_ARG, name() + " is compressed without indices");
+        if (m.bitsPerIndex > 32) return fail(CRX_INVALID_ARG, name() + " has bitsPerIndex > 32");
     }
     return CRX_OK;
 }
@@ -240,7 +241,17 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
     for (uint32_t v = 0; v < numViews; v++) CRX_TRY(check_target(targets + v));
     DrawPlan& D = *plan;
-    D = DrawPlan{};
+    {
+        // scalars back to their defaults; the record vectors keep their
+        // storage (a context plans every draw into the same DrawPlan, and
+        // fresh multi-hundred-KB vectors per draw cost page faults)
+        DrawPlan fresh;
+        fresh.items.swap(D.items);
+        fresh.inst.swap(D.inst);
+        fresh.xf.swap(D.xf);
+        fresh.targets.swap(D.targets);
+        D = std::move(fresh);
+    }
     D.numViews = numViews;
     D.numInstances = numInstances;
     D.targets.assign(targets, targets + numViews);
@@ -277,6 +288,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         if (off < 0 || cnt < 0 || off + cnt > (int64_t)numInstances)
             return fail(CRX_INVALID_ARG, "instances{offset,count} of item " + std::to_string(k) + " out of range");
         DrawItem& it = D.items[k];
+        it = DrawItem{};
         // a non-indexed item reads its positions in place of an index buffer
         // (a valid address of at least 36 bytes per triangle) and masks the
         // words off, so stage 1 issues the same loads for both kinds
@@ -399,7 +411,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
                       const uint8_t* skip, const std::function<crx_status()>* afterStages) {
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (skip && numViews < 2) return fail(CRX_INVALID_ARG, "per-view instance culling needs a multi-view draw");
-    DrawPlan D;
+    DrawPlan& D = ctx->plan;
     CRX_TRY(plan_draw(ctx, meshes, numMeshes, instances, numInstances, transforms, targets, numViews, instanced,
                       skip, &D));
     if (stats) stats->num_triangles = D.tris * numViews;
@@ -482,9 +494,13 @@ size_t resolve_image_layout(uint32_t numInstances, uint32_t numViews, size_t* of
     return *offViews + (size_t)numViews * sizeof(ResolveParams);
 }
 
-crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
-                         const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
-                         char* dev, std::vector<char>* image, size_t* offViews, uint32_t* blocks) {
+// Writes the resolve image of device base `dev` into `out` (host memory of
+// resolve_image_layout bytes): the records go straight to their places, with
+// no intermediate copies (a scene of thousands of instances makes this a
+// memory-bound loop on the host).
+crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                              const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                              char* dev, char* out, size_t* offViews, uint32_t* blocks) {
     if (numViews == 0 || numViews > kMaxViews)
         return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
     if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
@@ -496,17 +512,17 @@ crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, cons
         return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
     if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
     if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
-    std::vector<InstanceRec> inst(numInstances);
+    size_t offPrefix = 0;
+    resolve_image_layout(numInstances, numViews, &offPrefix, offViews);
+    InstanceRec* inst = reinterpret_cast<InstanceRec*>(out);
     for (uint32_t i = 0; i < numInstances; i++) {
         CRX_TRY(check_mesh(instances[i], "instances", i));
         inst[i] = make_instance(instances[i]);
     }
-    const size_t offInst = 0;
-    size_t offPrefix = 0;
-    const size_t bytes = resolve_image_layout(numInstances, numViews, &offPrefix, offViews);
-    const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(dev + offInst);
+    if (numInstances) std::memcpy(out + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
+    const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(dev);
     const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(dev + offPrefix);
-    std::vector<ResolveParams> rv(numViews);
+    ResolveParams* rv = reinterpret_cast<ResolveParams*>(out + *offViews);
     uint64_t nb = 0;
     for (uint32_t v = 0; v < numViews; v++) {
         rv[v] = make_resolve_params(targets[v], instDev, prefixDev, numInstances, attribute);
@@ -515,11 +531,16 @@ crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, cons
     }
     if (nb >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
     *blocks = (uint32_t)nb;
-    image->assign(bytes, 0);
-    std::memcpy(image->data() + offInst, inst.data(), inst.size() * sizeof(InstanceRec));
-    if (numInstances) std::memcpy(image->data() + offPrefix, prefix, (size_t)numInstances * sizeof(uint64_t));
-    std::memcpy(image->data() + *offViews, rv.data(), rv.size() * sizeof(ResolveParams));
     return CRX_OK;
+}
+
+crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                         const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                         char* dev, std::vector<char>* image, size_t* offViews, uint32_t* blocks) {
+    size_t offPrefix = 0;
+    image->resize(resolve_image_layout(numInstances, numViews, &offPrefix, offViews));
+    return resolve_image_into(instances, numInstances, prefix, targets, numViews, attribute, dev, image->data(),
+                              offViews, blocks);
 }
 
 // kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141) for
@@ -529,18 +550,19 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
                          hipStream_t stream) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
     CRX_HIP(hipSetDevice(ctx->device));
-    std::vector<char> image;
     size_t offPrefix = 0, offViews = 0;
     uint32_t blocks = 0;
-    CRX_TRY(ctx->resolveMeta.reserve(resolve_image_layout(numInstances, numViews, &offPrefix, &offViews)));
+    const size_t bytes = resolve_image_layout(numInstances, numViews, &offPrefix, &offViews);
+    CRX_TRY(ctx->resolveMeta.reserve(bytes));
     char* meta = static_cast<char*>(ctx->resolveMeta.ptr);
-    CRX_TRY(resolve_image(instances, numInstances, prefix, targets, numViews, attribute, meta, &image, &offViews,
-                          &blocks));
-    const ResolveParams* rv = reinterpret_cast<const ResolveParams*>(image.data() + offViews);
+    // the image is written straight into the page-locked staging buffer
+    CRX_TRY(ctx->resolveStaging.acquire(bytes));
+    char* image = static_cast<char*>(ctx->resolveStaging.ptr);
+    CRX_TRY(resolve_image_into(instances, numInstances, prefix, targets, numViews, attribute, meta, image, &offViews,
+                               &blocks));
+    const ResolveParams* rv = reinterpret_cast<const ResolveParams*>(image + offViews);
     if (numInstances || numViews > 1) {
-        CRX_TRY(ctx->resolveStaging.acquire(image.size()));
-        std::memcpy(ctx->resolveStaging.ptr, image.data(), image.size());
-        CRX_HIP(hipMemcpyAsync(meta, ctx->resolveStaging.ptr, image.size(), hipMemcpyHostToDevice, stream));
+        CRX_HIP(hipMemcpyAsync(meta, image, bytes, hipMemcpyHostToDevice, stream));
         CRX_TRY(ctx->resolveStaging.mark(stream));
     }
     record(ctx, T_RESOLVE, stream);
@@ -674,12 +696,14 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
     }
     // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
     // instances of one mesh keep node order.
-    std::stable_sort(kept.begin(), kept.end(), [&](uint32_t a, uint32_t b) {
+    const auto before = [&](uint32_t a, uint32_t b) {
         const crx_mesh* A = scene->nodes[a].mesh;
         const crx_mesh* B = scene->nodes[b].mesh;
         if (A->numTriangles == B->numTriangles) return A->seq < B->seq;
         return A->numTriangles > B->numTriangles;
-    });
+    };
+    // scenes are usually built mesh by mesh: then the order already holds
+    if (!std::is_sorted(kept.begin(), kept.end(), before)) std::stable_sort(kept.begin(), kept.end(), before);
     const uint32_t n = (uint32_t)kept.size();
     scene->unique.resize(n);
     scene->instances.resize(n);

@@ -110,9 +110,24 @@ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 enum TimerSlot { T_CLEAR = 0, T_S1, T_S2, T_S3, T_S3_END, T_RESOLVE, T_RESOLVE_END, T_COUNT };
 
+// Host side of one draw: records, metadata layout and grid sizes.
+struct DrawPlan {
+    std::vector<DrawItem> items;
+    std::vector<InstanceRec> inst;
+    std::vector<Transform> xf;                 // view-major
+    std::vector<crx_render_target> targets;
+    uint32_t numViews = 0, numInstances = 0, numItems = 0;
+    uint64_t batches = 0, tris = 0;
+    bool pu16 = false;
+    uint32_t block = 256, trisPerLane = 1, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
+    bool instanceBoxes = false;   // some item loops over >= 4 instances
+    size_t offItems = 0, offInst = 0, offViews = 0, offXf = 0, metaBytes = 0;
+};
+
 }  // namespace crx
 
 struct crx_context {
+    crx::DrawPlan plan;        // draw records reused across draws (vectors keep their capacity)
     int device = 0;
     int numCUs = 0;
     crx_tuning tuning{};
@@ -205,19 +220,6 @@ void frustum_planes(const crx_view& view, float out[16]);
 crx_box3 world_box(const crx_box3& b, const crx_mat4& world);
 bool box_in_frustum(const float planes[16], const crx_box3& worldBox);
 
-// Host side of one draw: records, metadata layout and grid sizes.
-struct DrawPlan {
-    std::vector<DrawItem> items;
-    std::vector<InstanceRec> inst;
-    std::vector<Transform> xf;                 // view-major
-    std::vector<crx_render_target> targets;
-    uint32_t numViews = 0, numInstances = 0, numItems = 0;
-    uint64_t batches = 0, tris = 0;
-    bool pu16 = false;
-    uint32_t block = 256, trisPerLane = 1, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
-    bool instanceBoxes = false;   // some item loops over >= 4 instances
-    size_t offItems = 0, offInst = 0, offViews = 0, offXf = 0, metaBytes = 0;
-};
 
 crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                      uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
