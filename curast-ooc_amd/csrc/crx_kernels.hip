@@ -1428,8 +1428,14 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
     S2Setup* setup = sh_setup[wid];
     const uint32_t count = min(*P.ctrNontrivial, P.ntCap);
     const int TILE = P.tile;
-    // first claim is static (wave w takes entries [64w, 64w+64)), later claims
-    // come from the shared cursor, so waves beyond the queue exit without an atomic
+    // Entries per claim: 64 (one per lane) once the queue fills the grid's
+    // waves; a shorter queue is spread over more waves in smaller claims, as
+    // each wave rasterises its claim's triangles one after the other.  The
+    // first claim is static (wave w takes entries [w*chunk, (w+1)*chunk)),
+    // later claims come from the shared cursor, so waves beyond the queue
+    // exit without an atomic.
+    const uint32_t chunk =
+        (uint32_t)min((uint64_t)kWave, max((uint64_t)1, ((uint64_t)count + P.numWaves2 - 1u) / P.numWaves2));
     const uint32_t waveId = blockIdx.x * 4u + wid;
     bool first = true;
 
@@ -1437,10 +1443,10 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
         wave_sync();
         uint32_t base = 0;
         if (first) {
-            base = waveId * (uint32_t)kWave;
+            base = waveId * chunk;
             first = false;
         } else {
-            if (lane == 0) base = P.numWaves2 * (uint32_t)kWave + atomicAdd(P.cursor2, (uint32_t)kWave);
+            if (lane == 0) base = P.numWaves2 * chunk + atomicAdd(P.cursor2, chunk);
             base = __shfl(base, 0);
         }
         if (base >= count) break;
@@ -1449,7 +1455,7 @@ __global__ __launch_bounds__(256) void k_stage2(RasterParams P) {
         uint32_t numTiles = 0;
         int tile_x = 0, tile_y = 0, tiles_x = 0;
         uint32_t meshIndex = 0, triangleIndex = 0;
-        if (idx < count) {
+        if (lane < chunk && idx < count) {
             const uint64_t packed = P.nontrivial[idx];
             meshIndex = (uint32_t)(packed >> 32);    // view << 24 | instance, kept for the tile entries
             triangleIndex = (uint32_t)packed;
