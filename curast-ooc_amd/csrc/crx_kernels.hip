@@ -632,9 +632,13 @@ __device__ __forceinline__ f3 pu16_vertex_issue(const S1Item& it, uint32_t i) {
     const auto base = reinterpret_cast<const char*>(it.positions);
     const uint64_t bytes = 6ull * it.numVertices;
     if (it.numVertices >= 3u && bytes < 0xFFFFFFFFull) {
+        // an odd vertex's window ends at 6i + 6 <= 6V; an even one's at 6i + 8,
+        // inside the buffer unless i is the last vertex.  (An index at or past
+        // V, which no valid stream holds, reads some in-range window or zeros:
+        // the buffer load never leaves the buffer.)
         const uint32_t odd = i & 1u;
-        const bool fast = (i + 2u <= it.numVertices && i + 2u > i) || (i + 1u == it.numVertices && odd);
-        const uint32_t at = fast ? 6u * i - 2u * odd : 6u * i - 4u;
+        const bool fast = odd != 0u || i + 2u <= it.numVertices;
+        const uint32_t at = 6u * i - (fast ? 2u * odd : 4u);
         const uint2 w = buffer_load_u2(buffer_rsrc(base, (uint32_t)bytes), at);
         lo = w.x;
         hi = w.y;
@@ -659,12 +663,12 @@ __device__ __forceinline__ f3 pu16_vertex_decode(const S1Item& it, const f3& raw
         hi = h[2];
         flag = 0u;
     }
-    // even window: lo = x | y << 16, hi = z | ..; odd window: lo = .. | x << 16, hi = y | z << 16
-    const uint32_t odd = flag;
-    const uint32_t mid = __builtin_amdgcn_alignbit(hi, lo, 16u);
-    const uint32_t x = (odd ? mid : lo) & 0xFFFFu;
-    const uint32_t y = (mid >> (odd << 4)) & 0xFFFFu;
-    const uint32_t z = (odd ? hi : mid) >> 16;
+    // the three uint16 sit at bits 16*odd + {0, 16, 32} of the window hi:lo
+    const uint32_t sh = flag << 4;
+    const uint32_t m = __builtin_amdgcn_alignbit(hi, lo, sh);
+    const uint32_t x = m & 0xFFFFu;
+    const uint32_t y = m >> 16;
+    const uint32_t z = __builtin_amdgcn_ubfe(hi, sh, 16u);
     return f3{(float)x * it.cfx + it.mnx, (float)y * it.cfy + it.mny, (float)z * it.cfz + it.mnz};
 }
 
