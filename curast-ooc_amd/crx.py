@@ -252,6 +252,8 @@ _SIGS = {
     "crx_copy_buffer": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_probe_atomic_min": (C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), _VP]),
     "crx_probe_read": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _VP]),
+    "crx_check_division": (C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint64)]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
     "crx_raster_stage1": (C.c_int, [_VP, C.POINTER(RasterArgs), C.c_uint32, _VP]),
     "crx_raster_stage2": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
@@ -483,6 +485,14 @@ class Context:
 
     def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int, stream=None) -> None:
         _check(lib().crx_copy_buffer(self.handle, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, stream))
+
+    def check_division(self, pairs: int) -> tuple[int, int, int]:
+        """Mismatches of stage 1's division sequences against the IEEE
+        division: (reciprocal over every float in range, quotient over
+        `pairs` operand pairs, screen term over `pairs` small numerators)."""
+        r, d, t = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().crx_check_division(self.handle, pairs, C.byref(r), C.byref(d), C.byref(t)))
+        return r.value, d.value, t.value
 
     def probe_read(self, src: DeviceBuffer, nbytes: int, sink: DeviceBuffer, stream=None) -> None:
         """Enqueue the streaming-read bandwidth probe over nbytes of src."""

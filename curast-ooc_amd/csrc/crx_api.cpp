@@ -1011,6 +1011,25 @@ crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t 
     return CRX_OK;
 }
 
+crx_status crx_check_division(crx_context* ctx, uint64_t pairs, uint64_t* rcpMismatches, uint64_t* divMismatches,
+                              uint64_t* smallNumeratorMismatches) {
+    if (!ctx || !rcpMismatches || !divMismatches || !smallNumeratorMismatches)
+        return fail(CRX_INVALID_ARG, "NULL argument");
+    CRX_HIP(hipSetDevice(ctx->device));
+    unsigned long long* d = nullptr;
+    CRX_HIP(hipMalloc(&d, 3 * sizeof(unsigned long long)));
+    unsigned long long h[3] = {0ull, 0ull, 0ull};
+    hipError_t e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_check_division(pairs, ctx->numCUs, d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CRX_HIP_ERROR, std::string("crx_check_division: ") + hipGetErrorString(e));
+    *rcpMismatches = h[0];
+    *divMismatches = h[1];
+    *smallNumeratorMismatches = h[2];
+    return CRX_OK;
+}
+
 crx_status crx_probe_read(crx_context* ctx, const void* src, size_t bytes, uint32_t* sink, void* stream) {
     if (!ctx || !src || !sink) return fail(CRX_INVALID_ARG, "NULL argument");
     if ((bytes % 16) != 0 || ((uintptr_t)src % 16) != 0)

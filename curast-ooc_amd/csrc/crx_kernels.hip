@@ -19,6 +19,8 @@
 #include "crx_device.h"
 #include "crx_internal.h"
 
+#include <type_traits>
+
 namespace crx {
 
 namespace {
@@ -44,6 +46,51 @@ __device__ __forceinline__ const T* kconst(const T* p) { return p; }
 template <typename T>
 __device__ __forceinline__ const T* kglobal(const T* p) { return p; }
 #endif
+
+// Short reciprocal and quotient sequences with the IEEE (round-to-nearest-
+// even) results on the operand ranges where stage 1 uses them.
+//
+// rcp_refined(b): v_rcp_f32 (error below 1 ulp) refined by one Newton step
+// with fused multiply-adds, e = 1 - b*r0 (exact), r1 = r0 + e*r0.  For every
+// float b with kRcpLo <= |b| < kRcpHi it equals RN(1/b): crx_check_division
+// compares it with the IEEE division for all 2 * 250 * 2^23 such floats.
+// div_fast(a, b, y) with y = rcp_refined(b) = RN(1/b): q0 = RN(a*y) is within
+// one ulp of a/b, rem = a - b*q0 is exact (one FMA) and q1 = RN(q0 + rem*y)
+// is RN(a/b) (Markstein's theorem: P. Markstein, IBM J. Res. Dev. 34(1),
+// 1990; Muller et al., Handbook of Floating-Point Arithmetic, on FMA-based
+// division) when kDivLo <= |a|, |b| <= kDivHi, where nothing underflows or
+// overflows; crx_check_division also compares it with the IEEE division on
+// random operand pairs of that range.  For |a| below kDivLo (zero and
+// subnormals included) both quotients are below 2^-26 in magnitude, so
+// q * 0.5 + 0.5 -- the only use in stage 1 -- is 0.5 for both (also
+// checked).
+constexpr float kRcpLo = 0x1p-125f, kRcpHi = 0x1p125f;
+constexpr float kDivLo = 0x1p-60f, kDivHi = 0x1p64f;
+
+__device__ __forceinline__ float rcp_refined(float b) {
+    const float r0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r0, 1.0f);
+    return __builtin_fmaf(e, r0, r0);
+}
+
+__device__ __forceinline__ float div_fast(float a, float b, float y) {
+    const float q0 = a * y;
+    const float rem = __builtin_fmaf(-b, q0, a);
+    return __builtin_fmaf(rem, y, q0);
+}
+
+// RN(1/b) for any b: the short sequence, or the IEEE division behind a
+// wave-uniform branch for lanes out of range (so it is not evaluated
+// speculatively for every lane).
+__device__ __forceinline__ float rcp_rn(float b) {
+    float r = rcp_refined(b);
+    const float m = fabsf(b);
+    const bool ok = m >= kRcpLo && m < kRcpHi;
+    if (__ballot(!ok) != 0ull) {
+        if (!ok) r = 1.0f / b;
+    }
+    return r;
+}
 
 // Orders LDS traffic between the lanes of one wave (no hardware barrier: the
 // waves of a workgroup run independent loops here).
@@ -204,13 +251,39 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     }
 
     // viewToNDC (rasterization_helpers.cuh:95-101) and ndcToScreen - 0.5 (:103-108, :132-134)
+    // The quotients here, the depth reciprocals and every per-pixel depth
+    // reciprocal below take the short sequences (rcp_refined, div_fast) when
+    // every active lane of the wave has its depths in [kDivLo, kDivHi] and
+    // its numerators within kDivHi: then all of them are the IEEE values (a
+    // per-pixel inv_depth is a convex combination of the depth reciprocals,
+    // inside [2^-66, 2^62]).  Other waves take the IEEE division throughout.
     const float Wf = V.Wf, Hf = V.Hf;
-    const float ax = ((pxa / da) * 0.5f + 0.5f) * Wf - 0.5f;
-    const float bx = ((pxb / db) * 0.5f + 0.5f) * Wf - 0.5f;
-    const float cx = ((pxc / dc) * 0.5f + 0.5f) * Wf - 0.5f;
-    const float ay = ((pya / da) * 0.5f + 0.5f) * Hf - 0.5f;
-    const float by = ((pyb / db) * 0.5f + 0.5f) * Hf - 0.5f;
-    const float cy = ((pyc / dc) * 0.5f + 0.5f) * Hf - 0.5f;
+    const float dmin = fminf(da, fminf(db, dc)), dmax = fmaxf(da, fmaxf(db, dc));
+    const float pmax = fmaxf(fmaxf(fmaxf(fabsf(pxa), fabsf(pxb)), fmaxf(fabsf(pxc), fabsf(pya))),
+                             fmaxf(fabsf(pyb), fabsf(pyc)));
+    const bool fast = __ballot(!(dmin >= kDivLo && dmax <= kDivHi && pmax <= kDivHi)) == 0ull;
+    float ax, bx, cx, ay, by, cy, iza, izb, izc;
+    if (fast) {
+        iza = rcp_refined(da);
+        izb = rcp_refined(db);
+        izc = rcp_refined(dc);
+        ax = (div_fast(pxa, da, iza) * 0.5f + 0.5f) * Wf - 0.5f;
+        bx = (div_fast(pxb, db, izb) * 0.5f + 0.5f) * Wf - 0.5f;
+        cx = (div_fast(pxc, dc, izc) * 0.5f + 0.5f) * Wf - 0.5f;
+        ay = (div_fast(pya, da, iza) * 0.5f + 0.5f) * Hf - 0.5f;
+        by = (div_fast(pyb, db, izb) * 0.5f + 0.5f) * Hf - 0.5f;
+        cy = (div_fast(pyc, dc, izc) * 0.5f + 0.5f) * Hf - 0.5f;
+    } else {
+        ax = ((pxa / da) * 0.5f + 0.5f) * Wf - 0.5f;
+        bx = ((pxb / db) * 0.5f + 0.5f) * Wf - 0.5f;
+        cx = ((pxc / dc) * 0.5f + 0.5f) * Wf - 0.5f;
+        ay = ((pya / da) * 0.5f + 0.5f) * Hf - 0.5f;
+        by = ((pyb / db) * 0.5f + 0.5f) * Hf - 0.5f;
+        cy = ((pyc / dc) * 0.5f + 0.5f) * Hf - 0.5f;
+        iza = 1.0f / da;
+        izb = 1.0f / db;
+        izc = 1.0f / dc;
+    }
 
     float min_x = fminf(ax, fminf(bx, cx));
     float max_x = fmaxf(ax, fmaxf(bx, cx));
@@ -243,8 +316,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     // per-lane incremental rasterisation (:200-269)
     const float vabx = bx - ax, vaby = by - ay;
     const float vacx = cx - ax, vacy = cy - ay;
-    const float factor = 1.0f / (vabx * vacy - vaby * vacx);
-    const float iza = 1.0f / da, izb = 1.0f / db, izc = 1.0f / dc;
+    const float factor = rcp_rn(vabx * vacy - vaby * vacx);
     const float ds_dx = vacy * factor;
     const float ds_dy = -vacx * factor;
     const float dt_dx = -vaby * factor;
@@ -257,24 +329,31 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     float t_row = (vabx * s0y - vaby * s0x) * factor;
     uint32_t rowPix = (uint32_t)f2i(start_x) + (uint32_t)V.W * (uint32_t)f2i(start_y);
     const uint32_t numPixels = (uint32_t)V.W * (uint32_t)V.H;
-    for (int fy = 0; fy < size_y; ++fy) {
-        float s = s_row, t = t_row;
-        uint32_t pix = rowPix;
-        for (int fx = 0; fx < size_x; ++fx) {
-            const float v = 1.0f - (s + t);
-            if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
-                const float inv_depth = (v * iza + s * izb) + t * izc;
-                const float depth = 1.0f / inv_depth;
-                atomic_min_fb(V.fb, pix, pack_pixel(depth, id));
+    // the loop is compiled twice, so that neither copy carries a branch per pixel
+    const auto raster = [&](auto shortDivision) __attribute__((always_inline)) {
+        for (int fy = 0; fy < size_y; ++fy) {
+            float s = s_row, t = t_row;
+            uint32_t pix = rowPix;
+            for (int fx = 0; fx < size_x; ++fx) {
+                const float v = 1.0f - (s + t);
+                if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
+                    const float inv_depth = (v * iza + s * izb) + t * izc;
+                    float depth;
+                    if constexpr (decltype(shortDivision)::value) depth = rcp_refined(inv_depth);
+                    else depth = 1.0f / inv_depth;
+                    atomic_min_fb(V.fb, pix, pack_pixel(depth, id));
+                }
+                s += ds_dx;
+                t += dt_dx;
+                pix++;
             }
-            s += ds_dx;
-            t += dt_dx;
-            pix++;
+            s_row += ds_dy;
+            t_row += dt_dy;
+            rowPix += (uint32_t)V.W;
         }
-        s_row += ds_dy;
-        t_row += dt_dy;
-        rowPix += (uint32_t)V.W;
-    }
+    };
+    if (fast) raster(std::true_type{});
+    else raster(std::false_type{});
 }
 
 }  // namespace
@@ -1900,6 +1979,52 @@ __global__ __launch_bounds__(256) void k_copy(u32x4* __restrict__ dst, const u32
     for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// Check of rcp_refined / div_fast against the IEEE division: lanes sweep all
+// floats b with kRcpLo <= |b| < kRcpHi (biased exponents 2..251, both signs)
+// and count b with rcp_refined(b) != 1/b (out[0]); then `pairs` hashed
+// operand pairs (a, b) with exponents in [-60, 63] and any mantissa and sign
+// count div_fast(a, b, rcp_refined(b)) != a/b (out[1]), and as many pairs
+// with |a| below kDivLo (zero and subnormals included) count
+// q * 0.5 + 0.5 differing between the two quotients (out[2]).
+__global__ __launch_bounds__(256) void k_check_division(uint64_t pairs, unsigned long long* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr uint64_t kPerSign = 250ull << 23;          // exponents 2..251
+    uint32_t bad = 0u;
+    for (uint64_t k = tid; k < 2ull * kPerSign; k += stride) {
+        const uint64_t j = k % kPerSign;
+        const uint32_t bits = (uint32_t)((k / kPerSign) << 31) | (uint32_t)((j + (2ull << 23)));
+        const float b = __builtin_bit_cast(float, bits);
+        const float r = rcp_refined(b);
+        const float q = 1.0f / b;
+        bad += __builtin_bit_cast(uint32_t, r) != __builtin_bit_cast(uint32_t, q) ? 1u : 0u;
+    }
+    if (bad) atomicAdd(out, (unsigned long long)bad);
+    bad = 0u;
+    uint32_t bad2 = 0u;
+    for (uint64_t k = tid; k < pairs; k += stride) {
+        uint64_t h = (k + 1ull) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 29; h *= 0x94D049BB133111EBull; h ^= h >> 32;
+        const uint32_t ha = (uint32_t)h, hb = (uint32_t)(h >> 32);
+        // exponent in [-60, 63] from 7 bits, mantissa and sign from the rest
+        const uint32_t ea = (uint32_t)(127 - 60) + ((ha >> 24) & 127u) % 124u;
+        const uint32_t eb = (uint32_t)(127 - 60) + ((hb >> 24) & 127u) % 124u;
+        const float a = __builtin_bit_cast(float, (ha & 0x807FFFFFu) | (ea << 23));
+        const float b = __builtin_bit_cast(float, (hb & 0x807FFFFFu) | (eb << 23));
+        const float q = div_fast(a, b, rcp_refined(b));
+        const float w = a / b;
+        bad += __builtin_bit_cast(uint32_t, q) != __builtin_bit_cast(uint32_t, w) ? 1u : 0u;
+        // a tiny numerator: biased exponent 0..66 (|a| < 2^-60)
+        const uint32_t et = ((ha >> 24) & 127u) % 67u;
+        const float at = __builtin_bit_cast(float, (ha & 0x807FFFFFu) | (et << 23));
+        const float qt = div_fast(at, b, rcp_refined(b)) * 0.5f + 0.5f;
+        const float wt = (at / b) * 0.5f + 0.5f;
+        bad2 += __builtin_bit_cast(uint32_t, qt) != __builtin_bit_cast(uint32_t, wt) ? 1u : 0u;
+    }
+    if (bad) atomicAdd(out + 1, (unsigned long long)bad);
+    if (bad2) atomicAdd(out + 2, (unsigned long long)bad2);
+}
+
 // Plain streaming read, 16 bytes per lane and four nontemporal loads in
 // flight per lane: the measured read bandwidth of HBM.  The lanes' XOR of
 // the data is stored only when it equals a fixed value, so the loads stay.
@@ -2068,6 +2193,11 @@ hipError_t launch_read(const void* src, size_t bytes, int numCUs, uint32_t* sink
     uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)numCUs * 32);
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_read, dim3(blocks), dim3(256), 0, stream, (const u32x4*)src, n, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_division(uint64_t pairs, int numCUs, unsigned long long* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_check_division, dim3((uint32_t)numCUs * 32u), dim3(256), 0, stream, pairs, out);
     return hipGetLastError();
 }
 

@@ -375,6 +375,16 @@ crx_status crx_stream_synchronize(crx_context* ctx, void* stream);
  * the reference: it is the known-size kernel that calibrates HBM traffic
  * counters (FETCH_SIZE / WRITE_SIZE) for the roofline measurement. */
 crx_status crx_copy_buffer(crx_context* ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* Check of stage 1's short reciprocal and quotient sequences against the
+ * IEEE division on the device (no counterpart in the reference: stage 1
+ * divides with v_rcp_f32 refined by FMAs, and this shows the results are the
+ * IEEE ones).  Every float b with 2^-125 <= |b| < 2^125 is checked for the
+ * reciprocal; `pairs` hashed operand pairs with exponents in [-60, 63] for
+ * the quotient, and as many with a numerator below 2^-60 for the screen
+ * coordinate term q * 0.5 + 0.5.  The three mismatch counts are returned.
+ * Blocking. */
+crx_status crx_check_division(crx_context* ctx, uint64_t pairs, uint64_t* rcpMismatches, uint64_t* divMismatches,
+                              uint64_t* smallNumeratorMismatches);
 /* Streaming read of `bytes` (16-byte aligned) of device memory with
  * nontemporal loads (no counterpart in the reference): the measured read
  * bandwidth against which stage 1's traffic is read.  sink is a device word

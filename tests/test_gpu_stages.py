@@ -183,3 +183,13 @@ def test_read_probe(ctx):
     assert np.array_equal(src.download(np.uint32, n // 4), np.arange(n // 4, dtype=np.uint32))
     with pytest.raises(crx.CrxError):
         ctx.probe_read(src, 20, sink)
+
+
+def test_division_sequences_are_ieee(ctx):
+    """Stage 1 computes reciprocals as v_rcp_f32 plus one FMA Newton step and
+    quotients with Markstein's correction; both must equal the IEEE division
+    bit for bit: every float in the reciprocal's range (4.2e9 values), 2^30
+    hashed quotient operand pairs, and 2^30 pairs with a numerator below
+    2^-60 through the screen term q * 0.5 + 0.5."""
+    rcp_bad, div_bad, small_bad = ctx.check_division(1 << 30)
+    assert rcp_bad == 0 and div_bad == 0 and small_bad == 0, (rcp_bad, div_bad, small_bad)
