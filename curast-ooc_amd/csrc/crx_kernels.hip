@@ -19,8 +19,6 @@
 #include "crx_device.h"
 #include "crx_internal.h"
 
-#include <type_traits>
-
 namespace crx {
 
 namespace {
@@ -329,31 +327,26 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     float t_row = (vabx * s0y - vaby * s0x) * factor;
     uint32_t rowPix = (uint32_t)f2i(start_x) + (uint32_t)V.W * (uint32_t)f2i(start_y);
     const uint32_t numPixels = (uint32_t)V.W * (uint32_t)V.H;
-    // the loop is compiled twice, so that neither copy carries a branch per pixel
-    const auto raster = [&](auto shortDivision) __attribute__((always_inline)) {
-        for (int fy = 0; fy < size_y; ++fy) {
-            float s = s_row, t = t_row;
-            uint32_t pix = rowPix;
-            for (int fx = 0; fx < size_x; ++fx) {
-                const float v = 1.0f - (s + t);
-                if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
-                    const float inv_depth = (v * iza + s * izb) + t * izc;
-                    float depth;
-                    if constexpr (decltype(shortDivision)::value) depth = rcp_refined(inv_depth);
-                    else depth = 1.0f / inv_depth;
-                    atomic_min_fb(V.fb, pix, pack_pixel(depth, id));
-                }
-                s += ds_dx;
-                t += dt_dx;
-                pix++;
+    for (int fy = 0; fy < size_y; ++fy) {
+        float s = s_row, t = t_row;
+        uint32_t pix = rowPix;
+        for (int fx = 0; fx < size_x; ++fx) {
+            const float v = 1.0f - (s + t);
+            if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
+                const float inv_depth = (v * iza + s * izb) + t * izc;
+                float depth;
+                if (fast) depth = rcp_refined(inv_depth);   // wave-uniform branch
+                else depth = 1.0f / inv_depth;
+                atomic_min_fb(V.fb, pix, pack_pixel(depth, id));
             }
-            s_row += ds_dy;
-            t_row += dt_dy;
-            rowPix += (uint32_t)V.W;
+            s += ds_dx;
+            t += dt_dx;
+            pix++;
         }
-    };
-    if (fast) raster(std::true_type{});
-    else raster(std::false_type{});
+        s_row += ds_dy;
+        t_row += dt_dy;
+        rowPix += (uint32_t)V.W;
+    }
 }
 
 }  // namespace
