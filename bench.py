@@ -335,7 +335,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
-              "(the visibility-buffer path here is single-GPU; see DESIGN.md for how it would shard)",
+              "(the visibility-buffer path here runs on one GPU; several GPUs would split triangles or views "
+              "and min-reduce the visibility buffers)",
               file=sys.stderr)
         return 2
 

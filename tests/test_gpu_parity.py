@@ -52,7 +52,7 @@ def _check(ctx, spec, W, H, view, proj, instanced=True, attribute=crx.DISPLAY_VE
 
 
 def test_sphere_1m_1080p(ctx):
-    """Config 1: 1,048,576-triangle displaced sphere, one 1920x1080 view, resolve to vertex colours."""
+    """The 1,048,576-triangle displaced sphere, one 1920x1080 view, resolve to vertex colours."""
     pos, idx, col = wl.sphere_mesh()
     spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
     view, proj = wl.sphere_view(1920, 1080)
@@ -188,7 +188,7 @@ def test_queue_overflow_regrows(ctx):
 
 
 def test_heightfield_oblique(ctx):
-    """Scaled-down config 2: 2049^2-vertex terrain (8.4M triangles), oblique view."""
+    """The oblique heightfield scene scaled down: 2049^2-vertex terrain (8.4M triangles)."""
     n = 2049
     pos, idx, col = wl.heightfield_mesh(n)
     spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
