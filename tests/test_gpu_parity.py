@@ -286,6 +286,34 @@ def test_schedule_variants_bit_identical(ctx, cull_order, fast_tiny, order, bloc
     _check(ctx, spec, 1280, 720, view, proj, tuning=t)
 
 
+@pytest.mark.parametrize("scale", [1e19, 3e20])
+def test_extreme_magnitudes_take_ieee_division(ctx, scale):
+    """Stage 1 takes its short division sequences only for waves whose depths
+    lie in [2^-60, 2^64] and whose numerators stay within 2^64.  Instances
+    scaled and placed at ~1e19-1e20 units (numerators past 2^64 in some or
+    all lanes), one hugging the camera plane at depth ~1e-19 (below 2^-60)
+    and ordinary ones in the same draw must still match the oracle bit for
+    bit."""
+    pos, idx, col = wl.sphere_mesh(rings=24, segments=48, seed=81)
+    view = crx.look_at((0.0, 0.0, 10.0), (0.0, 0.0, 0.0))
+    vi = np.array(crx.mat4_inverse(view).m, dtype=np.float32).reshape(4, 4).T
+
+    def at(x, y, z, s):          # an instance at view-space (x, y, z), scale s
+        w = np.eye(4, dtype=np.float32) * np.float32(s)
+        w[3, 3] = 1.0
+        c = vi @ np.array([x, y, z, 1.0], dtype=np.float32)
+        w[:3, 3] = c[:3]
+        return crx.Mat4.from_numpy(w)
+
+    worlds = [at(0.0, 0.0, -5.0, 1.0), at(2.0, 1.0, -8.0, 0.5),                 # ordinary
+              at(0.2 * scale, 0.1 * scale, -scale, 0.3 * scale),                  # huge and far
+              at(0.0, 0.0, -1.2 * scale, 0.05 * scale),
+              at(0.0, 0.0, -2e-19, 1e-19),                                         # at the camera plane
+              at(3e-19, 0.0, -5e-19, 2e-19)]
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), worlds)
+    _check(ctx, spec, 640, 400, view, wl.projection(640, 400))
+
+
 @pytest.mark.parametrize("order,per_lane,instanced", [
     (0, 1, False), (1, 2, False), (2, 4, False), (2, 1, True), (0, 2, True), (1, 4, True)])
 def test_many_items_every_batch_order(ctx, order, per_lane, instanced):
