@@ -684,7 +684,7 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
         const crx_scene::Node& nd = scene->nodes[i];
         bool any = true;
         if (settings.frustum_culling) {
-            const crx_box3 wb = world_box(nd.mesh->aabb, nd.world);
+            const crx_box3& wb = nd.wbox;
             any = false;
             for (uint32_t v = 0; v < numViews; v++) {
                 const bool in = box_in_frustum(&planes[(size_t)v * 16], wb);
@@ -1374,7 +1374,7 @@ crx_status crx_scene_destroy(crx_scene* scene) {
 crx_status crx_scene_add_node(crx_scene* scene, crx_mesh* mesh, const crx_mat4* world, uint32_t* node_id) {
     if (!scene || !mesh || !world) return fail(CRX_INVALID_ARG, "NULL argument");
     if (mesh->ctx != scene->ctx) return fail(CRX_INVALID_ARG, "mesh belongs to another context");
-    scene->nodes.push_back(crx_scene::Node{mesh, *world});
+    scene->nodes.push_back(crx_scene::Node{mesh, *world, world_box(mesh->aabb, *world)});
     if (node_id) *node_id = (uint32_t)(scene->nodes.size() - 1);
     return CRX_OK;
 }
@@ -1383,6 +1383,7 @@ crx_status crx_scene_set_transform(crx_scene* scene, uint32_t node_id, const crx
     if (!scene || !world) return fail(CRX_INVALID_ARG, "NULL argument");
     if (node_id >= scene->nodes.size()) return fail(CRX_INVALID_ARG, "node_id out of range");
     scene->nodes[node_id].world = *world;
+    scene->nodes[node_id].wbox = world_box(scene->nodes[node_id].mesh->aabb, *world);
     return CRX_OK;
 }
 

@@ -180,6 +180,7 @@ struct crx_scene {
     struct Node {
         crx_mesh* mesh;
         crx_mat4 world;
+        crx_box3 wbox;   // world-space box of the mesh's box (kept in step with world)
     };
     std::vector<Node> nodes;
     // frame-preparation scratch, kept across frames like the reference's statics
