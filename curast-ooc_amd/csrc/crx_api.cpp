@@ -863,7 +863,7 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage3_waves_per_cu = 16;
     out->stage1_cull_order = 3;
     out->stage1_order = 1;
-    out->stage1_fast_tiny = 1;
+    out->stage1_fast_tiny = 0;          // the exact path divides cheaply (short sequences)
     out->stage1_tris_per_lane = 0;
 }
 

@@ -189,7 +189,8 @@ typedef struct crx_tuning {
     uint32_t stage1_order;        /* batch-to-workgroup order: 0 = contiguous share per XCD,
                                      1 = grid-stride, 2 = contiguous share per workgroup */
     uint32_t stage1_fast_tiny;    /* 1 = decide sure tiny-triangle culls from v_rcp_f32 screen
-                                     coordinates with a proven error margin (exact either way) */
+                                     coordinates with a proven error margin before the exact
+                                     path (exact either way; default 0) */
     uint32_t stage1_tris_per_lane; /* triangles each stage-1 lane takes per batch: 1, 2 or 4 (a batch
                                      is stage1_block * this consecutive triangles of one item);
                                      0 = per draw: a draw with a mesh drawn over >= 4 instances
