@@ -406,3 +406,42 @@ def test_random_index_order(ctx, per_lane):
     t = crx.default_tuning()
     t.stage1_tris_per_lane = per_lane
     _check(ctx, spec, 960, 540, view, proj, tuning=t)
+
+
+@pytest.mark.parametrize("seed", [101, 202, 303, 404, 505, 606])
+def test_random_scenes(ctx, seed):
+    """Seeded random scenes: two to five meshes (spheres and terrain patches
+    of random sizes, some non-indexed), one to forty instances each with
+    random scale (some mirrored), a random camera that may sit among the
+    instances, and a random schedule; every bit must be the oracle's."""
+    rng = np.random.default_rng(seed)
+    spec = SceneSpec()
+    for m in range(int(rng.integers(2, 6))):
+        if rng.random() < 0.6:
+            pos, idx, col = wl.sphere_mesh(rings=int(rng.integers(4, 64)), segments=int(rng.integers(6, 96)),
+                                           radius=float(rng.uniform(0.2, 2.0)), seed=seed * 10 + m)
+        else:
+            n = int(rng.integers(5, 80))
+            pos, idx, col = wl.heightfield_mesh(n, spacing=float(rng.uniform(0.05, 0.5)),
+                                                amplitude=float(rng.uniform(0.1, 2.0)), scale=16.0,
+                                                seed=seed * 10 + m)
+        if rng.random() < 0.25:                                   # non-indexed copy
+            flat = idx.reshape(-1)
+            pos, idx, col = pos[flat], None, col[flat]
+        cnt = int(rng.integers(1, 41))
+        worlds = list(wl.instance_transforms(cnt, extent=12.0, smin=0.2, smax=1.5, seed=seed * 100 + m))
+        for k in range(0, cnt, 7):                                # mirror some
+            for r in range(3):
+                worlds[k].m[r] = -worlds[k].m[r]
+        spec.add(orc.HostMesh(crx, pos, idx, col), worlds)
+    eye = tuple(float(v) for v in rng.uniform(-4.0, 16.0, 3))
+    target = tuple(float(v) for v in rng.uniform(2.0, 10.0, 3))
+    W, H = int(rng.choice([320, 640, 777])), int(rng.choice([200, 400, 333]))
+    view = crx.look_at(eye, target)
+    t = crx.default_tuning()
+    t.stage1_tris_per_lane = int(rng.choice([0, 1, 2, 4]))
+    t.stage1_order = int(rng.integers(0, 3))
+    t.stage1_cull_order = int(rng.integers(0, 4))
+    t.stage1_fast_tiny = int(rng.integers(0, 2))
+    t.stage1_block = int(rng.choice([128, 256, 512]))
+    _check(ctx, spec, W, H, view, wl.projection(W, H), instanced=bool(rng.random() < 0.7), tuning=t)
