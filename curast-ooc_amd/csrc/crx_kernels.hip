@@ -326,18 +326,24 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     float s_row = (s0x * vacy - s0y * vacx) * factor;
     float t_row = (vabx * s0y - vaby * s0x) * factor;
     uint32_t rowPix = (uint32_t)f2i(start_x) + (uint32_t)V.W * (uint32_t)f2i(start_y);
-    const uint32_t numPixels = (uint32_t)V.W * (uint32_t)V.H;
+    // The reference also tests pixelID < W*H; every sample here is inside
+    // the screen (the box is clamped to [0, W-1] x [0, H-1] before ceil, and
+    // a NaN bound clamps to an edge), so that test is always true.
     for (int fy = 0; fy < size_y; ++fy) {
         float s = s_row, t = t_row;
         uint32_t pix = rowPix;
         for (int fx = 0; fx < size_x; ++fx) {
             const float v = 1.0f - (s + t);
-            if (s > 0.0f && t > 0.0f && v > 0.0f && pix < numPixels) {
+            if (s > 0.0f && t > 0.0f && v > 0.0f) {
                 const float inv_depth = (v * iza + s * izb) + t * izc;
-                float depth;
-                if (fast) depth = rcp_refined(inv_depth);   // wave-uniform branch
-                else depth = 1.0f / inv_depth;
-                atomic_min_fb(V.fb, pix, pack_pixel(depth, id));
+                uint64_t packed;
+                if (fast) {   // wave-uniform branch: a finite depth, no infinity to map
+                    const uint32_t bits = __builtin_bit_cast(uint32_t, rcp_refined(inv_depth)) & 0x7FFFFFF8u;
+                    packed = ((uint64_t)bits << 33) | id;
+                } else {
+                    packed = pack_pixel(1.0f / inv_depth, id);
+                }
+                atomic_min_fb(V.fb, pix, packed);
             }
             s += ds_dx;
             t += dt_dx;
