@@ -660,6 +660,9 @@ __device__ __forceinline__ void pu16_load_words(const S1Item& it, uint32_t group
     w1 = v.y;
     w2 = v.z;
     w3 = v.w;
+#else
+    (void)it;
+    (void)groupBase;
 #endif
 }
 
@@ -726,6 +729,9 @@ __device__ __forceinline__ f3 pu16_vertex_issue(const S1Item& it, uint32_t i) {
         lo = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
         hi = h[2];
     }
+#else
+    (void)it;
+    (void)i;
 #endif
     return f3{__builtin_bit_cast(float, lo), __builtin_bit_cast(float, hi), __builtin_bit_cast(float, flag)};
 }
