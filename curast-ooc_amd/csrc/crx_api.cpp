@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -234,7 +235,8 @@ ViewParams make_view_params(const crx_render_target& t, const Transform* transfo
 // one instance list (transforms view-major).  No device work.
 crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                      uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
-                     uint32_t numViews, int instanced, const uint8_t* skip, DrawPlan* plan) {
+                     uint32_t numViews, int instanced, const uint8_t* skip, DrawPlan* plan, bool instUnchanged,
+                     bool prefix) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
     if (numViews == 0 || numViews > kMaxViews)
         return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
@@ -254,6 +256,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     }
     D.numViews = numViews;
     D.numInstances = numInstances;
+    D.prefix = prefix;
     D.targets.assign(targets, targets + numViews);
     // drawTrianglesVisbuffer returns early for an empty instance list (CuRast_render.h:231)
     if (numInstances == 0) return CRX_OK;
@@ -310,13 +313,17 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         D.tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
         if (D.batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
     }
-    D.inst.resize(numInstances);
-    for (uint32_t i = 0; i < numInstances; i++) {
-        CRX_TRY(check_mesh(instances[i], "instances", i));
-        const int64_t ti = instances[i].instances.offset;
-        if (ti < 0 || ti >= (int64_t)numInstances)
-            return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
-        D.inst[i] = make_instance(instances[i]);
+    // instUnchanged: the list is the one the plan's records were built (and
+    // checked) from, so they stand
+    if (!instUnchanged || D.inst.size() != numInstances) {
+        D.inst.resize(numInstances);
+        for (uint32_t i = 0; i < numInstances; i++) {
+            CRX_TRY(check_mesh(instances[i], "instances", i));
+            const int64_t ti = instances[i].instances.offset;
+            if (ti < 0 || ti >= (int64_t)numInstances)
+                return fail(CRX_INVALID_ARG, "instances[" + std::to_string(i) + "].instances.offset out of range");
+            D.inst[i] = make_instance(instances[i]);
+        }
     }
     const size_t numXf = (size_t)numViews * numInstances;
     D.xf.resize(numXf);
@@ -329,7 +336,8 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     }
     D.offItems = 0;
     D.offInst = align_up(D.offItems + D.items.size() * sizeof(DrawItem), 256);
-    D.offViews = align_up(D.offInst + D.inst.size() * sizeof(InstanceRec), 256);
+    D.offPrefix = align_up(D.offInst + D.inst.size() * sizeof(InstanceRec), 256);
+    D.offViews = align_up(D.offPrefix + (prefix ? D.inst.size() * sizeof(uint64_t) : 0), 256);
     D.offXf = align_up(D.offViews + (size_t)numViews * sizeof(ViewParams), 256);
     D.metaBytes = D.offXf + D.xf.size() * sizeof(Transform);
     if (sched.wavesPerCU == 0) {
@@ -346,14 +354,21 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
 
 // Writes the plan's metadata image (to be copied to device address `dev`)
 // into `host` and returns the kernel parameters; the queues are the
-// context's.
-RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* dev) {
+// context's.  Without writeInst the instance records and prefix are left out
+// of the image (the device already holds them).
+RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* dev, bool writeInst) {
     const Transform* xfDev = reinterpret_cast<const Transform*>(dev + D.offXf);
     std::vector<ViewParams> views(D.numViews);
     for (uint32_t v = 0; v < D.numViews; v++)
         views[v] = make_view_params(D.targets[v], xfDev + (size_t)v * D.numInstances);
     std::memcpy(host + D.offItems, D.items.data(), D.items.size() * sizeof(DrawItem));
-    std::memcpy(host + D.offInst, D.inst.data(), D.inst.size() * sizeof(InstanceRec));
+    if (writeInst) {
+        std::memcpy(host + D.offInst, D.inst.data(), D.inst.size() * sizeof(InstanceRec));
+        if (D.prefix) {
+            uint64_t* prefix = reinterpret_cast<uint64_t*>(host + D.offPrefix);
+            for (size_t i = 0; i < D.inst.size(); i++) prefix[i] = D.inst[i].cumm;
+        }
+    }
     std::memcpy(host + D.offViews, views.data(), views.size() * sizeof(ViewParams));
     std::memcpy(host + D.offXf, D.xf.data(), D.xf.size() * sizeof(Transform));
     const crx_tuning& tu = ctx->tuning;
@@ -401,6 +416,14 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
     return hipSuccess;
 }
 
+// Orders `stream` after the last draw's work that may still read the
+// context's metadata buffers (a resolve enqueued before an early return on
+// another stream).
+crx_status wait_meta_free(crx_context* ctx, hipStream_t stream) {
+    if (ctx->metaFreePending && stream != ctx->metaFreeStream) CRX_HIP(hipStreamWaitEvent(stream, ctx->metaFree, 0));
+    return CRX_OK;
+}
+
 // drawTrianglesVisbuffer (CuRast_render.h:211-264) for numViews >= 1 views
 // sharing one instance list: transforms[v * numInstances + i] is view v's
 // view * world of instance i.  One view runs the plain stage 1; several views
@@ -408,22 +431,63 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
 crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                       uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
                       uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
-                      const uint8_t* skip, const std::function<crx_status()>* afterStages) {
+                      const DrawExtras& ex) {
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (skip && numViews < 2) return fail(CRX_INVALID_ARG, "per-view instance culling needs a multi-view draw");
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    if (ex.skip && numViews < 2) return fail(CRX_INVALID_ARG, "per-view instance culling needs a multi-view draw");
     DrawPlan& D = ctx->plan;
+    // the plan's instance records stand when the list is the one they were
+    // last built from; the key is void until this draw's records are in place
+    const bool sameLists = ex.listsKey != 0 && ex.listsKey == ctx->metaListsKey;
+    ctx->metaListsKey = 0;
     CRX_TRY(plan_draw(ctx, meshes, numMeshes, instances, numInstances, transforms, targets, numViews, instanced,
-                      skip, &D));
+                      ex.skip, &D, sameLists, ex.prefix));
     if (stats) stats->num_triangles = D.tris * numViews;
-    if (D.batches == 0) return CRX_OK;
+    if (D.batches == 0 && !ex.afterStages) return CRX_OK;
     CRX_HIP(hipSetDevice(ctx->device));
-    CRX_TRY(ctx->drawMeta.reserve(D.metaBytes));
-    CRX_TRY(ctx->drawStaging.acquire(D.metaBytes));
+    CRX_TRY(wait_meta_free(ctx, stream));
+
+    RasterParams P{};
+    if (D.numInstances > 0) {
+        CRX_TRY(ctx->drawMeta.reserve(D.metaBytes));
+        CRX_TRY(ctx->drawStaging.acquire(D.metaBytes));
+        char* meta = static_cast<char*>(ctx->drawMeta.ptr);
+        char* image = static_cast<char*>(ctx->drawStaging.ptr);
+        // records (and prefix) already in the buffer at the same place: copy
+        // only the draw items and the per-view records and transforms
+        const bool held = sameLists && ctx->metaListsPtr == meta && ctx->metaListsBegin == D.offInst &&
+                          ctx->metaListsEnd == D.offViews && ctx->metaListsPrefix == D.prefix;
+        P = plan_write(ctx, D, image, meta, !held);
+        if (held) {
+            CRX_HIP(hipMemcpyAsync(meta, image, D.offInst, hipMemcpyHostToDevice, stream));
+            CRX_HIP(hipMemcpyAsync(meta + D.offViews, image + D.offViews, D.metaBytes - D.offViews,
+                                   hipMemcpyHostToDevice, stream));
+        } else {
+            CRX_HIP(hipMemcpyAsync(meta, image, D.metaBytes, hipMemcpyHostToDevice, stream));
+        }
+        CRX_TRY(ctx->drawStaging.mark(stream));
+        ctx->metaListsKey = ex.listsKey;
+        ctx->metaListsPtr = meta;
+        ctx->metaListsBegin = D.offInst;
+        ctx->metaListsEnd = D.offViews;
+        ctx->metaListsPrefix = D.prefix;
+    }
+    const auto finish = [&]() -> crx_status {
+        if (ex.earlyReturn && ex.afterStages) {
+            CRX_HIP(hipEventRecord(ctx->metaFree, stream));
+            ctx->metaFreePending = true;
+            ctx->metaFreeStream = stream;
+        }
+        return CRX_OK;
+    };
+    if (D.batches == 0) {
+        // nothing to draw; the work that follows still runs (a resolve
+        // writes every pixel)
+        CRX_TRY((*ex.afterStages)());
+        if (!ex.earlyReturn) CRX_HIP(hipStreamSynchronize(stream));
+        return finish();
+    }
     CRX_TRY(ensure_queues(ctx, ctx->tuning.max_nontrivial, ctx->tuning.max_huge));
-    char* meta = static_cast<char*>(ctx->drawMeta.ptr);
-    RasterParams P = plan_write(ctx, D, static_cast<char*>(ctx->drawStaging.ptr), meta);
-    CRX_HIP(hipMemcpyAsync(meta, ctx->drawStaging.ptr, D.metaBytes, hipMemcpyHostToDevice, stream));
-    CRX_TRY(ctx->drawStaging.mark(stream));
 
     uint32_t redraws = 0;
     for (;;) {
@@ -433,10 +497,12 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         P.hugeCap = ctx->hugeCap;
         CRX_HIP(enqueue_stages(ctx, D, P, false, stream));
         CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        CRX_HIP(hipEventRecord(ctx->countersBack, stream));
         // work that follows the stages (the resolve) is prepared and enqueued
         // while they run; after a redraw it is enqueued again
-        if (afterStages) CRX_TRY((*afterStages)());
-        CRX_HIP(hipStreamSynchronize(stream));
+        if (ex.afterStages) CRX_TRY((*ex.afterStages)());
+        if (ex.earlyReturn) CRX_HIP(hipEventSynchronize(ctx->countersBack));
+        else CRX_HIP(hipStreamSynchronize(stream));
         const uint32_t nNt = ctx->ctrHost[CTR_NONTRIVIAL];
         const uint32_t nHuge = ctx->ctrHost[CTR_HUGE];
         if (stats) {
@@ -455,7 +521,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
             return fail(CRX_QUEUE_OVERFLOW, "stage queue would exceed 2^32 entries");
         CRX_TRY(ensure_queues(ctx, (uint32_t)ntWant, (uint32_t)hgWant));
     }
-    return CRX_OK;
+    return finish();
 }
 
 // Per-view resolve record (the instance list and prefix are shared).
@@ -498,9 +564,7 @@ size_t resolve_image_layout(uint32_t numInstances, uint32_t numViews, size_t* of
 // resolve_image_layout bytes): the records go straight to their places, with
 // no intermediate copies (a scene of thousands of instances makes this a
 // memory-bound loop on the host).
-crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
-                              const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
-                              char* dev, char* out, size_t* offViews, uint32_t* blocks) {
+crx_status check_resolve(const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute) {
     if (numViews == 0 || numViews > kMaxViews)
         return fail(CRX_INVALID_ARG, "numViews must be in [1, " + std::to_string(kMaxViews) + "]");
     if (!targets) return fail(CRX_INVALID_ARG, "targets is NULL");
@@ -511,6 +575,25 @@ crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances,
     if (attribute == CRX_DISPLAY_TEXTURE || attribute == CRX_DISPLAY_STAGE)
         return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
     if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
+    return CRX_OK;
+}
+
+// First 64x4 block of each view in a multi-view resolve grid; returns the total.
+crx_status resolve_blocks(const crx_render_target* targets, uint32_t numViews, ResolveParams* rv, uint32_t* blocks) {
+    uint64_t nb = 0;
+    for (uint32_t v = 0; v < numViews; v++) {
+        rv[v].firstBlock = (uint32_t)nb;
+        nb += (uint64_t)((targets[v].width + 63) / 64) * (uint64_t)((targets[v].height + 3) / 4);
+    }
+    if (nb >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
+    *blocks = (uint32_t)nb;
+    return CRX_OK;
+}
+
+crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
+                              const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
+                              char* dev, char* out, size_t* offViews, uint32_t* blocks) {
+    CRX_TRY(check_resolve(targets, numViews, attribute));
     if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
     size_t offPrefix = 0;
     resolve_image_layout(numInstances, numViews, &offPrefix, offViews);
@@ -523,15 +606,9 @@ crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances,
     const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(dev);
     const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(dev + offPrefix);
     ResolveParams* rv = reinterpret_cast<ResolveParams*>(out + *offViews);
-    uint64_t nb = 0;
-    for (uint32_t v = 0; v < numViews; v++) {
+    for (uint32_t v = 0; v < numViews; v++)
         rv[v] = make_resolve_params(targets[v], instDev, prefixDev, numInstances, attribute);
-        rv[v].firstBlock = (uint32_t)nb;
-        nb += (uint64_t)((targets[v].width + 63) / 64) * (uint64_t)((targets[v].height + 3) / 4);
-    }
-    if (nb >= (1ull << 31)) return fail(CRX_INVALID_ARG, "too many pixels for one resolve");
-    *blocks = (uint32_t)nb;
-    return CRX_OK;
+    return resolve_blocks(targets, numViews, rv, blocks);
 }
 
 crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
@@ -550,6 +627,7 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
                          hipStream_t stream) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
     CRX_HIP(hipSetDevice(ctx->device));
+    CRX_TRY(wait_meta_free(ctx, stream));
     size_t offPrefix = 0, offViews = 0;
     uint32_t blocks = 0;
     const size_t bytes = resolve_image_layout(numInstances, numViews, &offPrefix, &offViews);
@@ -571,6 +649,38 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
     } else {
         CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews, blocks, stream));
     }
+    record(ctx, T_RESOLVE_END, stream);
+    return CRX_OK;
+}
+
+crx_status resolve_views_device(crx_context* ctx, const InstanceRec* instDev, const uint64_t* prefixDev,
+                                uint32_t numInstances, const crx_render_target* targets, uint32_t numViews,
+                                crx_display_attribute attribute, hipStream_t stream) {
+    if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
+    CRX_TRY(check_resolve(targets, numViews, attribute));
+    if (numInstances > 0 && (!instDev || !prefixDev)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
+    CRX_HIP(hipSetDevice(ctx->device));
+    if (numViews == 1) {
+        // one view: its record is the kernel argument, nothing to upload
+        const ResolveParams R = make_resolve_params(targets[0], instDev, prefixDev, numInstances, attribute);
+        record(ctx, T_RESOLVE, stream);
+        CRX_HIP(launch_resolve(R, stream));
+        record(ctx, T_RESOLVE_END, stream);
+        return CRX_OK;
+    }
+    CRX_TRY(wait_meta_free(ctx, stream));
+    const size_t bytes = (size_t)numViews * sizeof(ResolveParams);
+    CRX_TRY(ctx->resolveMeta.reserve(bytes));
+    CRX_TRY(ctx->resolveStaging.acquire(bytes));
+    ResolveParams* rv = static_cast<ResolveParams*>(ctx->resolveStaging.ptr);
+    for (uint32_t v = 0; v < numViews; v++)
+        rv[v] = make_resolve_params(targets[v], instDev, prefixDev, numInstances, attribute);
+    uint32_t blocks = 0;
+    CRX_TRY(resolve_blocks(targets, numViews, rv, &blocks));
+    CRX_HIP(hipMemcpyAsync(ctx->resolveMeta.ptr, rv, bytes, hipMemcpyHostToDevice, stream));
+    CRX_TRY(ctx->resolveStaging.mark(stream));
+    record(ctx, T_RESOLVE, stream);
+    CRX_HIP(launch_resolve_views(static_cast<const ResolveParams*>(ctx->resolveMeta.ptr), numViews, blocks, stream));
     record(ctx, T_RESOLVE_END, stream);
     return CRX_OK;
 }
@@ -665,6 +775,13 @@ bool box_in_frustum(const float planes[16], const crx_box3& wb) {
     return true;
 }
 
+// A content key for a scene's instance list, new on every rebuild and unique
+// across scenes.
+uint64_t next_lists_key() {
+    static std::atomic<uint64_t> key{0};
+    return ++key;
+}
+
 // CuRast::draw's frame preparation (CuRast_render.h:338-435) for numViews
 // views: the node cull of each view (SNTriangles::update, SNTriangles.h:29-105),
 // the nodes kept (visible in any view, or all with keepAll), sorted by
@@ -705,53 +822,68 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
     // scenes are usually built mesh by mesh: then the order already holds
     if (!std::is_sorted(kept.begin(), kept.end(), before)) std::stable_sort(kept.begin(), kept.end(), before);
     const uint32_t n = (uint32_t)kept.size();
-    scene->unique.resize(n);
-    scene->instances.resize(n);
+    const int instancedFlag = settings.instanced ? 1 : 0;
+    // the per-node records depend on the node set, the instancing switch and
+    // the kept nodes only: a frame that changes none of them keeps them
+    if (scene->listsKey == 0 || scene->listsVersion != scene->version || scene->listsInstanced != instancedFlag ||
+        scene->listsKept != kept) {
+        scene->unique.resize(n);
+        scene->instances.resize(n);
+        scene->prefix.resize(n);
+        scene->nodeOf.resize(n);
+        uint64_t sum = 0;
+        uint32_t u = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const crx_scene::Node& nd = scene->nodes[kept[i]];
+            crx_cmesh cm;
+            std::memset(&cm, 0, sizeof(cm));
+            cm.numTriangles = nd.mesh->numTriangles;
+            cm.indices = nd.mesh->indices;
+            cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
+            cm.colors = nd.mesh->colors;
+            cm.aabb = nd.mesh->aabb;
+            cm.world = nd.world;
+            cm.id = (int32_t)kept[i];
+            cm.numVertices = nd.mesh->numVertices;
+            cm.address = nd.mesh->seq;
+            cm.isLoaded = 1;
+            cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
+            cm.compressed = nd.mesh->compressed ? 1 : 0;
+            cm.bitsPerIndex = nd.mesh->bitsPerIndex;
+            cm.index_min = nd.mesh->indexMin;
+            cm.index_max = nd.mesh->indexMax;
+            cm.compressionFactor = nd.mesh->compressionFactor;
+            cm.cummulativeTriangleCount = sum;
+            cm.instances.offset = (int32_t)i;
+            cm.instances.count = 0;
+            scene->prefix[i] = sum;
+            scene->nodeOf[i] = kept[i];
+            sum += cm.numTriangles;
+            scene->instances[i] = cm;
+            if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
+                scene->unique[u] = cm;
+                u++;
+            }
+            scene->unique[u - 1].instances.count++;
+        }
+        scene->unique.resize(u);
+        scene->listsVersion = scene->version;
+        scene->listsInstanced = instancedFlag;
+        scene->listsKept.swap(kept);
+        scene->listsKey = next_lists_key();
+    }
+    // per view: view * world of each listed node and the view's own cull
     scene->transforms.resize((size_t)n * numViews);
     scene->skip.assign((size_t)n * numViews, 0);
-    scene->prefix.resize(n);
-    scene->nodeOf.resize(n);
-    uint64_t sum = 0;
-    uint32_t u = 0;
     for (uint32_t i = 0; i < n; i++) {
-        const crx_scene::Node& nd = scene->nodes[kept[i]];
-        crx_cmesh cm;
-        std::memset(&cm, 0, sizeof(cm));
-        cm.numTriangles = nd.mesh->numTriangles;
-        cm.indices = nd.mesh->indices;
-        cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
-        cm.colors = nd.mesh->colors;
-        cm.aabb = nd.mesh->aabb;
-        cm.world = nd.world;
-        cm.id = (int32_t)kept[i];
-        cm.numVertices = nd.mesh->numVertices;
-        cm.address = nd.mesh->seq;
-        cm.isLoaded = 1;
-        cm.flipTriangles = mirror_flag(nd.world) ? 1 : 0;
-        cm.compressed = nd.mesh->compressed ? 1 : 0;
-        cm.bitsPerIndex = nd.mesh->bitsPerIndex;
-        cm.index_min = nd.mesh->indexMin;
-        cm.index_max = nd.mesh->indexMax;
-        cm.compressionFactor = nd.mesh->compressionFactor;
-        cm.cummulativeTriangleCount = sum;
-        cm.instances.offset = (int32_t)i;
-        cm.instances.count = 0;
+        const uint32_t node = scene->nodeOf[i];
+        const crx_mat4& world = scene->nodes[node].world;
         for (uint32_t v = 0; v < numViews; v++) {
-            scene->transforms[(size_t)v * n + i] = mat_mul(views[v].view, nd.world);
-            scene->skip[(size_t)v * n + i] = vis[(size_t)kept[i] * numViews + v] ? 0 : 1;
+            scene->transforms[(size_t)v * n + i] = mat_mul(views[v].view, world);
+            scene->skip[(size_t)v * n + i] = vis[(size_t)node * numViews + v] ? 0 : 1;
         }
-        scene->prefix[i] = sum;
-        scene->nodeOf[i] = kept[i];
-        sum += cm.numTriangles;
-        scene->instances[i] = cm;
-        if (u == 0 || scene->unique[u - 1].address != cm.address || !settings.instanced) {
-            scene->unique[u] = cm;
-            u++;
-        }
-        scene->unique[u - 1].instances.count++;
     }
-    scene->unique.resize(u);
-    return u;
+    return (uint32_t)scene->unique.size();
 }
 
 // Render targets for views (framebuffers / colourbuffers per view) and the
@@ -819,13 +951,22 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
     CRX_TRY(clear_buffers(ctx, clearBufs.data(), clearPx.data(), (uint32_t)clearBufs.size(), stream));
     bool anySkip = false;
     for (uint8_t k : scene->skip) anySkip = anySkip || k != 0;
+    // the resolve reads the instance records and prefix the draw uploaded
     const std::function<crx_status()> resolve = [&]() {
-        return resolve_views(ctx, scene->instances.data(), n, scene->prefix.data(), targets.data(), numViews,
-                             (crx_display_attribute)settings.attribute, stream);
+        const DrawPlan& D = ctx->plan;
+        const char* meta = static_cast<const char*>(ctx->drawMeta.ptr);
+        return resolve_views_device(ctx, n ? reinterpret_cast<const InstanceRec*>(meta + D.offInst) : nullptr,
+                                    n ? reinterpret_cast<const uint64_t*>(meta + D.offPrefix) : nullptr, n,
+                                    targets.data(), numViews, (crx_display_attribute)settings.attribute, stream);
     };
+    DrawExtras ex;
+    ex.skip = (anySkip && numViews > 1) ? scene->skip.data() : nullptr;
+    ex.afterStages = settings.resolve ? &resolve : nullptr;
+    ex.listsKey = scene->listsKey;
+    ex.prefix = settings.resolve != 0;
+    ex.earlyReturn = true;
     CRX_TRY(draw_views(ctx, scene->unique.data(), u, scene->instances.data(), n, scene->transforms.data(),
-                       targets.data(), numViews, settings.instanced, stream, stats,
-                       (anySkip && numViews > 1) ? scene->skip.data() : nullptr, settings.resolve ? &resolve : nullptr));
+                       targets.data(), numViews, settings.instanced, stream, stats, ex));
     return CRX_OK;
 }
 
@@ -888,6 +1029,8 @@ crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context*
     if (e == hipSuccess) e = hipMemset(ctx->ctr, 0, CTR_COUNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->ctrHost, CTR_COUNT * sizeof(uint32_t), hipHostMallocDefault);
     for (int i = 0; e == hipSuccess && i < T_COUNT; i++) e = hipEventCreate(&ctx->ev[i]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->countersBack, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->metaFree, hipEventDisableTiming);
     if (e != hipSuccess) {
         crx_context_destroy(ctx);
         return fail(CRX_HIP_ERROR, std::string("context allocation: ") + hipGetErrorString(e));
@@ -914,6 +1057,8 @@ crx_status crx_context_destroy(crx_context* ctx) {
     if (ctx->ctrHost) (void)hipHostFree(ctx->ctrHost);
     for (int i = 0; i < T_COUNT; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->countersBack) (void)hipEventDestroy(ctx->countersBack);
+    if (ctx->metaFree) (void)hipEventDestroy(ctx->metaFree);
     delete ctx;
     return CRX_OK;
 }
@@ -1375,6 +1520,7 @@ crx_status crx_scene_add_node(crx_scene* scene, crx_mesh* mesh, const crx_mat4* 
     if (!scene || !mesh || !world) return fail(CRX_INVALID_ARG, "NULL argument");
     if (mesh->ctx != scene->ctx) return fail(CRX_INVALID_ARG, "mesh belongs to another context");
     scene->nodes.push_back(crx_scene::Node{mesh, *world, world_box(mesh->aabb, *world)});
+    scene->version++;
     if (node_id) *node_id = (uint32_t)(scene->nodes.size() - 1);
     return CRX_OK;
 }
@@ -1384,6 +1530,7 @@ crx_status crx_scene_set_transform(crx_scene* scene, uint32_t node_id, const crx
     if (node_id >= scene->nodes.size()) return fail(CRX_INVALID_ARG, "node_id out of range");
     scene->nodes[node_id].world = *world;
     scene->nodes[node_id].wbox = world_box(scene->nodes[node_id].mesh->aabb, *world);
+    scene->version++;
     return CRX_OK;
 }
 

@@ -121,7 +121,24 @@ struct DrawPlan {
     bool pu16 = false;
     uint32_t block = 256, trisPerLane = 1, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
     bool instanceBoxes = false;   // some item loops over >= 4 instances
-    size_t offItems = 0, offInst = 0, offViews = 0, offXf = 0, metaBytes = 0;
+    bool prefix = false;          // the image carries the instances' triangle prefix (InstanceRec::cumm)
+    size_t offItems = 0, offInst = 0, offPrefix = 0, offViews = 0, offXf = 0, metaBytes = 0;
+};
+
+// What a caller adds to a draw.  listsKey != 0 names the content of the
+// instance list (equal keys, bitwise-equal lists): a draw whose key matches
+// the records the context's metadata buffer already holds neither rebuilds
+// nor uploads them again.  prefix adds the instances' triangle prefix to the
+// metadata, for a resolve that reads the draw's records.  afterStages is
+// enqueued after the stages (and again after a redraw); with earlyReturn the
+// draw returns once the queue counters are back, while that work may still
+// run on the stream.
+struct DrawExtras {
+    const uint8_t* skip = nullptr;
+    const std::function<crx_status()>* afterStages = nullptr;
+    uint64_t listsKey = 0;
+    bool prefix = false;
+    bool earlyReturn = false;
 };
 
 }  // namespace crx
@@ -140,6 +157,16 @@ struct crx_context {
     crx::DeviceBuffer resolveMeta;         // instances | prefix | per-view records
     crx::DeviceBuffer clearMeta;           // per-buffer clear records
     crx::Staging drawStaging, resolveStaging, clearStaging;
+    // instance records (and prefix) that drawMeta holds: the list key they
+    // were uploaded under (0 = none), the buffer and their byte range
+    uint64_t metaListsKey = 0;
+    void* metaListsPtr = nullptr;
+    size_t metaListsBegin = 0, metaListsEnd = 0;
+    bool metaListsPrefix = false;
+    hipEvent_t countersBack = nullptr;   // the counter readback of the last draw
+    hipEvent_t metaFree = nullptr;       // recorded after the last draw's work that reads the metadata buffers
+    hipStream_t metaFreeStream = nullptr;
+    bool metaFreePending = false;
     bool timing = false;
     hipEvent_t ev[crx::T_COUNT] = {};
     bool recorded[crx::T_COUNT] = {};
@@ -189,6 +216,14 @@ struct crx_scene {
     std::vector<uint64_t> prefix;
     std::vector<uint8_t> skip;        // [view][instance]: dropped by that view's node cull
     std::vector<uint32_t> nodeOf;     // node of each listed instance
+    // node-set version (every node change bumps it) and what the lists were
+    // built from; a frame with the same version, instancing and kept nodes
+    // keeps them and recomputes only the per-view transforms and skips
+    uint64_t version = 0;
+    uint64_t listsVersion = ~0ull;
+    int listsInstanced = -1;
+    std::vector<uint32_t> listsKept;
+    uint64_t listsKey = 0;            // content key of the lists (new on every rebuild)
 };
 
 
@@ -224,20 +259,26 @@ bool box_in_frustum(const float planes[16], const crx_box3& worldBox);
 
 crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                      uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
-                     uint32_t numViews, int instanced, const uint8_t* skip, DrawPlan* plan);
-RasterParams plan_write(crx_context* ctx, const DrawPlan& plan, char* host, char* dev);
+                     uint32_t numViews, int instanced, const uint8_t* skip, DrawPlan* plan,
+                     bool instUnchanged = false, bool prefix = false);
+RasterParams plan_write(crx_context* ctx, const DrawPlan& plan, char* host, char* dev, bool writeInst = true);
 hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& plan, const RasterParams& P, bool viewsKernel,
                           hipStream_t stream);
 crx_status ensure_queues(crx_context* ctx, uint32_t nt, uint32_t hg);
 
 // drawTrianglesVisbuffer for numViews >= 1 targets sharing one instance list
-// (transforms view-major).  skip (nullable, numViews * numInstances bytes,
-// view-major): a non-zero byte drops instance i from view v, as the host node
-// cull of that view would; it needs numViews > 1.
+// (transforms view-major).  extras.skip (nullable, numViews * numInstances
+// bytes, view-major): a non-zero byte drops instance i from view v, as the
+// host node cull of that view would; it needs numViews > 1.
 crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
                       uint32_t numInstances, const crx_mat4* transforms, const crx_render_target* targets,
                       uint32_t numViews, int instanced, hipStream_t stream, crx_draw_stats* stats,
-                      const uint8_t* skip = nullptr, const std::function<crx_status()>* afterStages = nullptr);
+                      const DrawExtras& extras = DrawExtras{});
+// The resolve of numViews targets over instance records and prefix already in
+// device memory (a draw's metadata); uploads at most the per-view records.
+crx_status resolve_views_device(crx_context* ctx, const InstanceRec* instDev, const uint64_t* prefixDev,
+                                uint32_t numInstances, const crx_render_target* targets, uint32_t numViews,
+                                crx_display_attribute attribute, hipStream_t stream);
 crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
                          hipStream_t stream);

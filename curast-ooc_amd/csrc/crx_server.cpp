@@ -278,7 +278,6 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
     do {
         const uint32_t U = (uint32_t)need.size();
         std::vector<crx_cmesh> inst(U);
-        std::vector<uint64_t> prefix(U);
         {
             std::lock_guard<std::mutex> lk(s->cacheMu);   // slot pointers are stable while pinned
             for (uint32_t k = 0; k < U; k++) {
@@ -301,7 +300,6 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
                 m.numVertices = n.numVertices;
                 m.isLoaded = 1;
                 m.flipTriangles = mirror_flag(n.world) ? 1 : 0;
-                prefix[k] = n.cumm;
                 // the lane's stream waits for the node's upload
                 if (n.ready) {
                     hipError_t e = hipStreamWaitEvent(lane.stream, n.ready, 0);
@@ -349,12 +347,20 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
         st = clear_buffers(lane.ctx, bufs.data(), px.data(), (uint32_t)bufs.size(), lane.stream);
         if (st != CRX_OK) break;
         crx_draw_stats ds;
+        // the resolve reads the instance records and their triangle prefix
+        // (cummulativeTriangleCount) from the draw's metadata
         const std::function<crx_status()> resolve = [&]() {
-            return resolve_views(lane.ctx, inst.data(), U, prefix.data(), targets.data(), V, CRX_DISPLAY_VERTEX_COLORS,
-                                 lane.stream);
+            const DrawPlan& D = lane.ctx->plan;
+            const char* meta = static_cast<const char*>(lane.ctx->drawMeta.ptr);
+            return resolve_views_device(lane.ctx, U ? reinterpret_cast<const InstanceRec*>(meta + D.offInst) : nullptr,
+                                        U ? reinterpret_cast<const uint64_t*>(meta + D.offPrefix) : nullptr, U,
+                                        targets.data(), V, CRX_DISPLAY_VERTEX_COLORS, lane.stream);
         };
-        st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream, &ds,
-                        (anySkip && V > 1) ? skip.data() : nullptr, s->cfg.resolve ? &resolve : nullptr);
+        DrawExtras ex;
+        ex.skip = (anySkip && V > 1) ? skip.data() : nullptr;
+        ex.afterStages = s->cfg.resolve ? &resolve : nullptr;
+        ex.prefix = s->cfg.resolve != 0;
+        st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream, &ds, ex);
         if (st != CRX_OK) break;
         hipError_t e = hipStreamSynchronize(lane.stream);
         if (e != hipSuccess) st = fail(CRX_HIP_ERROR, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));

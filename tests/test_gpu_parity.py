@@ -216,39 +216,34 @@ def test_resolve_rejects_texture(ctx):
     assert e.value.status == 6
 
 
-def test_scene_api_matches_oracle_frame(ctx):
-    """crx_render (scene API: node cull, sort/group, prefix, clear, stages,
-    resolve) against the oracle's restatement of CuRast::draw."""
-    import ctypes as C
+def _two_mesh_scene(ctx, n=30):
+    """A scene of n nodes over two sphere meshes (every third node the larger
+    one), with the oracle's copies of the meshes."""
     pos, idx, col = wl.sphere_mesh(rings=32, segments=64, seed=21)
     pos2, idx2, col2 = wl.sphere_mesh(rings=24, segments=48, radius=2.0, seed=22)
     meshes = [crx.Mesh(ctx, pos, idx, col), crx.Mesh(ctx, pos2, idx2, col2)]
     hosts = [orc.HostMesh(crx, pos, idx, col), orc.HostMesh(crx, pos2, idx2, col2)]
     scene = crx.Scene(ctx)
-    worlds = list(wl.instance_transforms(30, extent=30.0, seed=23))
+    worlds = list(wl.instance_transforms(n, extent=30.0, seed=23))
     order = []
     for k, w in enumerate(worlds):
         mk = k % 3 == 0
         scene.add(meshes[1 if mk else 0], w)
         order.append(1 if mk else 0)
-    W, H = 800, 450
-    view, proj = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
-    v = crx.make_view(view, proj, W, H)
-    fb = ctx.alloc(8 * W * H)
-    cb = ctx.alloc(8 * W * H)
-    scene.render(v, fb, cb, crx.render_settings())
-    got_fb = fb.download(np.uint64, W * H)
-    got_cb = cb.download(np.uint64, W * H)
+    return scene, meshes, hosts, worlds, order
 
+
+def _oracle_scene_frame(meshes, hosts, worlds, order, v, view, proj, W, H, settings):
+    """The oracle's CuRast::draw of the node list (node cull, sort/group,
+    prefix), its three stages and its resolve."""
+    import ctypes as C
     L = orc.lib(crx)
     n = len(worlds)
     node_meshes = (crx.CMesh * n)()
     node_world = (crx.Mat4 * n)()
     for k in range(n):
-        hm = hosts[order[k]]
-        cm = hm.cmesh(crx)
-        b = meshes[order[k]].aabb()
-        cm.aabb = b
+        cm = hosts[order[k]].cmesh(crx)
+        cm.aabb = meshes[order[k]].aabb()
         cm.address = order[k] + 1
         node_meshes[k] = cm
         node_world[k] = worlds[k]
@@ -257,14 +252,97 @@ def test_scene_api_matches_oracle_frame(ctx):
     inst = (crx.CMesh * n)()
     xf = (crx.Mat4 * n)()
     pre = (C.c_uint64 * n)()
-    nv = L.crx_oracle_prepare_frame(node_meshes, node_world, n, C.byref(v), 1, 0, uniq, C.byref(nu), inst, xf, pre)
-    assert 0 < nv <= n
+    nv = L.crx_oracle_prepare_frame(node_meshes, node_world, n, C.byref(v), int(settings.frustum_culling),
+                                    0 if settings.instanced else 1, uniq, C.byref(nu), inst, xf, pre)
     for k in range(nv):
         inst[k].id = -1
-    fbw, _ = orc.draw(crx, list(uniq)[:nu.value], list(inst)[:nv], list(xf)[:nv], W, H, view, proj, True)
+    if nv:
+        fbw, _ = orc.draw(crx, list(uniq)[:nu.value], list(inst)[:nv], list(xf)[:nv], W, H, view, proj,
+                          bool(settings.instanced))
+    else:
+        fbw = np.empty(W * H, dtype=np.uint64)
+        L.crx_oracle_clear(fbw.ctypes.data, fbw.size)
+    cbw = None
+    if settings.resolve:
+        cbw = orc.resolve(crx, list(inst)[:nv], list(pre)[:nv], fbw, W, H, view, proj, settings.attribute)
+    return fbw, cbw, nv
+
+
+def test_scene_api_matches_oracle_frame(ctx):
+    """crx_render (scene API: node cull, sort/group, prefix, clear, stages,
+    resolve) against the oracle's restatement of CuRast::draw."""
+    scene, meshes, hosts, worlds, order = _two_mesh_scene(ctx)
+    W, H = 800, 450
+    view, proj = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
+    v = crx.make_view(view, proj, W, H)
+    fb = ctx.alloc(8 * W * H)
+    cb = ctx.alloc(8 * W * H)
+    settings = crx.render_settings()
+    scene.render(v, fb, cb, settings)
+    got_fb = fb.download(np.uint64, W * H)
+    got_cb = cb.download(np.uint64, W * H)
+    fbw, cbw, nv = _oracle_scene_frame(meshes, hosts, worlds, order, v, view, proj, W, H, settings)
+    assert 0 < nv <= len(worlds)
     assert np.array_equal(got_fb, fbw), describe_mismatch(got_fb, fbw, W)
-    cbw = orc.resolve(crx, list(inst)[:nv], list(pre)[:nv], fbw, W, H, view, proj, crx.DISPLAY_VERTEX_COLORS)
     assert np.array_equal(got_cb, cbw), describe_mismatch(got_cb, cbw, W)
+    scene.close()
+
+
+def test_scene_frames_keep_and_rebuild_lists(ctx):
+    """A run of crx_render frames over one scene: the per-node lists and the
+    uploaded instance records are kept while the node set, the instancing
+    switch and the culled set stay the same, and rebuilt when any changes (a
+    view that culls other nodes, a moved node, a frame that sees nothing, the
+    other instancing mode, a frame without resolve, a C-ABI draw in between).
+    Every frame equals the oracle's, colour buffer included."""
+    scene, meshes, hosts, worlds, order = _two_mesh_scene(ctx)
+    W, H = 640, 360
+    proj = wl.projection(W, H)
+    view_a, _ = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
+    view_b = crx.look_at((0.0, 0.0, 0.0), (30.0, 0.0, 5.0), (0.0, 1.0, 0.0))
+    view_e = crx.look_at((500.0, 0.0, 0.0), (1000.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    fb = ctx.alloc(8 * W * H)
+    cb = ctx.alloc(8 * W * H)
+    plain = crx.render_settings()
+    seen = set()
+
+    def frame(name, view, settings=plain):
+        print(f"[case] frame {name}", flush=True)
+        v = crx.make_view(view, proj, W, H)
+        scene.render(v, fb, cb if settings.resolve else None, settings)
+        got_fb = fb.download(np.uint64, W * H)
+        fbw, cbw, nv = _oracle_scene_frame(meshes, hosts, worlds, order, v, view, proj, W, H, settings)
+        assert np.array_equal(got_fb, fbw), f"{name}: " + describe_mismatch(got_fb, fbw, W)
+        if settings.resolve:
+            got_cb = cb.download(np.uint64, W * H)
+            assert np.array_equal(got_cb, cbw), f"{name}: " + describe_mismatch(got_cb, cbw, W)
+        seen.add(nv)
+
+    frame("A", view_a)
+    frame("A again", view_a)
+    frame("B", view_b)
+    frame("A after B", view_a)
+    w3 = worlds[3]
+    m = crx.Mat4()
+    for k in range(16):
+        m.m[k] = w3.m[k]
+    m.m[12] += 4.0
+    m.m[13] -= 3.0
+    scene.set_transform(3, m)
+    worlds[3] = m
+    frame("A after a moved node", view_a)
+    # a C-ABI draw of other lists on the same context between two frames
+    pos, idx, col = wl.sphere_mesh(rings=16, segments=32, seed=5)
+    other = SceneSpec().add(orc.HostMesh(crx, pos, idx, col), [crx.identity()])
+    sv, sp = wl.sphere_view(W, H)
+    GpuScene(ctx, other).render(W, H, sv, sp)
+    frame("A after a C-ABI draw", view_a)
+    frame("E (nothing visible)", view_e)
+    frame("A after E", view_a)
+    frame("A non-instanced", view_a, crx.render_settings(instanced=0))
+    frame("A without resolve", view_a, crx.render_settings(resolve=0))
+    frame("A with resolve", view_a)
+    assert 0 in seen and len(seen) >= 3
     scene.close()
 
 
