@@ -398,9 +398,12 @@ RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* d
 // Counter reset and the three stages.  viewsKernel forces the multi-view
 // stage 1 (which reads every view record from memory) for a single view.
 hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParams& P, bool viewsKernel,
-                          hipStream_t stream) {
-    hipError_t e = launch_reset_counters(P.ctrNontrivial - CTR_NONTRIVIAL, stream);   // the plan's counter block
-    if (e != hipSuccess) return e;
+                          hipStream_t stream, bool countersZeroed) {
+    hipError_t e = hipSuccess;
+    if (!countersZeroed) {
+        e = launch_reset_counters(P.ctrNontrivial - CTR_NONTRIVIAL, stream);   // the plan's counter block
+        if (e != hipSuccess) return e;
+    }
     record(ctx, T_S1, stream);
     if (D.numViews == 1 && !viewsKernel)
         e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, D.instanceBoxes, stream);
@@ -495,7 +498,8 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         P.huge = static_cast<int4*>(ctx->huge.ptr);
         P.ntCap = ctx->ntCap;
         P.hugeCap = ctx->hugeCap;
-        CRX_HIP(enqueue_stages(ctx, D, P, false, stream));
+        // the caller's clear zeroed the counters for the first pass only
+        CRX_HIP(enqueue_stages(ctx, D, P, false, stream, ex.countersZeroed && redraws == 0));
         CRX_HIP(hipMemcpyAsync(ctx->ctrHost, ctx->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         CRX_HIP(hipEventRecord(ctx->countersBack, stream));
         // work that follows the stages (the resolve) is prepared and enqueued
@@ -707,15 +711,19 @@ crx_status clear_records(uint64_t* const* buffers, const uint32_t* numPixels, ui
 
 // kernel_clearFramebuffer (resolve.cu:213-232) over several buffers in one launch.
 crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,
-                         hipStream_t stream) {
+                         hipStream_t stream, uint32_t* ctr, bool* zeroed) {
+    if (zeroed) *zeroed = false;
     std::vector<ClearView> cv;
     uint32_t blocks = 0;
     CRX_TRY(clear_records(buffers, numPixels, n, &cv, &blocks));
     if (cv.empty()) return CRX_OK;
     if (cv.size() <= 2) {
-        // one or two buffers (a view's visibility and colour buffers): a
-        // launch each, with no metadata to stage and copy
-        for (const ClearView& c : cv) CRX_HIP(launch_clear(c.fb, c.numPixels, nullptr, ctx->numCUs, stream));
+        // one or two buffers (a view's visibility and colour buffers): one
+        // launch, with no metadata to stage and copy
+        const bool two = cv.size() == 2;
+        CRX_HIP(launch_clear(cv[0].fb, cv[0].numPixels, ctr, ctx->numCUs, stream, two ? cv[1].fb : nullptr,
+                             two ? cv[1].numPixels : 0u));
+        if (zeroed) *zeroed = ctr != nullptr;
         return CRX_OK;
     }
     const size_t bytes = cv.size() * sizeof(ClearView);
@@ -724,8 +732,9 @@ crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint3
     std::memcpy(ctx->clearStaging.ptr, cv.data(), bytes);
     CRX_HIP(hipMemcpyAsync(ctx->clearMeta.ptr, ctx->clearStaging.ptr, bytes, hipMemcpyHostToDevice, stream));
     CRX_TRY(ctx->clearStaging.mark(stream));
-    CRX_HIP(launch_clear_views(static_cast<const ClearView*>(ctx->clearMeta.ptr), (uint32_t)cv.size(), blocks, nullptr,
+    CRX_HIP(launch_clear_views(static_cast<const ClearView*>(ctx->clearMeta.ptr), (uint32_t)cv.size(), blocks, ctr,
                                stream));
+    if (zeroed) *zeroed = ctr != nullptr;
     return CRX_OK;
 }
 
@@ -948,7 +957,10 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
 
     CRX_HIP(hipSetDevice(ctx->device));
     record(ctx, T_CLEAR, stream);
-    CRX_TRY(clear_buffers(ctx, clearBufs.data(), clearPx.data(), (uint32_t)clearBufs.size(), stream));
+    // the clear also zeroes the draw counters (stage 1 then needs no reset)
+    bool zeroed = false;
+    CRX_TRY(clear_buffers(ctx, clearBufs.data(), clearPx.data(), (uint32_t)clearBufs.size(), stream, ctx->ctr,
+                          &zeroed));
     bool anySkip = false;
     for (uint8_t k : scene->skip) anySkip = anySkip || k != 0;
     // the resolve reads the instance records and prefix the draw uploaded
@@ -965,6 +977,7 @@ crx_status render_views(crx_context* ctx, crx_scene* scene, const crx_view* view
     ex.listsKey = scene->listsKey;
     ex.prefix = settings.resolve != 0;
     ex.earlyReturn = true;
+    ex.countersZeroed = zeroed;
     CRX_TRY(draw_views(ctx, scene->unique.data(), u, scene->instances.data(), n, scene->transforms.data(),
                        targets.data(), numViews, settings.instanced, stream, stats, ex));
     return CRX_OK;

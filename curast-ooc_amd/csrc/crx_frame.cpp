@@ -119,10 +119,10 @@ crx_status frame_capture(crx_frame* F) {
         e = hipMemcpyAsync(F->resolveMeta.ptr, F->stageResolve, F->resolveBytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && F->clearBytes)
         e = hipMemcpyAsync(F->clearMeta.ptr, F->stageClear, F->clearBytes, hipMemcpyHostToDevice, s);
+    // the clear zeroes the frame's counters as well (else stage 1's reset does)
     if (e == hipSuccess && F->numClear)
-        e = launch_clear_views(static_cast<const ClearView*>(F->clearMeta.ptr), F->numClear, F->clearBlocks, nullptr, s);
-    if (e == hipSuccess) e = launch_reset_counters(F->ctr, s);
-    if (e == hipSuccess && F->plan.batches) e = enqueue_stages(ctx, F->plan, F->P, false, s);
+        e = launch_clear_views(static_cast<const ClearView*>(F->clearMeta.ptr), F->numClear, F->clearBlocks, F->ctr, s);
+    if (e == hipSuccess && F->plan.batches) e = enqueue_stages(ctx, F->plan, F->P, false, s, F->numClear > 0);
     if (e == hipSuccess)
         e = hipMemcpyAsync(F->ctrHost, F->ctr, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && F->settings.resolve)

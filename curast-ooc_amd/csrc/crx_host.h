@@ -132,13 +132,15 @@ struct DrawPlan {
 // metadata, for a resolve that reads the draw's records.  afterStages is
 // enqueued after the stages (and again after a redraw); with earlyReturn the
 // draw returns once the queue counters are back, while that work may still
-// run on the stream.
+// run on the stream.  countersZeroed: the caller's clear zeroed the counter
+// block in stream order before the draw.
 struct DrawExtras {
     const uint8_t* skip = nullptr;
     const std::function<crx_status()>* afterStages = nullptr;
     uint64_t listsKey = 0;
     bool prefix = false;
     bool earlyReturn = false;
+    bool countersZeroed = false;
 };
 
 }  // namespace crx
@@ -263,7 +265,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
                      bool instUnchanged = false, bool prefix = false);
 RasterParams plan_write(crx_context* ctx, const DrawPlan& plan, char* host, char* dev, bool writeInst = true);
 hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& plan, const RasterParams& P, bool viewsKernel,
-                          hipStream_t stream);
+                          hipStream_t stream, bool countersZeroed = false);
 crx_status ensure_queues(crx_context* ctx, uint32_t nt, uint32_t hg);
 
 // drawTrianglesVisbuffer for numViews >= 1 targets sharing one instance list
@@ -282,8 +284,10 @@ crx_status resolve_views_device(crx_context* ctx, const InstanceRec* instDev, co
 crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
                          hipStream_t stream);
+// Clears n buffers in one launch; a non-null ctr counter block is zeroed by
+// the same launch (*zeroed tells whether one ran).
 crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,
-                         hipStream_t stream);
+                         hipStream_t stream, uint32_t* ctr = nullptr, bool* zeroed = nullptr);
 uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews, const crx_render_settings& settings,
                      bool keepAll);
 void view_targets(const crx_view* views, uint32_t numViews, uint64_t* const* framebuffers,

@@ -8,7 +8,10 @@
 
 namespace crx {
 
-hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream);
+// Clears fb (and fb2, when given) to the clear pixel; a non-null ctr counter
+// block is zeroed too.
+hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream,
+                        uint64_t* fb2 = nullptr, uint32_t numPixels2 = 0);
 hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream);
 hipError_t launch_reset_counter_set(const CounterSet& c, hipStream_t stream);
 // device-built draw plan from caller-owned CMesh arrays (kernel-level entry points)

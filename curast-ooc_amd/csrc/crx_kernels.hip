@@ -357,13 +357,16 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
 
 }  // namespace
 
-// Clear the framebuffer to 0xFFFFFFF0'00000000 (depth +inf, ID 0), one pixel
-// per lane (a wave stores 512 contiguous bytes), and reset the draw counters
-// (graph-safe: no memset node).
-__global__ __launch_bounds__(256) void k_clear(uint64_t* __restrict__ fb, uint32_t numPixels,
-                                               uint32_t* __restrict__ ctr) {
+// Clear one or two buffers (a view's visibility and colour buffers) to
+// 0xFFFFFFF0'00000000 (depth +inf, ID 0), one pixel per lane (a wave stores
+// 512 contiguous bytes), and reset the draw counters (graph-safe: no memset
+// node).
+__global__ __launch_bounds__(256) void k_clear(uint64_t* __restrict__ fb, uint32_t numPixels, uint64_t* fb2,
+                                               uint32_t numPixels2, uint32_t* __restrict__ ctr) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < numPixels; i += stride) fb[i] = kClearPixel;
+    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = first; i < numPixels; i += stride) fb[i] = kClearPixel;
+    for (uint32_t i = first; i < numPixels2; i += stride) fb2[i] = kClearPixel;
     if (blockIdx.x == 0 && ctr && threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
 }
 
@@ -2081,12 +2084,13 @@ __global__ __launch_bounds__(256) void k_atomic_probe(uint64_t* __restrict__ buf
 // Launch wrappers used by the host glue.
 // ---------------------------------------------------------------------------
 
-hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream) {
-    uint32_t blocks = (numPixels + 255u) / 256u;
+hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream,
+                        uint64_t* fb2, uint32_t numPixels2) {
+    uint32_t blocks = (std::max(numPixels, numPixels2) + 255u) / 256u;
     const uint32_t cap = (uint32_t)numCUs * 16u;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, stream, fb, numPixels, ctr);
+    hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(256), 0, stream, fb, numPixels, fb2, fb2 ? numPixels2 : 0u, ctr);
     return hipGetLastError();
 }
 

@@ -487,7 +487,12 @@ void crx_render_settings_default(crx_render_settings* out);
 /* CuRast::draw for one view: host node cull, sort/group instances,
  * per-instance transforms and triangle prefix, clear, the three stages and
  * (optionally) resolve.  framebuffer / colorbuffer are caller-owned device
- * buffers of width*height uint64 (colorbuffer may be NULL when !resolve). */
+ * buffers of width*height uint64 (colorbuffer may be NULL when !resolve).
+ * Everything is enqueued on `stream`; the call returns once the stage queue
+ * sizes are read back, while the resolve may still run: order later work on
+ * the buffers after `stream`.  Frames that keep the node set, the instancing
+ * switch and the culled nodes reuse the instance records already on the
+ * device and upload only the per-view transforms. */
 crx_status crx_render(crx_context* ctx, crx_scene* scene, const crx_view* view,
                       const crx_render_settings* settings,
                       uint64_t* framebuffer, uint64_t* colorbuffer, void* stream,
