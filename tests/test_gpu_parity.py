@@ -346,6 +346,44 @@ def test_scene_frames_keep_and_rebuild_lists(ctx):
     scene.close()
 
 
+def test_scene_frames_on_two_streams(ctx):
+    """Two frames on two streams before either is read back: crx_render
+    returns while its resolve may still run, and the next frame's draw on the
+    other stream rewrites the context's metadata (another culled set) only
+    after that resolve.  Both frames equal the oracle's."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")   # the runtime libcrx already loaded (same soname)
+    streams = []
+    for _ in range(2):
+        h = C.c_void_p()
+        assert hip.hipStreamCreate(C.byref(h)) == 0
+        streams.append(h)
+    scene, meshes, hosts, worlds, order = _two_mesh_scene(ctx)
+    W, H = 640, 360
+    proj = wl.projection(W, H)
+    view_a, _ = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
+    view_b = crx.look_at((0.0, 0.0, 0.0), (30.0, 0.0, 5.0), (0.0, 1.0, 0.0))
+    bufs = [(ctx.alloc(8 * W * H), ctx.alloc(8 * W * H)) for _ in range(2)]
+    settings = crx.render_settings()
+    for rnd in range(3):
+        print(f"[case] two streams, round {rnd}", flush=True)
+        views = [crx.make_view(view_a, proj, W, H), crx.make_view(view_b, proj, W, H)]
+        for k in range(2):
+            scene.render(views[k], bufs[k][0], bufs[k][1], settings, streams[k])
+        for k, view in enumerate((view_a, view_b)):
+            got_fb = bufs[k][0].download(np.uint64, W * H, streams[k])
+            got_cb = bufs[k][1].download(np.uint64, W * H, streams[k])
+            fbw, cbw, nv = _oracle_scene_frame(meshes, hosts, worlds, order, views[k], view, proj, W, H, settings)
+            assert nv > 0
+            assert np.array_equal(got_fb, fbw), f"stream {k}: " + describe_mismatch(got_fb, fbw, W)
+            assert np.array_equal(got_cb, cbw), f"stream {k}: " + describe_mismatch(got_cb, cbw, W)
+    ctx.synchronize()
+    for h in streams:
+        assert hip.hipStreamSynchronize(h) == 0
+        assert hip.hipStreamDestroy(h) == 0
+    scene.close()
+
+
 @pytest.mark.parametrize("cull_order,fast_tiny,order,block,per_lane", [
     (0, 0, 0, 256, 1), (0, 1, 1, 512, 2), (1, 0, 2, 128, 2), (1, 1, 1, 512, 1), (2, 1, 0, 256, 2),
     (2, 0, 1, 512, 1), (1, 1, 1, 256, 2), (1, 1, 0, 128, 1), (1, 1, 2, 256, 4), (0, 1, 1, 512, 4),
