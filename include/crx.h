@@ -502,7 +502,8 @@ crx_status crx_render(crx_context* ctx, crx_scene* scene, const crx_view* view,
  * node cull keeps a node that is visible in ANY view, so all views share one
  * instance list (and one triangle-ID numbering); each view's buffers equal a
  * single-view draw of that list.  framebuffers[v] / colorbuffers[v] are
- * device buffers of views[v].width * views[v].height uint64. */
+ * device buffers of views[v].width * views[v].height uint64.  Returns as
+ * crx_render does: the resolve may still run on `stream`. */
 crx_status crx_render_views(crx_context* ctx, crx_scene* scene, const crx_view* views, uint32_t numViews,
                             const crx_render_settings* settings,
                             uint64_t* const* framebuffers, uint64_t* const* colorbuffers, void* stream,
