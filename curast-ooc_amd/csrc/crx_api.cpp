@@ -270,7 +270,10 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     bool manyInstances = false;
     for (uint32_t k = 0; instanced && k < D.numItems; k++)
         manyInstances = manyInstances || list[k].instances.count >= (int32_t)kBoxInstances;
-    const S1Schedule sched = s1_schedule(tu, manyInstances);
+    // the draw's stage-1 kernel is chosen by its first mesh, as the reference
+    // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
+    D.pu16 = D.numItems > 0 && list[0].compressed;
+    const S1Schedule sched = s1_schedule(tu, manyInstances, D.pu16);
     D.block = tu.stage1_block;
     D.trisPerLane = sched.trisPerLane;
     D.batchShift = 0;   // log2(triangles per batch)
@@ -278,9 +281,6 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     const uint64_t batchTris = 1ull << D.batchShift;
 
     D.items.resize(D.numItems);
-    // the draw's stage-1 kernel is chosen by its first mesh, as the reference
-    // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
-    D.pu16 = D.numItems > 0 && list[0].compressed;
     for (uint32_t k = 0; k < D.numItems; k++) {
         const crx_cmesh& m = list[k];
         CRX_TRY(check_mesh(m, instanced ? "meshes" : "instances", k));

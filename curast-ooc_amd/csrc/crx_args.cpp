@@ -82,7 +82,7 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* a, uint32_
     bind_queues(P, a);
     // instance counts live in device memory: an instanced draw takes the
     // many-instances choice
-    const S1Schedule sched = s1_schedule(tu, instanced);
+    const S1Schedule sched = s1_schedule(tu, instanced, pu16);
     const uint32_t perLane = sched.trisPerLane;
     uint32_t batchShift = 0;
     while ((1u << batchShift) < tu.stage1_block * perLane) batchShift++;

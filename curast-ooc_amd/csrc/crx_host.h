@@ -243,14 +243,18 @@ void set_counter_block(RasterParams& P, uint32_t* block);
 // that repeats a mesh over many instances takes one triangle per lane (its
 // sweep keeps instance lists per wave and runs at higher occupancy) and 64
 // waves per CU (its batches differ widely in cost, so more workgroups than
-// fit at once let the dispatcher even out the load); other draws take two
-// triangles per lane and the tuning's grid.
+// fit at once let the dispatcher even out the load); an IX_PU16 draw takes
+// two triangles per lane and 64 waves per CU (its index and vertex decode
+// waits on LDS and loads, and more waves hide it: 1.89 ms against 2.10 ms at
+// 32 on the 400 M-triangle heightfield); other draws take two triangles per
+// lane and the tuning's grid.
 struct S1Schedule {
     uint32_t trisPerLane, wavesPerCU;
 };
-inline S1Schedule s1_schedule(const crx_tuning& t, bool manyInstances) {
+inline S1Schedule s1_schedule(const crx_tuning& t, bool manyInstances, bool pu16) {
     if (t.stage1_tris_per_lane) return {t.stage1_tris_per_lane, t.stage1_waves_per_cu};
-    return manyInstances ? S1Schedule{1u, 64u} : S1Schedule{2u, t.stage1_waves_per_cu};
+    if (manyInstances) return S1Schedule{1u, 64u};
+    return pu16 ? S1Schedule{2u, 64u} : S1Schedule{2u, t.stage1_waves_per_cu};
 }
 crx_status check_tuning(const crx_tuning& t);
 crx_status check_target(const crx_render_target* t);

@@ -195,7 +195,8 @@ typedef struct crx_tuning {
                                      is stage1_block * this consecutive triangles of one item);
                                      0 = per draw: a draw with a mesh drawn over >= 4 instances
                                      (every instanced crx_raster_stage1 draw) takes 1 and a grid
-                                     of 64 waves per CU, others take 2 and stage1_waves_per_cu */
+                                     of 64 waves per CU, an IX_PU16 draw 2 and 64 waves per CU,
+                                     others 2 and stage1_waves_per_cu */
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */
