@@ -419,6 +419,10 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
     return hipSuccess;
 }
 
+// Instance records (and prefix) from which a draw that finds them already in
+// the metadata buffer copies the rest in two pieces rather than all in one.
+constexpr size_t kSplitUploadBytes = 64 * 1024;
+
 // Orders `stream` after the last draw's work that may still read the
 // context's metadata buffers (a resolve enqueued before an early return on
 // another stream).
@@ -457,9 +461,11 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         char* meta = static_cast<char*>(ctx->drawMeta.ptr);
         char* image = static_cast<char*>(ctx->drawStaging.ptr);
         // records (and prefix) already in the buffer at the same place: copy
-        // only the draw items and the per-view records and transforms
+        // only the draw items and the per-view records and transforms (two
+        // copies, so only when the records skipped outweigh a copy launch)
         const bool held = sameLists && ctx->metaListsPtr == meta && ctx->metaListsBegin == D.offInst &&
-                          ctx->metaListsEnd == D.offViews && ctx->metaListsPrefix == D.prefix;
+                          ctx->metaListsEnd == D.offViews && ctx->metaListsPrefix == D.prefix &&
+                          D.offViews - D.offInst >= kSplitUploadBytes;
         P = plan_write(ctx, D, image, meta, !held);
         if (held) {
             CRX_HIP(hipMemcpyAsync(meta, image, D.offInst, hipMemcpyHostToDevice, stream));

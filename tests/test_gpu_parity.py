@@ -288,14 +288,17 @@ def test_scene_api_matches_oracle_frame(ctx):
     scene.close()
 
 
-def test_scene_frames_keep_and_rebuild_lists(ctx):
+@pytest.mark.parametrize("nodes", [30, 400])
+def test_scene_frames_keep_and_rebuild_lists(ctx, nodes):
     """A run of crx_render frames over one scene: the per-node lists and the
     uploaded instance records are kept while the node set, the instancing
     switch and the culled set stay the same, and rebuilt when any changes (a
     view that culls other nodes, a moved node, a frame that sees nothing, the
     other instancing mode, a frame without resolve, a C-ABI draw in between).
-    Every frame equals the oracle's, colour buffer included."""
-    scene, meshes, hosts, worlds, order = _two_mesh_scene(ctx)
+    Every frame equals the oracle's, colour buffer included.  400 nodes make
+    the kept records large enough that a repeated frame copies only the
+    rest of the metadata."""
+    scene, meshes, hosts, worlds, order = _two_mesh_scene(ctx, nodes)
     W, H = 640, 360
     proj = wl.projection(W, H)
     view_a, _ = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
@@ -307,7 +310,7 @@ def test_scene_frames_keep_and_rebuild_lists(ctx):
     seen = set()
 
     def frame(name, view, settings=plain):
-        print(f"[case] frame {name}", flush=True)
+        print(f"[case] {nodes} nodes, frame {name}", flush=True)
         v = crx.make_view(view, proj, W, H)
         scene.render(v, fb, cb if settings.resolve else None, settings)
         got_fb = fb.download(np.uint64, W * H)
