@@ -463,7 +463,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         // records (and prefix) already in the buffer at the same place: copy
         // only the draw items and the per-view records and transforms (two
         // copies, so only when the records skipped outweigh a copy launch)
-        const bool held = sameLists && ctx->metaListsPtr == meta && ctx->metaListsBegin == D.offInst &&
+        const bool held = sameLists && ctx->metaListsGen == ctx->drawMeta.generation && ctx->metaListsBegin == D.offInst &&
                           ctx->metaListsEnd == D.offViews && ctx->metaListsPrefix == D.prefix &&
                           D.offViews - D.offInst >= kSplitUploadBytes;
         P = plan_write(ctx, D, image, meta, !held);
@@ -476,7 +476,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
         }
         CRX_TRY(ctx->drawStaging.mark(stream));
         ctx->metaListsKey = ex.listsKey;
-        ctx->metaListsPtr = meta;
+        ctx->metaListsGen = ctx->drawMeta.generation;
         ctx->metaListsBegin = D.offInst;
         ctx->metaListsEnd = D.offViews;
         ctx->metaListsPrefix = D.prefix;
@@ -602,7 +602,7 @@ crx_status resolve_blocks(const crx_render_target* targets, uint32_t numViews, R
 
 crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                               const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
-                              char* dev, char* out, size_t* offViews, uint32_t* blocks) {
+                              char* dev, char* out, size_t* offViews, uint32_t* blocks, bool ownedOnly = false) {
     CRX_TRY(check_resolve(targets, numViews, attribute));
     if (numInstances > 0 && (!instances || !prefix)) return fail(CRX_INVALID_ARG, "instances / prefix is NULL");
     size_t offPrefix = 0;
@@ -616,8 +616,10 @@ crx_status resolve_image_into(const crx_cmesh* instances, uint32_t numInstances,
     const InstanceRec* instDev = reinterpret_cast<const InstanceRec*>(dev);
     const uint64_t* prefixDev = reinterpret_cast<const uint64_t*>(dev + offPrefix);
     ResolveParams* rv = reinterpret_cast<ResolveParams*>(out + *offViews);
-    for (uint32_t v = 0; v < numViews; v++)
+    for (uint32_t v = 0; v < numViews; v++) {
         rv[v] = make_resolve_params(targets[v], instDev, prefixDev, numInstances, attribute);
+        rv[v].ownedOnly = ownedOnly ? 1u : 0u;
+    }
     return resolve_blocks(targets, numViews, rv, blocks);
 }
 
@@ -634,7 +636,7 @@ crx_status resolve_image(const crx_cmesh* instances, uint32_t numInstances, cons
 // numViews >= 1 targets sharing one instance list, in one launch.
 crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
-                         hipStream_t stream) {
+                         hipStream_t stream, bool ownedOnly) {
     if (!ctx) return fail(CRX_INVALID_ARG, "ctx is NULL");
     CRX_HIP(hipSetDevice(ctx->device));
     CRX_TRY(wait_meta_free(ctx, stream));
@@ -647,7 +649,7 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
     CRX_TRY(ctx->resolveStaging.acquire(bytes));
     char* image = static_cast<char*>(ctx->resolveStaging.ptr);
     CRX_TRY(resolve_image_into(instances, numInstances, prefix, targets, numViews, attribute, meta, image, &offViews,
-                               &blocks));
+                               &blocks, ownedOnly));
     const ResolveParams* rv = reinterpret_cast<const ResolveParams*>(image + offViews);
     if (numInstances || numViews > 1) {
         CRX_HIP(hipMemcpyAsync(meta, image, bytes, hipMemcpyHostToDevice, stream));
@@ -1070,8 +1072,8 @@ crx_status crx_context_destroy(crx_context* ctx) {
     ctx->drawMeta.release();
     ctx->resolveMeta.release();
     ctx->clearMeta.release();
-    ctx->argsStaging.release();
     ctx->argsMeta.release();
+    if (ctx->argsUse) (void)hipEventDestroy(ctx->argsUse);
     if (ctx->ctr) (void)hipFree(ctx->ctr);
     if (ctx->ctrHost) (void)hipHostFree(ctx->ctrHost);
     for (int i = 0; i < T_COUNT; i++)

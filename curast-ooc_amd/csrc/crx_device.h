@@ -142,6 +142,8 @@ struct ResolveParams {
     float viewI[16];           // inverse(view), GLM order, computed once per frame
     float origin[3];           // viewI * (0,0,0,1)
     uint32_t firstBlock;       // multi-view resolve: first 64x4 tile block of this view
+    uint32_t ownedOnly;        // 1: leave pixels whose triangle is not in the listed instances
+                               // (a resolve in passes over groups of instances)
 };
 
 // The five stage counters of the kernel-level entry points (RasterArgs:

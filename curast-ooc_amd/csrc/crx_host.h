@@ -35,8 +35,12 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t capacity = 0;
+    // bumped by every allocation: contents kept under an older generation
+    // are gone even when the new allocation has the same address
+    uint64_t generation = 0;
     crx_status reserve(size_t bytes) {
         if (bytes <= capacity) return CRX_OK;
+        generation++;
         if (ptr) {
             hipError_t e = hipFree(ptr);
             ptr = nullptr;
@@ -162,7 +166,7 @@ struct crx_context {
     // instance records (and prefix) that drawMeta holds: the list key they
     // were uploaded under (0 = none), the buffer and their byte range
     uint64_t metaListsKey = 0;
-    void* metaListsPtr = nullptr;
+    uint64_t metaListsGen = 0;        // drawMeta.generation the records were written under
     size_t metaListsBegin = 0, metaListsEnd = 0;
     bool metaListsPrefix = false;
     hipEvent_t countersBack = nullptr;   // the counter readback of the last draw
@@ -173,19 +177,13 @@ struct crx_context {
     hipEvent_t ev[crx::T_COUNT] = {};
     bool recorded[crx::T_COUNT] = {};
     uint64_t meshSeq = 0;
-    // kernel-level entry points: the device-built plan of the last
-    // crx_raster_stage1 call, read by crx_raster_stage2/3 of the same args
+    // kernel-level entry points: the records buffer every crx_raster_stage*
+    // call rebuilds its records in, and the event after the last call's
+    // launches (a call on another stream waits for it)
     crx::DeviceBuffer argsMeta;
-    crx::Staging argsStaging;
-    struct {
-        bool valid = false;
-        const crx_cmesh* instances = nullptr;
-        const crx_mat4* transforms = nullptr;
-        uint32_t numInstances = 0;
-        uint64_t* framebuffer = nullptr;
-        crx::RasterParams P{};
-        uint32_t g2 = 0, g3 = 0;
-    } args;
+    hipEvent_t argsUse = nullptr;
+    hipStream_t argsUseStream = nullptr;
+    bool argsUsePending = false;
 };
 
 struct crx_mesh {
@@ -285,9 +283,11 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
 crx_status resolve_views_device(crx_context* ctx, const InstanceRec* instDev, const uint64_t* prefixDev,
                                 uint32_t numInstances, const crx_render_target* targets, uint32_t numViews,
                                 crx_display_attribute attribute, hipStream_t stream);
+// ownedOnly: pixels whose triangle lies outside the listed instances are
+// left as they are (a resolve in passes over groups of instances).
 crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t numInstances, const uint64_t* prefix,
                          const crx_render_target* targets, uint32_t numViews, crx_display_attribute attribute,
-                         hipStream_t stream);
+                         hipStream_t stream, bool ownedOnly = false);
 // Clears n buffers in one launch; a non-null ctr counter block is zeroed by
 // the same launch (*zeroed tells whether one ran).
 crx_status clear_buffers(crx_context* ctx, uint64_t* const* buffers, const uint32_t* numPixels, uint32_t n,

@@ -19,6 +19,7 @@ hipError_t launch_plan(const crx_cmesh* list, uint32_t numItems, const crx_cmesh
                        const crx_mat4* transforms, uint32_t numInstances, int instanced, int pu16,
                        uint32_t batchShift, DrawItem* items, InstanceRec* inst, Transform* xf, uint32_t* batchCount,
                        hipStream_t stream);
+hipError_t launch_store_view(const ViewParams& v, ViewParams* dst, hipStream_t stream);
 // trisPerLane: 1, 2 or 4 (P.batchShift = log2(block * trisPerLane));
 // pu16: the draw's meshes are IX_PU16-compressed
 // instanceBoxes: the draw repeats meshes over many instances (wave boxes

@@ -439,6 +439,12 @@ __global__ __launch_bounds__(1024) void k_plan_items(const crx_cmesh* __restrict
     if (t == 1023u) *batchCount = (uint32_t)min(sums[1023], 0xFFFFFFFFull);
 }
 
+// The view record of a kernel-level call, stored from the kernel argument
+// (no host staging buffer, so the call needs no page-locked memory).
+__global__ __launch_bounds__(64) void k_store_view(ViewParams v, ViewParams* __restrict__ dst) {
+    if (threadIdx.x == 0) *dst = v;
+}
+
 // One lane per instance: its InstanceRec (make_instance on the host) and its
 // Transform with the mirror flag det(worldView[0..2]) < 0 computed in the
 // same float operations (triangles_visbuffer.cu:179-183).
@@ -1910,13 +1916,19 @@ __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int
         // a wave whose covered pixels share one instance reads its record
         // with scalar loads; otherwise every lane loads its own
         const uint32_t f0 = (uint32_t)__builtin_amdgcn_readfirstlane(found);
+        // (ownedOnly: a pixel whose triangle lies outside the listed
+        // instances belongs to another pass and is left as it is)
         if (__ballot((uint32_t)found != f0) == 0ull) {
             const InstanceRec mesh = kconst(R.inst)[f0];
+            if (R.ownedOnly && total - mesh.cumm >= (uint64_t)mesh.numTriangles) return;
             color = shade_pixel(R, mesh, total, x, y);
         } else {
             const InstanceRec mesh = R.inst[found];
+            if (R.ownedOnly && total - mesh.cumm >= (uint64_t)mesh.numTriangles) return;
             color = shade_pixel(R, mesh, total, x, y);
         }
+    } else if (R.ownedOnly && !__builtin_isinf(depth)) {
+        return;
     }
     if (depth != __builtin_inff()) {
         R.cb[pixelID] = ((uint64_t)__builtin_bit_cast(uint32_t, depth) << 32) | (uint64_t)color;
@@ -2101,6 +2113,11 @@ hipError_t launch_reset_counters(uint32_t* ctr, hipStream_t stream) {
 
 hipError_t launch_reset_counter_set(const CounterSet& c, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset_counter_set, dim3(1), dim3(64), 0, stream, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_view(const ViewParams& v, ViewParams* dst, hipStream_t stream) {
+    hipLaunchKernelGGL(k_store_view, dim3(1), dim3(64), 0, stream, v, dst);
     return hipGetLastError();
 }
 

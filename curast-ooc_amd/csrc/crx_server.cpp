@@ -13,7 +13,16 @@
 // slot on the upload stream, and the drawing lane's stream waits for that
 // copy's event (the async scheme of the reference's point-cloud octree,
 // structureUpdate.cpp:256 and SNCOctree.h:102-119, here for triangle nodes).
-// Nodes in use by a batch in flight are pinned and never evicted.
+// Nodes in use by a batch in flight are pinned and never evicted, and a
+// slot is refilled only after the kernels that read its previous node: the
+// lane that releases a node records its stream's release event, and the
+// upload into that slot waits for the release events of the lanes that read
+// it.
+//
+// A view that sees more nodes than the cache holds is drawn in passes over
+// groups of nodes into the same visibility buffer (the 64-bit atomic minimum
+// makes the result independent of the grouping), then resolved in passes
+// that each shade only the pixels whose triangle lies in the group's nodes.
 //
 // Triangle ids are global: node i's ids start at the triangle count of the
 // nodes before it, and a view's per-node culling (SNTriangles.h:29-105) does
@@ -121,6 +130,7 @@ struct crx_server {
     size_t slotBytes = 0;
     uint32_t numSlots = 0;
     std::vector<int> slotNode;      // node in each slot, -1 = free
+    std::vector<uint32_t> slotReaders;   // lanes (bit mask) whose reads of the slot's node may be in flight
     hipStream_t upload = nullptr;
     std::mutex cacheMu;
     std::condition_variable cacheCv;
@@ -139,6 +149,8 @@ struct crx_server {
     struct Lane {
         crx_context* ctx = nullptr;
         hipStream_t stream = nullptr;
+        hipEvent_t released = nullptr;   // recorded on `stream` whenever the lane releases nodes
+        uint32_t index = 0;
         std::thread thread;
     };
     std::vector<Lane> lanes;
@@ -177,6 +189,11 @@ crx_status upload_node(crx_server* s, uint32_t id, int slot) {
     float* pos;
     uint32_t *idx, *col;
     slot_pointers(s, n, &pos, &idx, &col);
+    // the copy waits for the kernels that read the slot's previous node
+    for (uint32_t l = 0; l < s->lanes.size(); l++) {
+        if (s->slotReaders[slot] & (1u << l)) CRX_HIP(hipStreamWaitEvent(s->upload, s->lanes[l].released, 0));
+    }
+    s->slotReaders[slot] = 0;
     if (n.numVertices) CRX_HIP(hipMemcpyAsync(pos, n.hpos, 12ull * n.numVertices, hipMemcpyHostToDevice, s->upload));
     if (n.hidx && n.numTriangles)
         CRX_HIP(hipMemcpyAsync(idx, n.hidx, 12ull * n.numTriangles, hipMemcpyHostToDevice, s->upload));
@@ -243,19 +260,74 @@ crx_status acquire_nodes(crx_server* s, const std::vector<uint32_t>& need, uint3
     return CRX_OK;
 }
 
-void release_nodes(crx_server* s, const std::vector<uint32_t>& need) {
+// Unpins `need` after the lane has enqueued every kernel that reads those
+// nodes: the lane's release event marks that point of its stream, and an
+// upload into one of their slots waits for it.  Runs on every path, errors
+// included.
+crx_status release_nodes(crx_server* s, crx_server::Lane& lane, const std::vector<uint32_t>& need) {
+    crx_status st = CRX_OK;
     {
         std::lock_guard<std::mutex> lk(s->cacheMu);
-        for (uint32_t id : need) s->nodes[id].pins--;
+        hipError_t e = hipEventRecord(lane.released, lane.stream);
+        if (e != hipSuccess) {
+            // without the event the slots cannot be ordered after the reads
+            st = fail(CRX_HIP_ERROR, std::string("hipEventRecord: ") + hipGetErrorString(e));
+            (void)hipStreamSynchronize(lane.stream);
+        }
+        for (uint32_t id : need) {
+            Node& n = s->nodes[id];
+            n.pins--;
+            if (e == hipSuccess) s->slotReaders[n.slot] |= 1u << lane.index;
+        }
     }
     s->cacheCv.notify_all();
+    return st;
 }
 
-// Draws one batch of requests on a lane.  Returns the status for all of them.
-// Draws a group of requests whose nodes fit the cache together.
+// CMesh of node `id` in its slot (scene-wide triangle ids), listed at `k`.
+crx_cmesh node_cmesh(crx_server* s, uint32_t id, uint32_t k) {
+    const Node& n = s->nodes[id];
+    float* pos;
+    uint32_t *idx, *col;
+    slot_pointers(s, n, &pos, &idx, &col);
+    crx_cmesh m;
+    std::memset(&m, 0, sizeof(m));
+    m.numTriangles = n.numTriangles;
+    m.indices = n.hidx ? idx : nullptr;
+    m.positions = reinterpret_cast<crx_vec3*>(pos);
+    m.colors = n.hcol ? col : nullptr;
+    m.cummulativeTriangleCount = n.cumm;
+    m.aabb = n.aabb;
+    m.instances.offset = (int32_t)k;
+    m.instances.count = 1;
+    m.world = n.world;
+    m.id = (int32_t)id;
+    m.numVertices = n.numVertices;
+    m.isLoaded = 1;
+    m.flipTriangles = mirror_flag(n.world) ? 1 : 0;
+    return m;
+}
+
+// CMesh records of the pinned nodes `ids`, and `stream` ordered after their uploads.
+crx_status pinned_cmeshes(crx_server* s, const std::vector<uint32_t>& ids, hipStream_t stream,
+                          std::vector<crx_cmesh>* inst) {
+    std::lock_guard<std::mutex> lk(s->cacheMu);   // slot pointers are stable while pinned
+    inst->resize(ids.size());
+    for (uint32_t k = 0; k < ids.size(); k++) {
+        const Node& n = s->nodes[ids[k]];
+        (*inst)[k] = node_cmesh(s, ids[k], k);
+        if (n.ready) CRX_HIP(hipStreamWaitEvent(stream, n.ready, 0));
+    }
+    return CRX_OK;
+}
+
+// Draws a group of requests on a lane: one pass when the nodes the group
+// sees fit the cache, else (a single view that sees more nodes than the
+// cache holds) passes over groups of nodes.  Returns the status for all of them.
 crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<Request*>& batch, uint32_t* uploads) {
     const uint32_t V = (uint32_t)batch.size();
     const uint32_t N = (uint32_t)s->nodes.size();
+    *uploads = 0;
     // per-view node cull (SNTriangles::update for each view)
     std::vector<uint8_t> vis((size_t)V * N, 0);
     std::vector<uint8_t> any(N, 0);
@@ -272,106 +344,136 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
     std::vector<uint32_t> need;
     for (uint32_t i = 0; i < N; i++)
         if (any[i]) need.push_back(i);
-    CRX_TRY(acquire_nodes(s, need, uploads));
 
-    crx_status st = CRX_OK;
-    do {
-        const uint32_t U = (uint32_t)need.size();
-        std::vector<crx_cmesh> inst(U);
-        {
-            std::lock_guard<std::mutex> lk(s->cacheMu);   // slot pointers are stable while pinned
-            for (uint32_t k = 0; k < U; k++) {
-                const Node& n = s->nodes[need[k]];
-                float* pos;
-                uint32_t *idx, *col;
-                slot_pointers(s, n, &pos, &idx, &col);
-                crx_cmesh& m = inst[k];
-                std::memset(&m, 0, sizeof(m));
-                m.numTriangles = n.numTriangles;
-                m.indices = n.hidx ? idx : nullptr;
-                m.positions = reinterpret_cast<crx_vec3*>(pos);
-                m.colors = n.hcol ? col : nullptr;
-                m.cummulativeTriangleCount = n.cumm;
-                m.aabb = n.aabb;
-                m.instances.offset = (int32_t)k;
-                m.instances.count = 1;
-                m.world = n.world;
-                m.id = (int32_t)need[k];
-                m.numVertices = n.numVertices;
-                m.isLoaded = 1;
-                m.flipTriangles = mirror_flag(n.world) ? 1 : 0;
-                // the lane's stream waits for the node's upload
-                if (n.ready) {
-                    hipError_t e = hipStreamWaitEvent(lane.stream, n.ready, 0);
-                    if (e != hipSuccess) { st = fail(CRX_HIP_ERROR, std::string("hipStreamWaitEvent: ") + hipGetErrorString(e)); break; }
-                }
-            }
+    std::vector<crx_render_target> targets(V);
+    std::vector<uint64_t*> bufs;
+    std::vector<uint32_t> px;
+    for (uint32_t v = 0; v < V; v++) {
+        const crx_view& cv = batch[v]->view;
+        crx_render_target& t = targets[v];
+        std::memset(&t, 0, sizeof(t));
+        const size_t pixels = (size_t)cv.width * (size_t)cv.height;
+        t.framebuffer = reinterpret_cast<uint64_t*>(s->frames + batch[v]->offset);
+        t.colorbuffer = s->cfg.resolve ? reinterpret_cast<uint64_t*>(s->frames + batch[v]->offset +
+                                                                     align_up(8 * pixels, kAlign))
+                                       : nullptr;
+        t.width = cv.width;
+        t.height = cv.height;
+        t.view = cv.view;
+        t.viewI = mat_inverse(cv.view);
+        t.proj = cv.proj;
+        t.f = cv.proj.m[5];
+        t.aspect = (float)cv.width / (float)cv.height;
+        bufs.push_back(t.framebuffer);
+        px.push_back((uint32_t)pixels);
+        if (t.colorbuffer) {
+            bufs.push_back(t.colorbuffer);
+            px.push_back((uint32_t)pixels);
         }
-        if (st != CRX_OK) break;
-        std::vector<crx_mat4> xf((size_t)V * U);
-        std::vector<uint8_t> skip((size_t)V * U, 0);
-        std::vector<crx_render_target> targets(V);
-        std::vector<uint64_t*> bufs;
-        std::vector<uint32_t> px;
+    }
+    // view-major transforms and per-view node culls of the nodes `ids`
+    std::vector<crx_mat4> xf;
+    std::vector<uint8_t> skip;
+    const auto lists = [&](const std::vector<uint32_t>& ids) -> bool {
+        const uint32_t U = (uint32_t)ids.size();
+        xf.assign((size_t)V * U, crx_mat4{});
+        skip.assign((size_t)V * U, 0);
         bool anySkip = false;
         for (uint32_t v = 0; v < V; v++) {
-            const crx_view& cv = batch[v]->view;
             for (uint32_t k = 0; k < U; k++) {
-                xf[(size_t)v * U + k] = mat_mul(cv.view, s->nodes[need[k]].world);
-                if (!vis[(size_t)v * N + need[k]]) {
+                xf[(size_t)v * U + k] = mat_mul(batch[v]->view.view, s->nodes[ids[k]].world);
+                if (!vis[(size_t)v * N + ids[k]]) {
                     skip[(size_t)v * U + k] = 1;
                     anySkip = true;
                 }
             }
-            crx_render_target& t = targets[v];
-            std::memset(&t, 0, sizeof(t));
-            const size_t pixels = (size_t)cv.width * (size_t)cv.height;
-            t.framebuffer = reinterpret_cast<uint64_t*>(s->frames + batch[v]->offset);
-            t.colorbuffer = s->cfg.resolve ? reinterpret_cast<uint64_t*>(s->frames + batch[v]->offset +
-                                                                         align_up(8 * pixels, kAlign))
-                                           : nullptr;
-            t.width = cv.width;
-            t.height = cv.height;
-            t.view = cv.view;
-            t.viewI = mat_inverse(cv.view);
-            t.proj = cv.proj;
-            t.f = cv.proj.m[5];
-            t.aspect = (float)cv.width / (float)cv.height;
-            bufs.push_back(t.framebuffer);
-            px.push_back((uint32_t)pixels);
-            if (t.colorbuffer) {
-                bufs.push_back(t.colorbuffer);
-                px.push_back((uint32_t)pixels);
+        }
+        return anySkip && V > 1;
+    };
+    std::vector<crx_cmesh> inst;
+
+    if (need.size() <= s->numSlots) {
+        uint32_t up = 0;
+        CRX_TRY(acquire_nodes(s, need, &up));
+        *uploads += up;
+        crx_status st = CRX_OK;
+        do {
+            st = pinned_cmeshes(s, need, lane.stream, &inst);
+            if (st != CRX_OK) break;
+            const uint32_t U = (uint32_t)need.size();
+            const bool useSkip = lists(need);
+            st = clear_buffers(lane.ctx, bufs.data(), px.data(), (uint32_t)bufs.size(), lane.stream);
+            if (st != CRX_OK) break;
+            crx_draw_stats ds;
+            // the resolve reads the instance records and their triangle prefix
+            // (cummulativeTriangleCount) from the draw's metadata
+            const std::function<crx_status()> resolve = [&]() {
+                const DrawPlan& D = lane.ctx->plan;
+                const char* meta = static_cast<const char*>(lane.ctx->drawMeta.ptr);
+                return resolve_views_device(lane.ctx, U ? reinterpret_cast<const InstanceRec*>(meta + D.offInst) : nullptr,
+                                            U ? reinterpret_cast<const uint64_t*>(meta + D.offPrefix) : nullptr, U,
+                                            targets.data(), V, CRX_DISPLAY_VERTEX_COLORS, lane.stream);
+            };
+            DrawExtras ex;
+            ex.skip = useSkip ? skip.data() : nullptr;
+            ex.afterStages = s->cfg.resolve ? &resolve : nullptr;
+            ex.prefix = s->cfg.resolve != 0;
+            st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream, &ds,
+                            ex);
+        } while (false);
+        const crx_status rs = release_nodes(s, lane, need);
+        if (st == CRX_OK) st = rs;
+        if (st == CRX_OK) {
+            hipError_t e = hipStreamSynchronize(lane.stream);
+            if (e != hipSuccess) st = fail(CRX_HIP_ERROR, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        }
+        return st;
+    }
+
+    // Passes over groups of at most a lane's share of the cache.
+    const size_t group = std::max<size_t>(1, s->numSlots / std::max<size_t>(1, s->lanes.size()));
+    std::vector<std::vector<uint32_t>> groups;
+    for (size_t g0 = 0; g0 < need.size(); g0 += group)
+        groups.emplace_back(need.begin() + g0, need.begin() + std::min(need.size(), g0 + group));
+    CRX_TRY(clear_buffers(lane.ctx, bufs.data(), px.data(), (uint32_t)bufs.size(), lane.stream));
+    const auto pass = [&](const std::vector<uint32_t>& ids, bool resolvePass) -> crx_status {
+        uint32_t up = 0;
+        CRX_TRY(acquire_nodes(s, ids, &up));
+        *uploads += up;
+        crx_status st = pinned_cmeshes(s, ids, lane.stream, &inst);
+        if (st == CRX_OK) {
+            const uint32_t U = (uint32_t)ids.size();
+            if (!resolvePass) {
+                const bool useSkip = lists(ids);
+                DrawExtras ex;
+                ex.skip = useSkip ? skip.data() : nullptr;
+                crx_draw_stats ds;
+                st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream,
+                                &ds, ex);
+            } else {
+                std::vector<uint64_t> prefix(U);
+                for (uint32_t k = 0; k < U; k++) prefix[k] = s->nodes[ids[k]].cumm;
+                st = resolve_views(lane.ctx, inst.data(), U, prefix.data(), targets.data(), V,
+                                   CRX_DISPLAY_VERTEX_COLORS, lane.stream, true);
             }
         }
-        st = clear_buffers(lane.ctx, bufs.data(), px.data(), (uint32_t)bufs.size(), lane.stream);
-        if (st != CRX_OK) break;
-        crx_draw_stats ds;
-        // the resolve reads the instance records and their triangle prefix
-        // (cummulativeTriangleCount) from the draw's metadata
-        const std::function<crx_status()> resolve = [&]() {
-            const DrawPlan& D = lane.ctx->plan;
-            const char* meta = static_cast<const char*>(lane.ctx->drawMeta.ptr);
-            return resolve_views_device(lane.ctx, U ? reinterpret_cast<const InstanceRec*>(meta + D.offInst) : nullptr,
-                                        U ? reinterpret_cast<const uint64_t*>(meta + D.offPrefix) : nullptr, U,
-                                        targets.data(), V, CRX_DISPLAY_VERTEX_COLORS, lane.stream);
-        };
-        DrawExtras ex;
-        ex.skip = (anySkip && V > 1) ? skip.data() : nullptr;
-        ex.afterStages = s->cfg.resolve ? &resolve : nullptr;
-        ex.prefix = s->cfg.resolve != 0;
-        st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream, &ds, ex);
-        if (st != CRX_OK) break;
-        hipError_t e = hipStreamSynchronize(lane.stream);
-        if (e != hipSuccess) st = fail(CRX_HIP_ERROR, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-    } while (false);
-    release_nodes(s, need);
-    return st;
+        const crx_status rs = release_nodes(s, lane, ids);
+        return st != CRX_OK ? st : rs;
+    };
+    for (const auto& ids : groups) CRX_TRY(pass(ids, false));
+    // every pixel's triangle is final: shade each group's pixels, the last
+    // drawn group (still resident) first
+    if (s->cfg.resolve) {
+        for (size_t g = groups.size(); g-- > 0;) CRX_TRY(pass(groups[g], true));
+    }
+    CRX_HIP(hipStreamSynchronize(lane.stream));
+    return CRX_OK;
 }
 
 // Draws one batch: views are grouped greedily (in order) so that the nodes
 // each group sees fit the cache together; a view that alone sees more nodes
-// than the cache holds fails with CRX_REJECTED.  Sets every request's status.
+// than the cache holds is drawn on its own, in passes.  Sets every request's
+// status.
 void draw_batch(crx_server* s, crx_server::Lane& lane, const std::vector<Request*>& batch, uint32_t* uploads) {
     const uint32_t N = (uint32_t)s->nodes.size();
     *uploads = 0;
@@ -395,8 +497,9 @@ void draw_batch(crx_server* s, crx_server::Lane& lane, const std::vector<Request
         for (uint32_t i = 0; i < N; i++)
             if (box_in_frustum(planes, s->nodes[i].wbox)) mine.push_back(i);
         if (mine.size() > s->numSlots) {
-            r->status = fail(CRX_REJECTED, "a view sees " + std::to_string(mine.size()) +
-                                               " nodes but the node cache has " + std::to_string(s->numSlots) + " slots");
+            flush();
+            group.push_back(r);
+            flush();
             continue;
         }
         uint32_t extra = 0;
@@ -562,6 +665,7 @@ crx_status crx_server_start(crx_server* s) {
         if (e != hipSuccess) { s->cache = nullptr; return fail(CRX_OUT_OF_MEMORY, std::string("node cache: ") + hipGetErrorString(e)); }
     }
     s->slotNode.assign(s->numSlots, -1);
+    s->slotReaders.assign(s->numSlots, 0u);
     hipError_t e = hipMalloc(&s->frames, s->cfg.frame_bytes);
     if (e != hipSuccess) { s->frames = nullptr; return fail(CRX_OUT_OF_MEMORY, std::string("frame arena: ") + hipGetErrorString(e)); }
     s->arena.init(s->cfg.frame_bytes);
@@ -578,9 +682,12 @@ crx_status crx_server_start(crx_server* s) {
     }
     CRX_HIP(hipStreamSynchronize(s->upload));
     s->lanes.resize(s->cfg.lanes);
-    for (auto& lane : s->lanes) {
+    for (uint32_t l = 0; l < s->lanes.size(); l++) {
+        auto& lane = s->lanes[l];
+        lane.index = l;
         CRX_TRY(crx_context_create(s->ctx->device, &s->ctx->tuning, &lane.ctx));
         CRX_HIP(hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking));
+        CRX_HIP(hipEventCreateWithFlags(&lane.released, hipEventDisableTiming));
     }
     s->started = true;
     for (auto& lane : s->lanes) lane.thread = std::thread(lane_main, s, &lane);
@@ -600,6 +707,7 @@ crx_status crx_server_destroy(crx_server* s) {
     for (auto& lane : s->lanes) {
         if (lane.stream) (void)hipStreamSynchronize(lane.stream);
         if (lane.stream) (void)hipStreamDestroy(lane.stream);
+        if (lane.released) (void)hipEventDestroy(lane.released);
         if (lane.ctx) crx_context_destroy(lane.ctx);
     }
     if (s->upload) {

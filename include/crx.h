@@ -296,10 +296,12 @@ typedef struct crx_raster_args {
  * a counter above its capacity after the stage reports that. */
 crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* args, uint32_t flags, void* stream);
 /* kernel_stage2_drawMediumTriangles_* and kernel_stage3_drawHugeTriangles_*:
- * each consumes its queue on `stream`.  They use the draw records of the last
- * crx_raster_stage1 on this context, which must have had the same args (same
- * instances, transforms and framebuffer), with no other draw on the context
- * in between. */
+ * each consumes its queue on `stream`.  Like the reference's stage kernels
+ * they depend only on their args: each call builds the instance and view
+ * records it reads from the args' device arrays, so draws with different
+ * args may interleave on one context (each with its own counters and
+ * queues).  Calls on one context that use different streams are ordered on
+ * the device: a call waits for the launches of the call before it. */
 crx_status crx_raster_stage2(crx_context* ctx, const crx_raster_args* args, void* stream);
 crx_status crx_raster_stage3(crx_context* ctx, const crx_raster_args* args, void* stream);
 

@@ -64,6 +64,33 @@ def _fetch(srv, ticket, view):
     return fb, cb, r
 
 
+def _oracle_view(nodes, ids, v):
+    """The oracle's draw and resolve of the nodes `ids` for view v, with the
+    scene-wide triangle prefix."""
+    prefix = np.concatenate([[0], np.cumsum([nd["nt"] for nd in nodes])[:-1]]).astype(np.uint64)
+    inst = []
+    for j, i in enumerate(ids):
+        nd = nodes[i]
+        m = crx.CMesh()
+        m.numTriangles = nd["nt"]
+        m.positions = nd["pos"]
+        m.indices = nd["idx"]
+        m.colors = nd["col"]
+        m.cummulativeTriangleCount = int(prefix[i])
+        m.instances.offset = j
+        m.instances.count = 1
+        m.world = crx.identity()
+        m.id = i
+        m.numVertices = nd["nv"]
+        m.isLoaded = 1
+        inst.append(m)
+    xf = [crx.mat4_mul(v.view, crx.identity()) for _ in ids]
+    fb, _ = orc.draw(crx, inst, inst, xf, v.width, v.height, v.view, v.proj, False)
+    cb = orc.resolve(crx, inst, prefix[ids] if len(ids) else np.zeros(0, np.uint64), fb, v.width, v.height,
+                     v.view, v.proj, crx.DISPLAY_VERTEX_COLORS)
+    return fb, cb
+
+
 def test_alone_vs_mix_and_oracle(ctx, terrain):
     views = _requests(terrain, 24)
     srv = _server(ctx, terrain, slots=20, frame_mb=2, batch=8, lanes=2)
@@ -87,31 +114,9 @@ def test_alone_vs_mix_and_oracle(ctx, terrain):
             assert np.array_equal(mix[k][1], alone[k][1]), f"request {k} colours"
         assert sum(int((fb != crx.CLEAR_PIXEL).sum()) for fb, _ in alone) > 50_000
         # oracle: the nodes the view sees, numbered with the scene-wide prefix
-        prefix = np.concatenate([[0], np.cumsum([nd["nt"] for nd in terrain])[:-1]]).astype(np.uint64)
         for k in range(0, len(views), 4):
-            v = views[k]
-            ids = srv.view_nodes(v)
-            inst = []
-            for j, i in enumerate(ids):
-                nd = terrain[i]
-                m = crx.CMesh()
-                m.numTriangles = nd["nt"]
-                m.positions = nd["pos"]
-                m.indices = nd["idx"]
-                m.colors = nd["col"]
-                m.cummulativeTriangleCount = int(prefix[i])
-                m.instances.offset = j
-                m.instances.count = 1
-                m.world = crx.identity()
-                m.id = i
-                m.numVertices = nd["nv"]
-                m.isLoaded = 1
-                inst.append(m)
-            xf = [crx.mat4_mul(v.view, crx.identity()) for _ in ids]
-            want_fb, _ = orc.draw(crx, inst, inst, xf, v.width, v.height, v.view, v.proj, False)
-            assert np.array_equal(alone[k][0], want_fb), f"request {k} vs oracle\n" + describe_mismatch(alone[k][0], want_fb, v.width)
-            want_cb = orc.resolve(crx, inst, prefix[ids] if ids else np.zeros(0, np.uint64), want_fb, v.width,
-                                  v.height, v.view, v.proj, crx.DISPLAY_VERTEX_COLORS)
+            want_fb, want_cb = _oracle_view(terrain, srv.view_nodes(views[k]), views[k])
+            assert np.array_equal(alone[k][0], want_fb), f"request {k} vs oracle\n" + describe_mismatch(alone[k][0], want_fb, views[k].width)
             assert np.array_equal(alone[k][1], want_cb), f"request {k} colours vs oracle"
     finally:
         srv.close()
@@ -134,13 +139,35 @@ def test_admission_waits_for_memory(ctx, terrain):
         srv.close()
 
 
-def test_view_seeing_too_many_nodes_is_rejected(ctx, terrain):
-    srv = _server(ctx, terrain, slots=2, frame_mb=4, batch=4, lanes=1)
+def test_view_seeing_more_nodes_than_the_cache(ctx, terrain):
+    """An overview that sees all 64 nodes with a 16-slot cache: it is drawn
+    in passes over groups of nodes into one visibility buffer and resolved
+    in passes, each shading the pixels of its group's triangles.  Equal to
+    the oracle's one-pass draw of all 64 nodes, and the same alone and
+    between other requests on two lanes."""
+    srv = _server(ctx, terrain, slots=16, frame_mb=8, batch=4, lanes=2)
     try:
-        far = crx.make_view(crx.look_at((256.0, 400.0, -200.0), (256.0, 0.0, 256.0)), wl.projection(128, 128), 128, 128)
-        t = srv.submit(far)
-        r = srv.wait(t)
-        assert r.status == 7       # CRX_REJECTED
-        srv.release(t)
+        W, H = 320, 240
+        far = crx.make_view(crx.look_at((256.0, 400.0, -200.0), (256.0, 0.0, 256.0)), wl.projection(W, H), W, H)
+        ids = srv.view_nodes(far)
+        assert len(ids) == len(terrain) == 64
+        fb, cb, r = _fetch(srv, srv.submit(far), far)
+        st = srv.stats()
+        assert st.cache_slots == 16 and st.evictions > 0
+        print(f"[server] overview: uploads {r.uploads} evictions {st.evictions}", flush=True)
+        want_fb, want_cb = _oracle_view(terrain, ids, far)
+        assert np.array_equal(fb, want_fb), describe_mismatch(fb, want_fb, W)
+        assert np.array_equal(cb, want_cb), "colours\n" + describe_mismatch(cb, want_cb, W)
+        assert int((fb != crx.CLEAR_PIXEL).sum()) > W * H // 4
+        # between other requests: the overview and its neighbours as when alone
+        views = _requests(terrain, 6)
+        alone = [_fetch(srv, srv.submit(v), v)[:2] for v in views]
+        mix_views = views[:3] + [far] + views[3:]
+        tickets = [srv.submit(v) for v in mix_views]
+        got = [_fetch(srv, t, v)[:2] for t, v in zip(tickets, mix_views)]
+        want = alone[:3] + [(fb, cb)] + alone[3:]
+        for k, v in enumerate(mix_views):
+            assert np.array_equal(got[k][0], want[k][0]), f"request {k}\n" + describe_mismatch(got[k][0], want[k][0], v.width)
+            assert np.array_equal(got[k][1], want[k][1]), f"request {k} colours"
     finally:
         srv.close()
