@@ -349,6 +349,41 @@ def test_scene_frames_keep_and_rebuild_lists(ctx, nodes):
     scene.close()
 
 
+def test_scene_frames_grow_metadata_with_kept_lists():
+    """A one-view frame and then multi-view frames whose views keep the same
+    400 nodes: the kept instance records stand, but the metadata grows with
+    the views (per-view records and transforms) and its device buffer is
+    reallocated, so the records must be written again into the new buffer.
+    A fresh context makes the first frame size the buffer.  Every view equals
+    the oracle's frame."""
+    c = crx.Context(0)
+    try:
+        scene, meshes, hosts, worlds, order = _two_mesh_scene(c, 400)
+        W, H = 640, 360
+        proj = wl.projection(W, H)
+        view_a, _ = wl.instances_view(extent=30.0, width=W, height=H, distance=25.0)
+        settings = crx.render_settings()
+        v = crx.make_view(view_a, proj, W, H)
+        fbw, cbw, nv = _oracle_scene_frame(meshes, hosts, worlds, order, v, view_a, proj, W, H, settings)
+        assert nv > 100
+        fbs = [c.alloc(8 * W * H) for _ in range(8)]
+        cbs = [c.alloc(8 * W * H) for _ in range(8)]
+        for count in (1, 2, 8, 1, 8):
+            print(f"[case] {count} view(s) over the kept node set", flush=True)
+            if count == 1:
+                scene.render(v, fbs[0], cbs[0], settings)
+            else:
+                scene.render_views([v] * count, fbs[:count], cbs[:count], settings)
+            for k in range(count):
+                got_fb = fbs[k].download(np.uint64, W * H)
+                got_cb = cbs[k].download(np.uint64, W * H)
+                assert np.array_equal(got_fb, fbw), f"{count} views, view {k}: " + describe_mismatch(got_fb, fbw, W)
+                assert np.array_equal(got_cb, cbw), f"{count} views, view {k} colours"
+        scene.close()
+    finally:
+        c.close()
+
+
 def test_scene_frames_on_two_streams(ctx):
     """Two frames on two streams before either is read back: crx_render
     returns while its resolve may still run, and the next frame's draw on the
