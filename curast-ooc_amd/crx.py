@@ -258,6 +258,7 @@ _SIGS = {
     "crx_raster_stage1": (C.c_int, [_VP, C.POINTER(RasterArgs), C.c_uint32, _VP]),
     "crx_raster_stage2": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
     "crx_raster_stage3": (C.c_int, [_VP, C.POINTER(RasterArgs), _VP]),
+    "crx_raster_visbuffer_noalloc": (C.c_int, [_VP, C.POINTER(RasterArgs), C.c_uint32, _VP]),
     "crx_draw_triangles_visbuffer": (C.c_int, [_VP, C.POINTER(CMesh), C.c_uint32, C.POINTER(CMesh),
                                                C.c_uint32, C.POINTER(Mat4), C.POINTER(RenderTarget),
                                                C.c_int, _VP, C.POINTER(DrawStats)]),
@@ -521,6 +522,11 @@ class Context:
 
     def raster_stage3(self, args: RasterArgs, stream=None) -> None:
         _check(lib().crx_raster_stage3(self.handle, C.byref(args), stream))
+
+    def raster_visbuffer_noalloc(self, args: RasterArgs, flags: int, stream=None) -> None:
+        """The three stages of `args` with the draw records in the library's
+        static device scratch (crx_raster_visbuffer_noalloc): no allocation."""
+        _check(lib().crx_raster_visbuffer_noalloc(self.handle, C.byref(args), int(flags), stream))
 
     def draw_triangles_visbuffer(self, meshes, instances, transforms, target: RenderTarget,
                                  instanced: bool, stream=None) -> DrawStats:

@@ -1056,6 +1056,11 @@ crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context*
         crx_context_destroy(ctx);
         return fail(CRX_HIP_ERROR, std::string("context allocation: ") + hipGetErrorString(e));
     }
+    const crx_status ss = noalloc_scratch_init(device);
+    if (ss != CRX_OK) {
+        crx_context_destroy(ctx);
+        return ss;
+    }
     *out = ctx;
     return CRX_OK;
 }

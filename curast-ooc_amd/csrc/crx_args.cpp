@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "crx.h"
@@ -162,7 +163,34 @@ crx_status run_late_stage(crx_context* ctx, const crx_raster_args* a, int stage,
     return CRX_OK;
 }
 
+// Scratch of crx_raster_visbuffer_noalloc: the draw records of one call
+// (items | instances | transforms | view | batch count), in device memory the
+// library never allocates: a file-scope array that each GPU holds until the
+// process ends.  Host code uses the address hipGetSymbolAddress returns for
+// the current device, never the array's name.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void* g_scratch[kMaxDevices] = {};
+
 }  // namespace
+
+// Looks up this device's copy of the scratch array once (the device must be
+// current); crx_context_create calls it at set-up.
+crx_status noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(CRX_INVALID_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (g_scratch[device]) return CRX_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(CRX_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = p;
+    return CRX_OK;
+}
+
 }  // namespace crx
 
 using namespace crx;
@@ -210,6 +238,39 @@ crx_status crx_raster_stage3(crx_context* ctx, const crx_raster_args* a, void* s
     CRX_TRY(args_buffer(ctx, args_image(0, a->numInstances).bytes, s, &dev));
     CRX_TRY(run_late_stage(ctx, a, 3, dev, s));
     return args_done(ctx, s);
+}
+
+crx_status crx_raster_visbuffer_noalloc(crx_context* ctx, const crx_raster_args* a, uint32_t flags, void* stream) {
+    CRX_TRY(check_args(ctx, a));
+    uint32_t numItems = 0;
+    CRX_TRY(check_flags(a, flags, &numItems));
+    const bool instanced = (flags & CRX_RASTER_INSTANCED) != 0;
+    const bool pu16 = (flags & CRX_RASTER_COMPRESSED) != 0;
+    const ArgsImage L = args_image(numItems, a->numInstances);
+    if (L.bytes > kNoAllocScratchBytes)
+        return fail(CRX_INVALID_ARG, "crx_raster_visbuffer_noalloc: the draw records take " + std::to_string(L.bytes) +
+                                         " bytes, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                         "-byte static scratch holds (fewer meshes / instances per call)");
+    char* dev = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) dev = static_cast<char*>(g_scratch[ctx->device]);
+    }
+    if (!dev) return fail(CRX_INVALID_ARG, "crx_raster_visbuffer_noalloc: no scratch address for the context's device");
+    hipStream_t s = as_stream(stream);
+    CRX_HIP(hipSetDevice(ctx->device));
+    CounterSet cs{{a->numProcessedBatches, a->numProcessedBatches_nontrivial, a->hugeTrianglesCounter,
+                   a->nontrivialTrianglesCounter, a->numProcessedHugeTriangles}};
+    CRX_HIP(launch_reset_counter_set(cs, s));
+    if (numItems == 0 || a->numInstances == 0) return CRX_OK;
+    // one image for all three stages: stages 2 and 3 read its instances,
+    // transforms and view, which stage 1's records include
+    const ArgsLaunch r = args_launch(ctx, a, instanced, pu16, numItems, dev, L);
+    CRX_HIP(build_records(a, instanced, pu16, numItems, r.P.batchShift, dev, L, s));
+    CRX_HIP(launch_stage1(r.P, ctx->tuning.stage1_block, r.perLane, r.g1, pu16, instanced, s));
+    CRX_HIP(launch_stage2(r.P, r.g2, s));
+    CRX_HIP(launch_stage3(r.P, r.g3, s));
+    return CRX_OK;
 }
 
 }  // extern "C"

@@ -255,6 +255,9 @@ inline S1Schedule s1_schedule(const crx_tuning& t, bool manyInstances, bool pu16
     return pu16 ? S1Schedule{2u, 64u} : S1Schedule{2u, t.stage1_waves_per_cu};
 }
 crx_status check_tuning(const crx_tuning& t);
+// The static scratch of crx_raster_visbuffer_noalloc on `device` (current):
+// its address is looked up once per device.
+crx_status noalloc_scratch_init(int device);
 crx_status check_target(const crx_render_target* t);
 void frustum_planes(const crx_view& view, float out[16]);
 crx_box3 world_box(const crx_box3& b, const crx_mat4& world);

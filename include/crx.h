@@ -305,6 +305,21 @@ crx_status crx_raster_stage1(crx_context* ctx, const crx_raster_args* args, uint
 crx_status crx_raster_stage2(crx_context* ctx, const crx_raster_args* args, void* stream);
 crx_status crx_raster_stage3(crx_context* ctx, const crx_raster_args* args, void* stream);
 
+/* crx_raster_stage1, crx_raster_stage2 and crx_raster_stage3 of `args` in one
+ * call on `stream`, allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  The draw
+ * records live in a static device array of 4 MiB per GPU, held until the
+ * process ends: 344 bytes per mesh drawn over one instance (about 12,000 of
+ * them), or 256 bytes per instance plus 88 per mesh with CRX_RASTER_INSTANCED
+ * (about 16,000 instances of one mesh); larger args fail with
+ * CRX_INVALID_ARG.  The caller owns
+ * counters, queues and framebuffer as for crx_raster_stage1.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (on different
+ * streams, or from different threads) race; calls on one stream do not. */
+crx_status crx_raster_visbuffer_noalloc(crx_context* ctx, const crx_raster_args* args, uint32_t flags,
+                                        void* stream);
+
 /* kernel_resolve_visbuffer_to_colorbuffer2D (resolve.cu:640-1141).
  * `instances` is a HOST array of CMesh, `prefix` a HOST array of the per-
  * instance cumulative triangle counts (CuRast_render.h:412-435).  Writes
