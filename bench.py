@@ -13,8 +13,19 @@ default workload is the 400M-triangle heightfield (14142 x 14142 grid, seeded
               its algorithmic bytes 12*T (indices) + 12*V (positions) per
               launch over the stage-1 duration measured with HIP events on
               the stream the kernel runs on
-* cpu_baseline  the CPU oracle timed on this host (OpenMP) rendering the same
-              frame, repeated until at least 10 s of wall time
+* parity      the benchmarked frame's visibility and colour buffers against
+              the CPU oracle's frame of the same workload, bit for bit
+* cpu_baseline  the CPU oracle timed on this host rendering the same frame:
+              on every CPU this process may use (capped by OMP_NUM_THREADS,
+              the host's CPU share per GPU), and on one thread
+* batched_multiview  64 mixed-resolution views of the heightfield: the
+              batched frame (eager and as a captured hipGraph) against the
+              plain single-view frame looped over the views
+* instances   3000 instances of the 1M-triangle sphere: frame time and stage
+              1 against the VALU issue rate
+* serving     the 2G-triangle tiled heightfield served out of core at 200
+              Poisson requests/s: p50 / p99 latency of batched and plain
+              serving, and saturated throughput
 """
 from __future__ import annotations
 
@@ -68,15 +79,49 @@ def stage1_bytes(T: int, V: int) -> int:
     return 12 * T + 12 * V
 
 
-def load_traffic(workload: str):
-    """HBM bytes per stage-1 launch measured with rocprofv3 --pmc (profiles/)."""
-    path = ROOT / "profiles" / f"stage1_{workload}_traffic.json"
+def kernel_source_sha16() -> str:
+    """Hash of the kernel sources: a counter summary taken on other kernels is stale."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("crx_kernels.hip", "crx_device.h"):
+        h.update((ROOT / "curast-ooc_amd" / "csrc" / f).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def load_profile(name: str):
+    """A figure measured with rocprofv3 counters (profiles/<name>.json) if it
+    was taken on the current kernel sources, else None."""
+    path = ROOT / "profiles" / f"{name}.json"
     if not path.exists():
         return None
     try:
-        return json.loads(path.read_text()).get("traffic_bytes")
+        d = json.loads(path.read_text())
     except (OSError, ValueError):
         return None
+    return d if d.get("kernel_source_sha16") == kernel_source_sha16() else None
+
+
+def cpu_info():
+    """CPUs this process may use on this host and the CPU model."""
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if q != "max":
+            quota = round(int(q) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    share = int(os.environ["OMP_NUM_THREADS"]) if os.environ.get("OMP_NUM_THREADS", "").isdigit() else affinity
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity, "cgroup_cpus": quota,
+            "threads": max(1, min(affinity, share))}
 
 
 def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int = 64, autotune: bool = True):
@@ -138,12 +183,33 @@ def bench_multiview(ctx, scene, pos, n: int, steps: int, warmup: int, count: int
     tb = timed(batched, reps_b)
     reps_p = max(2, steps // 40)
     tp = timed(plain, reps_p)
+    # the same batched frame recorded once into a hipGraph (crx_frame_*):
+    # one graph launch per frame, no per-frame host work
+    graph_fbs = [ctx.alloc(8 * v.width * v.height) for v in views]
+    graph_cbs = [ctx.alloc(8 * v.width * v.height) for v in views]
+    frame = crx.Frame(scene, views, graph_fbs, graph_cbs, settings)
+    frame.launch()
+    frame.wait()
+    graph_same = all(np.array_equal(g, b.download(np.uint64, v.width * v.height))
+                     for g, b, v in zip(got, graph_fbs, views))
+
+    def graph():
+        frame.launch()
+        frame.wait()
+
+    tg = timed(graph, reps_b)
+    frame.close()
+    for b in graph_fbs + graph_cbs:
+        b.free()
     out = {
         "views": count, "pixels": pixels, "closeups": int(sum(c for *_, c in vs)),
         "batched_views_per_s": count * reps_b / tb, "batched_ms_per_frame": 1e3 * tb / reps_b,
         "plain_views_per_s": count * reps_p / tp, "plain_ms_per_frame": 1e3 * tp / reps_p,
         "speedup": (count * reps_b / tb) / (count * reps_p / tp),
         "bit_identical_to_plain": bool(same),
+        "graph_views_per_s": count * reps_b / tg, "graph_ms_per_frame": 1e3 * tg / reps_b,
+        "graph_speedup": (count * reps_b / tg) / (count * reps_p / tp),
+        "graph_bit_identical": bool(graph_same),
         "batched_stage_ms": {k: round(x, 4) for k, x in stage.items()},
         "batched_nontrivial": st.num_nontrivial, "batched_huge": st.num_huge,
         "autotune": tuned,
@@ -221,9 +287,9 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
     views = [crx.make_view(v, p, W, H) for (_, v, p, W, H, _) in reqs]
     arrivals = [t for (t, *_) in reqs]
 
-    def run(max_batch, nlanes, paced=True):
+    def run(max_batch, nlanes, paced=True, shared_stream=1):
         cfg = crx.server_config(node_cache_bytes=max(node_bytes) * (len(nodes) // 2), frame_bytes=8 << 30,
-                                max_batch_views=max_batch, lanes=nlanes, resolve=1)
+                                max_batch_views=max_batch, lanes=nlanes, resolve=1, shared_stream=shared_stream)
         srv = crx.Server(ctx, cfg)
         for k, nd in enumerate(nodes):
             srv.add_node(nd["pos"], nd["nv"], nd["idx"], nd["nt"], nd["col"], nd["aabb"], resident=(k % 2 == 0))
@@ -231,7 +297,7 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
         srv.start()
         log(f"[serve] server started (batch {max_batch}, lanes {nlanes}) in {time.perf_counter() - t1:.1f} s")
         tickets = []
-        lat, batches, results = [], [], {}
+        lat, batches, ups, results = [], [], [], {}
         done = threading.Event()
 
         def reaper():
@@ -243,6 +309,7 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
                 results[k] = r.status
                 lat.append(r.done_time - r.submit_time)
                 batches.append(r.batch_views)
+                ups.append(r.uploads)
                 srv.release(tickets[k])
                 k += 1
             done.set()
@@ -261,10 +328,19 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
         srv.close()
         lat_ms = np.array(lat) * 1e3
         ok = sum(1 for s in results.values() if s == 0)
+        pct = lambda a, q: round(float(np.percentile(a, q)), 3) if len(a) else None
+        # requests whose batch streamed no node in, and requests after the
+        # first second (the cache warmed up along the cameras' paths)
+        no_up = lat_ms[np.array(ups) == 0]
+        warm = lat_ms[np.array(arrivals[:len(lat_ms)]) >= 1.0] if paced else lat_ms
         return {"requests": len(views), "ok": ok, "wall_s": round(wall, 3), "views_per_s": len(views) / wall,
-                "p50_ms": round(float(np.percentile(lat_ms, 50)), 3), "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                "p50_ms": pct(lat_ms, 50), "p90_ms": pct(lat_ms, 90), "p99_ms": pct(lat_ms, 99),
+                "p999_ms": pct(lat_ms, 99.9), "max_ms": round(float(lat_ms.max()), 3),
+                "requests_with_uploads": int(np.count_nonzero(np.array(ups))),
+                "p99_without_uploads_ms": pct(no_up, 99), "p99_after_first_second_ms": pct(warm, 99),
                 "mean_ms": round(float(lat_ms.mean()), 3), "mean_batch": round(float(np.mean(batches)), 2),
-                "max_batch": st.max_batch, "uploads": st.uploads, "upload_gb": round(st.upload_bytes / 1e9, 3),
+                "max_batch": st.max_batch, "uploads": st.uploads, "prefetches": st.prefetches,
+                "upload_gb": round(st.upload_bytes / 1e9, 3),
                 "evictions": st.evictions, "cache_slots": st.cache_slots}
 
     out = {"workload": f"tiled-heightfield-{n}x{n}-{len(nodes)}nodes", "triangles": total_tris,
@@ -274,6 +350,11 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
     # capacity: every request queued at once
     out["batched_saturated"] = run(batch, lanes, paced=False)
     log(f"[serve] batched saturated {out['batched_saturated']}")
+    # the lanes' batches side by side on separate streams instead of in order on one
+    out["batched_separate_streams"] = run(batch, lanes, shared_stream=0)
+    log(f"[serve] batched, separate streams {out['batched_separate_streams']}")
+    out["batched_separate_streams_saturated"] = run(batch, lanes, paced=False, shared_stream=0)
+    log(f"[serve] batched saturated, separate streams {out['batched_separate_streams_saturated']}")
     if plain:
         out["plain"] = run(1, 1)
         log(f"[serve] plain {out['plain']}")
@@ -285,26 +366,92 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
     return out
 
 
-def cpu_baseline(pos, idx, col, worlds, view, proj, threads: int, compressed: bool = False):
-    """The CPU oracle rendering the same frame (stages + resolve), repeated for >= 10 s."""
+def cpu_baseline(pos, idx, col, worlds, view, proj, compressed: bool = False, seconds: float = 8.0):
+    """The CPU oracle rendering the same frame (stages + resolve): repeated
+    for >= `seconds` on every CPU this process may use (up to the host's CPU
+    share per GPU, OMP_NUM_THREADS), then once on one thread.  Also returns
+    the last frame's buffers (the parity check of the GPU frame)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     sys.path.insert(0, str(ROOT / "tests"))
     import crx
     import oracle as orc
     from crxtest import CompressedMesh, SceneSpec, oracle_render
+    info = cpu_info()
+    threads = info["threads"]
     mesh = CompressedMesh(pos, idx, col) if compressed else orc.HostMesh(crx, pos, idx, col)
     spec = SceneSpec().add(mesh, worlds)
     T = len(idx) * len(worlds)
     reps, t0 = 0, time.perf_counter()
     while True:
-        oracle_render(spec, WIDTH, HEIGHT, view, proj, attribute=crx.DISPLAY_VERTEX_COLORS, threads=threads)
+        fb, cb, _ = oracle_render(spec, WIDTH, HEIGHT, view, proj, attribute=crx.DISPLAY_VERTEX_COLORS,
+                                  threads=threads)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= 10.0 or reps >= 50:
+        if el >= seconds or reps >= 50:
             break
-    return {"value": T * reps / el, "unit": "triangles/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} full frame(s) of the same workload (stages + resolve) on the CPU oracle, "
-                      f"{threads} OpenMP threads, {el:.1f} s"}
+    t1 = time.perf_counter()
+    oracle_render(spec, WIDTH, HEIGHT, view, proj, attribute=crx.DISPLAY_VERTEX_COLORS, threads=1)
+    one = time.perf_counter() - t1
+    out = {"value": T * reps / el, "unit": "triangles/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} full frame(s) of the same workload (stages + resolve) on the CPU oracle on "
+                     f"{threads} OpenMP threads in {el:.1f} s, then one frame on 1 thread in {one:.1f} s",
+           "one_thread_value": T / one, "one_thread_ms_per_frame": 1e3 * one,
+           "ms_per_frame": 1e3 * el / reps, "cpu": info}
+    return out, fb, cb
+
+
+def bench_instances(ctx, steps: int, warmup: int, count: int = 3000):
+    """3000 instances of the 1M-triangle sphere at 1920x1080 (one mesh read
+    once, transformed per instance): frame time over `steps` frames, stage
+    times from HIP events, and stage 1 against the VALU issue rate from the
+    committed counter summary when it was taken on these kernels."""
+    import crx
+    import workloads as wl
+    pos, idx, col = wl.sphere_mesh()
+    worlds = list(wl.instance_transforms(count))
+    view, proj = wl.instances_view()
+    mesh = crx.Mesh(ctx, pos, idx, col)
+    scene = crx.Scene(ctx)
+    for w in worlds:
+        scene.add(mesh, w)
+    v = crx.make_view(view, proj, WIDTH, HEIGHT)
+    fb, cb = ctx.alloc(8 * WIDTH * HEIGHT), ctx.alloc(8 * WIDTH * HEIGHT)
+    settings = crx.render_settings()
+    for _ in range(warmup):
+        scene.render(v, fb, cb, settings)
+    ctx.enable_timing(True)
+    s1, pipe = [], []
+    for _ in range(min(steps, 60)):
+        scene.render(v, fb, cb, settings)
+        t = ctx.timings()
+        s1.append(t.stage1_ms)
+        pipe.append(t.pipeline_ms)
+    ctx.enable_timing(False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        scene.render(v, fb, cb, settings)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    T = len(idx) * count
+    out = {"workload": f"sphere-1M-x{count}-instances-1080p", "triangle_instances": T,
+           "ms_per_frame": 1e3 * el / steps, "triangle_instances_per_s": T * steps / el,
+           "stage1_ms_mean": round(float(np.mean(s1)), 4), "pipeline_ms_median": round(float(np.median(pipe)), 4)}
+    prof = load_profile("stage1_instances_3000x1M_1080p_valu")
+    if prof:
+        # wave64 VALU instructions per launch over the live stage-1 time, against
+        # 1024 SIMDs issuing one wave64 VALU instruction per 4 cycles at 2.4 GHz
+        peak = prof["simds"] * prof["clock_ghz"] * 1e9 / prof["cycles_per_valu_inst"]
+        achieved = prof["valu_insts_per_launch"] / (out["stage1_ms_mean"] * 1e-3)
+        out["roofline"] = {"bound": "valu", "achieved": round(achieved / 1e9, 2), "peak": round(peak / 1e9, 1),
+                           "unit": "G wave64 VALU instructions/s", "frac": round(achieved / peak, 4),
+                           "kernel": prof["kernel"], "valu_insts_per_launch": prof["valu_insts_per_launch"],
+                           "source": prof["source"]}
+    for b in (fb, cb):
+        b.free()
+    scene.close()
+    mesh.close()
+    return out
 
 
 def main() -> int:
@@ -325,6 +472,8 @@ def main() -> int:
     ap.add_argument("--serve-n", type=int, default=31623, help="serving terrain grid size (vertices per side)")
     ap.add_argument("--serve-tile", type=int, default=707, help="cells per node side (707 -> ~1M triangles)")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served out-of-core leg")
+    ap.add_argument("--no-instances", action="store_true", help="skip the 3000-instance leg")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run the 1 GiB copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -453,13 +602,22 @@ def main() -> int:
         algo = mesh_bytes + 64 * len(worlds)
     achieved = algo / (mean_s1 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None if args.compressed else load_traffic(args.workload),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_stage1", "algorithmic_bytes_per_launch": algo,
                 "stage1_ms_mean": round(mean_s1, 4)}
+    # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, used only
+    # when they were measured on these kernel sources
+    prof = None if args.compressed else load_profile(f"stage1_{args.workload}_traffic")
+    if prof:
+        roofline["traffic"] = prof["traffic_bytes"]
+        roofline["traffic_source"] = prof["source"]
+    else:
+        roofline["traffic_note"] = "no counter summary measured on these kernel sources"
     if args.workload == "instances":
         roofline["note"] = ("the mesh is read once for all 3000 instances: this stage 1 is bound by the "
                             "per-instance arithmetic (VALU), not by HBM")
+    got_fb = fb.download(np.uint64, WIDTH * HEIGHT)
+    got_cb = cb.download(np.uint64, WIDTH * HEIGHT)
 
     multiview = None
     if args.workload == "heightfield" and not args.no_multiview:
@@ -468,12 +626,33 @@ def main() -> int:
                                     autotune=not args.no_autotune)
         log(f"[bench] multi-view {multiview}")
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "16"))
-        log(f"[bench] CPU oracle baseline on {threads} threads ...")
-        cpu = cpu_baseline(pos, idx, col, worlds, view, proj, threads, compressed=args.compressed)
-        log(f"[bench] CPU baseline {cpu['value']:.3e} triangles/s ({cpu['sample']})")
+        log(f"[bench] CPU oracle baseline on {cpu_info()['threads']} threads and on 1 thread ...")
+        cpu, want_fb, want_cb = cpu_baseline(pos, idx, col, worlds, view, proj, compressed=args.compressed)
+        parity = {"fb_equal": bool(np.array_equal(got_fb, want_fb)), "cb_equal": bool(np.array_equal(got_cb, want_cb)),
+                  "pixels": WIDTH * HEIGHT, "covered": int((got_fb != crx.CLEAR_PIXEL).sum()),
+                  "fb_mismatches": int((got_fb != want_fb).sum()), "cb_mismatches": int((got_cb != want_cb).sum())}
+        log(f"[bench] CPU baseline {cpu['value']:.3e} triangles/s ({cpu['sample']}); parity {parity}")
+        del want_fb, want_cb
+
+    scene.close()
+    mesh.close()
+    fb.free()
+    cb.free()
+    del pos, idx, col
+
+    instances = None
+    if args.workload == "heightfield" and not args.no_instances:
+        log("[bench] 3000 instances of the 1M-triangle sphere ...")
+        instances = bench_instances(ctx, args.steps, args.warmup)
+        log(f"[bench] instances {instances}")
+
+    serving = None
+    if args.workload == "heightfield" and not args.no_serve:
+        log("[bench] served out-of-core 2G-triangle heightfield at 200 requests/s ...")
+        serving = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate,
+                                lanes=2, batch=64)
 
     out = {
         "metric": "triangles/sec & ms/frame @1080p",
@@ -496,7 +675,10 @@ def main() -> int:
         "cpu_baseline": cpu,
         "atomic_min_peak": atomics,
         "hbm_probe": hbm,
+        "parity": parity,
         "batched_multiview": multiview,
+        "instances": instances,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

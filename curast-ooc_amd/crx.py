@@ -158,12 +158,13 @@ class Tuning(C.Structure):
                 ("stage1_block", C.c_uint32), ("stage1_waves_per_cu", C.c_uint32),
                 ("stage2_waves_per_cu", C.c_uint32), ("stage3_waves_per_cu", C.c_uint32),
                 ("stage1_cull_order", C.c_uint32), ("stage1_order", C.c_uint32),
-                ("stage1_fast_tiny", C.c_uint32), ("stage1_tris_per_lane", C.c_uint32)]
+                ("stage1_fast_tiny", C.c_uint32), ("stage1_tris_per_lane", C.c_uint32),
+                ("stage1_layout", C.c_uint32), ("stage1_strip_rows", C.c_uint32)]
 
 
 class DrawStats(C.Structure):
     _fields_ = [("num_triangles", C.c_uint64), ("num_nontrivial", C.c_uint32),
-                ("num_huge", C.c_uint32), ("redraws", C.c_uint32), ("reserved", C.c_uint32)]
+                ("num_huge", C.c_uint32), ("redraws", C.c_uint32), ("stage1_kernel", C.c_uint32)]
 
 
 class AutotuneResult(C.Structure):
@@ -176,7 +177,8 @@ TUNE_SCHEDULE, TUNE_ROUTING, TUNE_QUEUES = 1, 2, 4
 
 class ServerConfig(C.Structure):
     _fields_ = [("node_cache_bytes", C.c_uint64), ("frame_bytes", C.c_uint64), ("max_batch_views", C.c_uint32),
-                ("lanes", C.c_uint32), ("resolve", C.c_int32), ("reserved", C.c_uint32)]
+                ("lanes", C.c_uint32), ("resolve", C.c_int32), ("prefetch", C.c_uint32),
+                ("shared_stream", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class RequestResult(C.Structure):
@@ -188,7 +190,7 @@ class RequestResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("batches", C.c_uint64), ("uploads", C.c_uint64),
                 ("upload_bytes", C.c_uint64), ("evictions", C.c_uint64), ("resident_nodes", C.c_uint32),
-                ("cache_slots", C.c_uint32), ("max_batch", C.c_uint32), ("reserved", C.c_uint32)]
+                ("cache_slots", C.c_uint32), ("max_batch", C.c_uint32), ("prefetches", C.c_uint32)]
 
 
 class Timings(C.Structure):
@@ -308,6 +310,7 @@ _SIGS = {
                                          C.POINTER(_VP)]),
     "crx_mesh_destroy": (C.c_int, [_VP]),
     "crx_mesh_info": (C.c_int, [_VP, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(Box3)]),
+    "crx_mesh_row_triangles": (C.c_int, [_VP, C.POINTER(C.c_uint32)]),
     "crx_mesh_buffers": (C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
     "crx_scene_create": (C.c_int, [_VP, C.POINTER(_VP)]),
     "crx_scene_destroy": (C.c_int, [_VP]),
@@ -696,6 +699,12 @@ class Mesh:
         b = Box3()
         _check(lib().crx_mesh_info(self.handle, None, None, C.byref(b)))
         return b
+
+    def row_triangles(self) -> int:
+        """Triangles per row found in the index buffer (0 = none)."""
+        n = C.c_uint32()
+        _check(lib().crx_mesh_row_triangles(self.handle, C.byref(n)))
+        return n.value
 
     def close(self) -> None:
         if self.handle:

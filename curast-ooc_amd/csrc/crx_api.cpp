@@ -16,7 +16,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "crx_device.h"
@@ -127,6 +129,9 @@ crx_status check_tuning(const crx_tuning& t) {
     if (t.stage1_order > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_order must be 0, 1 or 2");
     if (t.stage1_tris_per_lane > 4 || t.stage1_tris_per_lane == 3)
         return fail(CRX_INVALID_ARG, "tuning.stage1_tris_per_lane must be 0, 1, 2 or 4");
+    if (t.stage1_layout > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_layout must be 0, 1 or 2");
+    if (t.stage1_strip_rows == 0 || t.stage1_strip_rows > 256)
+        return fail(CRX_INVALID_ARG, "tuning.stage1_strip_rows must be in [1, 256]");
     if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
     if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
         t.stage3_waves_per_cu > 64 || t.stage1_waves_per_cu > 64)
@@ -207,6 +212,7 @@ void set_counter_block(RasterParams& P, uint32_t* block) {
     P.ctrHuge = block + CTR_HUGE;
     P.cursor2 = block + CTR_CURSOR2;
     P.cursor3 = block + CTR_CURSOR3;
+    P.claims = block + CTR_CLAIM;
 }
 
 // Per-view record of a draw: size, projection and the view's transforms.
@@ -229,6 +235,53 @@ ViewParams make_view_params(const crx_render_target& t, const Transform* transfo
     // the margin is 5x that plus an absolute 2^-6
     V.tinyMargin = (float)std::max(t.width, t.height) * 0x1p-18f + 0x1p-6f;
     return V;
+}
+
+// Row length of meshes stored in rows, by their device index buffer: filled
+// by crx_mesh_create, read when a draw plans its items (a draw of CMesh
+// arrays whose index buffer is not a crx mesh walks contiguously).  The hint
+// only orders the stage-1 walk; no result depends on it.
+std::mutex g_rowHintsMu;
+std::unordered_map<const void*, uint32_t> g_rowHints;
+
+uint32_t row_hint_of(const void* indices) {
+    if (!indices) return 0;
+    std::lock_guard<std::mutex> lk(g_rowHintsMu);
+    auto it = g_rowHints.find(indices);
+    return it == g_rowHints.end() ? 0u : it->second;
+}
+
+// Triangles per row of a mesh stored in rows (a grid, a lat-long sphere):
+// over a window of triangles in the middle of the index buffer, the most
+// common distance from a vertex's first use to its first use more than 64
+// triangles later (the same vertex in the next row); 0 when no distance
+// covers a quarter of the vertices seen.
+uint32_t detect_row_triangles(const uint32_t* idx, uint64_t T) {
+    const uint64_t W = std::min<uint64_t>(T, 1ull << 18);
+    if (W < 1024) return 0;
+    const uint64_t t0 = T / 2 - W / 2;
+    std::unordered_map<uint32_t, uint64_t> first;
+    first.reserve(2 * W);
+    std::unordered_map<uint64_t, uint32_t> gaps;
+    std::unordered_map<uint32_t, uint8_t> done;
+    for (uint64_t t = t0; t < t0 + W; t++) {
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = idx[3 * t + k];
+            auto f = first.find(v);
+            if (f == first.end()) {
+                first.emplace(v, t);
+            } else if (t - f->second > 64 && !done[v]) {
+                done[v] = 1;
+                gaps[t - f->second]++;
+            }
+        }
+    }
+    uint64_t best = 0;
+    uint32_t count = 0;
+    for (const auto& g : gaps)
+        if (g.second > count || (g.second == count && g.first < best)) { best = g.first; count = g.second; }
+    if (count * 4ull < first.size() || best > 0xFFFFFFFFull) return 0;
+    return (uint32_t)best;
 }
 
 // Host records and metadata layout of a draw of numViews >= 1 views sharing
@@ -279,6 +332,18 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     D.batchShift = 0;   // log2(triangles per batch)
     while ((1u << D.batchShift) < D.block * D.trisPerLane) D.batchShift++;
     const uint64_t batchTris = 1ull << D.batchShift;
+    // pairs of consecutive triangles per lane (k_stage1_pairs), for draws
+    // large enough that every wave of the grid claims many 128-triangle
+    // batches (a small draw keeps the short per-batch walk of k_stage1)
+    D.runs = tu.stage1_layout != 0 && numViews == 1 && !manyInstances && !D.pu16 && D.block == 256 &&
+             D.trisPerLane == 2;
+    if (D.runs) {
+        uint64_t tris = 0;
+        for (uint32_t k = 0; k < D.numItems; k++) tris += list[k].numTriangles;
+        const uint64_t waves = (uint64_t)grid_for(ctx, sched.wavesPerCU ? sched.wavesPerCU : 32u, D.block / 64) *
+                               (D.block / 64);
+        D.runs = tris >= 16ull * 128ull * waves;
+    }
 
     D.items.resize(D.numItems);
     for (uint32_t k = 0; k < D.numItems; k++) {
@@ -309,7 +374,20 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
         it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
         it.numVertices = m.numVertices;
-        D.batches += ((uint64_t)m.numTriangles + batchTris - 1) >> D.batchShift;
+        // pair layout (batches of one wave's 128 triangles): units of
+        // stage1_strip_rows rows of a 128-triangle strip of the mesh's rows
+        // (a mesh stored in rows of at least eight strips), else of rows of
+        // 128 triangles (a contiguous walk)
+        constexpr uint64_t waveTris = 128;
+        const uint32_t rowHint = (D.runs && tu.stage1_layout == 2) ? row_hint_of(m.indices) : 0u;
+        const bool strips = rowHint >= 8 * waveTris;
+        it.rowTris = strips ? rowHint : (uint32_t)waveTris;
+        it.strips = strips ? (uint32_t)((rowHint + waveTris - 1) / waveTris) : 1u;
+        it.unitRows = tu.stage1_strip_rows;
+        const uint64_t rows = ((uint64_t)m.numTriangles + it.rowTris - 1) / it.rowTris;
+        it.unitSegs = (uint32_t)((rows + it.unitRows - 1) / it.unitRows);
+        const uint64_t units = (uint64_t)it.strips * it.unitSegs;
+        D.batches += D.runs ? units : ((uint64_t)m.numTriangles + batchTris - 1) >> D.batchShift;
         D.tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
         if (D.batches >= (1ull << 32)) return fail(CRX_INVALID_ARG, "too many triangle batches for one draw");
     }
@@ -405,7 +483,9 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
         if (e != hipSuccess) return e;
     }
     record(ctx, T_S1, stream);
-    if (D.numViews == 1 && !viewsKernel)
+    if (D.numViews == 1 && !viewsKernel && D.runs)
+        e = launch_stage1_pairs(P, D.block, D.g1, stream);
+    else if (D.numViews == 1 && !viewsKernel)
         e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, D.instanceBoxes, stream);
     else e = launch_stage1_views(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
     if (e != hipSuccess) return e;
@@ -519,6 +599,7 @@ crx_status draw_views(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMes
             stats->num_nontrivial = nNt;
             stats->num_huge = nHuge;
             stats->redraws = redraws;
+            stats->stage1_kernel = D.numViews > 1 ? 1u : D.runs ? 2u : 0u;
         }
         if (nNt <= ctx->ntCap && nHuge <= ctx->hugeCap) break;
         // Queue overflow: grow and draw again.  Every value the truncated pass
@@ -1027,6 +1108,8 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage1_order = 1;
     out->stage1_fast_tiny = 0;          // the exact path divides cheaply (short sequences)
     out->stage1_tris_per_lane = 0;
+    out->stage1_layout = 2;
+    out->stage1_strip_rows = 16;
 }
 
 crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {
@@ -1365,6 +1448,13 @@ crx_status crx_mesh_create(crx_context* ctx, const float* positions, uint32_t nu
         crx_mesh_destroy(m);
         return s;
     }
+    if (indices && m->indices) {
+        m->rowTris = detect_row_triangles(indices, numTriangles);
+        if (m->rowTris) {
+            std::lock_guard<std::mutex> lk(g_rowHintsMu);
+            g_rowHints[m->indices] = m->rowTris;
+        }
+    }
     *out = m;
     return CRX_OK;
 }
@@ -1503,12 +1593,22 @@ crx_status crx_mesh_create_device(crx_context* ctx, const float* d_positions, ui
 
 crx_status crx_mesh_destroy(crx_mesh* mesh) {
     if (!mesh) return CRX_OK;
+    if (mesh->rowTris && mesh->indices) {
+        std::lock_guard<std::mutex> lk(g_rowHintsMu);
+        g_rowHints.erase(mesh->indices);
+    }
     if (mesh->owned) {
         if (mesh->positions) (void)hipFree(mesh->positions);
         if (mesh->indices) (void)hipFree(mesh->indices);
         if (mesh->colors) (void)hipFree(mesh->colors);
     }
     delete mesh;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_row_triangles(const crx_mesh* mesh, uint32_t* out) {
+    if (!mesh || !out) return fail(CRX_INVALID_ARG, "NULL argument");
+    *out = mesh->rowTris;
     return CRX_OK;
 }
 
@@ -1634,6 +1734,8 @@ crx_status crx_autotune(crx_context* ctx, crx_scene* scene, const crx_view* view
         {&crx_tuning::stage1_order, {0, 1, 2}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_cull_order, {0, 1, 2, 3}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_fast_tiny, {0, 1}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_layout, {0, 1, 2}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_strip_rows, {2, 4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage2_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage3_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::small_threshold, {32, 64, 128, 256, 512}, CRX_TUNE_ROUTING},

@@ -33,6 +33,16 @@ struct DrawItem {
     float cf[3];
     float amin[3];
     uint32_t numVertices;       // CMesh::numVertices (IX_PU16: bounds the 8-byte vertex reads)
+    // Pair layout (k_stage1_pairs, whose batch is one wave's 128
+    // triangles): the triangles as rows of rowTris triangles cut into
+    // `strips` strips of 128; a work unit is strip s = unit / unitSegs over
+    // rows [g * unitRows, (g + 1) * unitRows), g = unit % unitSegs, drawn by
+    // one wave row after row.  A contiguous walk is rowTris = 128, strips = 1.
+    // firstBatch then counts units.
+    uint32_t rowTris;
+    uint32_t strips;
+    uint32_t unitRows;
+    uint32_t unitSegs;
 };
 
 // Per-instance record for stage 2, stage 3 and resolve.
@@ -74,8 +84,12 @@ enum Counter : int {
     CTR_HUGE = 1,         // stage-3 tile entries produced by stage 2
     CTR_CURSOR2 = 2,      // stage-2 work cursor
     CTR_CURSOR3 = 3,      // stage-3 work cursor
-    CTR_COUNT = 16
+    // k_stage1_pairs: eight unit counters, one per XCD, each on its own
+    // 128-byte line (word CTR_CLAIM + kClaimStride * xcd)
+    CTR_CLAIM = 32,
+    CTR_COUNT = 32 + 8 * 32
 };
+constexpr uint32_t kClaimStride = 32;
 
 // One render target of a draw.  A plain draw has one; a multi-view draw has
 // one per view, all sharing the draw items and instances.  Stage-2 and
@@ -127,6 +141,7 @@ struct RasterParams {
     uint32_t numWaves2;      // waves in the stage-2 grid
     uint32_t numWaves3;      // waves in the stage-3 grid
     uint32_t pad0;
+    uint32_t* claims;        // k_stage1_pairs: the per-XCD unit counters (CTR_CLAIM of the counter block)
 };
 
 struct ResolveParams {

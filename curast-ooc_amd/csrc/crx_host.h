@@ -125,6 +125,7 @@ struct DrawPlan {
     bool pu16 = false;
     uint32_t block = 256, trisPerLane = 1, batchShift = 8, g1 = 0, g2 = 0, g3 = 0;
     bool instanceBoxes = false;   // some item loops over >= 4 instances
+    bool runs = false;            // k_stage1_runs (items carry the run layout; batches counts its units)
     bool prefix = false;          // the image carries the instances' triangle prefix (InstanceRec::cumm)
     size_t offItems = 0, offInst = 0, offPrefix = 0, offViews = 0, offXf = 0, metaBytes = 0;
 };
@@ -200,6 +201,7 @@ struct crx_mesh {
     bool compressed = false;
     uint32_t bitsPerIndex = 0, indexMin = 0, indexMax = 0;
     crx_vec3 compressionFactor{};
+    uint32_t rowTris = 0;             // triangles per row of a mesh stored in rows (0 = none found)
 };
 
 struct crx_scene {

@@ -167,12 +167,23 @@ __device__ __forceinline__ uint32_t item_cull_order(const RasterParams& P, uint3
 //    the quotient error 2^-21*|ndc| scaled by W/2 plus the roundings of the
 //    screen transform, with slack); only triangles that are culled for sure
 //    leave early, all others take the exact path.
-__device__ __forceinline__ void stage1_triangle(const RasterParams& P, const ViewParams& V,
-                                                const float* __restrict__ m,
-                                                bool flip, f3 A, f3 B, f3 C,
-                                                uint64_t id, uint64_t queueEntry, uint32_t cullOrder) {
-    // depth first: ndc.z = depth = -view.z
-    const float da = -xform_row(m, 2, A), db = -xform_row(m, 2, B), dc = -xform_row(m, 2, C);
+//
+// stage1_core takes each vertex's depth d = -view.z and, with PRE, its view
+// y row (VQ), from a caller that computed them once for vertices shared
+// between triangles: the same operations, so the same bits.
+struct VQ {
+    float d, y;
+};
+
+__device__ __forceinline__ VQ vertex_q(const float* m, f3 p) {
+    return VQ{-xform_row(m, 2, p), xform_row(m, 1, p)};
+}
+
+template <bool PRE>
+__device__ __forceinline__ void stage1_core(const RasterParams& P, const ViewParams& V, const float* __restrict__ m,
+                                            bool flip, f3 A, f3 B, f3 C, const VQ& qa, const VQ& qb, const VQ& qc,
+                                            uint64_t id, uint64_t queueEntry, uint32_t cullOrder) {
+    const float da = qa.d, db = qb.d, dc = qc.d;
     // all vertices behind the camera (:119)
     if (da <= 0.0f && db <= 0.0f && dc <= 0.0f) return;
     bool nontrivial = (da <= 0.0f || db <= 0.0f || dc <= 0.0f);
@@ -182,11 +193,11 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     // frustum culling (:113-118), only when every depth is positive; the axis
     // the view culls most is tested before the other coordinate is computed
     const bool frustumFirst = cullOrder != 0;
-    float vya = 0.0f, vyb = 0.0f, vyc = 0.0f, vxa = 0.0f, vxb = 0.0f, vxc = 0.0f;
-    bool haveX = false, haveY = false;
+    float vya = qa.y, vyb = qb.y, vyc = qc.y, vxa = 0.0f, vxb = 0.0f, vxc = 0.0f;
+    bool haveX = false, haveY = PRE;
     if (frustumFirst && !nontrivial && safeDepth) {
         if (cullOrder == 1) {
-            vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
+            if (!PRE) { vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C); }
             haveY = true;
             if (frustum_culled_axis(f * vya, f * vyb, f * vyc, da, db, dc)) return;
             vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C);
@@ -196,7 +207,7 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
             vxa = xform_row(m, 0, A); vxb = xform_row(m, 0, B); vxc = xform_row(m, 0, C);
             haveX = true;
             if (frustum_culled_axis(fa * vxa, fa * vxb, fa * vxc, da, db, dc)) return;
-            vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C);
+            if (!PRE) { vya = xform_row(m, 1, A); vyb = xform_row(m, 1, B); vyc = xform_row(m, 1, C); }
             haveY = true;
             if (frustum_culled_axis(f * vya, f * vyb, f * vyc, da, db, dc)) return;
         }
@@ -355,6 +366,18 @@ __device__ __forceinline__ void stage1_triangle(const RasterParams& P, const Vie
     }
 }
 
+__device__ __forceinline__ void stage1_triangle(const RasterParams& P, const ViewParams& V,
+                                                const float* __restrict__ m,
+                                                bool flip, f3 A, f3 B, f3 C,
+                                                uint64_t id, uint64_t queueEntry, uint32_t cullOrder) {
+    // depth first: ndc.z = depth = -view.z
+    VQ qa{}, qb{}, qc{};
+    qa.d = -xform_row(m, 2, A);
+    qb.d = -xform_row(m, 2, B);
+    qc.d = -xform_row(m, 2, C);
+    stage1_core<false>(P, V, m, flip, A, B, C, qa, qb, qc, id, queueEntry, cullOrder);
+}
+
 }  // namespace
 
 // Clear one or two buffers (a view's visibility and colour buffers) to
@@ -367,11 +390,12 @@ __global__ __launch_bounds__(256) void k_clear(uint64_t* __restrict__ fb, uint32
     const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = first; i < numPixels; i += stride) fb[i] = kClearPixel;
     for (uint32_t i = first; i < numPixels2; i += stride) fb2[i] = kClearPixel;
-    if (blockIdx.x == 0 && ctr && threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && ctr)
+        for (uint32_t i = threadIdx.x; i < CTR_COUNT; i += blockDim.x) ctr[i] = 0u;
 }
 
 __global__ __launch_bounds__(64) void k_reset_counters(uint32_t* __restrict__ ctr) {
-    if (threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
+    for (uint32_t i = threadIdx.x; i < CTR_COUNT; i += blockDim.x) ctr[i] = 0u;
 }
 
 // Counter reset of the kernel-level entry points: the five counters the
@@ -433,6 +457,11 @@ __global__ __launch_bounds__(1024) void k_plan_items(const crx_cmesh* __restrict
         it.cf[0] = m.compressionFactor.x; it.cf[1] = m.compressionFactor.y; it.cf[2] = m.compressionFactor.z;
         it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
         it.numVertices = m.numVertices;
+        // (the pair layout is not built here: these plans run k_stage1)
+        it.rowTris = 128u;
+        it.strips = 1u;
+        it.unitRows = 1u;
+        it.unitSegs = 1u;
         items[k] = it;
         first += ((uint64_t)n + (1ull << batchShift) - 1u) >> batchShift;
     }
@@ -1291,6 +1320,229 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BOXES && 
     });
 }
 
+// ---------------------------------------------------------------------------
+// Stage 1 over pairs of consecutive triangles (k_stage1_pairs).
+//
+// Lane t of a wave takes triangles base + 2t and base + 2t + 1.  Meshes
+// stored in order put the two triangles of a quad next to each other, and
+// those share two vertices: B = (A.a, A.c, n) for a grid cut along one
+// diagonal, B = (A.a, n, A.b) for the other (a lat-long sphere).  When every
+// lane of a wave holds one of these two patterns (the same one), the wave
+// gathers 4 vertices instead of 6 and computes each vertex's depth and view
+// y row once for both triangles (stage1_core<true>: the same operations, so
+// the same bits); other waves draw both triangles separately.  The six
+// indices are read with one 16- and one 8-byte buffer load.  Each wave
+// claims units of unitRows consecutive rows of a 128-triangle strip (see
+// DrawItem::rowTris) and draws them row after row, so for a grid the vertex
+// row two neighbouring rows share is still in its XCD's L2 when the second
+// row reads it; claims are dynamic, which keeps the view-dependent cost of
+// the units balanced.  Only the order and the sharing of identical values
+// differ from k_stage1.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kPairNone = 0, kPairGrid = 1, kPairFan = 2;
+
+// The layout of the item a unit walk is in (wave-uniform).
+struct RunLayout {
+    uint32_t rowTris, unitRows, unitSegs, rows, firstUnit;
+};
+
+// A batch of the unit walk: unit (absolute), its strip, current row and the
+// unit's row end (wave-uniform scalars).
+struct RunBatch {
+    uint32_t unit, strip, row, rowEnd;
+};
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Triangles [base, end) of batch b (one wave's 128-triangle strip of a row).
+__device__ __forceinline__ void run_bounds(const S1Item& it, const RunLayout& L, const RunBatch& b, uint32_t& base,
+                                           uint32_t& end) {
+    constexpr uint32_t waveTris = 2u * kWave;
+    const uint64_t start = (uint64_t)b.row * L.rowTris + (uint64_t)b.strip * waveTris;
+    const uint64_t w = min((uint64_t)waveTris, (uint64_t)L.rowTris - (uint64_t)b.strip * waveTris);
+    base = uni((uint32_t)min(start, (uint64_t)it.numTriangles));
+    end = uni((uint32_t)min(start + w, (uint64_t)it.numTriangles));
+}
+
+// First batch of unit u.
+__device__ __forceinline__ RunBatch run_first(const RunLayout& L, uint32_t u) {
+    const uint32_t rel = u - L.firstUnit;
+    const uint32_t s = uni(rel / L.unitSegs);
+    const uint32_t g = rel - s * L.unitSegs;
+    RunBatch b;
+    b.unit = uni(u);
+    b.strip = s;
+    b.row = uni(g * L.unitRows);
+    b.rowEnd = uni(min(L.rows, g * L.unitRows + L.unitRows));
+    return b;
+}
+
+// Claims this wave's next unit: units x, x + 8, x + 16, ... belong to the
+// waves of XCD x (workgroup b runs on XCD b % 8), counted on that XCD's own
+// counter, so the claims of different XCDs never meet on one address, and a
+// unit is drawn row after row by one wave (the vertex row two rows share is
+// read again from the XCD's L2).  The reference claims batches from one
+// atomic cursor (triangles_visbuffer.cu:313).
+__device__ __forceinline__ uint32_t run_claim(uint32_t* claims) {
+    const uint32_t x = blockIdx.x & 7u;
+    uint32_t k = 0u;
+    if ((threadIdx.x & (kWave - 1)) == 0) k = atomicAdd(claims + kClaimStride * x, 1u);
+    k = __builtin_amdgcn_readfirstlane(k);
+    return x + 8u * k;
+}
+
+// The batch after `a` in this wave's walk of the item: the unit's next row,
+// else the first row of a newly claimed unit of the same item; more = 0 when
+// the claimed unit lies past the item (left in `pending` for the next pass;
+// the batch returned is then `a` again, a valid address to reload).
+__device__ __forceinline__ RunBatch run_next(const RunLayout& L, uint32_t* claims, uint32_t iend, const RunBatch& a,
+                                             uint32_t& more, uint32_t& pending) {
+    more = 1u;
+    if (a.row + 1u < a.rowEnd) {
+        RunBatch b = a;
+        b.row = uni(a.row + 1u);
+        return b;
+    }
+    const uint32_t u = run_claim(claims);
+    if (u < iend) return run_first(L, u);
+    pending = u;
+    more = 0u;
+    return a;
+}
+
+// Indices of the lane's two triangles (lanes past `end` read zeros).  A
+// non-indexed item reads the same words of its positions (at least 36
+// bytes per triangle) and replaces them with the vertex numbers 3t + v.
+__device__ __forceinline__ void pair_indices(const S1Item& it, uint32_t base, uint32_t end, uint32_t lane,
+                                             uint32_t (&I)[6]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(it.indices + 3ull * base, 12u * (end - base));
+    const uint4 v = buffer_load_u4(rs, 24u * lane);
+    const uint2 w = buffer_load_u2(rs, 24u * lane + 16u);
+    I[0] = v.x; I[1] = v.y; I[2] = v.z; I[3] = v.w; I[4] = w.x; I[5] = w.y;
+    if (it.indexMask == 0u) {          // wave-uniform; no loads in this branch
+        const uint32_t t = min(base + 2u * lane, it.numTriangles - 1u), t1 = min(t + 1u, it.numTriangles - 1u);
+        I[0] = 3u * t; I[1] = 3u * t + 1u; I[2] = 3u * t + 2u;
+        I[3] = 3u * t1; I[4] = 3u * t1 + 1u; I[5] = 3u * t1 + 2u;
+    }
+#else
+    (void)it; (void)base; (void)end; (void)lane;
+    for (int j = 0; j < 6; j++) I[j] = 0u;
+#endif
+}
+
+// The sharing pattern of the wave's pairs, and the gathers: A's three
+// vertices, then B's new vertex (a pattern) or B's three (none).
+__device__ __forceinline__ uint32_t pair_vertices(const S1Item& it, const uint32_t (&I)[6], f3 (&S)[6]) {
+    const bool grid = I[3] == I[0] && I[4] == I[2];
+    const bool fan = I[3] == I[0] && I[5] == I[1];
+    const uint32_t pat = __ballot(!grid) == 0ull ? kPairGrid : __ballot(!fan) == 0ull ? kPairFan : kPairNone;
+    S[0] = load_pos_g(it.positions, I[0]);
+    S[1] = load_pos_g(it.positions, I[1]);
+    S[2] = load_pos_g(it.positions, I[2]);
+    S[3] = load_pos_g(it.positions, pat == kPairGrid ? I[5] : pat == kPairFan ? I[4] : I[3]);
+    if (pat == kPairNone) {
+        S[4] = load_pos_g(it.positions, I[4]);
+        S[5] = load_pos_g(it.positions, I[5]);
+    }
+    return pat;
+}
+
+// The batch walk of k_stage1_pairs, software-pipelined two deep like
+// s1_sweep: while batch i is drawn, the gathers of batch i+1 and the index
+// loads of batch i+2 are in flight.  Units (strip, group of rows) are dealt
+// to the workgroups by P.order as s1_range deals batches; a workgroup walks
+// its unit's rows in turn.  body(it, base, end, S, pattern) draws one batch.
+template <typename Body>
+__device__ __forceinline__ void s1_pairs_sweep(const RasterParams& P, Body&& body) {
+    const uint32_t numUnits = P.batchCount ? *kconst(P.batchCount) : P.numBatches;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const auto items = kconst(P.items);
+    uint32_t unit = run_claim(P.claims);
+    while (unit < numUnits) {
+        const uint32_t item = find_item(P, unit);
+        const S1Item it = s1_item(P, item, numUnits);
+        RunLayout L;
+        L.rowTris = items[item].rowTris;
+        L.unitRows = items[item].unitRows;
+        L.unitSegs = items[item].unitSegs;
+        L.rows = (uint32_t)(((uint64_t)it.numTriangles + L.rowTris - 1u) / L.rowTris);
+        L.firstUnit = it.firstBatch;
+        const uint32_t iend = it.endBatch;
+        uint32_t pending = numUnits;
+        uint32_t base0, end0, base1, end1, base2, end2;
+        const RunBatch b0 = run_first(L, unit);
+        run_bounds(it, L, b0, base0, end0);
+        uint32_t I[6], J[6];
+        pair_indices(it, base0, end0, lane, I);
+        uint32_t more1;
+        RunBatch b1 = run_next(L, P.claims, iend, b0, more1, pending);
+        run_bounds(it, L, b1, base1, end1);
+        pair_indices(it, base1, end1, lane, J);
+        f3 S[6];
+        uint32_t pat = pair_vertices(it, I, S);
+        for (;;) {
+            f3 Sn[6];
+            const uint32_t patN = pair_vertices(it, J, Sn);
+            uint32_t more2 = 0u;
+            const RunBatch b2 = more1 ? run_next(L, P.claims, iend, b1, more2, pending) : b1;
+            run_bounds(it, L, b2, base2, end2);
+            pair_indices(it, base2, end2, lane, J);
+            body(it, base0, end0, S, pat);
+            if (!more1) break;
+            base0 = base1;
+            end0 = end1;
+            b1 = b2;
+            base1 = base2;
+            end1 = end2;
+            more1 = more2;
+            #pragma unroll
+            for (int j = 0; j < 6; j++) S[j] = Sn[j];
+            pat = patN;
+        }
+        // a unit claimed past this item (or past the end) starts the next pass
+        unit = pending;
+    }
+}
+
+// Stage 1 over pairs of consecutive triangles for draws of one view whose
+// items are drawn over few instances (the others take k_stage1).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_stage1_pairs(RasterParams P) {
+    const ViewParams V0 = kconst(P.views)[0];
+    const auto transforms = kconst(V0.transforms);
+    s1_pairs_sweep(P, [&](const S1Item& it, uint32_t base, uint32_t end, const f3 (&S)[6], uint32_t pat)
+                          __attribute__((always_inline)) {
+        const uint32_t ta = base + 2u * (threadIdx.x & (kWave - 1)), tb = ta + 1u;
+        for (uint32_t j = 0; j < it.instCount; j++) {
+            const uint32_t inst = it.instOffset + j;
+            const auto T = transforms + inst;
+            float m[16];
+            #pragma unroll
+            for (int e = 0; e < 16; e++) m[e] = T->m[e];
+            const bool flip = T->flip != 0u;
+            const uint32_t cullOrder = item_cull_order(P, it.instCount);
+            const uint64_t off = it.cumm + (uint64_t)(j * it.numTriangles);
+            const uint64_t hi = (uint64_t)inst << 32;
+            if (pat != kPairNone) {
+                // the four distinct vertices' quantities, once
+                const VQ q0 = vertex_q(m, S[0]), q1 = vertex_q(m, S[1]), q2 = vertex_q(m, S[2]), q3 = vertex_q(m, S[3]);
+                if (ta < end) stage1_core<true>(P, V0, m, flip, S[0], S[1], S[2], q0, q1, q2, off + ta, hi | ta, cullOrder);
+                // B = (A.a, A.c, n) or (A.a, n, A.b): a wave-uniform choice
+                if (pat == kPairGrid) {
+                    if (tb < end) stage1_core<true>(P, V0, m, flip, S[0], S[2], S[3], q0, q2, q3, off + tb, hi | tb, cullOrder);
+                } else {
+                    if (tb < end) stage1_core<true>(P, V0, m, flip, S[0], S[3], S[1], q0, q3, q1, off + tb, hi | tb, cullOrder);
+                }
+            } else {
+                if (ta < end) stage1_triangle(P, V0, m, flip, S[0], S[1], S[2], off + ta, hi | ta, cullOrder);
+                if (tb < end) stage1_triangle(P, V0, m, flip, S[3], S[4], S[5], off + tb, hi | tb, cullOrder);
+            }
+        }
+    });
+}
+
 // Stage 1 of a multi-view draw.  Each batch of triangles is fetched once and
 // drawn into every view that may see it: a wave reduces the object-space box
 // of its 64 triangles, each lane tests that box against one view
@@ -1984,7 +2236,8 @@ __global__ __launch_bounds__(256) void k_clear_views(const ClearView* __restrict
     const uint32_t base = (blockIdx.x - c.firstBlock) * kClearChunk;
     const uint32_t end = min(c.numPixels, base + kClearChunk);
     for (uint32_t i = base + threadIdx.x; i < end; i += 256u) c.fb[i] = kClearPixel;
-    if (blockIdx.x == 0 && ctr && threadIdx.x < CTR_COUNT) ctr[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && ctr)
+        for (uint32_t i = threadIdx.x; i < CTR_COUNT; i += blockDim.x) ctr[i] = 0u;
 }
 
 // Plain streaming copy, 16 bytes per lane, four nontemporal loads in flight
@@ -2176,6 +2429,12 @@ hipError_t launch_s1_any(const RasterParams& P, uint32_t block, uint32_t k, uint
 hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
                          bool instanceBoxes, hipStream_t stream) {
     return launch_s1_any(P, block, trisPerLane, grid, pu16, false, instanceBoxes, stream);
+}
+
+hipError_t launch_stage1_pairs(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
+    if (block != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stage1_pairs<256>), dim3(grid), dim3(256), 0, stream, P);
+    return hipGetLastError();
 }
 
 hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,

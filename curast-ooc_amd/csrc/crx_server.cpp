@@ -19,6 +19,12 @@
 // upload into that slot waits for the release events of the lanes that read
 // it.
 //
+// With prefetch on, the nodes whose boxes touch those of a batch's nodes are
+// queued after the batch, and a prefetch thread uploads them on its own
+// stream, one at a time, into free slots or slots whose node no batch used
+// recently: a camera that moves on finds the nodes coming into view already
+// resident, so its requests do not wait for uploads.
+//
 // A view that sees more nodes than the cache holds is drawn in passes over
 // groups of nodes into the same visibility buffer (the 64-bit atomic minimum
 // makes the result independent of the grouping), then resolved in passes
@@ -70,6 +76,7 @@ struct Node {
 
 struct Request {
     crx_view view{};
+    std::vector<uint32_t> nodes;     // the nodes the view sees (its node cull), ascending
     size_t offset = 0, bytes = 0;
     double submit = 0, start = 0, done = 0;
     uint32_t batchViews = 0, uploads = 0;
@@ -150,13 +157,26 @@ struct crx_server {
         crx_context* ctx = nullptr;
         hipStream_t stream = nullptr;
         hipEvent_t released = nullptr;   // recorded on `stream` whenever the lane releases nodes
+        hipEvent_t done = nullptr;       // recorded after a batch's last launch; the lane waits on it
         uint32_t index = 0;
         std::thread thread;
     };
     std::vector<Lane> lanes;
+    hipStream_t sharedStream = nullptr;   // cfg.shared_stream: the one stream of all lanes
+
+    // prefetch of the neighbours of drawn nodes (cfg.prefetch): queue and
+    // flags guarded by cacheMu
+    std::vector<std::vector<uint32_t>> neighbors;
+    hipStream_t prefetchStream = nullptr;
+    std::thread prefetcher;
+    std::deque<uint32_t> prefetchQueue;
+    std::vector<uint8_t> prefetchQueued;
+    std::condition_variable prefetchCv;
+    bool prefetchStop = false;
 
     // statistics (guarded by mu, except the cache counters: cacheMu)
     uint64_t statRequests = 0, statBatches = 0, statUploads = 0, statUploadBytes = 0, statEvictions = 0;
+    uint64_t statPrefetches = 0;
     uint32_t statMaxBatch = 0;
 
     double now() const {
@@ -180,9 +200,10 @@ void slot_pointers(crx_server* s, const Node& n, float** pos, uint32_t** idx, ui
                                        align_up(12ull * n.numTriangles, kAlign));
 }
 
-// Copies node `id` into `slot` on the upload stream and records its event.
-// Caller holds cacheMu.
-crx_status upload_node(crx_server* s, uint32_t id, int slot) {
+// Copies node `id` into `slot` on `stream` (the upload stream by default)
+// and records its event.  Caller holds cacheMu.
+crx_status upload_node(crx_server* s, uint32_t id, int slot, hipStream_t stream = nullptr) {
+    if (!stream) stream = s->upload;
     Node& n = s->nodes[id];
     n.slot = slot;
     s->slotNode[slot] = (int)id;
@@ -191,16 +212,16 @@ crx_status upload_node(crx_server* s, uint32_t id, int slot) {
     slot_pointers(s, n, &pos, &idx, &col);
     // the copy waits for the kernels that read the slot's previous node
     for (uint32_t l = 0; l < s->lanes.size(); l++) {
-        if (s->slotReaders[slot] & (1u << l)) CRX_HIP(hipStreamWaitEvent(s->upload, s->lanes[l].released, 0));
+        if (s->slotReaders[slot] & (1u << l)) CRX_HIP(hipStreamWaitEvent(stream, s->lanes[l].released, 0));
     }
     s->slotReaders[slot] = 0;
-    if (n.numVertices) CRX_HIP(hipMemcpyAsync(pos, n.hpos, 12ull * n.numVertices, hipMemcpyHostToDevice, s->upload));
+    if (n.numVertices) CRX_HIP(hipMemcpyAsync(pos, n.hpos, 12ull * n.numVertices, hipMemcpyHostToDevice, stream));
     if (n.hidx && n.numTriangles)
-        CRX_HIP(hipMemcpyAsync(idx, n.hidx, 12ull * n.numTriangles, hipMemcpyHostToDevice, s->upload));
+        CRX_HIP(hipMemcpyAsync(idx, n.hidx, 12ull * n.numTriangles, hipMemcpyHostToDevice, stream));
     if (n.hcol && n.numVertices)
-        CRX_HIP(hipMemcpyAsync(col, n.hcol, 4ull * n.numVertices, hipMemcpyHostToDevice, s->upload));
+        CRX_HIP(hipMemcpyAsync(col, n.hcol, 4ull * n.numVertices, hipMemcpyHostToDevice, stream));
     if (!n.ready) CRX_HIP(hipEventCreateWithFlags(&n.ready, hipEventDisableTiming));
-    CRX_HIP(hipEventRecord(n.ready, s->upload));
+    CRX_HIP(hipEventRecord(n.ready, stream));
     s->statUploads++;
     s->statUploadBytes += node_bytes(n);
     return CRX_OK;
@@ -284,6 +305,97 @@ crx_status release_nodes(crx_server* s, crx_server::Lane& lane, const std::vecto
     return st;
 }
 
+// Batches a node may go unused and still not be evicted by a prefetch.
+constexpr uint64_t kPrefetchKeep = 64;
+
+// Queues the non-resident neighbours of the nodes a batch drew for the
+// prefetch thread.
+void note_drawn(crx_server* s, const std::vector<uint32_t>& ids) {
+    if (!s->cfg.prefetch) return;
+    bool any = false;
+    {
+        std::lock_guard<std::mutex> lk(s->cacheMu);
+        for (uint32_t id : ids) {
+            for (uint32_t nb : s->neighbors[id]) {
+                if (s->nodes[nb].slot >= 0 || s->prefetchQueued[nb]) continue;
+                s->prefetchQueued[nb] = 1;
+                s->prefetchQueue.push_back(nb);
+                any = true;
+            }
+        }
+    }
+    if (any) s->prefetchCv.notify_one();
+}
+
+// Uploads queued nodes one at a time on the prefetch stream, into a free
+// slot or the slot of the least recently used unpinned node that no batch
+// used in the last kPrefetchKeep batches; a node with no such slot is dropped.
+void prefetch_main(crx_server* s) {
+    (void)hipSetDevice(s->ctx->device);
+    for (;;) {
+        hipEvent_t ready = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->cacheMu);
+            s->prefetchCv.wait(lk, [&] { return s->prefetchStop || !s->prefetchQueue.empty(); });
+            if (s->prefetchStop) return;
+            const uint32_t id = s->prefetchQueue.front();
+            s->prefetchQueue.pop_front();
+            s->prefetchQueued[id] = 0;
+            Node& n = s->nodes[id];
+            if (n.slot >= 0) continue;
+            int slot = -1;
+            for (uint32_t k = 0; k < s->numSlots && slot < 0; k++)
+                if (s->slotNode[k] < 0) slot = (int)k;
+            if (slot < 0) {
+                uint64_t best = UINT64_MAX;
+                for (uint32_t k = 0; k < s->numSlots; k++) {
+                    const Node& c = s->nodes[s->slotNode[k]];
+                    if (c.pins == 0 && c.lastUse + kPrefetchKeep <= s->useClock && c.lastUse < best) {
+                        best = c.lastUse;
+                        slot = (int)k;
+                    }
+                }
+                if (slot < 0) continue;
+                s->nodes[s->slotNode[slot]].slot = -1;
+                s->slotNode[slot] = -1;
+                s->statEvictions++;
+            }
+            if (upload_node(s, id, slot, s->prefetchStream) != CRX_OK) {
+                // the slot's contents are unknown: leave it free
+                n.slot = -1;
+                s->slotNode[slot] = -1;
+                continue;
+            }
+            n.lastUse = s->useClock;
+            s->statPrefetches++;
+            ready = n.ready;
+        }
+        (void)hipEventSynchronize(ready);
+    }
+}
+
+// Nodes whose world boxes touch (closed boxes that intersect), by a sweep
+// over the boxes sorted by their lower x.
+void find_neighbors(crx_server* s) {
+    const uint32_t N = (uint32_t)s->nodes.size();
+    s->neighbors.assign(N, {});
+    std::vector<uint32_t> order(N);
+    for (uint32_t i = 0; i < N; i++) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t a, uint32_t b) { return s->nodes[a].wbox.min.x < s->nodes[b].wbox.min.x; });
+    for (uint32_t a = 0; a < N; a++) {
+        const crx_box3& A = s->nodes[order[a]].wbox;
+        for (uint32_t b = a + 1; b < N; b++) {
+            const crx_box3& B = s->nodes[order[b]].wbox;
+            if (B.min.x > A.max.x) break;
+            if (B.min.y <= A.max.y && A.min.y <= B.max.y && B.min.z <= A.max.z && A.min.z <= B.max.z) {
+                s->neighbors[order[a]].push_back(order[b]);
+                s->neighbors[order[b]].push_back(order[a]);
+            }
+        }
+    }
+}
+
 // CMesh of node `id` in its slot (scene-wide triangle ids), listed at `k`.
 crx_cmesh node_cmesh(crx_server* s, uint32_t id, uint32_t k) {
     const Node& n = s->nodes[id];
@@ -321,6 +433,15 @@ crx_status pinned_cmeshes(crx_server* s, const std::vector<uint32_t>& ids, hipSt
     return CRX_OK;
 }
 
+// Waits for the lane's last batch: an event after its last launch, so a
+// lane sharing its stream does not wait for the batches other lanes
+// enqueued after it.
+crx_status lane_finish(crx_server::Lane& lane) {
+    CRX_HIP(hipEventRecord(lane.done, lane.stream));
+    CRX_HIP(hipEventSynchronize(lane.done));
+    return CRX_OK;
+}
+
 // Draws a group of requests on a lane: one pass when the nodes the group
 // sees fit the cache, else (a single view that sees more nodes than the
 // cache holds) passes over groups of nodes.  Returns the status for all of them.
@@ -328,17 +449,13 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
     const uint32_t V = (uint32_t)batch.size();
     const uint32_t N = (uint32_t)s->nodes.size();
     *uploads = 0;
-    // per-view node cull (SNTriangles::update for each view)
+    // per-view node cull (SNTriangles::update for each view, done by draw_batch)
     std::vector<uint8_t> vis((size_t)V * N, 0);
     std::vector<uint8_t> any(N, 0);
     for (uint32_t v = 0; v < V; v++) {
-        float planes[16];
-        frustum_planes(batch[v]->view, planes);
-        for (uint32_t i = 0; i < N; i++) {
-            if (box_in_frustum(planes, s->nodes[i].wbox)) {
-                vis[(size_t)v * N + i] = 1;
-                any[i] = 1;
-            }
+        for (uint32_t i : batch[v]->nodes) {
+            vis[(size_t)v * N + i] = 1;
+            any[i] = 1;
         }
     }
     std::vector<uint32_t> need;
@@ -418,15 +535,14 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
             ex.skip = useSkip ? skip.data() : nullptr;
             ex.afterStages = s->cfg.resolve ? &resolve : nullptr;
             ex.prefix = s->cfg.resolve != 0;
+            ex.earlyReturn = true;   // lane_finish waits for the resolve
             st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream, &ds,
                             ex);
         } while (false);
         const crx_status rs = release_nodes(s, lane, need);
         if (st == CRX_OK) st = rs;
-        if (st == CRX_OK) {
-            hipError_t e = hipStreamSynchronize(lane.stream);
-            if (e != hipSuccess) st = fail(CRX_HIP_ERROR, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-        }
+        note_drawn(s, need);
+        if (st == CRX_OK) st = lane_finish(lane);
         return st;
     }
 
@@ -447,6 +563,7 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
                 const bool useSkip = lists(ids);
                 DrawExtras ex;
                 ex.skip = useSkip ? skip.data() : nullptr;
+                ex.earlyReturn = true;
                 crx_draw_stats ds;
                 st = draw_views(lane.ctx, inst.data(), U, inst.data(), U, xf.data(), targets.data(), V, 0, lane.stream,
                                 &ds, ex);
@@ -466,8 +583,7 @@ crx_status draw_group(crx_server* s, crx_server::Lane& lane, const std::vector<R
     if (s->cfg.resolve) {
         for (size_t g = groups.size(); g-- > 0;) CRX_TRY(pass(groups[g], true));
     }
-    CRX_HIP(hipStreamSynchronize(lane.stream));
-    return CRX_OK;
+    return lane_finish(lane);
 }
 
 // Draws one batch: views are grouped greedily (in order) so that the nodes
@@ -493,7 +609,8 @@ void draw_batch(crx_server* s, crx_server::Lane& lane, const std::vector<Request
     for (Request* r : batch) {
         float planes[16];
         frustum_planes(r->view, planes);
-        std::vector<uint32_t> mine;
+        std::vector<uint32_t>& mine = r->nodes;
+        mine.clear();
         for (uint32_t i = 0; i < N; i++)
             if (box_in_frustum(planes, s->nodes[i].wbox)) mine.push_back(i);
         if (mine.size() > s->numSlots) {
@@ -584,6 +701,8 @@ void crx_server_config_default(crx_server_config* out) {
     out->max_batch_views = 64;
     out->lanes = 2;
     out->resolve = 1;
+    out->prefetch = 1;
+    out->shared_stream = 1;
     out->reserved = 0;
 }
 
@@ -686,10 +805,22 @@ crx_status crx_server_start(crx_server* s) {
         auto& lane = s->lanes[l];
         lane.index = l;
         CRX_TRY(crx_context_create(s->ctx->device, &s->ctx->tuning, &lane.ctx));
-        CRX_HIP(hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking));
+        if (s->cfg.shared_stream) {
+            if (!s->sharedStream) CRX_HIP(hipStreamCreateWithFlags(&s->sharedStream, hipStreamNonBlocking));
+            lane.stream = s->sharedStream;
+        } else {
+            CRX_HIP(hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking));
+        }
         CRX_HIP(hipEventCreateWithFlags(&lane.released, hipEventDisableTiming));
+        CRX_HIP(hipEventCreateWithFlags(&lane.done, hipEventDisableTiming));
     }
     s->started = true;
+    if (s->cfg.prefetch) {
+        find_neighbors(s);
+        s->prefetchQueued.assign(s->nodes.size(), 0);
+        CRX_HIP(hipStreamCreateWithFlags(&s->prefetchStream, hipStreamNonBlocking));
+        s->prefetcher = std::thread(prefetch_main, s);
+    }
     for (auto& lane : s->lanes) lane.thread = std::thread(lane_main, s, &lane);
     return CRX_OK;
 }
@@ -703,16 +834,28 @@ crx_status crx_server_destroy(crx_server* s) {
     s->workCv.notify_all();
     for (auto& lane : s->lanes)
         if (lane.thread.joinable()) lane.thread.join();
+    {
+        std::lock_guard<std::mutex> lk(s->cacheMu);
+        s->prefetchStop = true;
+    }
+    s->prefetchCv.notify_all();
+    if (s->prefetcher.joinable()) s->prefetcher.join();
     (void)hipSetDevice(s->ctx->device);
     for (auto& lane : s->lanes) {
         if (lane.stream) (void)hipStreamSynchronize(lane.stream);
-        if (lane.stream) (void)hipStreamDestroy(lane.stream);
+        if (lane.stream && lane.stream != s->sharedStream) (void)hipStreamDestroy(lane.stream);
         if (lane.released) (void)hipEventDestroy(lane.released);
+        if (lane.done) (void)hipEventDestroy(lane.done);
         if (lane.ctx) crx_context_destroy(lane.ctx);
     }
     if (s->upload) {
         (void)hipStreamSynchronize(s->upload);
         (void)hipStreamDestroy(s->upload);
+    }
+    if (s->sharedStream) (void)hipStreamDestroy(s->sharedStream);
+    if (s->prefetchStream) {
+        (void)hipStreamSynchronize(s->prefetchStream);
+        (void)hipStreamDestroy(s->prefetchStream);
     }
     for (Node& n : s->nodes)
         if (n.ready) (void)hipEventDestroy(n.ready);
@@ -796,6 +939,7 @@ crx_status crx_server_get_stats(const crx_server* s_, crx_server_stats* out) {
     out->uploads = s->statUploads;
     out->upload_bytes = s->statUploadBytes;
     out->evictions = s->statEvictions;
+    out->prefetches = (uint32_t)std::min<uint64_t>(s->statPrefetches, UINT32_MAX);
     out->cache_slots = s->numSlots;
     uint32_t res = 0;
     for (int k : s->slotNode) res += k >= 0 ? 1u : 0u;

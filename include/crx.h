@@ -197,6 +197,20 @@ typedef struct crx_tuning {
                                      (every instanced crx_raster_stage1 draw) takes 1 and a grid
                                      of 64 waves per CU, an IX_PU16 draw 2 and 64 waves per CU,
                                      others 2 and stage1_waves_per_cu */
+    uint32_t stage1_layout;       /* draws of one view over uncompressed meshes drawn over < 4
+                                     instances, with 2 triangles per lane, a block of 256 and at
+                                     least 16 batches of 128 triangles per wave of the grid:
+                                     0 = lane t takes triangles t and t + stage1_block;
+                                     1 = lane t takes triangles 2t and 2t + 1, and a wave whose pairs
+                                     all share two vertices as the two triangles of a quad do gathers
+                                     and transforms each shared vertex once; waves claim units of
+                                     stage1_strip_rows batches of 128 triangles;
+                                     2 (default) = 1, and a mesh stored in rows (a grid: the row
+                                     length detected from the index buffer by crx_mesh_create) is
+                                     walked in units of stage1_strip_rows consecutive rows of a
+                                     128-triangle strip, so the vertex row two rows share is read
+                                     once */
+    uint32_t stage1_strip_rows;   /* rows (or batches) per unit of layouts 1 and 2, 1..256 (default 16) */
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */
@@ -205,7 +219,9 @@ typedef struct crx_draw_stats {
     uint32_t num_nontrivial;       /* entries in the stage-2 queue */
     uint32_t num_huge;             /* tile entries in the stage-3 queue */
     uint32_t redraws;              /* redraws caused by queue growth */
-    uint32_t reserved;
+    uint32_t stage1_kernel;        /* the stage-1 kernel the draw ran: 0 = k_stage1 (lanes take
+                                      triangles a block apart), 1 = k_stage1_views (several views),
+                                      2 = k_stage1_pairs (consecutive pairs, tuning.stage1_layout) */
 } crx_draw_stats;
 
 typedef struct crx_context crx_context;
@@ -473,6 +489,12 @@ crx_status crx_mesh_create_device(crx_context* ctx, const float* d_positions, ui
                                   const uint32_t* d_colors, const crx_box3* aabb, crx_mesh** out);
 crx_status crx_mesh_destroy(crx_mesh* mesh);
 crx_status crx_mesh_info(const crx_mesh* mesh, uint32_t* numTriangles, uint32_t* numVertices, crx_box3* aabb);
+/* Triangles per row that crx_mesh_create found in the index buffer of a mesh
+ * stored in rows (0 = none): the stage-1 walk of tuning.stage1_layout 2
+ * uses it to read the vertices two rows share once.  No counterpart in the
+ * reference, whose stage 1 walks every mesh contiguously; it orders the
+ * walk and changes no result. */
+crx_status crx_mesh_row_triangles(const crx_mesh* mesh, uint32_t* out);
 /* Device pointers of the mesh buffers (colors may be NULL). */
 crx_status crx_mesh_buffers(const crx_mesh* mesh, const float** d_positions, const uint32_t** d_indices,
                             const uint32_t** d_colors);
@@ -575,6 +597,15 @@ typedef struct crx_server_config {
     uint32_t lanes;              /* batches in flight at once (each its own stream, queues and host
                                     thread), 1..8 */
     int32_t resolve;             /* also resolve every request to RGBA8 vertex colours */
+    uint32_t prefetch;           /* 1 (default): after each batch, upload the nodes whose boxes touch
+                                    the drawn nodes' boxes while the cache has slots not used in the
+                                    last 64 batches (own stream, one node at a time); 0: upload on
+                                    demand only */
+    uint32_t shared_stream;      /* 1 (default): the lanes admit, cull, upload and enqueue their
+                                    batches concurrently but launch them on one stream, so batches
+                                    run on the device in the order they were enqueued (a batch is
+                                    not slowed by the next one); 0: each lane its own stream, and
+                                    batches run side by side */
     uint32_t reserved;
 } crx_server_config;
 
@@ -641,7 +672,7 @@ typedef struct crx_server_stats {
     uint32_t resident_nodes;
     uint32_t cache_slots;
     uint32_t max_batch;        /* largest batch drawn */
-    uint32_t reserved;
+    uint32_t prefetches;       /* of the uploads, nodes streamed ahead of the requests that drew them */
 } crx_server_stats;
 
 crx_status crx_server_get_stats(const crx_server* server, crx_server_stats* out);
