@@ -264,7 +264,18 @@ def bench_hbm(ctx, nbytes: int = 1 << 30, reps: int = 20):
     return out
 
 
-def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: int, batch: int, plain: bool = True):
+SERVING_LEGS = (
+    # name, views per batch, lanes, paced (else every request queued at once), lanes share one stream
+    ("batched", 64, 2, True, 0),
+    ("batched_saturated", 64, 2, False, 0),
+    ("plain", 1, 1, True, 0),
+    ("plain_saturated", 1, 1, False, 0),
+    ("batched_again", 64, 2, True, 0),
+    ("plain_again", 1, 1, True, 0),
+)
+
+
+def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, legs=SERVING_LEGS):
     """Out-of-core serving workload: an n x n heightfield cut into tile x tile
     nodes in pinned host memory, half of them device-resident at start, the
     device node cache holding half of the scene; a seeded Poisson request
@@ -344,22 +355,11 @@ def bench_serving(ctx, n: int, tile: int, duration: float, rate: float, lanes: i
                 "evictions": st.evictions, "cache_slots": st.cache_slots}
 
     out = {"workload": f"tiled-heightfield-{n}x{n}-{len(nodes)}nodes", "triangles": total_tris,
-           "pinned_gb": round(sum(node_bytes) / 1e9, 2), "rate_per_s": rate, "duration_s": duration,
-           "batched": run(batch, lanes)}
-    log(f"[serve] batched {out['batched']}")
-    # capacity: every request queued at once
-    out["batched_saturated"] = run(batch, lanes, paced=False)
-    log(f"[serve] batched saturated {out['batched_saturated']}")
-    # the lanes' batches side by side on separate streams instead of in order on one
-    out["batched_separate_streams"] = run(batch, lanes, shared_stream=0)
-    log(f"[serve] batched, separate streams {out['batched_separate_streams']}")
-    out["batched_separate_streams_saturated"] = run(batch, lanes, paced=False, shared_stream=0)
-    log(f"[serve] batched saturated, separate streams {out['batched_separate_streams_saturated']}")
-    if plain:
-        out["plain"] = run(1, 1)
-        log(f"[serve] plain {out['plain']}")
-        out["plain_saturated"] = run(1, 1, paced=False)
-        log(f"[serve] plain saturated {out['plain_saturated']}")
+           "pinned_gb": round(sum(node_bytes) / 1e9, 2), "rate_per_s": rate, "duration_s": duration}
+    for name, max_batch, nlanes, paced, shared in legs:
+        out[name] = run(max_batch, nlanes, paced=paced, shared_stream=shared)
+        log(f"[serve] {name} {out[name]}")
+    if "batched_saturated" in out and "plain_saturated" in out:
         out["capacity_ratio"] = out["batched_saturated"]["views_per_s"] / out["plain_saturated"]["views_per_s"]
     for b in bufs:
         b.free()
@@ -492,7 +492,7 @@ def main() -> int:
     import crx
     if args.serve:
         ctx = crx.Context(0)
-        sv = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate, lanes=2, batch=64)
+        sv = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate)
         b = sv["batched"]
         print(json.dumps({
             "metric": "served views/sec with p50/p99 latency (out-of-core, Poisson arrivals)",
@@ -651,8 +651,7 @@ def main() -> int:
     serving = None
     if args.workload == "heightfield" and not args.no_serve:
         log("[bench] served out-of-core 2G-triangle heightfield at 200 requests/s ...")
-        serving = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate,
-                                lanes=2, batch=64)
+        serving = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate)
 
     out = {
         "metric": "triangles/sec & ms/frame @1080p",

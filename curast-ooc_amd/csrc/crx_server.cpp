@@ -702,7 +702,7 @@ void crx_server_config_default(crx_server_config* out) {
     out->lanes = 2;
     out->resolve = 1;
     out->prefetch = 1;
-    out->shared_stream = 1;
+    out->shared_stream = 0;
     out->reserved = 0;
 }
 

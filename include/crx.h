@@ -601,11 +601,10 @@ typedef struct crx_server_config {
                                     the drawn nodes' boxes while the cache has slots not used in the
                                     last 64 batches (own stream, one node at a time); 0: upload on
                                     demand only */
-    uint32_t shared_stream;      /* 1 (default): the lanes admit, cull, upload and enqueue their
-                                    batches concurrently but launch them on one stream, so batches
-                                    run on the device in the order they were enqueued (a batch is
-                                    not slowed by the next one); 0: each lane its own stream, and
-                                    batches run side by side */
+    uint32_t shared_stream;      /* 0 (default): each lane its own stream, and batches run side by
+                                    side; 1: the lanes admit, cull, upload and enqueue their batches
+                                    concurrently but launch them on one stream, so batches run on
+                                    the device in the order they were enqueued */
     uint32_t reserved;
 } crx_server_config;
 
