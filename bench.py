@@ -264,6 +264,35 @@ def bench_hbm(ctx, nbytes: int = 1 << 30, reps: int = 20):
     return out
 
 
+VALU_PROBE_ITERS = 16384
+
+
+def bench_valu(ctx, iters: int = VALU_PROBE_ITERS, reps: int = 10):
+    """Measured VALU issue rate (crx_probe_valu): v_fma_f32 chains and
+    32-bit add/xor chains on every lane, and the fma chains on 32 lanes and
+    on 1 lane of each wave, as wall-clock launch times; the
+    instructions per launch come from the SQ_INSTS_VALU counter of the same
+    launch, recorded with the instanced stage-1 counters."""
+    sink = ctx.alloc(16)
+    out = {"iters": iters}
+    # VALU instructions per wave and round as compiled: 8 v_fma_f32, or 8 v_add_u32 + 8 v_xor_b32;
+    # the fma chains also with 32 lanes and with 1 lane of each wave active
+    for mode, name, per_round in ((0, "fma_f32", 8), (1, "add_xor_u32", 16), (2, "fma_f32_32_lanes", 8),
+                                  (3, "fma_f32_1_lane", 8), (4, "fma_and_int_interleaved", 12),
+                                  (5, "fma_and_int_alternate_waves", 12)):
+        ctx.probe_valu(iters, mode, sink)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            waves = ctx.probe_valu(iters, mode, sink)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"waves": waves, "ms": round(dt * 1e3, 4),
+                     "nominal_G_insts_per_s": round(waves * per_round * iters / dt / 1e9, 1)}
+    sink.free()
+    return out
+
+
 SERVING_LEGS = (
     # name, views per batch, lanes, paced (else every request queued at once), lanes share one stream
     ("batched", 64, 2, True, 0),
@@ -477,6 +506,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run the 1 GiB copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--probe-valu", action="store_true",
+                    help="only the VALU issue-rate probes (crx_probe_valu), as one JSON object")
     ap.add_argument("--no-autotune", action="store_true",
                     help="multi-view: keep the default tuning instead of running crx_autotune first")
     ap.add_argument("--tuning", default="",
@@ -490,6 +521,11 @@ def main() -> int:
         return 2
 
     import crx
+    if args.probe_valu:
+        ctx = crx.Context(0)
+        print(json.dumps({"valu_probe": bench_valu(ctx)}), flush=True)
+        ctx.close()
+        return 0
     if args.serve:
         ctx = crx.Context(0)
         sv = bench_serving(ctx, args.serve_n, args.serve_tile, args.serve_seconds, args.serve_rate)

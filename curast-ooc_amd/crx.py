@@ -254,6 +254,7 @@ _SIGS = {
     "crx_copy_buffer": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP]),
     "crx_probe_atomic_min": (C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), _VP]),
     "crx_probe_read": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _VP]),
+    "crx_probe_valu": (C.c_int, [_VP, C.c_uint32, C.c_int, _VP, C.POINTER(C.c_uint64), _VP]),
     "crx_check_division": (C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64)]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
@@ -508,6 +509,15 @@ class Context:
         n = C.c_uint64()
         _check(lib().crx_probe_atomic_min(self.handle, C.c_void_p(buf.ptr), log_entries, per_lane,
                                           int(bool(scattered)), C.byref(n), stream))
+        return n.value
+
+    def probe_valu(self, iters: int, mode: int, sink: DeviceBuffer, stream=None) -> int:
+        """Enqueue the VALU issue-rate probe (mode 0 fp32 fma, 1 int32
+        add/xor, 2 fma on 32 lanes, 3 fma on 1 lane, 4 fma and int
+        interleaved, 5 fma and int waves alternating); returns the number of
+        waves launched."""
+        n = C.c_uint64()
+        _check(lib().crx_probe_valu(self.handle, iters, mode, C.c_void_p(sink.ptr), C.byref(n), stream))
         return n.value
 
     # --- kernel-level entry points ------------------------------------------------

@@ -1304,6 +1304,19 @@ crx_status crx_probe_atomic_min(crx_context* ctx, uint64_t* buffer, uint32_t log
     return CRX_OK;
 }
 
+crx_status crx_probe_valu(crx_context* ctx, uint32_t iters, int mode, uint32_t* sink, uint64_t* waves,
+                          void* stream) {
+    if (!ctx || !sink) return fail(CRX_INVALID_ARG, "NULL argument");
+    if (mode < 0 || mode > 5)
+        return fail(CRX_INVALID_ARG, "mode must be 0 (fp32 fma), 1 (int32 add/xor), 2 (fma, 32 lanes), 3 (fma, 1 lane), "
+                                     "4 (fma and int interleaved), 5 (fma and int waves alternating)");
+    CRX_HIP(hipSetDevice(ctx->device));
+    const uint32_t blocks = (uint32_t)ctx->numCUs * 8u;
+    CRX_HIP(launch_valu_probe(iters, mode, blocks, sink, as_stream(stream)));
+    if (waves) *waves = (uint64_t)blocks * 4ull;
+    return CRX_OK;
+}
+
 crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void* stream) {
     if (!framebuffer && numPixels) return fail(CRX_INVALID_ARG, "framebuffer is NULL");
     if (numPixels == 0) return CRX_OK;

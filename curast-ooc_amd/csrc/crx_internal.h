@@ -41,6 +41,7 @@ hipError_t launch_clear_views(const ClearView* views, uint32_t numViews, uint32_
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, int numCUs, hipStream_t stream);
 hipError_t launch_read(const void* src, size_t bytes, int numCUs, uint32_t* sink, hipStream_t stream);
 hipError_t launch_check_division(uint64_t pairs, int numCUs, unsigned long long* out, hipStream_t stream);
+hipError_t launch_valu_probe(uint32_t iters, int mode, uint32_t blocks, uint32_t* sink, hipStream_t stream);
 hipError_t launch_atomic_probe(uint64_t* buf, uint32_t logEntries, uint32_t perLane, int scattered, uint32_t blocks,
                                hipStream_t stream);
 

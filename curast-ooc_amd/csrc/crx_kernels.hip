@@ -2345,6 +2345,78 @@ __global__ __launch_bounds__(256) void k_atomic_probe(uint64_t* __restrict__ buf
     }
 }
 
+// VALU issue-rate probe: every lane runs `iters` rounds of 8 independent
+// dependency chains, so enough waves per SIMD hide each chain's latency and
+// the issue rate is what is measured.  mode 0: v_fma_f32 chains (8 VALU
+// instructions per round); mode 1: 32-bit integer add and xor chains (16
+// per round as compiled: the address and bit arithmetic that dominates
+// stage 1); modes 2 and 3: the fma chains on 32 lanes or one lane of each
+// wave; mode 4: 4 fma and 4 integer chains interleaved in every wave; mode
+// 5: fma chains on even waves and integer chains on odd ones (the two
+// instruction mixes under which a SIMD may issue two VALU instructions in
+// one quad-cycle).  The SQ_INSTS_VALU counter of a launch confirms the
+// count.  The
+// result is stored only when it matches a value no input produces, which
+// keeps the chains live.
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, int mode, uint32_t s1, uint32_t s2,
+                                                    uint32_t* __restrict__ sink) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    // modes 2 and 3 run the fma chains with half of each wave's lanes, or
+    // one lane, active: whether a VALU instruction's cost depends on EXEC
+    if ((mode == 2 && (threadIdx.x & 63u) >= 32u) || (mode == 3 && (threadIdx.x & 63u) != 0u)) return;
+    // mode 5: even waves run the fma chains, odd waves the integer chains
+    if (mode == 5) mode = (threadIdx.x & 64u) ? 1 : 0;
+    if (mode == 4) {
+        // 4 fma chains and 4 integer add/xor chains interleaved in each wave
+        float f[4];
+        uint32_t u[4];
+        const float m = __uint_as_float(0x3F7FFFF0u | (s1 & 15u)), c = __uint_as_float(0x3A800000u | (s2 & 255u));
+        #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            f[k] = (float)(gid + (uint32_t)k) * 1e-6f;
+            u[k] = gid * 8u + (uint32_t)k;
+        }
+        for (uint32_t j = 0; j < iters; j++) {
+            #pragma unroll
+            for (int k = 0; k < 4; k++) {
+                f[k] = __builtin_fmaf(f[k], m, c);
+                u[k] = (u[k] + s1) ^ s2;
+            }
+        }
+        uint32_t r = 0u;
+        #pragma unroll
+        for (int k = 0; k < 4; k++) r ^= u[k] ^ __float_as_uint(f[k]);
+        if (r == 0x9E3779B9u && s1 == 0u) *sink = gid;
+        return;
+    }
+    if (mode != 1) {
+        float a[8];
+        const float m = __uint_as_float(0x3F7FFFF0u | (s1 & 15u)), c = __uint_as_float(0x3A800000u | (s2 & 255u));
+        #pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = (float)(gid + (uint32_t)k) * 1e-6f;
+        for (uint32_t j = 0; j < iters; j++) {
+            #pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = __builtin_fmaf(a[k], m, c);
+        }
+        float r = 0.0f;
+        #pragma unroll
+        for (int k = 0; k < 8; k++) r += a[k];
+        if (__float_as_uint(r) == 0x7F800001u) *sink = gid;
+    } else {
+        uint32_t a[8];
+        #pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = gid * 8u + (uint32_t)k;
+        for (uint32_t j = 0; j < iters; j++) {
+            #pragma unroll
+            for (int k = 0; k < 8; k++) a[k] = (a[k] + s1) ^ s2;
+        }
+        uint32_t r = 0u;
+        #pragma unroll
+        for (int k = 0; k < 8; k++) r ^= a[k];
+        if (r == 0x9E3779B9u && s1 == 0u) *sink = gid;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launch wrappers used by the host glue.
 // ---------------------------------------------------------------------------
@@ -2468,6 +2540,11 @@ hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t strea
 hipError_t launch_atomic_probe(uint64_t* buf, uint32_t logEntries, uint32_t perLane, int scattered, uint32_t blocks,
                                hipStream_t stream) {
     hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, stream, buf, logEntries, perLane, scattered);
+    return hipGetLastError();
+}
+
+hipError_t launch_valu_probe(uint32_t iters, int mode, uint32_t blocks, uint32_t* sink, hipStream_t stream) {
+    hipLaunchKernelGGL(k_valu_probe, dim3(blocks), dim3(256), 0, stream, iters, mode, 0x2545F491u, 0x6C8E9CF5u, sink);
     return hipGetLastError();
 }
 

@@ -433,6 +433,18 @@ crx_status crx_probe_read(crx_context* ctx, const void* src, size_t bytes, uint3
  * against which the stages' atomic rates are read. */
 crx_status crx_probe_atomic_min(crx_context* ctx, uint64_t* buffer, uint32_t logEntries, uint32_t atomicsPerLane,
                                 int scattered, uint64_t* issued, void* stream);
+/* VALU issue-rate probe (no counterpart in the reference): a CUs * 8 x 256
+ * grid whose lanes each run `iters` rounds of 8 independent dependency
+ * chains, mode 0 of v_fma_f32 and mode 1 of 32-bit integer add and xor;
+ * modes 2 and 3 run the fma chains with 32 lanes or 1 lane of each wave
+ * active; mode 4 interleaves 4 fma and 4 integer chains in every wave and
+ * mode 5 runs fma chains on even waves and integer chains on odd ones.
+ * *waves receives the number of waves launched; the caller times the stream
+ * and reads the instruction count from the SQ_INSTS_VALU counter.  The
+ * measured issue peak against which the instanced stage 1 (an
+ * ALU-bound kernel) is read.  sink is a device word written only for a
+ * result no input produces. */
+crx_status crx_probe_valu(crx_context* ctx, uint32_t iters, int mode, uint32_t* sink, uint64_t* waves, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Camera helpers                                                       */
