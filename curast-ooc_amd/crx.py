@@ -159,7 +159,8 @@ class Tuning(C.Structure):
                 ("stage2_waves_per_cu", C.c_uint32), ("stage3_waves_per_cu", C.c_uint32),
                 ("stage1_cull_order", C.c_uint32), ("stage1_order", C.c_uint32),
                 ("stage1_fast_tiny", C.c_uint32), ("stage1_tris_per_lane", C.c_uint32),
-                ("stage1_layout", C.c_uint32), ("stage1_strip_rows", C.c_uint32)]
+                ("stage1_layout", C.c_uint32), ("stage1_strip_rows", C.c_uint32),
+                ("stage1_strip_batches", C.c_uint32)]
 
 
 class DrawStats(C.Structure):

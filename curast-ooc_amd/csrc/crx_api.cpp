@@ -132,6 +132,8 @@ crx_status check_tuning(const crx_tuning& t) {
     if (t.stage1_layout > 2) return fail(CRX_INVALID_ARG, "tuning.stage1_layout must be 0, 1 or 2");
     if (t.stage1_strip_rows == 0 || t.stage1_strip_rows > 256)
         return fail(CRX_INVALID_ARG, "tuning.stage1_strip_rows must be in [1, 256]");
+    if (t.stage1_strip_batches == 0 || t.stage1_strip_batches > 8)
+        return fail(CRX_INVALID_ARG, "tuning.stage1_strip_batches must be in [1, 8]");
     if (t.max_nontrivial == 0 || t.max_huge == 0) return fail(CRX_INVALID_ARG, "tuning queue capacities must be > 0");
     if (t.stage2_waves_per_cu == 0 || t.stage3_waves_per_cu == 0 || t.stage2_waves_per_cu > 64 ||
         t.stage3_waves_per_cu > 64 || t.stage1_waves_per_cu > 64)
@@ -254,8 +256,13 @@ uint32_t row_hint_of(const void* indices) {
 // Triangles per row of a mesh stored in rows (a grid, a lat-long sphere):
 // over a window of triangles in the middle of the index buffer, the most
 // common distance from a vertex's first use to its first use more than 64
-// triangles later (the same vertex in the next row); 0 when no distance
-// covers a quarter of the vertices seen.
+// triangles later (the same vertex in the next row), rounded down to even;
+// 0 when no distance covers a quarter of the vertices seen.  In a row of
+// quads cut along either diagonal a vertex first appears one triangle
+// further along in its second row than in its first, so the distance is the
+// row's length plus one; a row of quads holds an even number of triangles,
+// and rows of even length keep every row's pairs of consecutive triangles
+// on quads.
 uint32_t detect_row_triangles(const uint32_t* idx, uint64_t T) {
     const uint64_t W = std::min<uint64_t>(T, 1ull << 18);
     if (W < 1024) return 0;
@@ -281,7 +288,7 @@ uint32_t detect_row_triangles(const uint32_t* idx, uint64_t T) {
     for (const auto& g : gaps)
         if (g.second > count || (g.second == count && g.first < best)) { best = g.first; count = g.second; }
     if (count * 4ull < first.size() || best > 0xFFFFFFFFull) return 0;
-    return (uint32_t)best;
+    return (uint32_t)best & ~1u;
 }
 
 // Host records and metadata layout of a draw of numViews >= 1 views sharing
@@ -334,9 +341,12 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     const uint64_t batchTris = 1ull << D.batchShift;
     // pairs of consecutive triangles per lane (k_stage1_pairs), for draws
     // large enough that every wave of the grid claims many 128-triangle
-    // batches (a small draw keeps the short per-batch walk of k_stage1)
-    D.runs = tu.stage1_layout != 0 && numViews == 1 && !manyInstances && !D.pu16 && D.block == 256 &&
-             D.trisPerLane == 2;
+    // batches (a small draw keeps the short per-batch walk of k_stage1).  An
+    // IX_PU16 draw takes them only with stage1_tris_per_lane set to 2: its
+    // sweep is bound by the decode's VALU work, which the pairs do not cut
+    // (1.98 ms against 1.91 ms for k_stage1 on the 400 M-triangle heightfield)
+    D.runs = tu.stage1_layout != 0 && numViews == 1 && !manyInstances && D.block == 256 && D.trisPerLane == 2 &&
+             (!D.pu16 || tu.stage1_tris_per_lane == 2);
     if (D.runs) {
         uint64_t tris = 0;
         for (uint32_t k = 0; k < D.numItems; k++) tris += list[k].numTriangles;
@@ -375,14 +385,16 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         it.amin[0] = m.aabb.min.x; it.amin[1] = m.aabb.min.y; it.amin[2] = m.aabb.min.z;
         it.numVertices = m.numVertices;
         // pair layout (batches of one wave's 128 triangles): units of
-        // stage1_strip_rows rows of a 128-triangle strip of the mesh's rows
-        // (a mesh stored in rows of at least eight strips), else of rows of
-        // 128 triangles (a contiguous walk)
+        // stage1_strip_rows rows of a strip of stage1_strip_batches batches
+        // of the mesh's rows (a mesh stored in rows of at least eight
+        // batches), else of rows of 128 triangles (a contiguous walk)
         constexpr uint64_t waveTris = 128;
         const uint32_t rowHint = (D.runs && tu.stage1_layout == 2) ? row_hint_of(m.indices) : 0u;
         const bool strips = rowHint >= 8 * waveTris;
+        const uint64_t stripTris = waveTris * (strips ? tu.stage1_strip_batches : 1u);
         it.rowTris = strips ? rowHint : (uint32_t)waveTris;
-        it.strips = strips ? (uint32_t)((rowHint + waveTris - 1) / waveTris) : 1u;
+        it.unitCols = strips ? tu.stage1_strip_batches : 1u;
+        it.strips = strips ? (uint32_t)((rowHint + stripTris - 1) / stripTris) : 1u;
         it.unitRows = tu.stage1_strip_rows;
         const uint64_t rows = ((uint64_t)m.numTriangles + it.rowTris - 1) / it.rowTris;
         it.unitSegs = (uint32_t)((rows + it.unitRows - 1) / it.unitRows);
@@ -484,7 +496,7 @@ hipError_t enqueue_stages(crx_context* ctx, const DrawPlan& D, const RasterParam
     }
     record(ctx, T_S1, stream);
     if (D.numViews == 1 && !viewsKernel && D.runs)
-        e = launch_stage1_pairs(P, D.block, D.g1, stream);
+        e = launch_stage1_pairs(P, D.block, D.g1, D.pu16, stream);
     else if (D.numViews == 1 && !viewsKernel)
         e = launch_stage1(P, D.block, D.trisPerLane, D.g1, D.pu16, D.instanceBoxes, stream);
     else e = launch_stage1_views(P, D.block, D.trisPerLane, D.g1, D.pu16, stream);
@@ -1109,7 +1121,8 @@ void crx_tuning_default(crx_tuning* out) {
     out->stage1_fast_tiny = 0;          // the exact path divides cheaply (short sequences)
     out->stage1_tris_per_lane = 0;
     out->stage1_layout = 2;
-    out->stage1_strip_rows = 16;
+    out->stage1_strip_rows = 8;
+    out->stage1_strip_batches = 1;
 }
 
 crx_status crx_context_create(int device, const crx_tuning* tuning, crx_context** out) {
@@ -1579,6 +1592,14 @@ crx_status crx_mesh_create_compressed(crx_context* ctx, const float* positions, 
         crx_mesh_destroy(m);
         return st;
     }
+    // the row length of the uncompressed index order, kept for the packed stream
+    if (m->indices) {
+        m->rowTris = detect_row_triangles(indices, numTriangles);
+        if (m->rowTris) {
+            std::lock_guard<std::mutex> lk(g_rowHintsMu);
+            g_rowHints[m->indices] = m->rowTris;
+        }
+    }
     *out = m;
     return CRX_OK;
 }
@@ -1749,6 +1770,7 @@ crx_status crx_autotune(crx_context* ctx, crx_scene* scene, const crx_view* view
         {&crx_tuning::stage1_fast_tiny, {0, 1}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_layout, {0, 1, 2}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage1_strip_rows, {2, 4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
+        {&crx_tuning::stage1_strip_batches, {1, 2, 4}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage2_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::stage3_waves_per_cu, {4, 8, 16, 32}, CRX_TUNE_SCHEDULE},
         {&crx_tuning::small_threshold, {32, 64, 128, 256, 512}, CRX_TUNE_ROUTING},

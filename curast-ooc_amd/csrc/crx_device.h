@@ -35,14 +35,16 @@ struct DrawItem {
     uint32_t numVertices;       // CMesh::numVertices (IX_PU16: bounds the 8-byte vertex reads)
     // Pair layout (k_stage1_pairs, whose batch is one wave's 128
     // triangles): the triangles as rows of rowTris triangles cut into
-    // `strips` strips of 128; a work unit is strip s = unit / unitSegs over
-    // rows [g * unitRows, (g + 1) * unitRows), g = unit % unitSegs, drawn by
-    // one wave row after row.  A contiguous walk is rowTris = 128, strips = 1.
-    // firstBatch then counts units.
+    // `strips` strips of unitCols batches of 128; a work unit is strip
+    // s = unit / unitSegs over rows [g * unitRows, (g + 1) * unitRows),
+    // g = unit % unitSegs, drawn by one wave row after row, each row's
+    // batches left to right.  A contiguous walk is rowTris = 128, strips = 1,
+    // unitCols = 1.  firstBatch then counts units.
     uint32_t rowTris;
     uint32_t strips;
     uint32_t unitRows;
     uint32_t unitSegs;
+    uint32_t unitCols;
 };
 
 // Per-instance record for stage 2, stage 3 and resolve.

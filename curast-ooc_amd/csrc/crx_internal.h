@@ -28,7 +28,7 @@ hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPer
                          bool instanceBoxes, hipStream_t stream);
 // Pairs of consecutive triangles per lane, block 256 (k_stage1_pairs): the
 // items carry the strip layout (DrawItem::rowTris ...).
-hipError_t launch_stage1_pairs(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream);
+hipError_t launch_stage1_pairs(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream);
 hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t trisPerLane, uint32_t grid, bool pu16,
                                hipStream_t stream);
 hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t stream);

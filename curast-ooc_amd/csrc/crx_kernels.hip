@@ -462,6 +462,7 @@ __global__ __launch_bounds__(1024) void k_plan_items(const crx_cmesh* __restrict
         it.strips = 1u;
         it.unitRows = 1u;
         it.unitSegs = 1u;
+        it.unitCols = 1u;
         items[k] = it;
         first += ((uint64_t)n + (1ull << batchShift) - 1u) >> batchShift;
     }
@@ -1344,23 +1345,25 @@ constexpr uint32_t kPairNone = 0, kPairGrid = 1, kPairFan = 2;
 
 // The layout of the item a unit walk is in (wave-uniform).
 struct RunLayout {
-    uint32_t rowTris, unitRows, unitSegs, rows, firstUnit;
+    uint32_t rowTris, unitRows, unitSegs, unitCols, rows, firstUnit;
 };
 
-// A batch of the unit walk: unit (absolute), its strip, current row and the
-// unit's row end (wave-uniform scalars).
+// A batch of the unit walk: unit (absolute), its strip, current row, the
+// batch's column in the strip and the unit's row end (wave-uniform scalars).
 struct RunBatch {
-    uint32_t unit, strip, row, rowEnd;
+    uint32_t unit, strip, row, col, rowEnd;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Triangles [base, end) of batch b (one wave's 128-triangle strip of a row).
+// Triangles [base, end) of batch b (one wave's 128 triangles of a row; a
+// column past the row's end, in the last strip, is empty).
 __device__ __forceinline__ void run_bounds(const S1Item& it, const RunLayout& L, const RunBatch& b, uint32_t& base,
                                            uint32_t& end) {
     constexpr uint32_t waveTris = 2u * kWave;
-    const uint64_t start = (uint64_t)b.row * L.rowTris + (uint64_t)b.strip * waveTris;
-    const uint64_t w = min((uint64_t)waveTris, (uint64_t)L.rowTris - (uint64_t)b.strip * waveTris);
+    const uint64_t x = ((uint64_t)b.strip * L.unitCols + b.col) * waveTris;
+    const uint64_t start = (uint64_t)b.row * L.rowTris + x;
+    const uint64_t w = x < L.rowTris ? min((uint64_t)waveTris, (uint64_t)L.rowTris - x) : 0ull;
     base = uni((uint32_t)min(start, (uint64_t)it.numTriangles));
     end = uni((uint32_t)min(start + w, (uint64_t)it.numTriangles));
 }
@@ -1373,6 +1376,7 @@ __device__ __forceinline__ RunBatch run_first(const RunLayout& L, uint32_t u) {
     RunBatch b;
     b.unit = uni(u);
     b.strip = s;
+    b.col = 0u;
     b.row = uni(g * L.unitRows);
     b.rowEnd = uni(min(L.rows, g * L.unitRows + L.unitRows));
     return b;
@@ -1392,16 +1396,23 @@ __device__ __forceinline__ uint32_t run_claim(uint32_t* claims) {
     return x + 8u * k;
 }
 
-// The batch after `a` in this wave's walk of the item: the unit's next row,
-// else the first row of a newly claimed unit of the same item; more = 0 when
-// the claimed unit lies past the item (left in `pending` for the next pass;
-// the batch returned is then `a` again, a valid address to reload).
+// The batch after `a` in this wave's walk of the item: the next column of
+// the row (while it starts inside the row), else the unit's next row, else
+// the first row of a newly claimed unit of the same item; more = 0 when the
+// claimed unit lies past the item (left in `pending` for the next pass; the
+// batch returned is then `a` again, a valid address to reload).
 __device__ __forceinline__ RunBatch run_next(const RunLayout& L, uint32_t* claims, uint32_t iend, const RunBatch& a,
                                              uint32_t& more, uint32_t& pending) {
     more = 1u;
+    if (a.col + 1u < L.unitCols && ((uint64_t)a.strip * L.unitCols + a.col + 1u) * (2u * kWave) < L.rowTris) {
+        RunBatch b = a;
+        b.col = uni(a.col + 1u);
+        return b;
+    }
     if (a.row + 1u < a.rowEnd) {
         RunBatch b = a;
         b.row = uni(a.row + 1u);
+        b.col = 0u;
         return b;
     }
     const uint32_t u = run_claim(claims);
@@ -1414,14 +1425,41 @@ __device__ __forceinline__ RunBatch run_next(const RunLayout& L, uint32_t* claim
 // Indices of the lane's two triangles (lanes past `end` read zeros).  A
 // non-indexed item reads the same words of its positions (at least 36
 // bytes per triangle) and replaces them with the vertex numbers 3t + v.
+// IX_PU16: the raw stream words, cut into indices by pair_pu16_decode.  The
+// wave's triangles [base, end) hold bits [3*bits*base, 3*bits*end) of the
+// stream, at most 384 words from the word holding the first bit; lane l
+// loads words 6l .. 6l + 5 of that range (zeros past the range or the
+// stream), with the same two buffer loads.
+constexpr uint32_t kPairPu16Words = 392;   // 384 words, the word after them and padding
+
+__device__ __forceinline__ void pair_pu16_range(const S1Item& it, uint32_t base, uint32_t end, uint64_t& firstWord,
+                                                uint32_t& bit0, uint32_t& avail) {
+    const uint64_t b = (uint64_t)it.bits * 3ull * base;
+    firstWord = b >> 5;
+    bit0 = (uint32_t)b & 31u;
+    const uint64_t numWords = ((uint64_t)it.numTriangles * 3ull * it.bits + 31u) >> 5;
+    const uint64_t need = ((uint64_t)bit0 + 3ull * it.bits * (end - base) + 31u) >> 5;
+    avail = (uint32_t)min(need, numWords - firstWord);
+}
+
+template <bool PU16>
 __device__ __forceinline__ void pair_indices(const S1Item& it, uint32_t base, uint32_t end, uint32_t lane,
                                              uint32_t (&I)[6]) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(it.indices + 3ull * base, 12u * (end - base));
+    __amdgpu_buffer_rsrc_t rs;
+    if constexpr (PU16) {
+        uint64_t firstWord;
+        uint32_t bit0, avail;
+        pair_pu16_range(it, base, end, firstWord, bit0, avail);
+        if (it.bits == 0u || it.bits > 32u) avail = 0u;     // BitEdit::readU32 reads nothing
+        rs = buffer_rsrc(it.indices + firstWord, 4u * avail);
+    } else {
+        rs = buffer_rsrc(it.indices + 3ull * base, 12u * (end - base));
+    }
     const uint4 v = buffer_load_u4(rs, 24u * lane);
     const uint2 w = buffer_load_u2(rs, 24u * lane + 16u);
     I[0] = v.x; I[1] = v.y; I[2] = v.z; I[3] = v.w; I[4] = w.x; I[5] = w.y;
-    if (it.indexMask == 0u) {          // wave-uniform; no loads in this branch
+    if (!PU16 && it.indexMask == 0u) {          // wave-uniform; no loads in this branch
         const uint32_t t = min(base + 2u * lane, it.numTriangles - 1u), t1 = min(t + 1u, it.numTriangles - 1u);
         I[0] = 3u * t; I[1] = 3u * t + 1u; I[2] = 3u * t + 2u;
         I[3] = 3u * t1; I[4] = 3u * t1 + 1u; I[5] = 3u * t1 + 2u;
@@ -1432,19 +1470,88 @@ __device__ __forceinline__ void pair_indices(const S1Item& it, uint32_t base, ui
 #endif
 }
 
+// IX_PU16: the six indices of the lane's two triangles from the stream words
+// of pair_indices, through the wave's LDS words (BitEdit::readU32's
+// arithmetic, BitEdit.h:8-26, plus index_min).  A lane past the batch's end
+// cuts the batch's last triangle, so every read stays inside the words
+// loaded.
+__device__ __forceinline__ void pair_pu16_decode(const S1Item& it, uint32_t base, uint32_t end, uint32_t lane,
+                                                 uint32_t* words, uint32_t (&I)[6]) {
+    const uint32_t bits = it.bits;
+    if (bits == 0u || bits > 32u) {                     // wave-uniform
+        #pragma unroll
+        for (int j = 0; j < 6; j++) I[j] = it.indexMin;
+        return;
+    }
+    uint64_t firstWord;
+    uint32_t bit0, avail;
+    pair_pu16_range(it, base, end, firstWord, bit0, avail);
+    uint32_t* row = words + 6u * lane;
+    *reinterpret_cast<uint2*>(row) = make_uint2(I[0], I[1]);
+    *reinterpret_cast<uint2*>(row + 2) = make_uint2(I[2], I[3]);
+    *reinterpret_cast<uint2*>(row + 4) = make_uint2(I[4], I[5]);
+    wave_sync();
+    const uint32_t last = end > base ? end - base - 1u : 0u;
+    const uint32_t mask = bits >= 32u ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    uint32_t v[6];
+    #pragma unroll
+    for (int h = 0; h < 2; h++) {
+        // bit offset of the triangle's first index from the range's first word
+        uint32_t off = bit0 + 3u * bits * min(2u * lane + (uint32_t)h, last);
+        #pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const uint32_t wd = off >> 5;
+            v[3 * h + j] = __builtin_amdgcn_alignbit(words[wd + 1u], words[wd], off & 31u) & mask;
+            off += bits;
+        }
+    }
+    wave_sync();
+    #pragma unroll
+    for (int j = 0; j < 6; j++) I[j] = v[j] + it.indexMin;
+}
+
 // The sharing pattern of the wave's pairs, and the gathers: A's three
-// vertices, then B's new vertex (a pattern) or B's three (none).
-__device__ __forceinline__ uint32_t pair_vertices(const S1Item& it, const uint32_t (&I)[6], f3 (&S)[6]) {
+// vertices, then B's new vertex (a pattern) or B's three (none).  A lane
+// whose second triangle (or both) lies past the batch's end takes indices
+// that fit both patterns, so a partly filled batch keeps the pattern of its
+// drawn pairs (those triangles are not drawn).  IX_PU16: the indices are
+// cut out of the stream words first, and the gathers are raw windows until
+// pu16_vertex_decode.
+template <bool PU16>
+__device__ __forceinline__ uint32_t pair_vertices(const S1Item& it, uint32_t base, uint32_t end, uint32_t lane,
+                                                  uint32_t* words, uint32_t (&I)[6], f3 (&S)[6]) {
+    if constexpr (PU16) pair_pu16_decode(it, base, end, lane, words, I);
+    const uint32_t rel = 2u * lane;
+    if (base + rel >= end) {
+        I[1] = I[2] = I[0];
+    }
+    if (base + rel + 1u >= end) {
+        I[3] = I[0];
+        I[4] = I[2];
+        I[5] = I[1];
+    }
     const bool grid = I[3] == I[0] && I[4] == I[2];
     const bool fan = I[3] == I[0] && I[5] == I[1];
     const uint32_t pat = __ballot(!grid) == 0ull ? kPairGrid : __ballot(!fan) == 0ull ? kPairFan : kPairNone;
-    S[0] = load_pos_g(it.positions, I[0]);
-    S[1] = load_pos_g(it.positions, I[1]);
-    S[2] = load_pos_g(it.positions, I[2]);
-    S[3] = load_pos_g(it.positions, pat == kPairGrid ? I[5] : pat == kPairFan ? I[4] : I[3]);
-    if (pat == kPairNone) {
-        S[4] = load_pos_g(it.positions, I[4]);
-        S[5] = load_pos_g(it.positions, I[5]);
+    const uint32_t i3 = pat == kPairGrid ? I[5] : pat == kPairFan ? I[4] : I[3];
+    if constexpr (PU16) {
+        S[0] = pu16_vertex_issue(it, I[0]);
+        S[1] = pu16_vertex_issue(it, I[1]);
+        S[2] = pu16_vertex_issue(it, I[2]);
+        S[3] = pu16_vertex_issue(it, i3);
+        if (pat == kPairNone) {
+            S[4] = pu16_vertex_issue(it, I[4]);
+            S[5] = pu16_vertex_issue(it, I[5]);
+        }
+    } else {
+        S[0] = load_pos_g(it.positions, I[0]);
+        S[1] = load_pos_g(it.positions, I[1]);
+        S[2] = load_pos_g(it.positions, I[2]);
+        S[3] = load_pos_g(it.positions, i3);
+        if (pat == kPairNone) {
+            S[4] = load_pos_g(it.positions, I[4]);
+            S[5] = load_pos_g(it.positions, I[5]);
+        }
     }
     return pat;
 }
@@ -1454,8 +1561,8 @@ __device__ __forceinline__ uint32_t pair_vertices(const S1Item& it, const uint32
 // loads of batch i+2 are in flight.  Units (strip, group of rows) are dealt
 // to the workgroups by P.order as s1_range deals batches; a workgroup walks
 // its unit's rows in turn.  body(it, base, end, S, pattern) draws one batch.
-template <typename Body>
-__device__ __forceinline__ void s1_pairs_sweep(const RasterParams& P, Body&& body) {
+template <bool PU16, typename Body>
+__device__ __forceinline__ void s1_pairs_sweep(const RasterParams& P, uint32_t* words, Body&& body) {
     const uint32_t numUnits = P.batchCount ? *kconst(P.batchCount) : P.numBatches;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const auto items = kconst(P.items);
@@ -1467,6 +1574,7 @@ __device__ __forceinline__ void s1_pairs_sweep(const RasterParams& P, Body&& bod
         L.rowTris = items[item].rowTris;
         L.unitRows = items[item].unitRows;
         L.unitSegs = items[item].unitSegs;
+        L.unitCols = items[item].unitCols;
         L.rows = (uint32_t)(((uint64_t)it.numTriangles + L.rowTris - 1u) / L.rowTris);
         L.firstUnit = it.firstBatch;
         const uint32_t iend = it.endBatch;
@@ -1475,20 +1583,28 @@ __device__ __forceinline__ void s1_pairs_sweep(const RasterParams& P, Body&& bod
         const RunBatch b0 = run_first(L, unit);
         run_bounds(it, L, b0, base0, end0);
         uint32_t I[6], J[6];
-        pair_indices(it, base0, end0, lane, I);
+        pair_indices<PU16>(it, base0, end0, lane, I);
         uint32_t more1;
         RunBatch b1 = run_next(L, P.claims, iend, b0, more1, pending);
         run_bounds(it, L, b1, base1, end1);
-        pair_indices(it, base1, end1, lane, J);
+        pair_indices<PU16>(it, base1, end1, lane, J);
         f3 S[6];
-        uint32_t pat = pair_vertices(it, I, S);
+        uint32_t pat = pair_vertices<PU16>(it, base0, end0, lane, words, I, S);
         for (;;) {
             f3 Sn[6];
-            const uint32_t patN = pair_vertices(it, J, Sn);
+            const uint32_t patN = pair_vertices<PU16>(it, base1, end1, lane, words, J, Sn);
             uint32_t more2 = 0u;
             const RunBatch b2 = more1 ? run_next(L, P.claims, iend, b1, more2, pending) : b1;
             run_bounds(it, L, b2, base2, end2);
-            pair_indices(it, base2, end2, lane, J);
+            pair_indices<PU16>(it, base2, end2, lane, J);
+            if constexpr (PU16) {
+                #pragma unroll
+                for (int j = 0; j < 4; j++) S[j] = pu16_vertex_decode(it, S[j]);
+                if (pat == kPairNone) {
+                    S[4] = pu16_vertex_decode(it, S[4]);
+                    S[5] = pu16_vertex_decode(it, S[5]);
+                }
+            }
             body(it, base0, end0, S, pat);
             if (!more1) break;
             base0 = base1;
@@ -1508,11 +1624,16 @@ __device__ __forceinline__ void s1_pairs_sweep(const RasterParams& P, Body&& bod
 
 // Stage 1 over pairs of consecutive triangles for draws of one view whose
 // items are drawn over few instances (the others take k_stage1).
-template <int BLOCK>
+template <int BLOCK, bool PU16>
 __global__ __launch_bounds__(BLOCK) void k_stage1_pairs(RasterParams P) {
     const ViewParams V0 = kconst(P.views)[0];
     const auto transforms = kconst(V0.transforms);
-    s1_pairs_sweep(P, [&](const S1Item& it, uint32_t base, uint32_t end, const f3 (&S)[6], uint32_t pat)
+    uint32_t* words = nullptr;
+    if constexpr (PU16) {
+        __shared__ __align__(16) uint32_t pairWords[BLOCK / kWave][kPairPu16Words];
+        words = pairWords[threadIdx.x / kWave];
+    }
+    s1_pairs_sweep<PU16>(P, words, [&](const S1Item& it, uint32_t base, uint32_t end, const f3 (&S)[6], uint32_t pat)
                           __attribute__((always_inline)) {
         const uint32_t ta = base + 2u * (threadIdx.x & (kWave - 1)), tb = ta + 1u;
         for (uint32_t j = 0; j < it.instCount; j++) {
@@ -2503,9 +2624,10 @@ hipError_t launch_stage1(const RasterParams& P, uint32_t block, uint32_t trisPer
     return launch_s1_any(P, block, trisPerLane, grid, pu16, false, instanceBoxes, stream);
 }
 
-hipError_t launch_stage1_pairs(const RasterParams& P, uint32_t block, uint32_t grid, hipStream_t stream) {
+hipError_t launch_stage1_pairs(const RasterParams& P, uint32_t block, uint32_t grid, bool pu16, hipStream_t stream) {
     if (block != 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_stage1_pairs<256>), dim3(grid), dim3(256), 0, stream, P);
+    if (pu16) hipLaunchKernelGGL((k_stage1_pairs<256, true>), dim3(grid), dim3(256), 0, stream, P);
+    else hipLaunchKernelGGL((k_stage1_pairs<256, false>), dim3(grid), dim3(256), 0, stream, P);
     return hipGetLastError();
 }
 

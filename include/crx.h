@@ -197,9 +197,10 @@ typedef struct crx_tuning {
                                      (every instanced crx_raster_stage1 draw) takes 1 and a grid
                                      of 64 waves per CU, an IX_PU16 draw 2 and 64 waves per CU,
                                      others 2 and stage1_waves_per_cu */
-    uint32_t stage1_layout;       /* draws of one view over uncompressed meshes drawn over < 4
-                                     instances, with 2 triangles per lane, a block of 256 and at
-                                     least 16 batches of 128 triangles per wave of the grid:
+    uint32_t stage1_layout;       /* draws of one view over meshes drawn over < 4 instances (IX_PU16
+                                     meshes only with stage1_tris_per_lane = 2), with 2 triangles
+                                     per lane, a block of 256 and at least 16 batches of 128
+                                     triangles per wave of the grid:
                                      0 = lane t takes triangles t and t + stage1_block;
                                      1 = lane t takes triangles 2t and 2t + 1, and a wave whose pairs
                                      all share two vertices as the two triangles of a quad do gathers
@@ -208,9 +209,11 @@ typedef struct crx_tuning {
                                      2 (default) = 1, and a mesh stored in rows (a grid: the row
                                      length detected from the index buffer by crx_mesh_create) is
                                      walked in units of stage1_strip_rows consecutive rows of a
-                                     128-triangle strip, so the vertex row two rows share is read
-                                     once */
-    uint32_t stage1_strip_rows;   /* rows (or batches) per unit of layouts 1 and 2, 1..256 (default 16) */
+                                     strip of stage1_strip_batches batches of 128 triangles, so the
+                                     vertex row two rows share is read once */
+    uint32_t stage1_strip_rows;   /* rows (or batches) per unit of layouts 1 and 2, 1..256 (default 8) */
+    uint32_t stage1_strip_batches; /* layout 2: batches of 128 triangles side by side in a strip, 1..8
+                                      (default 1); a unit's wave draws each row's batches left to right */
 } crx_tuning;
 
 /* Per-draw counters read back after a draw. */
@@ -502,7 +505,8 @@ crx_status crx_mesh_create_device(crx_context* ctx, const float* d_positions, ui
 crx_status crx_mesh_destroy(crx_mesh* mesh);
 crx_status crx_mesh_info(const crx_mesh* mesh, uint32_t* numTriangles, uint32_t* numVertices, crx_box3* aabb);
 /* Triangles per row that crx_mesh_create found in the index buffer of a mesh
- * stored in rows (0 = none): the stage-1 walk of tuning.stage1_layout 2
+ * stored in rows (0 = none; an even count: a row of quads holds two
+ * triangles per quad): the stage-1 walk of tuning.stage1_layout 2
  * uses it to read the vertices two rows share once.  No counterpart in the
  * reference, whose stage 1 walks every mesh contiguously; it orders the
  * walk and changes no result. */
