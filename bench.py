@@ -429,6 +429,26 @@ def cpu_baseline(pos, idx, col, worlds, view, proj, compressed: bool = False, se
     return out, fb, cb
 
 
+def instances_valu_roofline(stage1_ms: float):
+    """VALU roofline of the 3000-instance stage 1: wave64 VALU instructions
+    per launch (SQ_INSTS_VALU, from the committed counter summary when it was
+    taken on these kernels) over the live stage-1 time, against 1024 SIMDs
+    issuing at most two wave64 VALU instructions per quad-cycle (4 clocks)
+    at 2.4 GHz; issue_slots_busy is the counter pass's share of quad-cycles
+    that issued any."""
+    prof = load_profile("stage1_instances_3000x1M_1080p_valu")
+    if not prof:
+        return None
+    peak = prof["simds"] * prof["clock_ghz"] * 1e9 * prof["valu_insts_per_simd_quad_cycle"] / 4.0
+    single = prof["simds"] * prof["clock_ghz"] * 1e9 / 4.0
+    achieved = prof["valu_insts_per_launch"] / (stage1_ms * 1e-3)
+    return {"bound": "valu", "achieved": round(achieved / 1e9, 2), "peak": round(peak / 1e9, 1),
+            "unit": "G wave64 VALU instructions/s", "frac": round(achieved / peak, 4),
+            "single_issue_peak": round(single / 1e9, 1), "issue_slots_busy": prof["issue_slots_busy"],
+            "kernel": prof["kernel"], "valu_insts_per_launch": prof["valu_insts_per_launch"],
+            "stage1_ms_mean": round(stage1_ms, 4), "source": prof["source"]}
+
+
 def bench_instances(ctx, steps: int, warmup: int, count: int = 3000):
     """3000 instances of the 1M-triangle sphere at 1920x1080 (one mesh read
     once, transformed per instance): frame time over `steps` frames, stage
@@ -466,16 +486,9 @@ def bench_instances(ctx, steps: int, warmup: int, count: int = 3000):
     out = {"workload": f"sphere-1M-x{count}-instances-1080p", "triangle_instances": T,
            "ms_per_frame": 1e3 * el / steps, "triangle_instances_per_s": T * steps / el,
            "stage1_ms_mean": round(float(np.mean(s1)), 4), "pipeline_ms_median": round(float(np.median(pipe)), 4)}
-    prof = load_profile("stage1_instances_3000x1M_1080p_valu")
-    if prof:
-        # wave64 VALU instructions per launch over the live stage-1 time, against
-        # 1024 SIMDs issuing one wave64 VALU instruction per 4 cycles at 2.4 GHz
-        peak = prof["simds"] * prof["clock_ghz"] * 1e9 / prof["cycles_per_valu_inst"]
-        achieved = prof["valu_insts_per_launch"] / (out["stage1_ms_mean"] * 1e-3)
-        out["roofline"] = {"bound": "valu", "achieved": round(achieved / 1e9, 2), "peak": round(peak / 1e9, 1),
-                           "unit": "G wave64 VALU instructions/s", "frac": round(achieved / peak, 4),
-                           "kernel": prof["kernel"], "valu_insts_per_launch": prof["valu_insts_per_launch"],
-                           "source": prof["source"]}
+    rl = instances_valu_roofline(out["stage1_ms_mean"])
+    if rl:
+        out["roofline"] = rl
     for b in (fb, cb):
         b.free()
     scene.close()
@@ -582,9 +595,14 @@ def main() -> int:
         ctx.close()
         return 0
 
-    for _ in range(args.warmup):
-        scene.render(v, fb, cb, settings)
+    draw_stats = None
+    for _ in range(max(1, args.warmup)):
+        draw_stats = scene.render(v, fb, cb, settings)
     ctx.synchronize()
+    # the stage-1 kernel the draw took (crx_draw_stats::stage1_kernel)
+    kernel = {0: ("k_stage1<256, 2, true, false>" if args.compressed else "k_stage1"), 1: "k_stage1_views",
+              2: "k_stage1_pairs<256, %s>" % ("true" if args.compressed else "false")}.get(
+        draw_stats.stage1_kernel, "k_stage1")
 
     atomics = bench_atomics(ctx)
     log(f"[bench] atomicMin probe: {atomics}")
@@ -639,11 +657,11 @@ def main() -> int:
     achieved = algo / (mean_s1 * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_stage1", "algorithmic_bytes_per_launch": algo,
+                "kernel": kernel, "algorithmic_bytes_per_launch": algo,
                 "stage1_ms_mean": round(mean_s1, 4)}
     # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, used only
     # when they were measured on these kernel sources
-    prof = None if args.compressed else load_profile(f"stage1_{args.workload}_traffic")
+    prof = load_profile(f"stage1_{args.workload}{'_ix_pu16' if args.compressed else ''}_traffic")
     if prof:
         roofline["traffic"] = prof["traffic_bytes"]
         roofline["traffic_source"] = prof["source"]
@@ -652,6 +670,10 @@ def main() -> int:
     if args.workload == "instances":
         roofline["note"] = ("the mesh is read once for all 3000 instances: this stage 1 is bound by the "
                             "per-instance arithmetic (VALU), not by HBM")
+        valu = instances_valu_roofline(mean_s1)
+        if valu:
+            valu["hbm"] = roofline
+            roofline = valu
     got_fb = fb.download(np.uint64, WIDTH * HEIGHT)
     got_cb = cb.download(np.uint64, WIDTH * HEIGHT)
 

@@ -179,7 +179,8 @@ TUNE_SCHEDULE, TUNE_ROUTING, TUNE_QUEUES = 1, 2, 4
 class ServerConfig(C.Structure):
     _fields_ = [("node_cache_bytes", C.c_uint64), ("frame_bytes", C.c_uint64), ("max_batch_views", C.c_uint32),
                 ("lanes", C.c_uint32), ("resolve", C.c_int32), ("prefetch", C.c_uint32),
-                ("shared_stream", C.c_uint32), ("reserved", C.c_uint32)]
+                ("shared_stream", C.c_uint32), ("lane_backlog", C.c_uint32), ("merge_backlog", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class RequestResult(C.Structure):

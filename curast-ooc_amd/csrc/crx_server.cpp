@@ -152,6 +152,7 @@ struct crx_server {
     std::unordered_map<uint64_t, Request> reqs;
     uint64_t nextTicket = 1;
     bool stopping = false;
+    uint32_t drawing = 0;   // lanes between admitting a batch and finishing it
 
     struct Lane {
         crx_context* ctx = nullptr;
@@ -643,8 +644,14 @@ void lane_main(crx_server* s, crx_server::Lane* lane) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
-                // admit the oldest requests whose buffers fit (FIFO: stop at the first that does not)
-                while (!s->pending.empty() && batch.size() < s->cfg.max_batch_views) {
+                // admit the oldest requests whose buffers fit (FIFO: stop at the first that does not);
+                // while another lane draws, only a backlog of lane_backlog requests starts this one
+                // (a lone request waits for the drawing lane instead of sharing the device with it)
+                // and several requests share one batch only from a backlog of merge_backlog (the
+                // views of one batch finish together, so merging a short queue delays its head)
+                const bool mayStart = s->drawing == 0 || s->stopping || s->pending.size() >= s->cfg.lane_backlog;
+                const uint32_t cap = s->pending.size() >= s->cfg.merge_backlog ? s->cfg.max_batch_views : 1u;
+                while (mayStart && !s->pending.empty() && batch.size() < cap) {
                     Request& r = s->reqs[s->pending.front()];
                     size_t off;
                     if (!s->arena.alloc(r.bytes, &off)) break;
@@ -669,6 +676,7 @@ void lane_main(crx_server* s, crx_server::Lane* lane) {
             }
             const double t = s->now();
             for (Request* r : batch) r->start = t;
+            s->drawing++;
         }
         uint32_t uploads = 0;
         draw_batch(s, *lane, batch, &uploads);
@@ -681,6 +689,7 @@ void lane_main(crx_server* s, crx_server::Lane* lane) {
                 r->uploads = uploads;
                 r->finished = true;
             }
+            s->drawing--;
             s->statRequests += batch.size();
             s->statBatches++;
             s->statMaxBatch = std::max<uint32_t>(s->statMaxBatch, (uint32_t)batch.size());
@@ -703,6 +712,8 @@ void crx_server_config_default(crx_server_config* out) {
     out->resolve = 1;
     out->prefetch = 1;
     out->shared_stream = 0;
+    out->lane_backlog = 2;
+    out->merge_backlog = 4;
     out->reserved = 0;
 }
 
@@ -733,6 +744,8 @@ crx_status crx_server_create(crx_context* ctx, const crx_server_config* config, 
     if (cfg.max_batch_views == 0 || cfg.max_batch_views > kMaxViews)
         return fail(CRX_INVALID_ARG, "max_batch_views must be in [1, 256]");
     if (cfg.lanes == 0 || cfg.lanes > 8) return fail(CRX_INVALID_ARG, "lanes must be in [1, 8]");
+    if (cfg.lane_backlog == 0 || cfg.merge_backlog == 0)
+        return fail(CRX_INVALID_ARG, "lane_backlog and merge_backlog must be >= 1");
     if (cfg.frame_bytes == 0 || cfg.node_cache_bytes == 0) return fail(CRX_INVALID_ARG, "memory budgets must be > 0");
     crx_server* s = new crx_server();
     s->ctx = ctx;

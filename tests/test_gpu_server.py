@@ -35,11 +35,11 @@ def terrain(ctx):
         b.free()
 
 
-def _server(ctx, nodes, slots, frame_mb, batch, lanes):
+def _server(ctx, nodes, slots, frame_mb, batch, lanes, **extra):
     slot = max((n["nv"] * 12 + 255) // 256 * 256 + (n["nt"] * 12 + 255) // 256 * 256 + (n["nv"] * 4 + 255) // 256 * 256
                for n in nodes)
     cfg = crx.server_config(node_cache_bytes=slot * slots, frame_bytes=frame_mb << 20, max_batch_views=batch,
-                            lanes=lanes, resolve=1)
+                            lanes=lanes, resolve=1, **extra)
     srv = crx.Server(ctx, cfg)
     for k, nd in enumerate(nodes):
         srv.add_node(nd["pos"], nd["nv"], nd["idx"], nd["nt"], nd["col"], nd["aabb"], resident=(k % 2 == 0))
@@ -171,3 +171,28 @@ def test_view_seeing_more_nodes_than_the_cache(ctx, terrain):
             assert np.array_equal(got[k][1], want[k][1]), f"request {k} colours"
     finally:
         srv.close()
+
+
+def test_short_queues_are_not_merged(ctx, terrain):
+    """merge_backlog: a backlog shorter than it is drawn one request per
+    batch (the views of a batch finish together), a longer one is merged;
+    every result equals the request drawn alone."""
+    views = _requests(terrain, 12)
+    for merge, lane_backlog in ((16, 2), (2, 1)):
+        srv = _server(ctx, terrain, slots=40, frame_mb=4, batch=8, lanes=2, merge_backlog=merge,
+                      lane_backlog=lane_backlog)
+        try:
+            alone = [_fetch(srv, srv.submit(v), v)[:2] for v in views]
+            tickets = [srv.submit(v) for v in views]
+            batches = []
+            for k, (t, v) in enumerate(zip(tickets, views)):
+                fb, cb, r = _fetch(srv, t, v)
+                batches.append(r.batch_views)
+                assert np.array_equal(fb, alone[k][0]) and np.array_equal(cb, alone[k][1]), f"request {k}"
+            print(f"[server] merge_backlog {merge}: batches {batches}", flush=True)
+            if merge > len(views):
+                assert max(batches) == 1
+            else:
+                assert max(batches) > 1
+        finally:
+            srv.close()
