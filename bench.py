@@ -80,10 +80,11 @@ def stage1_bytes(T: int, V: int) -> int:
 
 
 def kernel_source_sha16() -> str:
-    """Hash of the kernel sources: a counter summary taken on other kernels is stale."""
+    """Hash of the kernel sources and of the host code that plans their
+    launches: a counter summary taken on other sources is stale."""
     import hashlib
     h = hashlib.sha256()
-    for f in ("crx_kernels.hip", "crx_device.h"):
+    for f in ("crx_kernels.hip", "crx_device.h", "crx_api.cpp"):
         h.update((ROOT / "curast-ooc_amd" / "csrc" / f).read_bytes())
     return h.hexdigest()[:16]
 

@@ -398,6 +398,11 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
         it.unitRows = tu.stage1_strip_rows;
         const uint64_t rows = ((uint64_t)m.numTriangles + it.rowTris - 1) / it.rowTris;
         it.unitSegs = (uint32_t)((rows + it.unitRows - 1) / it.unitRows);
+        // the groups of a strip rounded up to a multiple of 8 (the units past
+        // the last row are empty): unit s * unitSegs + g then runs on the XCD
+        // of its group g for every strip s, so the waves that draw the two
+        // sides of a strip edge share their XCD's L2
+        if (strips) it.unitSegs = (it.unitSegs + 7u) & ~7u;
         const uint64_t units = (uint64_t)it.strips * it.unitSegs;
         D.batches += D.runs ? units : ((uint64_t)m.numTriangles + batchTris - 1) >> D.batchShift;
         D.tris += (uint64_t)m.numTriangles * (uint64_t)cnt;
