@@ -333,7 +333,7 @@ crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMesh
     // the draw's stage-1 kernel is chosen by its first mesh, as the reference
     // picks the _compressed / _uncompressed kernels (CuRast_render.h:235-241)
     D.pu16 = D.numItems > 0 && list[0].compressed;
-    const S1Schedule sched = s1_schedule(tu, manyInstances, D.pu16);
+    const S1Schedule sched = s1_schedule(tu, manyInstances, D.pu16, numViews > 1);
     D.block = tu.stage1_block;
     D.trisPerLane = sched.trisPerLane;
     D.batchShift = 0;   // log2(triangles per batch)
@@ -479,7 +479,9 @@ RasterParams plan_write(crx_context* ctx, const DrawPlan& D, char* host, char* d
     P.tile = (int32_t)tu.tile_size;
     P.batchShift = D.batchShift;
     P.order = tu.stage1_order;
-    P.cullOrder = tu.stage1_cull_order;
+    // cull order 3 decides per item; a draw of several views tests backfaces
+    // first (10.70 ms against 11.07 ms for frustum first on 64 views)
+    P.cullOrder = (tu.stage1_cull_order == 3u && D.numViews > 1) ? 0u : tu.stage1_cull_order;
     P.fastTiny = tu.stage1_fast_tiny;
     P.numWaves2 = D.g2 * 4u;
     P.numWaves3 = D.g3 * 4u;

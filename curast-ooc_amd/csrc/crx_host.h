@@ -247,13 +247,16 @@ void set_counter_block(RasterParams& P, uint32_t* block);
 // two triangles per lane and 64 waves per CU (its index and vertex decode
 // waits on LDS and loads, and more waves hide it: 1.89 ms against 2.10 ms at
 // 32 on the 400 M-triangle heightfield); other draws take two triangles per
-// lane and the tuning's grid.
+// lane and the tuning's grid; a draw of several views takes one triangle
+// per lane and 64 waves per CU (its per-(triangle, view) work varies most:
+// 11.07 ms against 12.25 ms at two per lane for 64 views of the 400 M-triangle
+// heightfield).
 struct S1Schedule {
     uint32_t trisPerLane, wavesPerCU;
 };
-inline S1Schedule s1_schedule(const crx_tuning& t, bool manyInstances, bool pu16) {
+inline S1Schedule s1_schedule(const crx_tuning& t, bool manyInstances, bool pu16, bool multiView = false) {
     if (t.stage1_tris_per_lane) return {t.stage1_tris_per_lane, t.stage1_waves_per_cu};
-    if (manyInstances) return S1Schedule{1u, 64u};
+    if (manyInstances || multiView) return S1Schedule{1u, 64u};
     return pu16 ? S1Schedule{2u, 64u} : S1Schedule{2u, t.stage1_waves_per_cu};
 }
 crx_status check_tuning(const crx_tuning& t);

@@ -184,8 +184,9 @@ typedef struct crx_tuning {
     uint32_t stage3_waves_per_cu; /* stage-3 persistent grid size in waves per CU */
     uint32_t stage1_cull_order;   /* 0 = backface test before the frustum test, 1 = frustum first
                                      testing y before x, 2 = frustum first testing x before y,
-                                     3 = per item: 0 for meshes drawn over >= 4 instances, else 1
-                                     (all are exact early-outs; the order only changes cost) */
+                                     3 = per item: 0 for meshes drawn over >= 4 instances and for
+                                     draws of several views, else 1 (all are exact early-outs;
+                                     the order only changes cost) */
     uint32_t stage1_order;        /* batch-to-workgroup order: 0 = contiguous share per XCD,
                                      1 = grid-stride, 2 = contiguous share per workgroup */
     uint32_t stage1_fast_tiny;    /* 1 = decide sure tiny-triangle culls from v_rcp_f32 screen
@@ -195,8 +196,9 @@ typedef struct crx_tuning {
                                      is stage1_block * this consecutive triangles of one item);
                                      0 = per draw: a draw with a mesh drawn over >= 4 instances
                                      (every instanced crx_raster_stage1 draw) takes 1 and a grid
-                                     of 64 waves per CU, an IX_PU16 draw 2 and 64 waves per CU,
-                                     others 2 and stage1_waves_per_cu */
+                                     of 64 waves per CU, so does a draw of several views, an
+                                     IX_PU16 draw 2 and 64 waves per CU, others 2 and
+                                     stage1_waves_per_cu */
     uint32_t stage1_layout;       /* draws of one view over meshes drawn over < 4 instances (IX_PU16
                                      meshes only with stage1_tris_per_lane = 2), with 2 triangles
                                      per lane, a block of 256 and at least 16 batches of 128
