@@ -645,11 +645,14 @@ void lane_main(crx_server* s, crx_server::Lane* lane) {
             std::unique_lock<std::mutex> lk(s->mu);
             for (;;) {
                 // admit the oldest requests whose buffers fit (FIFO: stop at the first that does not);
-                // while another lane draws, only a backlog of lane_backlog requests starts this one
-                // (a lone request waits for the drawing lane instead of sharing the device with it)
-                // and several requests share one batch only from a backlog of merge_backlog (the
-                // views of one batch finish together, so merging a short queue delays its head)
-                const bool mayStart = s->drawing == 0 || s->stopping || s->pending.size() >= s->cfg.lane_backlog;
+                // with every lane idle the first lane takes the work (so light traffic keeps to one
+                // lane, whose context keeps the node lists of the last frame); while another lane
+                // draws, only a backlog of lane_backlog requests starts this one (a lone request
+                // waits for the drawing lane instead of sharing the device with it); several
+                // requests share one batch only from a backlog of merge_backlog (the views of one
+                // batch finish together, so merging a short queue delays its head)
+                const bool mayStart = s->stopping || (s->drawing == 0 ? lane->index == 0
+                                                                        : s->pending.size() >= s->cfg.lane_backlog);
                 const uint32_t cap = s->pending.size() >= s->cfg.merge_backlog ? s->cfg.max_batch_views : 1u;
                 while (mayStart && !s->pending.empty() && batch.size() < cap) {
                     Request& r = s->reqs[s->pending.front()];
@@ -712,7 +715,7 @@ void crx_server_config_default(crx_server_config* out) {
     out->resolve = 1;
     out->prefetch = 1;
     out->shared_stream = 0;
-    out->lane_backlog = 2;
+    out->lane_backlog = 4;
     out->merge_backlog = 4;
     out->reserved = 0;
 }
@@ -896,7 +899,7 @@ crx_status crx_server_submit(crx_server* s, const crx_view* view, uint64_t* tick
         s->pending.push_back(t);
         *ticket = t;
     }
-    s->workCv.notify_one();
+    s->workCv.notify_all();   // the lane allowed to start may not be the one woken first
     return CRX_OK;
 }
 

@@ -623,10 +623,11 @@ typedef struct crx_server_config {
                                     side; 1: the lanes admit, cull, upload and enqueue their batches
                                     concurrently but launch them on one stream, so batches run on
                                     the device in the order they were enqueued */
-    uint32_t lane_backlog;       /* while a lane draws, another idle lane starts a batch only when at
-                                    least this many requests wait (default 2: a lone request waits
-                                    for the drawing lane rather than share the device with it); 1 =
-                                    every idle lane starts at once */
+    uint32_t lane_backlog;       /* with every lane idle the first lane takes the work; while a lane
+                                    draws, another idle lane starts a batch only when at least this
+                                    many requests wait (default 4: a short queue waits for the
+                                    drawing lane rather than slow its batch down by sharing the
+                                    device with it); 1 = an idle lane starts at once */
     uint32_t merge_backlog;      /* a batch takes several requests only when at least this many wait
                                     (default 4), else one: the views of a batch finish together, so
                                     merging a short queue delays its oldest request */
