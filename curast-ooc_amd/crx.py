@@ -319,6 +319,8 @@ _SIGS = {
     "crx_scene_destroy": (C.c_int, [_VP]),
     "crx_scene_add_node": (C.c_int, [_VP, _VP, C.POINTER(Mat4), C.POINTER(C.c_uint32)]),
     "crx_scene_set_transform": (C.c_int, [_VP, C.c_uint32, C.POINTER(Mat4)]),
+    "crx_scene_set_texture": (C.c_int, [_VP, C.c_uint32, C.POINTER(Texture)]),
+    "crx_mesh_set_attributes": (C.c_int, [_VP, _VP, _VP]),
     "crx_scene_num_nodes": (C.c_int, [_VP, C.POINTER(C.c_uint32)]),
     "crx_render_settings_default": (None, [C.POINTER(RenderSettings)]),
     "crx_render": (C.c_int, [_VP, _VP, C.POINTER(View), C.POINTER(RenderSettings), _VP, _VP, _VP,
@@ -707,6 +709,16 @@ class Mesh:
         _check(lib().crx_mesh_buffers(self.handle, C.byref(p), C.byref(i), C.byref(c)))
         return p.value, i.value, c.value
 
+    def set_attributes(self, uvs: np.ndarray | None = None, normals: np.ndarray | None = None) -> None:
+        """Per-vertex uvs (V,2) and normals (V,3) float32 for the resolve
+        (Mesh.h:17-19); None leaves an attribute as it is."""
+        uv = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        nr = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        for a in (uv, nr):
+            if a is not None and len(a) != self.num_vertices:
+                raise ValueError("one attribute per vertex")
+        _check(lib().crx_mesh_set_attributes(C.c_void_p(self.handle), _ptr(uv), _ptr(nr)))
+
     def aabb(self) -> Box3:
         b = Box3()
         _check(lib().crx_mesh_info(self.handle, None, None, C.byref(b)))
@@ -750,6 +762,10 @@ class Scene:
     def set_transform(self, node_id: int, world: Mat4) -> None:
         _check(lib().crx_scene_set_transform(self.handle, node_id, C.byref(world)))
         self.nodes[node_id] = (self.nodes[node_id][0], world)
+
+    def set_texture(self, node_id: int, texture: Texture | None) -> None:
+        """SNTriangles::texture of a node (a device mip chain); None clears it."""
+        _check(lib().crx_scene_set_texture(self.handle, node_id, C.byref(texture) if texture is not None else None))
 
     def render(self, view: View, fb: DeviceBuffer, cb: DeviceBuffer | None,
                settings: RenderSettings | None = None, stream=None) -> DrawStats:

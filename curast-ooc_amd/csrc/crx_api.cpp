@@ -160,6 +160,9 @@ crx_status check_mesh(const crx_cmesh& m, const char* what, uint32_t k) {
         if (m.numTriangles > 0 && !m.indices) return fail(CRX_INVALID_ARG, name() + " is compressed without indices");
         if (m.bitsPerIndex > 32) return fail(CRX_INVALID_ARG, name() + " has bitsPerIndex > 32");
     }
+    // a texture that TEXTURE would sample needs a level 0 of at least one texel
+    if (m.texture.data && !m.texture.huffmanTables && (m.texture.width <= 0 || m.texture.height <= 0))
+        return fail(CRX_INVALID_ARG, name() + " has a texture of width or height <= 0");
     return CRX_OK;
 }
 
@@ -182,6 +185,10 @@ InstanceRec make_instance(const crx_cmesh& m) {
     r.cf[0] = m.compressionFactor.x; r.cf[1] = m.compressionFactor.y; r.cf[2] = m.compressionFactor.z;
     r.amin[0] = m.aabb.min.x; r.amin[1] = m.aabb.min.y; r.amin[2] = m.aabb.min.z;
     r.amax[0] = m.aabb.max.x; r.amax[1] = m.aabb.max.y; r.amax[2] = m.aabb.max.z;
+    r.texData = m.texture.data;
+    r.texW = m.texture.width;
+    r.texH = m.texture.height;
+    r.texJpeg = m.texture.huffmanTables ? 1u : 0u;
     return r;
 }
 
@@ -682,8 +689,7 @@ crx_status check_resolve(const crx_render_target* targets, uint32_t numViews, cr
         CRX_TRY(check_target(targets + v));
         if (!targets[v].colorbuffer) return fail(CRX_INVALID_ARG, "target colorbuffer is NULL");
     }
-    if (attribute == CRX_DISPLAY_TEXTURE || attribute == CRX_DISPLAY_STAGE)
-        return fail(CRX_UNSUPPORTED, "display attribute TEXTURE / STAGE is not supported");
+    if (attribute == CRX_DISPLAY_STAGE) return fail(CRX_UNSUPPORTED, "display attribute STAGE is not supported");
     if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
     return CRX_OK;
 }
@@ -943,7 +949,7 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
     // the per-node records depend on the node set, the instancing switch and
     // the kept nodes only: a frame that changes none of them keeps them
     if (scene->listsKey == 0 || scene->listsVersion != scene->version || scene->listsInstanced != instancedFlag ||
-        scene->listsKept != kept) {
+        scene->listsAttrEpoch != scene->ctx->attrEpoch || scene->listsKept != kept) {
         scene->unique.resize(n);
         scene->instances.resize(n);
         scene->prefix.resize(n);
@@ -958,6 +964,9 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
             cm.indices = nd.mesh->indices;
             cm.positions = reinterpret_cast<crx_vec3*>(nd.mesh->positions);
             cm.colors = nd.mesh->colors;
+            cm.uvs = reinterpret_cast<crx_vec2*>(nd.mesh->uvs);
+            cm.normals = reinterpret_cast<crx_vec3*>(nd.mesh->normals);
+            cm.texture = nd.texture;
             cm.aabb = nd.mesh->aabb;
             cm.world = nd.world;
             cm.id = (int32_t)kept[i];
@@ -985,6 +994,7 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
         }
         scene->unique.resize(u);
         scene->listsVersion = scene->version;
+        scene->listsAttrEpoch = scene->ctx->attrEpoch;
         scene->listsInstanced = instancedFlag;
         scene->listsKept.swap(kept);
         scene->listsKey = next_lists_key();
@@ -1643,7 +1653,37 @@ crx_status crx_mesh_destroy(crx_mesh* mesh) {
         if (mesh->indices) (void)hipFree(mesh->indices);
         if (mesh->colors) (void)hipFree(mesh->colors);
     }
+    if (mesh->uvs) (void)hipFree(mesh->uvs);
+    if (mesh->normals) (void)hipFree(mesh->normals);
     delete mesh;
+    return CRX_OK;
+}
+
+crx_status crx_mesh_set_attributes(crx_mesh* mesh, const float* uvs, const float* normals) {
+    if (!mesh) return fail(CRX_INVALID_ARG, "mesh is NULL");
+    CRX_HIP(hipSetDevice(mesh->ctx->device));
+    // upload into a new buffer first: the mesh keeps its old one on failure
+    auto replace = [&](float** dst, const float* src, size_t bytes) -> crx_status {
+        if (!src || bytes == 0) return CRX_OK;
+        float* d = nullptr;
+        hipError_t e = hipMalloc((void**)&d, bytes);
+        if (e != hipSuccess) return fail(CRX_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        e = hipMemcpy(d, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return fail(CRX_HIP_ERROR, std::string("hipMemcpy: ") + hipGetErrorString(e));
+        }
+        // frames already enqueued may still read the old buffer
+        if (*dst) {
+            CRX_HIP(hipDeviceSynchronize());
+            (void)hipFree(*dst);
+        }
+        *dst = d;
+        return CRX_OK;
+    };
+    CRX_TRY(replace(&mesh->uvs, uvs, 8ull * mesh->numVertices));
+    CRX_TRY(replace(&mesh->normals, normals, 12ull * mesh->numVertices));
+    mesh->ctx->attrEpoch++;
     return CRX_OK;
 }
 
@@ -1686,7 +1726,7 @@ crx_status crx_scene_destroy(crx_scene* scene) {
 crx_status crx_scene_add_node(crx_scene* scene, crx_mesh* mesh, const crx_mat4* world, uint32_t* node_id) {
     if (!scene || !mesh || !world) return fail(CRX_INVALID_ARG, "NULL argument");
     if (mesh->ctx != scene->ctx) return fail(CRX_INVALID_ARG, "mesh belongs to another context");
-    scene->nodes.push_back(crx_scene::Node{mesh, *world, world_box(mesh->aabb, *world)});
+    scene->nodes.push_back(crx_scene::Node{mesh, *world, world_box(mesh->aabb, *world), crx_texture{}});
     scene->version++;
     if (node_id) *node_id = (uint32_t)(scene->nodes.size() - 1);
     return CRX_OK;
@@ -1697,6 +1737,18 @@ crx_status crx_scene_set_transform(crx_scene* scene, uint32_t node_id, const crx
     if (node_id >= scene->nodes.size()) return fail(CRX_INVALID_ARG, "node_id out of range");
     scene->nodes[node_id].world = *world;
     scene->nodes[node_id].wbox = world_box(scene->nodes[node_id].mesh->aabb, *world);
+    scene->version++;
+    return CRX_OK;
+}
+
+crx_status crx_scene_set_texture(crx_scene* scene, uint32_t node_id, const crx_texture* texture) {
+    if (!scene) return fail(CRX_INVALID_ARG, "scene is NULL");
+    if (node_id >= scene->nodes.size()) return fail(CRX_INVALID_ARG, "node_id out of range");
+    crx_texture t{};
+    if (texture) t = *texture;
+    if (t.data && !t.huffmanTables && (t.width <= 0 || t.height <= 0))
+        return fail(CRX_INVALID_ARG, "texture width and height must be positive");
+    scene->nodes[node_id].texture = t;
     scene->version++;
     return CRX_OK;
 }

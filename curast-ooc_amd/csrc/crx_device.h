@@ -67,6 +67,12 @@ struct InstanceRec {
     float cf[3];
     float amin[3];
     float amax[3];
+    // TEXTURE (CMesh::texture): RGBA8 mip chain, level 0 first, each level
+    // ((w + 1) / 2) x ((h + 1) / 2) of the one before; texJpeg = 1 for a
+    // JPEG-coded texture (CMesh::texture.huffmanTables set)
+    const uint32_t* texData;
+    int32_t texW, texH;
+    uint32_t texJpeg;
 };
 
 // world-to-view transform of one instance plus its mirror flag

@@ -178,6 +178,7 @@ struct crx_context {
     hipEvent_t ev[crx::T_COUNT] = {};
     bool recorded[crx::T_COUNT] = {};
     uint64_t meshSeq = 0;
+    uint64_t attrEpoch = 0;           // bumped by every crx_mesh_set_attributes on this context
     // kernel-level entry points: the records buffer every crx_raster_stage*
     // call rebuilds its records in, and the event after the last call's
     // launches (a call on another stream waits for it)
@@ -202,6 +203,9 @@ struct crx_mesh {
     uint32_t bitsPerIndex = 0, indexMin = 0, indexMax = 0;
     crx_vec3 compressionFactor{};
     uint32_t rowTris = 0;             // triangles per row of a mesh stored in rows (0 = none found)
+    // resolve-only vertex attributes, always owned by the mesh (crx_mesh_set_attributes)
+    float* uvs = nullptr;             // 2 floats per vertex
+    float* normals = nullptr;         // 3 floats per vertex
 };
 
 struct crx_scene {
@@ -210,6 +214,7 @@ struct crx_scene {
         crx_mesh* mesh;
         crx_mat4 world;
         crx_box3 wbox;   // world-space box of the mesh's box (kept in step with world)
+        crx_texture texture;   // SNTriangles::texture (zero: none)
     };
     std::vector<Node> nodes;
     // frame-preparation scratch, kept across frames like the reference's statics
@@ -226,6 +231,7 @@ struct crx_scene {
     int listsInstanced = -1;
     std::vector<uint32_t> listsKept;
     uint64_t listsKey = 0;            // content key of the lists (new on every rebuild)
+    uint64_t listsAttrEpoch = 0;      // the context's attrEpoch the lists were built at
 };
 
 

@@ -504,6 +504,10 @@ __global__ __launch_bounds__(256) void k_plan_instances(const crx_cmesh* __restr
     r.cf[0] = m.compressionFactor.x; r.cf[1] = m.compressionFactor.y; r.cf[2] = m.compressionFactor.z;
     r.amin[0] = m.aabb.min.x; r.amin[1] = m.aabb.min.y; r.amin[2] = m.aabb.min.z;
     r.amax[0] = m.aabb.max.x; r.amax[1] = m.aabb.max.y; r.amax[2] = m.aabb.max.z;
+    r.texData = m.texture.data;
+    r.texW = m.texture.width;
+    r.texH = m.texture.height;
+    r.texJpeg = m.texture.huffmanTables ? 1u : 0u;
     inst[i] = r;
     Transform x{};
     for (int e = 0; e < 16; e++) x.m[e] = transforms[i].m[e];
@@ -2162,6 +2166,58 @@ __device__ __forceinline__ float intersect_plane(f3 origin, f3 dir, f3 normal, f
 
 // Colour of a covered pixel whose triangle id `total` lies in `mesh`
 // (resolve.cu:736-1035).  `mesh` is a register copy of the instance record.
+// computeRayDir (resolve.cu:662-676) through the screen point (px, py).
+__device__ __forceinline__ f3 resolve_ray(const ResolveParams& R, const f3& origin, float px, float py) {
+    const float u = 2.0f * px / R.Wf - 1.0f;
+    const float v = 2.0f * py / R.Hf - 1.0f;
+    const f3 rview = normalize3(f3{R.p00I * u, R.p11I * v, -1.0f});
+    return normalize3(sub3(xform_point(R.viewI, rview), origin));
+}
+
+// sampleColor_linear (resolve.cu:148-195): the four texels around the sample
+// (fmodf wraps them into the level, float -> int truncates), weighted by the
+// fractions of the shifted texel coordinates.
+__device__ __forceinline__ uint32_t sample_linear(const uint32_t* data, int width, int height, float ux, float uy) {
+    const float fw = (float)width, fh = (float)height;
+    const float ftx = (ux - floorf(ux)) * fw;
+    const float fty = (uy - floorf(uy)) * fh;
+    const auto tex = kglobal(data);
+    const int last = width * height - 1;
+    auto texel = [&](float fx, float fy) -> uint32_t {
+        const int tx = f2i(fmodf(fx, fw));
+        const int ty = f2i(fmodf(fy, fh));
+        int id = tx + ty * width;
+        id = id < 0 ? 0 : (id > last ? last : id);
+        return tex[id];
+    };
+    const uint32_t t00 = texel(ftx - 0.5f, fty - 0.5f);
+    const uint32_t t01 = texel(ftx - 0.5f, fty + 0.5f);
+    const uint32_t t10 = texel(ftx + 0.5f, fty - 0.5f);
+    const uint32_t t11 = texel(ftx + 0.5f, fty + 0.5f);
+    const float wx = fmodf(ftx + 0.5f, 1.0f);
+    const float wy = fmodf(fty + 0.5f, 1.0f);
+    const float w00 = (1.0f - wx) * (1.0f - wy), w10 = wx * (1.0f - wy), w01 = (1.0f - wx) * wy, w11 = wx * wy;
+    uint32_t rgb = 0u;
+    #pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float c = ((w00 * (float)((t00 >> (8 * k)) & 0xffu) + w10 * (float)((t10 >> (8 * k)) & 0xffu)) +
+                         w01 * (float)((t01 >> (8 * k)) & 0xffu)) +
+                        w11 * (float)((t11 >> (8 * k)) & 0xffu);
+        rgb |= f2u8(c) << (8 * k);
+    }
+    return 0xff000000u | rgb;
+}
+
+// Mip level of a texture sample: trunc(0.5 * log2(m)) clamped to [0, 7]
+// (resolve.cu:849-856), taken exactly from m's exponent: for m >= 1 the
+// value lies in [e/2, (e+1)/2) with e the exponent of m, so the level is
+// e >> 1; m < 1, zero and NaN give level 0.
+__device__ __forceinline__ int mip_level(float m) {
+    if (!(m >= 1.0f)) return 0;
+    const int e = (int)((__builtin_bit_cast(uint32_t, m) >> 23) & 0xffu) - 127;
+    return min(e >> 1, 7);
+}
+
 __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const InstanceRec& mesh, uint64_t total,
                                                 int x, int y) {
     uint32_t color = 0;
@@ -2175,36 +2231,34 @@ __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const In
         triangle_indices(mesh, tri, i0, i1, i2);   // resolve.cu:744-760
         // ray through the pixel centre (computeRayDir, resolve.cu:662-680)
         const f3 origin{R.origin[0], R.origin[1], R.origin[2]};
-        const float u = 2.0f * ((float)x + 0.5f) / R.Wf - 1.0f;
-        const float v = 2.0f * ((float)y + 0.5f) / R.Hf - 1.0f;
-        const f3 rview = normalize3(f3{R.p00I * u, R.p11I * v, -1.0f});
-        const f3 rayDir = normalize3(sub3(xform_point(R.viewI, rview), origin));
+        const f3 rayDir = resolve_ray(R, origin, (float)x + 0.5f, (float)y + 0.5f);
 
         const f3 aw = xform_point(mesh.world, resolve_vertex(mesh, i0));
         const f3 bw = xform_point(mesh.world, resolve_vertex(mesh, i1));
         const f3 cw = xform_point(mesh.world, resolve_vertex(mesh, i2));
         const f3 v0 = sub3(bw, aw);
         const f3 v1 = sub3(cw, aw);
-        float t;
-        {   // resolve.cu:790-801
-            const f3 normal = normalize3(cross3(v0, v1));
-            const float d = dot3(aw, normal);
-            t = intersect_plane(origin, rayDir, normal, -d);
-        }
-        float stx, sty;
-        {   // resolve.cu:805-834
-            const float d00 = dot3(v0, v0);
-            const float d01 = dot3(v0, v1);
-            const float d11 = dot3(v1, v1);
-            const float denom = d00 * d11 - d01 * d01;
-            const float denomI = 1.0f / denom;
-            const f3 p = add3(origin, f3{t * rayDir.x, t * rayDir.y, t * rayDir.z});
+        // resolve.cu:790-801
+        const f3 normal = normalize3(cross3(v0, v1));
+        const float d = dot3(aw, normal);
+        const float t = intersect_plane(origin, rayDir, normal, -d);
+        // resolve.cu:805-834
+        const float d00 = dot3(v0, v0);
+        const float d01 = dot3(v0, v1);
+        const float d11 = dot3(v1, v1);
+        const float denom = d00 * d11 - d01 * d01;
+        const float denomI = 1.0f / denom;
+        // barycentric (s, t) of the ray's hit on the triangle's plane
+        auto stv = [&](float th, const f3& rd, float& s_, float& t_) {
+            const f3 p = add3(origin, f3{th * rd.x, th * rd.y, th * rd.z});
             const f3 v2 = sub3(p, aw);
             const float d20 = dot3(v2, v0);
             const float d21 = dot3(v2, v1);
-            stx = (d11 * d20 - d01 * d21) * denomI;
-            sty = (d00 * d21 - d01 * d20) * denomI;
-        }
+            s_ = (d11 * d20 - d01 * d21) * denomI;
+            t_ = (d00 * d21 - d01 * d20) * denomI;
+        };
+        float stx, sty;
+        stv(t, rayDir, stx, sty);
         const float w = 1.0f - stx - sty;
         color = 0xffff00ffu;
         const int attr = R.attribute;
@@ -2214,6 +2268,59 @@ __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const In
             color = tri * 12345678u;
         } else if (attr == 6) {                // MESH_ID
             color = ((uint32_t)mesh.id + 1u) * 12345678u;
+        } else if (attr == 1 && mesh.uvs) {    // TEXTURE (resolve.cu:783-876)
+            // the neighbouring pixels' rays, hits and texture coordinates give
+            // the sample's footprint, and with it the mip level
+            const f3 r10 = resolve_ray(R, origin, (float)x + 1.5f, (float)y + 0.5f);
+            const f3 r01 = resolve_ray(R, origin, (float)x + 0.5f, (float)y + 1.5f);
+            const f3 r11 = resolve_ray(R, origin, (float)x + 1.5f, (float)y + 1.5f);
+            const float t10 = intersect_plane(origin, r10, normal, -d);
+            const float t01 = intersect_plane(origin, r01, normal, -d);
+            const float t11 = intersect_plane(origin, r11, normal, -d);
+            float s10, u10, s01, u01, s11, u11;
+            stv(t10, r10, s10, u10);
+            stv(t01, r01, s01, u01);
+            stv(t11, r11, s11, u11);
+            const float* ua = mesh.uvs + 2ull * i0;
+            const float* ub = mesh.uvs + 2ull * i1;
+            const float* uc = mesh.uvs + 2ull * i2;
+            const float a0 = ua[0], a1 = ua[1], b0 = ub[0], b1 = ub[1], c0 = uc[0], c1 = uc[1];
+            // uv_a * (1 - s - t) + uv_b * s + uv_c * t per component
+            auto interp = [&](float s_, float t_, float& px, float& py) {
+                const float wq = 1.0f - s_ - t_;
+                px = (a0 * wq + b0 * s_) + c0 * t_;
+                py = (a1 * wq + b1 * s_) + c1 * t_;
+            };
+            float uvx, uvy, x10, y10, x01, y01, x11, y11;
+            interp(stx, sty, uvx, uvy);
+            interp(s10, u10, x10, y10);
+            interp(s01, u01, x01, y01);
+            interp(s11, u11, x11, y11);
+            const float maxx = fmaxf(fmaxf(uvx, x10), fmaxf(x01, x11));
+            const float maxy = fmaxf(fmaxf(uvy, y10), fmaxf(y01, y11));
+            const float minx = fminf(fminf(uvx, x10), fminf(x01, x11));
+            const float miny = fminf(fminf(uvy, y10), fminf(y01, y11));
+            uvx = uvx - floorf(uvx);
+            uvy = uvy - floorf(uvy);
+            const float tw = (float)mesh.texW, th = (float)mesh.texH;
+            const float dxx = (maxx - minx) * tw, dxy = (maxx - minx) * th;
+            const float dyx = (maxy - miny) * tw, dyy = (maxy - miny) * th;
+            const int level = mip_level(fmaxf(dxx * dxx + dxy * dxy, dyx * dyx + dyy * dyy));
+            if (!mesh.texJpeg && mesh.texData == nullptr) {
+                color = 0u;   // no texture: every level samples 0
+            } else if (!mesh.texJpeg) {
+                // level k of the chain: k levels of (w + 1) / 2 x (h + 1) / 2 in
+                const uint32_t* data = mesh.texData;
+                int w = mesh.texW, h = mesh.texH;
+                for (int i = 1; i <= level; i++) {
+                    data = data + (ptrdiff_t)w * h;
+                    w = (w + 1) / 2;
+                    h = (h + 1) / 2;
+                }
+                color = sample_linear(data, w, h, uvx, uvy);
+            }
+            // a JPEG-coded texture: the reference queues its blocks for the
+            // decoder and this pass leaves the pixel magenta (resolve.cu:858-876)
         } else if (attr == 2 && mesh.uvs) {    // UV
             const float* ua = mesh.uvs + 2ull * i0;
             const float* ub = mesh.uvs + 2ull * i1;

@@ -94,7 +94,12 @@ typedef struct crx_mat4 { float m[16]; } crx_mat4;
 /* Box3, src/kernels/HostDeviceInterface.h:57-105 */
 typedef struct crx_box3 { crx_vec3 min, max; } crx_box3;
 
-/* Texture, src/kernels/HostDeviceInterface.h:177-185 (carried, not sampled) */
+/* Texture, src/kernels/HostDeviceInterface.h:177-185.  data (device) holds
+ * the RGBA8 mip chain, level 0 first, each next level ((w + 1) / 2) x
+ * ((h + 1) / 2) texels, up to level 7 (the deepest CRX_DISPLAY_TEXTURE
+ * samples).  huffmanTables != NULL marks a JPEG-coded texture, which the
+ * resolve shades magenta (0xffff00ff) as the reference's resolve pass does
+ * before its decoder runs; the remaining members are carried, not read. */
 typedef struct crx_texture {
     int32_t width;
     int32_t height;
@@ -159,7 +164,7 @@ typedef struct crx_huge_triangle {
 /* DisplayAttribute, src/kernels/HostDeviceInterface.h:277-286 */
 typedef enum crx_display_attribute {
     CRX_DISPLAY_NONE = 0,
-    CRX_DISPLAY_TEXTURE = 1,       /* not supported: crx_resolve_visbuffer returns CRX_UNSUPPORTED */
+    CRX_DISPLAY_TEXTURE = 1,       /* bilinear sample of CMesh::texture at the mip level of the pixel's uv footprint */
     CRX_DISPLAY_UV = 2,
     CRX_DISPLAY_NORMAL = 3,
     CRX_DISPLAY_VERTEX_COLORS = 4,
@@ -513,6 +518,13 @@ crx_status crx_mesh_info(const crx_mesh* mesh, uint32_t* numTriangles, uint32_t*
  * reference, whose stage 1 walks every mesh contiguously; it orders the
  * walk and changes no result. */
 crx_status crx_mesh_row_triangles(const crx_mesh* mesh, uint32_t* out);
+/* Vertex attributes of a mesh (Mesh.h:17-19, cptr_uv / cptr_normal): uvs
+ * (2*numVertices floats) and normals (3*numVertices floats) are copied from
+ * host arrays into device memory owned by the mesh; a NULL array leaves that
+ * attribute as it is.  Only the resolve reads them (TEXTURE and UV the uvs,
+ * NORMAL the normals).  Replacing an attribute waits for the device before
+ * freeing the old buffer. */
+crx_status crx_mesh_set_attributes(crx_mesh* mesh, const float* uvs, const float* normals);
 /* Device pointers of the mesh buffers (colors may be NULL). */
 crx_status crx_mesh_buffers(const crx_mesh* mesh, const float** d_positions, const uint32_t** d_indices,
                             const uint32_t** d_colors);
@@ -522,6 +534,10 @@ crx_status crx_scene_destroy(crx_scene* scene);
 /* SNTriangles node: mesh drawn with a world transform (column-major). */
 crx_status crx_scene_add_node(crx_scene* scene, crx_mesh* mesh, const crx_mat4* world, uint32_t* node_id);
 crx_status crx_scene_set_transform(crx_scene* scene, uint32_t node_id, const crx_mat4* world);
+/* SNTriangles::texture (SNTriangles.h:20) of a node, copied into the node's
+ * CMesh as toCMesh does (CuRast_render.h:374-376).  texture->data is a
+ * caller-owned device mip chain (crx_texture); NULL clears the texture. */
+crx_status crx_scene_set_texture(crx_scene* scene, uint32_t node_id, const crx_texture* texture);
 crx_status crx_scene_num_nodes(const crx_scene* scene, uint32_t* out);
 
 /* View (VKRenderer.h:78-82): view matrix, projection and framebuffer size. */

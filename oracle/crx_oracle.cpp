@@ -666,6 +666,50 @@ inline float intersectPlane(V3 origin, V3 dir, V3 normal, float constant) {
     return (t >= 0.0f) ? t : kInf;
 }
 
+// sampleColor_linear, resolve.cu:148-195
+inline uint32_t sampleColor_linear(const uint32_t* textureData, int width, int height, V2 uv) {
+    if (textureData == nullptr) return 0;
+    float ftx = (uv.x - std::floor(uv.x)) * float(width);
+    float fty = (uv.y - std::floor(uv.y)) * float(height);
+    auto getTexel = [&](float fx, float fy, float rgba[4]) {
+        int tx = f2i(std::fmod(fx, float(width)));
+        int ty = f2i(std::fmod(fy, float(height)));
+        int texelID = tx + ty * width;
+        // clamp(texelID, 0, width * height - 1), utils.cuh:205-212
+        const int hi = width * height - 1;
+        if (texelID < 0) texelID = 0;
+        else if (texelID > hi) texelID = hi;
+        uint32_t texel = textureData[texelID];
+        for (int k = 0; k < 4; k++) rgba[k] = float((texel >> (8 * k)) & 0xffu);
+    };
+    float t00[4], t01[4], t10[4], t11[4];
+    getTexel(ftx - 0.5f, fty - 0.5f, t00);
+    getTexel(ftx - 0.5f, fty + 0.5f, t01);
+    getTexel(ftx + 0.5f, fty - 0.5f, t10);
+    getTexel(ftx + 0.5f, fty + 0.5f, t11);
+    float wx = std::fmod(ftx + 0.5f, 1.0f);
+    float wy = std::fmod(fty + 0.5f, 1.0f);
+    uint32_t color = 0xff000000u;
+    for (int k = 0; k < 3; k++) {
+        float interpolated = (1.0f - wx) * (1.0f - wy) * t00[k] + wx * (1.0f - wy) * t10[k] +
+                             (1.0f - wx) * wy * t01[k] + wx * wy * t11[k];
+        color |= f2u8(interpolated) << (8 * k);
+    }
+    return color;
+}
+
+// Mip level of a TEXTURE sample, resolve.cu:849-856: int(0.5 * log2(m))
+// clamped to [0, 7].  Taken as the exact value of the real function (log2
+// in double is exact enough: for a float m its trunc never rounds across an
+// integer); int(NaN) is 0 as in f2i.
+inline int mipLevelOf(float m) {
+    double ml = 0.5 * std::log2((double)m);
+    if (std::isnan(ml)) return 0;
+    ml = std::trunc(ml);
+    ml = std::min(ml, 7.0);
+    return ml < 0.0 ? 0 : (int)ml;
+}
+
 }  // namespace
 
 extern "C" {
@@ -815,7 +859,8 @@ int crx_oracle_draw(const crx_cmesh* meshes, uint32_t numMeshes,
 
 int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64_t* triangleCountPrefixSum,
                        const crx_render_target* target, int attribute, int nthreads) {
-    if (attribute == CRX_DISPLAY_TEXTURE || attribute == CRX_DISPLAY_STAGE) return -2;
+    if (attribute == CRX_DISPLAY_STAGE) return -2;
+    const bool texture = attribute == CRX_DISPLAY_TEXTURE;
     const int W = target->width, H = target->height;
     const M4 view = target->view;
     const M4 proj = target->proj;
@@ -827,6 +872,13 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
         for (int x = 0; x < W; x++) {
             int pixelID = x + W * y;
             V3 rayDir = computeRayDir(float(x) + 0.5f, float(y) + 0.5f, float(W), float(H), viewI, proj, origin);
+            // the neighbours' rays enter only TEXTURE's mip level (resolve.cu:678-680)
+            V3 rayDir_10{}, rayDir_01{}, rayDir_11{};
+            if (texture) {
+                rayDir_10 = computeRayDir(float(x) + 1.5f, float(y) + 0.5f, float(W), float(H), viewI, proj, origin);
+                rayDir_01 = computeRayDir(float(x) + 0.5f, float(y) + 1.5f, float(W), float(H), viewI, proj, origin);
+                rayDir_11 = computeRayDir(float(x) + 1.5f, float(y) + 1.5f, float(W), float(H), viewI, proj, origin);
+            }
             uint64_t pixel = target->framebuffer[pixelID];
             uint64_t pixel_colorbuffer = target->colorbuffer[pixelID];
             float depth;
@@ -889,15 +941,20 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
                     V3 a_world = xyz(mulmv(mesh.world, V4{pa.x, pa.y, pa.z, 1.0f}));
                     V3 b_world = xyz(mulmv(mesh.world, V4{pb.x, pb.y, pb.z, 1.0f}));
                     V3 c_world = xyz(mulmv(mesh.world, V4{pc.x, pc.y, pc.z, 1.0f}));
-                    float t;
+                    float t, t_10 = 0.0f, t_01 = 0.0f, t_11 = 0.0f;
                     {   // resolve.cu:790-801
                         V3 edge1 = sub(b_world, a_world);
                         V3 edge2 = sub(c_world, a_world);
                         V3 normal = normalize(cross(edge1, edge2));
                         float d = dot(a_world, normal);
                         t = intersectPlane(origin, rayDir, normal, -d);
+                        if (texture) {
+                            t_10 = intersectPlane(origin, rayDir_10, normal, -d);
+                            t_01 = intersectPlane(origin, rayDir_01, normal, -d);
+                            t_11 = intersectPlane(origin, rayDir_11, normal, -d);
+                        }
                     }
-                    V2 stv;
+                    V2 stv, stv_10{}, stv_01{}, stv_11{};
                     {   // resolve.cu:805-834
                         V3 v0 = sub(b_world, a_world);
                         V3 v1 = sub(c_world, a_world);
@@ -906,12 +963,19 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
                         float d11 = dot(v1, v1);
                         float denom = d00 * d11 - d01 * d01;
                         float denomI = 1.0f / denom;
-                        V3 p = add(origin, mul(t, rayDir));
-                        V3 v2 = sub(p, a_world);
-                        float d20 = dot(v2, v0);
-                        float d21 = dot(v2, v1);
-                        stv.x = (d11 * d20 - d01 * d21) * denomI;
-                        stv.y = (d00 * d21 - d01 * d20) * denomI;
+                        auto computeSTV = [&](float tt, V3 dir) {
+                            V3 p = add(origin, mul(tt, dir));
+                            V3 v2 = sub(p, a_world);
+                            float d20 = dot(v2, v0);
+                            float d21 = dot(v2, v1);
+                            return V2{(d11 * d20 - d01 * d21) * denomI, (d00 * d21 - d01 * d20) * denomI};
+                        };
+                        stv = computeSTV(t, rayDir);
+                        if (texture) {
+                            stv_10 = computeSTV(t_10, rayDir_10);
+                            stv_01 = computeSTV(t_01, rayDir_01);
+                            stv_11 = computeSTV(t_11, rayDir_11);
+                        }
                     }
                     color = 0xffff00ff;
                     uint32_t rgb[3];
@@ -922,6 +986,41 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
                         color = triangleIndex * 12345678u;
                     } else if (attribute == CRX_DISPLAY_MESH_ID) {
                         color = ((uint32_t)mesh.id + 1u) * 12345678u;
+                    } else if (texture && mesh.uvs) {
+                        // resolve.cu:836-882
+                        const crx_vec2 uv_a = mesh.uvs[i0], uv_b = mesh.uvs[i1], uv_c = mesh.uvs[i2];
+                        auto interp = [&](V2 b) {
+                            float w = 1.0f - b.x - b.y;
+                            return V2{(uv_a.x * w + uv_b.x * b.x) + uv_c.x * b.y,
+                                      (uv_a.y * w + uv_b.y * b.x) + uv_c.y * b.y};
+                        };
+                        V2 uv = interp(stv), uv_10 = interp(stv_10), uv_01 = interp(stv_01), uv_11 = interp(stv_11);
+                        // max / min of two floats are fmaxf / fminf in the reference's device code
+                        V2 uvmax{std::fmax(std::fmax(uv.x, uv_10.x), std::fmax(uv_01.x, uv_11.x)),
+                                 std::fmax(std::fmax(uv.y, uv_10.y), std::fmax(uv_01.y, uv_11.y))};
+                        V2 uvmin{std::fmin(std::fmin(uv.x, uv_10.x), std::fmin(uv_01.x, uv_11.x)),
+                                 std::fmin(std::fmin(uv.y, uv_10.y), std::fmin(uv_01.y, uv_11.y))};
+                        uv.x = uv.x - std::floor(uv.x);
+                        uv.y = uv.y - std::floor(uv.y);
+                        const float tw = float(mesh.texture.width), th = float(mesh.texture.height);
+                        V2 dx{(uvmax.x - uvmin.x) * tw, (uvmax.x - uvmin.x) * th};
+                        V2 dy{(uvmax.y - uvmin.y) * tw, (uvmax.y - uvmin.y) * th};
+                        int target_level = mipLevelOf(std::fmax(dx.x * dx.x + dx.y * dx.y, dy.x * dy.x + dy.y * dy.y));
+                        // normalisation: without texture data every level samples 0
+                        // (the reference steps the null pointer to level k and
+                        // reads there)
+                        if (!mesh.texture.data) target_level = 0;
+                        const uint32_t* data = mesh.texture.data;
+                        uint32_t width = (uint32_t)mesh.texture.width;
+                        uint32_t height = (uint32_t)mesh.texture.height;
+                        for (int i = 1; i <= target_level; i++) {
+                            data = data + (size_t)width * height;
+                            width = (width + 2 - 1) / 2;
+                            height = (height + 2 - 1) / 2;
+                        }
+                        // a JPEG-coded texture stays magenta: the reference's
+                        // resolve only queues its blocks for the decoder
+                        if (!mesh.texture.huffmanTables) color = sampleColor_linear(data, (int)width, (int)height, uv);
                     } else if (attribute == CRX_DISPLAY_UV && mesh.uvs) {
                         crx_vec2 ua = mesh.uvs[i0], ub = mesh.uvs[i1], uc = mesh.uvs[i2];
                         float w = 1.0f - stv.x - stv.y;

@@ -72,11 +72,27 @@ def lib(crx):
 class HostMesh:
     """Host-side arrays of one mesh plus a CMesh pointing at them."""
 
-    def __init__(self, crx, positions, indices, colors=None):
+    def __init__(self, crx, positions, indices, colors=None, uvs=None, texture=None, texture_size=None,
+                 jpeg=False):
+        """texture: the RGBA8 mip chain as one flat uint32 array (level 0 of
+        texture_size = (width, height) first); jpeg marks it JPEG-coded."""
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.indices = None if indices is None else np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
         self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+        self.uvs = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        self.texture = None if texture is None else np.ascontiguousarray(texture, dtype=np.uint32).reshape(-1)
+        self.texture_size = (0, 0) if texture_size is None else (int(texture_size[0]), int(texture_size[1]))
+        self.jpeg = bool(jpeg)
+        # stands in for CMesh::texture.huffmanTables, which is only tested for NULL
+        self._huffman = np.zeros(1, dtype=np.uint32)
         self.num_triangles = len(self.indices) if self.indices is not None else len(self.positions) // 3
+
+    def set_texture(self, m, uvs_ptr, texture_ptr):
+        """Fill CMesh m's uvs and texture from device or host addresses."""
+        m.uvs = uvs_ptr
+        m.texture.width, m.texture.height = self.texture_size
+        m.texture.data = texture_ptr
+        m.texture.huffmanTables = self._huffman.ctypes.data if self.jpeg else None
 
     def cmesh(self, crx, cumm=0, offset=0, count=1, world=None, mesh_id=0):
         m = crx.CMesh()
@@ -91,6 +107,8 @@ class HostMesh:
         m.id = mesh_id
         m.numVertices = len(self.positions)
         m.isLoaded = 1
+        self.set_texture(m, self.uvs.ctypes.data if self.uvs is not None else None,
+                         self.texture.ctypes.data if self.texture is not None else None)
         return m
 
 

@@ -15,10 +15,17 @@ class CompressedMesh:
     """IX_PU16 copy of a mesh: 3 x uint16 positions, bit-packed indices and
     the CMesh compression fields (crx_compress_mesh)."""
 
-    def __init__(self, positions, indices, colors=None):
+    def __init__(self, positions, indices, colors=None, uvs=None, texture=None, texture_size=None, jpeg=False):
         self.positions, self.indices, self.meta = crx.compress_mesh(positions, indices)
         self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
+        self.uvs = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        self.texture = None if texture is None else np.ascontiguousarray(texture, dtype=np.uint32).reshape(-1)
+        self.texture_size = (0, 0) if texture_size is None else (int(texture_size[0]), int(texture_size[1]))
+        self.jpeg = bool(jpeg)
+        self._huffman = np.zeros(1, dtype=np.uint32)
         self.num_triangles = len(np.asarray(indices).reshape(-1, 3))
+
+    set_texture = orc.HostMesh.set_texture
 
     def decoded_positions(self):
         """Float positions as the raster stages decode them: X * factor + min
@@ -62,15 +69,34 @@ class Frame:
     prefix: np.ndarray
 
 
+def mip_chain(levels):
+    """One flat uint32 array of RGBA8 mip levels, level 0 first; each level k
+    is an (h, w) array of ((w_(k-1) + 1) // 2, (h_(k-1) + 1) // 2) texels."""
+    for a, b in zip(levels, levels[1:]):
+        assert b.shape == ((a.shape[0] + 1) // 2, (a.shape[1] + 1) // 2), (a.shape, b.shape)
+    return np.concatenate([np.ascontiguousarray(l, dtype=np.uint32).reshape(-1) for l in levels])
+
+
+def mip_shapes(w, h, n=8):
+    """(h, w) of the first n levels of a w x h texture."""
+    out = []
+    for _ in range(n):
+        out.append((h, w))
+        w, h = (w + 1) // 2, (h + 1) // 2
+    return out
+
+
 def prepare(spec: SceneSpec, view: crx.Mat4, pointers, instanced_ids=True) -> Frame:
-    """pointers[k] = (positions, indices, colors) addresses of mesh k."""
+    """pointers[k] = (positions, indices, colors[, uvs, texture]) addresses of mesh k."""
     unique, instances, transforms, prefix = [], [], [], []
     total = 0
     for k, (mesh, worlds) in enumerate(zip(spec.meshes, spec.worlds)):
         first = len(instances)
         for w in worlds:
             m = crx.CMesh()
-            p, i, c = pointers[k]
+            p, i, c = pointers[k][:3]
+            uv, tex = pointers[k][3:5] if len(pointers[k]) > 3 else (None, None)
+            mesh.set_texture(m, uv, tex)
             m.numTriangles = mesh.num_triangles
             m.positions = p
             m.indices = i
@@ -94,12 +120,22 @@ def prepare(spec: SceneSpec, view: crx.Mat4, pointers, instanced_ids=True) -> Fr
     return Frame(unique, instances, transforms, np.asarray(prefix, dtype=np.uint64))
 
 
+def _uvs(mesh):
+    return getattr(mesh, "uvs", None)
+
+
+def _texture(mesh):
+    return getattr(mesh, "texture", None)
+
+
 def host_pointers(spec: SceneSpec):
     out = []
     for m in spec.meshes:
         out.append((m.positions.ctypes.data,
                     m.indices.ctypes.data if m.indices is not None else None,
-                    m.colors.ctypes.data if m.colors is not None else None))
+                    m.colors.ctypes.data if m.colors is not None else None,
+                    _uvs(m).ctypes.data if _uvs(m) is not None else None,
+                    _texture(m).ctypes.data if _texture(m) is not None else None))
     return out
 
 
@@ -125,7 +161,9 @@ class GpuScene:
             p = ctx.to_device(m.positions).ptr
             i = ctx.to_device(m.indices).ptr if m.indices is not None else None
             c = ctx.to_device(m.colors).ptr if m.colors is not None else None
-            self.pointers.append((p, i, c))
+            uv = ctx.to_device(_uvs(m)).ptr if _uvs(m) is not None else None
+            tex = ctx.to_device(_texture(m)).ptr if _texture(m) is not None else None
+            self.pointers.append((p, i, c, uv, tex))
 
     def render(self, W, H, view, proj, instanced=True, attribute=None, fb=None, cb=None, stream=None):
         ctx = self.ctx

@@ -206,16 +206,6 @@ def test_resolve_attributes(ctx, attribute):
     _check(ctx, spec, 640, 480, view, proj, attribute=attribute)
 
 
-def test_resolve_rejects_texture(ctx):
-    fb = ctx.alloc(8 * 16)
-    cb = ctx.alloc(8 * 16)
-    view, proj = wl.sphere_view(4, 4)
-    target = crx.make_target(fb, cb, 4, 4, view, proj)
-    with pytest.raises(crx.CrxError) as e:
-        ctx.resolve_visbuffer([], [], target, crx.DISPLAY_TEXTURE)
-    assert e.value.status == 6
-
-
 def _two_mesh_scene(ctx, n=30):
     """A scene of n nodes over two sphere meshes (every third node the larger
     one), with the oracle's copies of the meshes."""

@@ -365,3 +365,116 @@ def test_resolve_triangle_id_recovers_linear_scan():
         assert color == ((tri * 12345678) & 0xFFFFFFFF) | 0xFF000000
         checked += 1
     assert checked > 100
+
+
+# ---------------------------------------------------------------------------
+# TEXTURE (resolve.cu:148-195, 783-882)
+# ---------------------------------------------------------------------------
+
+def _textured_triangle(W, gx, gy, texture, size, jpeg=False, uvs=True, offset=(0.0, 0.0)):
+    """The screen triangle A=(10,10), B=(42,10), C=(10,42) at depth 2 with
+    uv_a = offset, uv_b = offset + (gx, 0), uv_c = offset + (0, gy): pixel
+    (x, y) then has uv = offset + (gx (x-10)/32, gy (y-10)/32), and the next
+    pixel right / down adds gx/32 / gy/32."""
+    pos, idx = screen_triangle_scene([(10.0, 10.0), (42.0, 10.0), (10.0, 42.0)], W, W, depth=2.0)
+    uv = np.array([[0.0, 0.0], [gx, 0.0], [0.0, gy]], dtype=np.float32) + np.float32(offset)
+    mesh = orc.HostMesh(crx, pos, idx, None, uvs=uv if uvs else None, texture=texture, texture_size=size,
+                        jpeg=jpeg)
+    view, proj = crx.identity(), identity_proj(1.0)
+    inst = [mesh.cmesh(crx)]
+    fb, _ = orc.draw(crx, inst, inst, [crx.identity()], W, W, view, proj, True, threads=1)
+    cb = orc.resolve(crx, inst, [0], fb, W, W, view, proj, crx.DISPLAY_TEXTURE, threads=1)
+    return fb, cb
+
+
+def _interior(W):
+    """Covered pixels well inside the triangle (s, t >= 1/32, s + t <= 30/32)."""
+    return [(x, y) for y in range(11, 40) for x in range(11, 40) if (x - 10) + (y - 10) <= 30]
+
+
+def _rgba(c):
+    return [(int(c) >> (8 * k)) & 0xFF for k in range(4)]
+
+
+def test_texture_mip_level_of_the_uv_footprint():
+    """Each level of a 100 x 60 chain is one colour.  The footprint of a pixel
+    is gx/32 by gy/32 in uv, so m = (max(gx, gy)/32)^2 (100^2 + 60^2); g is
+    chosen so that 0.5 log2 m = j + phase: level j, clamped to [0, 7].  The
+    phases 0.04 and 0.96 put m within 6% of a level boundary, so a dropped or
+    misplaced term of m moves the level.  Level k is ((w + 1) / 2) x
+    ((h + 1) / 2) of level k - 1, so a wrong offset into the chain reads
+    another level's colour."""
+    import crxtest
+    shapes = crxtest.mip_shapes(100, 60)
+    assert shapes[3] == (8, 13) and shapes[7] == (1, 1)
+    cols = [0xFF000000 | ((7 * k + 3) << 16) | ((200 - 20 * k) << 8) | (20 + 25 * k) for k in range(8)]
+    chain = crxtest.mip_chain([np.full(s, cols[k], dtype=np.uint32) for k, s in enumerate(shapes)])
+    W = 64
+    for j, phase in [(j, p) for j in (-2, 0, 1, 2, 3, 5, 7, 9) for p in (0.5, 0.04, 0.96)]:
+        g = 32.0 * math.sqrt(4.0 ** (j + phase) / (100 ** 2 + 60 ** 2))
+        # the longer footprint side decides: along u for even j, along v for odd j
+        gx, gy = (g, g / 1.5) if j % 2 == 0 else (g / 1.5, g)
+        fb, cb = _textured_triangle(W, gx, gy, chain, (100, 60))
+        want = _rgba(cols[min(max(j, 0), 7)])
+        for (x, y) in _interior(W)[::7]:
+            got = _rgba(cb[y * W + x])
+            assert got[3] == 255
+            assert all(abs(a - b) <= 1 for a, b in zip(got[:3], want[:3])), (j, phase, (x, y), got, want)
+
+
+@pytest.mark.parametrize("level, gx, gy, offset", [(0, 0.194, 0.15, (0.0, 0.0)),
+                                                   (2, 0.776, 0.6, (3.0, -2.0))])
+def test_texture_bilinear_on_a_gradient(level, gx, gy, offset):
+    """Level k holds red = texel column, green = texel row, blue = 40 k.  The
+    four texels at ftx -+ 0.5, fty -+ 0.5 weighted by the fractions of
+    ftx + 0.5 and fty + 0.5 give red = ftx - 0.5 and green = fty - 0.5 exactly
+    (ftx = frac(u) * level width), then truncated to 8 bits.  gx != gy keeps
+    the x and y weights apart, so swapped texels or weights fail; the
+    offset (3, -2) checks the wrap of uv into [0, 1)."""
+    import crxtest
+    shapes = crxtest.mip_shapes(200, 120)
+    levels = []
+    for k, (h, w) in enumerate(shapes):
+        ty, tx = np.mgrid[0:h, 0:w].astype(np.uint32)
+        levels.append(0xFF000000 | ((40 * k) << 16) | (ty << 8) | tx)
+    chain = crxtest.mip_chain(levels)
+    W = 64
+    # the footprint picks the level: m = (gx/32)^2 (200^2 + 120^2) in [4^k, 4^k * 4)
+    m = (max(gx, gy) / 32.0) ** 2 * (200 ** 2 + 120 ** 2)
+    assert int(0.5 * math.log2(m)) == level
+    fb, cb = _textured_triangle(W, gx, gy, chain, (200, 120), offset=offset)
+    lh, lw = shapes[level]
+    checked = 0
+    for (x, y) in _interior(W):
+        ftx = gx * (x - 10) / 32.0 * lw
+        fty = gy * (y - 10) / 32.0 * lh
+        if ftx < 0.6 or fty < 0.6:
+            continue
+        got = _rgba(cb[y * W + x])
+        for ch, f in ((0, ftx), (1, fty)):
+            assert got[ch] in (math.floor(f - 0.5 - 1e-3), math.floor(f - 0.5 + 1e-3)), ((x, y), ch, got, f)
+        assert abs(got[2] - 40 * level) <= 1 and got[3] == 255
+        checked += 1
+    assert checked > 200
+
+
+@pytest.mark.parametrize("g", [0.2, 3.0])
+def test_texture_jpeg_missing_data_and_missing_uvs(g):
+    """A JPEG-coded texture leaves the pixel magenta (the decoder's pass is
+    not part of the resolve); no texture data samples 0, shaded opaque, at
+    level 0 (g = 0.2) and at level 3 (g = 3: m = (3/32)^2 2 64^2 = 72); a
+    mesh without uvs takes the final branch of the attribute chain (blue)."""
+    W = 64
+    chain = np.full(64 * 64 * 2, 0xFF123456, dtype=np.uint32)
+    cases = [(dict(texture=chain, jpeg=True), 0xFFFF00FF),
+             (dict(texture=None), 0xFF000000),
+             (dict(texture=chain, uvs=False), 0xFF0000FF)]
+    for kw, want in cases:
+        tex = kw.pop("texture")
+        fb, cb = _textured_triangle(W, g, g, tex, (64, 64), **kw)
+        cov = np.nonzero(fb != crx.CLEAR_PIXEL)[0]
+        assert cov.size > 400
+        assert np.all((cb[cov] & 0xFFFFFFFF) == want), (kw, hex(int(cb[cov[0]])))
+        depth = (cb[cov] >> np.uint64(32)).astype(np.uint32).view(np.float32)
+        assert np.all(np.abs(depth - 2.0) < 1e-5)
+        assert int(cb[0]) == 0x7F80000000000000
