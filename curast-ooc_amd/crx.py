@@ -257,6 +257,8 @@ _SIGS = {
     "crx_probe_atomic_min": (C.c_int, [_VP, _VP, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), _VP]),
     "crx_probe_read": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _VP]),
     "crx_probe_valu": (C.c_int, [_VP, C.c_uint32, C.c_int, _VP, C.POINTER(C.c_uint64), _VP]),
+    "crx_texture_chain_texels": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_uint64)]),
+    "crx_texture_compute_mipmap": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, _VP]),
     "crx_check_division": (C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_uint64)]),
     "crx_clear_framebuffer": (C.c_int, [_VP, C.c_uint32, _VP]),
@@ -523,6 +525,13 @@ class Context:
         n = C.c_uint64()
         _check(lib().crx_probe_valu(self.handle, iters, mode, C.c_void_p(sink.ptr), C.byref(n), stream))
         return n.value
+
+    def compute_mipmap(self, data: DeviceBuffer | int, width: int, height: int, stream=None) -> None:
+        """Levels 1 .. of the RGBA8 mip chain in `data` from its level 0
+        (computeMipMap, textureTools.cu:138-178); data holds
+        texture_chain_texels(width, height) texels."""
+        ptr = data.ptr if isinstance(data, DeviceBuffer) else data
+        _check(lib().crx_texture_compute_mipmap(self.handle, C.c_void_p(ptr), width, height, stream))
 
     # --- kernel-level entry points ------------------------------------------------
 
@@ -795,6 +804,13 @@ class Scene:
         if self.handle:
             lib().crx_scene_destroy(C.c_void_p(self.handle))
             self.handle = None
+
+
+def texture_chain_texels(width: int, height: int) -> int:
+    """Texels of a width x height texture's mip chain (levels 0 .. max(L-1, 7))."""
+    n = C.c_uint64()
+    _check(lib().crx_texture_chain_texels(width, height, C.byref(n)))
+    return n.value
 
 
 def compress_mesh(positions: np.ndarray, indices: np.ndarray):

@@ -1347,6 +1347,60 @@ crx_status crx_probe_valu(crx_context* ctx, uint32_t iters, int mode, uint32_t* 
     return CRX_OK;
 }
 
+namespace {
+
+// Last level crx_texture_compute_mipmap writes: the reference computes levels
+// 1 .. L-1 with L = floor(log2(max(w, h))) (textureTools.cu:47, :59); levels up
+// to 7, the deepest TEXTURE samples, are always computed.
+uint32_t mip_last_level(int32_t width, int32_t height) {
+    const uint32_t m = (uint32_t)std::max(width, height);
+    const uint32_t L = 31u - (uint32_t)__builtin_clz(m);
+    return std::max(L, 8u) - 1u;
+}
+
+crx_status check_texture_size(int32_t width, int32_t height) {
+    if (width <= 0 || height <= 0) return fail(CRX_INVALID_ARG, "texture width and height must be positive");
+    if ((uint64_t)width * (uint64_t)height > 0x7fffffffull)
+        return fail(CRX_INVALID_ARG, "texture has more than 2^31 - 1 texels");
+    return CRX_OK;
+}
+
+}  // namespace
+
+crx_status crx_texture_chain_texels(int32_t width, int32_t height, uint64_t* out) {
+    if (!out) return fail(CRX_INVALID_ARG, "out is NULL");
+    CRX_TRY(check_texture_size(width, height));
+    uint64_t total = 0, w = (uint64_t)width, h = (uint64_t)height;
+    for (uint32_t level = 0; level <= mip_last_level(width, height); level++) {
+        total += w * h;
+        w = (w + 1) / 2;
+        h = (h + 1) / 2;
+    }
+    *out = total;
+    return CRX_OK;
+}
+
+crx_status crx_texture_compute_mipmap(crx_context* ctx, uint32_t* data, int32_t width, int32_t height, void* stream) {
+    if (!ctx || !data) return fail(CRX_INVALID_ARG, "NULL argument");
+    CRX_TRY(check_texture_size(width, height));
+    CRX_HIP(hipSetDevice(ctx->device));
+    uint32_t sw = (uint32_t)width, sh = (uint32_t)height;
+    uint64_t srcOffset = 0;
+    for (uint32_t level = 1; level <= mip_last_level(width, height); level++) {
+        const uint32_t tw = (sw + 1) / 2, th = (sh + 1) / 2;
+        const uint64_t dstOffset = srcOffset + (uint64_t)sw * sh;
+        // alpha: the reference starts each texel from a per-level debug colour
+        // (0x00RRGGBB for levels 1-9, 0xff0000ff beyond) and overwrites RGB
+        const uint32_t alpha = level >= 10 ? 0xffu : 0u;
+        CRX_HIP(launch_mip_level(data + srcOffset, data + dstOffset, sw, sh, tw, th, alpha, ctx->numCUs,
+                                 as_stream(stream)));
+        srcOffset = dstOffset;
+        sw = tw;
+        sh = th;
+    }
+    return CRX_OK;
+}
+
 crx_status crx_clear_framebuffer(uint64_t* framebuffer, uint32_t numPixels, void* stream) {
     if (!framebuffer && numPixels) return fail(CRX_INVALID_ARG, "framebuffer is NULL");
     if (numPixels == 0) return CRX_OK;

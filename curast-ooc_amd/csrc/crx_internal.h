@@ -41,6 +41,9 @@ hipError_t launch_clear_views(const ClearView* views, uint32_t numViews, uint32_
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, int numCUs, hipStream_t stream);
 hipError_t launch_read(const void* src, size_t bytes, int numCUs, uint32_t* sink, hipStream_t stream);
 hipError_t launch_check_division(uint64_t pairs, int numCUs, unsigned long long* out, hipStream_t stream);
+// One mip level: dst (tw x th, tw * th < 2^32) from src (sw x sh), 2x2 box sums / 4
+hipError_t launch_mip_level(const uint32_t* src, uint32_t* dst, uint32_t sw, uint32_t sh, uint32_t tw, uint32_t th,
+                            uint32_t alpha, int numCUs, hipStream_t stream);
 hipError_t launch_valu_probe(uint32_t iters, int mode, uint32_t blocks, uint32_t* sink, hipStream_t stream);
 hipError_t launch_atomic_probe(uint64_t* buf, uint32_t logEntries, uint32_t perLane, int scattered, uint32_t blocks,
                                hipStream_t stream);

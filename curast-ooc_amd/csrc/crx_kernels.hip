@@ -2645,9 +2645,49 @@ __global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, int mode, ui
     }
 }
 
+// One level of a texture's mip chain (kernel_computeMipMap,
+// textureTools.cu:41-133): texel (x, y) of the target level is the
+// per-channel sum of the source texels (2x..2x+1, 2y..2y+1) that lie inside
+// the source level (a missing one adds 0), divided by 4 and truncated; its
+// alpha byte is `alpha`.  One lane per target texel, grid-stride; neighbouring
+// lanes read neighbouring texel pairs of two source rows.
+__global__ __launch_bounds__(256) void k_mip_level(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                   uint32_t sw, uint32_t sh, uint32_t tw, uint32_t n,
+                                                   uint32_t alpha) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint32_t y = i / tw;
+        const uint32_t x = i - y * tw;
+        const uint32_t sx = 2u * x, sy = 2u * y;
+        uint32_t r = 0u, g = 0u, b = 0u;
+        auto add = [&](uint32_t px, uint32_t py) {
+            if (px >= sw || py >= sh) return;
+            const uint32_t t = src[(uint64_t)py * sw + px];
+            r += t & 0xffu;
+            g += (t >> 8) & 0xffu;
+            b += (t >> 16) & 0xffu;
+        };
+        add(sx, sy);
+        add(sx, sy + 1u);
+        add(sx + 1u, sy);
+        add(sx + 1u, sy + 1u);
+        dst[i] = (alpha << 24) | ((b >> 2) << 16) | ((g >> 2) << 8) | (r >> 2);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launch wrappers used by the host glue.
 // ---------------------------------------------------------------------------
+
+hipError_t launch_mip_level(const uint32_t* src, uint32_t* dst, uint32_t sw, uint32_t sh, uint32_t tw, uint32_t th,
+                            uint32_t alpha, int numCUs, hipStream_t stream) {
+    const uint32_t n = tw * th;
+    uint32_t blocks = (n + 255u) / 256u;
+    const uint32_t cap = (uint32_t)numCUs * 16u;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_mip_level, dim3(blocks), dim3(256), 0, stream, src, dst, sw, sh, tw, n, alpha);
+    return hipGetLastError();
+}
 
 hipError_t launch_clear(uint64_t* fb, uint32_t numPixels, uint32_t* ctr, int numCUs, hipStream_t stream,
                         uint64_t* fb2, uint32_t numPixels2) {

@@ -518,6 +518,21 @@ crx_status crx_mesh_info(const crx_mesh* mesh, uint32_t* numTriangles, uint32_t*
  * reference, whose stage 1 walks every mesh contiguously; it orders the
  * walk and changes no result. */
 crx_status crx_mesh_row_triangles(const crx_mesh* mesh, uint32_t* out);
+/* Texels a w x h texture's mip chain occupies in crx_texture.data: levels
+ * 0 .. max(L - 1, 7), L = floor(log2(max(w, h))), each ((w + 1) / 2) x
+ * ((h + 1) / 2) texels of the level before (createMipMappedTexture,
+ * LargeGlbLoader.h:97-111, sizes levels 0 .. L - 1; levels up to 7 are
+ * what CRX_DISPLAY_TEXTURE may sample).  w * h must be below 2^31. */
+crx_status crx_texture_chain_texels(int32_t width, int32_t height, uint64_t* texels);
+/* computeMipMap (textureTools.cu:138-178, kernel :41-133): with level 0 of
+ * the RGBA8 chain `data` (device) in place, computes levels 1 .. max(L - 1, 7)
+ * on `stream`, one launch per level.  A texel is the per-channel sum of the
+ * source level's 2 x 2 block (texels past the source's edge add 0) divided by
+ * 4 and truncated; alpha is 0 for levels 1-9 and 255 beyond, as the
+ * reference leaves it.  data holds crx_texture_chain_texels texels; levels
+ * 1-7 of a texture smaller than 256 texels on its longer side are computed
+ * here although the reference neither allocates nor computes them. */
+crx_status crx_texture_compute_mipmap(crx_context* ctx, uint32_t* data, int32_t width, int32_t height, void* stream);
 /* Vertex attributes of a mesh (Mesh.h:17-19, cptr_uv / cptr_normal): uvs
  * (2*numVertices floats) and normals (3*numVertices floats) are copied from
  * host arrays into device memory owned by the mesh; a NULL array leaves that

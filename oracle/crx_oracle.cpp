@@ -718,6 +718,48 @@ void crx_oracle_clear(uint64_t* fb, uint64_t numPixels) {
     for (uint64_t i = 0; i < numPixels; i++) fb[i] = 0xFFFFFFF000000000ull;
 }
 
+int crx_oracle_compute_mipmap(uint32_t* data, int32_t width, int32_t height) {
+    // textureTools.cu:47: levels = int(log2(max(w, h))), here the exact floor
+    // (the float log2 agrees below 2^21); levels 1 .. levels-1 are computed,
+    // and (normalisation) at least levels 1 .. 7, the ones the resolve samples
+    int64_t levels = 0;
+    for (int64_t m = std::max(width, height); m > 1; m >>= 1) levels++;
+    const int64_t last = std::max<int64_t>(levels - 1, 7);
+    int64_t source_width = width, source_height = height, source_pixelOffset = 0;
+    for (int64_t level = 1; level <= last; level++) {
+        const int64_t target_width = (source_width + 2 - 1) / 2;
+        const int64_t target_height = (source_height + 2 - 1) / 2;
+        const int64_t target_pixelOffset = source_pixelOffset + source_width * source_height;
+        for (int64_t target_y = 0; target_y < target_height; target_y++) {
+            for (int64_t target_x = 0; target_x < target_width; target_x++) {
+                // textureTools.cu:76-85: a debug colour per level 1-9, whose
+                // alpha byte (0) survives; 0xff0000ff beyond
+                uint32_t color = level <= 9 ? 0x00000000u : 0xff0000ffu;
+                int32_t sR = 0, sG = 0, sB = 0;
+                auto fetchAdd = [&](int64_t source_x, int64_t source_y) {
+                    if (source_x >= source_width) return;
+                    if (source_y >= source_height) return;
+                    uint32_t sample = data[source_pixelOffset + source_x + source_width * source_y];
+                    sR += sample & 0xff;
+                    sG += (sample >> 8) & 0xff;
+                    sB += (sample >> 16) & 0xff;
+                };
+                fetchAdd(2 * target_x + 0, 2 * target_y + 0);
+                fetchAdd(2 * target_x + 0, 2 * target_y + 1);
+                fetchAdd(2 * target_x + 1, 2 * target_y + 0);
+                fetchAdd(2 * target_x + 1, 2 * target_y + 1);
+                color = (color & 0xff000000u) | (uint32_t)(sR / 4) | ((uint32_t)(sG / 4) << 8) |
+                        ((uint32_t)(sB / 4) << 16);
+                data[target_pixelOffset + target_x + target_width * target_y] = color;
+            }
+        }
+        source_width = target_width;
+        source_height = target_height;
+        source_pixelOffset = target_pixelOffset;
+    }
+    return (int)last;
+}
+
 }  // extern "C"
 
 namespace {

@@ -60,6 +60,11 @@ void crx_oracle_stage3(const crx_cmesh* instances, const crx_mat4* transforms,
                        const crx_render_target* target, const crx_tuning* tuning,
                        const crx_huge_triangle* huge, uint64_t count, int nthreads);
 
+/* computeMipMap's kernel (textureTools.cu:41-133) over the chain `data`
+ * whose level 0 is in place: levels 1 .. max(L - 1, 7), L = floor(log2(max(w,
+ * h))).  Returns the last level written. */
+int crx_oracle_compute_mipmap(uint32_t* data, int32_t width, int32_t height);
+
 /* kernel_resolve_visbuffer_to_colorbuffer2D, resolve.cu:640-1141 */
 int crx_oracle_resolve(const crx_cmesh* instances, uint32_t numInstances,
                        const uint64_t* prefix, const crx_render_target* target,

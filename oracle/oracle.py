@@ -35,6 +35,7 @@ def lib(crx):
     P = C.POINTER
     sigs = {
         "crx_oracle_clear": (None, [VP, C.c_uint64]),
+        "crx_oracle_compute_mipmap": (C.c_int, [VP, C.c_int32, C.c_int32]),
         "crx_oracle_draw": (C.c_int, [P(crx.CMesh), C.c_uint32, P(crx.CMesh), C.c_uint32, P(crx.Mat4),
                                       P(crx.RenderTarget), C.c_int, P(crx.Tuning), C.c_int, P(OracleStats)]),
         "crx_oracle_stage1": (C.c_uint64, [P(crx.CMesh), C.c_uint32, P(crx.CMesh), C.c_uint32, P(crx.Mat4),

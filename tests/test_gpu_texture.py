@@ -221,6 +221,55 @@ def test_texture_through_the_scene_api(ctx):
     mesh.close()
 
 
+@pytest.mark.parametrize("w, h", [(1, 1), (5, 3), (100, 60), (255, 2), (1024, 3), (4096, 3), (1000, 1000),
+                                  (8192, 8192)])
+def test_compute_mipmap_matches_the_oracle(ctx, w, h):
+    """crx_texture_compute_mipmap against the oracle's computeMipMap, bit for
+    bit, and nothing written past the chain."""
+    print(f"[case] mipmap {w}x{h}", flush=True)
+    L = orc.lib(crx)
+    rng = np.random.default_rng(w * 31 + h)
+    n = crx.texture_chain_texels(w, h)
+    host = np.full(n + 64, 0xDEADBEEF, dtype=np.uint32)
+    host[:w * h] = rng.integers(0, 2 ** 32, size=w * h, dtype=np.uint32)
+    dev = ctx.to_device(host)
+    ctx.compute_mipmap(dev, w, h)
+    got = dev.download(np.uint32, n + 64)
+    L.crx_oracle_compute_mipmap(host.ctypes.data, w, h)
+    assert np.all(got[n:] == 0xDEADBEEF)
+    bad = np.nonzero(got != host)[0]
+    assert bad.size == 0, f"{bad.size} texels differ, first at {bad[:4]}"
+
+
+def test_texture_with_a_computed_chain(ctx):
+    """A terrain textured with a chain that crx_texture_compute_mipmap built
+    from a random level 0, against the oracle with the oracle's own chain."""
+    n = 513
+    pos, idx, col = wl.heightfield_mesh(n, amplitude=20.0, scale=128.0)
+    rng = np.random.default_rng(77)
+    w, h = 700, 500
+    texels = crx.texture_chain_texels(w, h)
+    host = np.zeros(texels, dtype=np.uint32)
+    host[:w * h] = rng.integers(0, 2 ** 32, size=w * h, dtype=np.uint32)
+    dev = ctx.to_device(host)
+    ctx.compute_mipmap(dev, w, h)
+    orc.lib(crx).crx_oracle_compute_mipmap(host.ctypes.data, w, h)
+    mesh = orc.HostMesh(crx, pos, idx, col, uvs=_terrain_uvs(pos, n, 6.0), texture=host, texture_size=(w, h))
+    spec = SceneSpec().add(mesh, [crx.identity()])
+    W, H = 1280, 720
+    view, proj = wl.heightfield_view(n, W, H, altitude=30.0, pitch_deg=15.0, setback=10.0)
+    print(f"[case] computed chain {w}x{h}", flush=True)
+    want_fb, want_cb, _ = oracle_render(spec, W, H, view, proj, attribute=crx.DISPLAY_TEXTURE)
+    g = GpuScene(ctx, spec)
+    uv_ptr = g.pointers[0][3]
+    g.pointers[0] = g.pointers[0][:4] + (dev.ptr,)     # the GPU-built chain in place of the uploaded copy
+    assert uv_ptr is not None
+    fb, cb, _ = g.render(W, H, view, proj, attribute=crx.DISPLAY_TEXTURE)
+    assert np.array_equal(fb, want_fb), describe_mismatch(fb, want_fb, W)
+    assert np.array_equal(cb, want_cb), describe_mismatch(cb, want_cb, W)
+    assert np.unique(cb[fb != crx.CLEAR_PIXEL] & 0xFFFFFFFF).size > 1000
+
+
 def test_resolve_rejects_stage_and_bad_textures(ctx):
     fb = ctx.alloc(8 * 16)
     cb = ctx.alloc(8 * 16)
@@ -251,5 +300,9 @@ def test_resolve_rejects_stage_and_bad_textures(ctx):
     assert e.value.status == 1
     with pytest.raises(ValueError):
         mesh.set_attributes(uvs=np.zeros((3, 2), np.float32))
+    for bad in ((0, 4), (4, -1), (65536, 65536)):
+        with pytest.raises(crx.CrxError) as e:
+            ctx.compute_mipmap(g.pointers[0][0], *bad)
+        assert e.value.status == 1
     scene.close()
     mesh.close()

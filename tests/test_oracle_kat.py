@@ -478,3 +478,70 @@ def test_texture_jpeg_missing_data_and_missing_uvs(g):
         depth = (cb[cov] >> np.uint64(32)).astype(np.uint32).view(np.float32)
         assert np.all(np.abs(depth - 2.0) < 1e-5)
         assert int(cb[0]) == 0x7F80000000000000
+
+
+# ---------------------------------------------------------------------------
+# mip chain (textureTools.cu:41-133)
+# ---------------------------------------------------------------------------
+
+def _numpy_chain(level0, last):
+    """Levels 0..last by whole-array 2x2 block sums: each level zero-padded
+    to even sides, reshaped into blocks, summed per channel, // 4."""
+    levels = [level0]
+    cur = level0
+    for k in range(1, last + 1):
+        h, w = cur.shape
+        pad = np.zeros((h + h % 2, w + w % 2), dtype=np.uint32)
+        pad[:h, :w] = cur
+        ch = [((pad >> np.uint32(8 * c)) & np.uint32(0xFF)).astype(np.int64) for c in range(3)]
+        sums = [x.reshape(pad.shape[0] // 2, 2, pad.shape[1] // 2, 2).sum(axis=(1, 3)) // 4 for x in ch]
+        alpha = 0 if k <= 9 else 0xFF
+        cur = (np.uint32(alpha << 24) | (sums[2].astype(np.uint32) << np.uint32(16)) |
+               (sums[1].astype(np.uint32) << np.uint32(8)) | sums[0].astype(np.uint32))
+        levels.append(cur)
+    return levels
+
+
+def _oracle_chain(level0):
+    h, w = level0.shape
+    n = crx.texture_chain_texels(w, h)
+    buf = np.full(n + 16, 0xDEADBEEF, dtype=np.uint32)
+    buf[:w * h] = level0.reshape(-1)
+    last = L.crx_oracle_compute_mipmap(buf.ctypes.data, w, h)
+    assert np.all(buf[n:] == 0xDEADBEEF)      # nothing past the chain
+    return buf[:n], last
+
+
+@pytest.mark.parametrize("w, h, last", [(1, 1, 7), (1, 7, 7), (5, 3, 7), (64, 64, 7), (100, 60, 7), (255, 2, 7),
+                                        (256, 9, 7), (1024, 3, 9), (4096, 3, 11), (37, 300, 7)])
+def test_mip_chain_is_the_2x2_block_average(w, h, last):
+    """Levels 1 .. max(floor(log2(max(w, h))) - 1, 7), each a 2x2 block sum
+    // 4 of the level before with missing texels as 0; alpha 0 through level
+    9, 255 beyond; the chain's size is the sum of its levels."""
+    import crxtest
+    rng = np.random.default_rng(w * 7919 + h)
+    level0 = rng.integers(0, 2 ** 32, size=(h, w), dtype=np.uint32)
+    chain, got_last = _oracle_chain(level0)
+    assert got_last == last
+    shapes = crxtest.mip_shapes(w, h, last + 1)
+    assert chain.size == sum(a * b for a, b in shapes)
+    want = np.concatenate([l.reshape(-1) for l in _numpy_chain(level0, last)])
+    assert np.array_equal(chain, want)
+
+
+def test_mip_chain_closed_forms():
+    """A constant colour c keeps c on even sides and becomes c/2 on the odd
+    edge and c/4 in the odd corner (missing texels add 0, the divisor stays
+    4); a red ramp r = x gives r = 2x on level 1 ((2x + 2x + 1) * 2 // 4)."""
+    c = 0x00C8A064            # r = 100, g = 160, b = 200
+    chain, _ = _oracle_chain(np.full((3, 5), c, dtype=np.uint32))
+    lvl1 = chain[15:15 + 6].reshape(2, 3)
+    rgb = lambda t: [(int(t) >> s) & 0xFF for s in (0, 8, 16)]
+    assert rgb(lvl1[0, 0]) == [100, 160, 200] and rgb(lvl1[0, 1]) == [100, 160, 200]
+    assert rgb(lvl1[0, 2]) == [50, 80, 100] and rgb(lvl1[1, 0]) == [50, 80, 100]
+    assert rgb(lvl1[1, 2]) == [25, 40, 50]
+    assert all(int(t) >> 24 == 0 for t in lvl1.reshape(-1))
+    ramp = np.tile(np.arange(256, dtype=np.uint32), (4, 1))
+    chain, _ = _oracle_chain(ramp)
+    lvl1 = chain[1024:1024 + 256].reshape(2, 128)
+    assert np.array_equal(lvl1 & 0xFF, np.tile(2 * np.arange(128, dtype=np.uint32), (2, 1)))
