@@ -765,7 +765,8 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
     if (numViews == 1) {
         CRX_HIP(launch_resolve(rv[0], stream));
     } else {
-        CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews, blocks, stream));
+        CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews, blocks,
+                                     attribute == CRX_DISPLAY_TEXTURE, stream));
     }
     record(ctx, T_RESOLVE_END, stream);
     return CRX_OK;
@@ -798,7 +799,8 @@ crx_status resolve_views_device(crx_context* ctx, const InstanceRec* instDev, co
     CRX_HIP(hipMemcpyAsync(ctx->resolveMeta.ptr, rv, bytes, hipMemcpyHostToDevice, stream));
     CRX_TRY(ctx->resolveStaging.mark(stream));
     record(ctx, T_RESOLVE, stream);
-    CRX_HIP(launch_resolve_views(static_cast<const ResolveParams*>(ctx->resolveMeta.ptr), numViews, blocks, stream));
+    CRX_HIP(launch_resolve_views(static_cast<const ResolveParams*>(ctx->resolveMeta.ptr), numViews, blocks,
+                                 attribute == CRX_DISPLAY_TEXTURE, stream));
     record(ctx, T_RESOLVE_END, stream);
     return CRX_OK;
 }

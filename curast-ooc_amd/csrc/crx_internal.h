@@ -35,7 +35,8 @@ hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t strea
 hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream);
 hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream);
 // views: device array of numViews records; numBlocks = total blocks over all views
-hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, hipStream_t stream);
+hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, bool texture,
+                                hipStream_t stream);
 hipError_t launch_clear_views(const ClearView* views, uint32_t numViews, uint32_t numBlocks, uint32_t* ctr,
                               hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, int numCUs, hipStream_t stream);

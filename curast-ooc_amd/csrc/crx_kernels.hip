@@ -1755,6 +1755,20 @@ __device__ __forceinline__ f2 sh_intersect(f2 p1, f2 p2, int e) {
 }
 
 __device__ void sh_clip_extend(f2 a, f2 b, f2 c, f2& bmin, f2& bmax) {
+    // A triangle inside all four edges passes every edge unchanged (each
+    // vertex is kept, no intersection is made): extend by a, b, c in order,
+    // as the loop below would, without its per-lane polygon buffers.
+    if (sh_inside(a, 0) && sh_inside(a, 1) && sh_inside(a, 2) && sh_inside(a, 3) &&
+        sh_inside(b, 0) && sh_inside(b, 1) && sh_inside(b, 2) && sh_inside(b, 3) &&
+        sh_inside(c, 0) && sh_inside(c, 1) && sh_inside(c, 2) && sh_inside(c, 3)) {
+        for (const f2 p : {a, b, c}) {
+            bmin.x = fminf(bmin.x, p.x);
+            bmin.y = fminf(bmin.y, p.y);
+            bmax.x = fmaxf(bmax.x, p.x);
+            bmax.y = fmaxf(bmax.y, p.y);
+        }
+        return;
+    }
     f2 bufA[8], bufB[8];
     f2* in = bufA;
     f2* out = bufB;
@@ -2218,6 +2232,9 @@ __device__ __forceinline__ int mip_level(float m) {
     return min(e >> 1, 7);
 }
 
+// TEX: the variant launched for the TEXTURE attribute; the other attributes
+// run without its code, whose registers would lower every resolve's occupancy.
+template <bool TEX>
 __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const InstanceRec& mesh, uint64_t total,
                                                 int x, int y) {
     uint32_t color = 0;
@@ -2268,7 +2285,7 @@ __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const In
             color = tri * 12345678u;
         } else if (attr == 6) {                // MESH_ID
             color = ((uint32_t)mesh.id + 1u) * 12345678u;
-        } else if (attr == 1 && mesh.uvs) {    // TEXTURE (resolve.cu:783-876)
+        } else if (TEX && attr == 1 && mesh.uvs) {    // TEXTURE (resolve.cu:783-876)
             // the neighbouring pixels' rays, hits and texture coordinates give
             // the sample's footprint, and with it the mip level
             const f3 r10 = resolve_ray(R, origin, (float)x + 1.5f, (float)y + 0.5f);
@@ -2368,6 +2385,7 @@ __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const In
 }
 
 // Resolve of one pixel (x, y) < (W, H).
+template <bool TEX>
 __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int y) {
     const uint32_t pixelID = (uint32_t)x + (uint32_t)R.W * (uint32_t)y;
 
@@ -2401,11 +2419,11 @@ __device__ __forceinline__ void resolve_pixel(const ResolveParams& R, int x, int
         if (__ballot((uint32_t)found != f0) == 0ull) {
             const InstanceRec mesh = kconst(R.inst)[f0];
             if (R.ownedOnly && total - mesh.cumm >= (uint64_t)mesh.numTriangles) return;
-            color = shade_pixel(R, mesh, total, x, y);
+            color = shade_pixel<TEX>(R, mesh, total, x, y);
         } else {
             const InstanceRec mesh = R.inst[found];
             if (R.ownedOnly && total - mesh.cumm >= (uint64_t)mesh.numTriangles) return;
-            color = shade_pixel(R, mesh, total, x, y);
+            color = shade_pixel<TEX>(R, mesh, total, x, y);
         }
     } else if (R.ownedOnly && !__builtin_isinf(depth)) {
         return;
@@ -2432,15 +2450,17 @@ __device__ __forceinline__ uint32_t find_view(uint32_t numViews, uint32_t b, Get
 
 // Resolve: one lane per pixel, 64x4 pixels per workgroup (a wave reads 512
 // contiguous bytes of the visibility buffer).
+template <bool TEX>
 __global__ __launch_bounds__(256) void k_resolve(ResolveParams R) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= R.W || y >= R.H) return;
-    resolve_pixel(R, x, y);
+    resolve_pixel<TEX>(R, x, y);
 }
 
 // Resolve of several views in one launch: block b resolves 64x4 tile
 // b - firstBlock[v] of view v, whose record is read by scalar loads.
+template <bool TEX>
 __global__ __launch_bounds__(256) void k_resolve_views(const ResolveParams* __restrict__ views, uint32_t numViews) {
     const auto rv = kconst(views);
     const uint32_t v = find_view(numViews, blockIdx.x, [&](uint32_t i) { return rv[i].firstBlock; });
@@ -2450,7 +2470,7 @@ __global__ __launch_bounds__(256) void k_resolve_views(const ResolveParams* __re
     const int x = (int)(local % tilesX) * 64 + (int)(threadIdx.x & 63);
     const int y = (int)(local / tilesX) * 4 + (int)(threadIdx.x >> 6);
     if (x >= R.W || y >= R.H) return;
-    resolve_pixel(R, x, y);
+    resolve_pixel<TEX>(R, x, y);
 }
 
 // Clear of several framebuffers in one launch: block b clears the 4096-pixel
@@ -2783,9 +2803,11 @@ hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t t
     return launch_s1_any(P, block, trisPerLane, grid, pu16, true, false, stream);
 }
 
-hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, hipStream_t stream) {
+hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, bool texture,
+                                hipStream_t stream) {
     if (numViews == 0 || numBlocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_resolve_views, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
+    if (texture) hipLaunchKernelGGL(k_resolve_views<true>, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
+    else hipLaunchKernelGGL(k_resolve_views<false>, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
     return hipGetLastError();
 }
 
@@ -2840,7 +2862,8 @@ hipError_t launch_check_division(uint64_t pairs, int numCUs, unsigned long long*
 
 hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream) {
     const dim3 grid((uint32_t)(R.W + 63) / 64u, (uint32_t)(R.H + 3) / 4u);
-    hipLaunchKernelGGL(k_resolve, grid, dim3(256), 0, stream, R);
+    if (R.attribute == (int32_t)CRX_DISPLAY_TEXTURE) hipLaunchKernelGGL(k_resolve<true>, grid, dim3(256), 0, stream, R);
+    else hipLaunchKernelGGL(k_resolve<false>, grid, dim3(256), 0, stream, R);
     return hipGetLastError();
 }
 
