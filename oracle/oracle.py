@@ -74,13 +74,14 @@ class HostMesh:
     """Host-side arrays of one mesh plus a CMesh pointing at them."""
 
     def __init__(self, crx, positions, indices, colors=None, uvs=None, texture=None, texture_size=None,
-                 jpeg=False):
+                 jpeg=False, normals=None):
         """texture: the RGBA8 mip chain as one flat uint32 array (level 0 of
         texture_size = (width, height) first); jpeg marks it JPEG-coded."""
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.indices = None if indices is None else np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1, 3)
         self.colors = None if colors is None else np.ascontiguousarray(colors, dtype=np.uint32).reshape(-1)
         self.uvs = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
         self.texture = None if texture is None else np.ascontiguousarray(texture, dtype=np.uint32).reshape(-1)
         self.texture_size = (0, 0) if texture_size is None else (int(texture_size[0]), int(texture_size[1]))
         self.jpeg = bool(jpeg)
@@ -110,6 +111,7 @@ class HostMesh:
         m.isLoaded = 1
         self.set_texture(m, self.uvs.ctypes.data if self.uvs is not None else None,
                          self.texture.ctypes.data if self.texture is not None else None)
+        m.normals = self.normals.ctypes.data if self.normals is not None else None
         return m
 
 

@@ -87,7 +87,7 @@ def mip_shapes(w, h, n=8):
 
 
 def prepare(spec: SceneSpec, view: crx.Mat4, pointers, instanced_ids=True) -> Frame:
-    """pointers[k] = (positions, indices, colors[, uvs, texture]) addresses of mesh k."""
+    """pointers[k] = (positions, indices, colors[, uvs, texture[, normals]]) addresses of mesh k."""
     unique, instances, transforms, prefix = [], [], [], []
     total = 0
     for k, (mesh, worlds) in enumerate(zip(spec.meshes, spec.worlds)):
@@ -97,6 +97,7 @@ def prepare(spec: SceneSpec, view: crx.Mat4, pointers, instanced_ids=True) -> Fr
             p, i, c = pointers[k][:3]
             uv, tex = pointers[k][3:5] if len(pointers[k]) > 3 else (None, None)
             mesh.set_texture(m, uv, tex)
+            m.normals = pointers[k][5] if len(pointers[k]) > 5 else None
             m.numTriangles = mesh.num_triangles
             m.positions = p
             m.indices = i
@@ -128,6 +129,10 @@ def _texture(mesh):
     return getattr(mesh, "texture", None)
 
 
+def _normals(mesh):
+    return getattr(mesh, "normals", None)
+
+
 def host_pointers(spec: SceneSpec):
     out = []
     for m in spec.meshes:
@@ -135,7 +140,8 @@ def host_pointers(spec: SceneSpec):
                     m.indices.ctypes.data if m.indices is not None else None,
                     m.colors.ctypes.data if m.colors is not None else None,
                     _uvs(m).ctypes.data if _uvs(m) is not None else None,
-                    _texture(m).ctypes.data if _texture(m) is not None else None))
+                    _texture(m).ctypes.data if _texture(m) is not None else None,
+                    _normals(m).ctypes.data if _normals(m) is not None else None))
     return out
 
 
@@ -163,7 +169,8 @@ class GpuScene:
             c = ctx.to_device(m.colors).ptr if m.colors is not None else None
             uv = ctx.to_device(_uvs(m)).ptr if _uvs(m) is not None else None
             tex = ctx.to_device(_texture(m)).ptr if _texture(m) is not None else None
-            self.pointers.append((p, i, c, uv, tex))
+            nr = ctx.to_device(_normals(m)).ptr if _normals(m) is not None else None
+            self.pointers.append((p, i, c, uv, tex, nr))
 
     def render(self, W, H, view, proj, instanced=True, attribute=None, fb=None, cb=None, stream=None):
         ctx = self.ctx

@@ -156,6 +156,56 @@ def test_uv_attribute(ctx):
     _check(ctx, spec, 640, 480, view, proj, attribute=crx.DISPLAY_UV)
 
 
+def test_normal_attribute_with_vertex_normals(ctx):
+    """DISPLAY_NORMAL with per-vertex normals (resolve.cu:927-943), turned by
+    each instance's world (mirrored and scaled ones among them), against the
+    oracle; and the same through the scene API with crx_mesh_set_attributes."""
+    pos, idx, col = wl.sphere_mesh(rings=48, segments=96, seed=90)
+    nrm = (pos / np.linalg.norm(pos, axis=1, keepdims=True)).astype(np.float32)
+    worlds = list(wl.instance_transforms(12, extent=6.0, seed=91))
+    for k in range(0, 12, 4):
+        for e in range(3):
+            worlds[k].m[e] = -worlds[k].m[e]
+    spec = SceneSpec().add(orc.HostMesh(crx, pos, idx, col, normals=nrm), worlds)
+    W, H = 800, 600
+    view, proj = wl.instances_view(extent=6.0, width=W, height=H, distance=12.0)
+    fb, cb = _check(ctx, spec, W, H, view, proj, attribute=crx.DISPLAY_NORMAL)
+    # the scene API's mesh carries the same normals
+    import ctypes as C
+    L = orc.lib(crx)
+    mesh = crx.Mesh(ctx, pos, idx, col)
+    mesh.set_attributes(normals=nrm)
+    host = spec.meshes[0]
+    scene = crx.Scene(ctx)
+    for w in worlds:
+        scene.add(mesh, w)
+    v = crx.make_view(view, proj, W, H)
+    settings = crx.render_settings()
+    settings.attribute = crx.DISPLAY_NORMAL
+    sfb, scb = ctx.alloc(8 * W * H), ctx.alloc(8 * W * H)
+    print("[case] scene NORMAL", flush=True)
+    scene.render(v, sfb, scb, settings)
+    got_fb, got_cb = sfb.download(np.uint64, W * H), scb.download(np.uint64, W * H)
+    n = len(worlds)
+    node_meshes, node_world = (crx.CMesh * n)(), (crx.Mat4 * n)()
+    for k in range(n):
+        cm = host.cmesh(crx)
+        cm.aabb = mesh.aabb()
+        cm.address = 1
+        node_meshes[k] = cm
+        node_world[k] = worlds[k]
+    uniq, nu, inst, xf, pre = (crx.CMesh * n)(), C.c_uint32(), (crx.CMesh * n)(), (crx.Mat4 * n)(), (C.c_uint64 * n)()
+    nv = L.crx_oracle_prepare_frame(node_meshes, node_world, n, C.byref(v), int(settings.frustum_culling), 0,
+                                    uniq, C.byref(nu), inst, xf, pre)
+    assert nv > 0
+    fbw, _ = orc.draw(crx, list(uniq)[:nu.value], list(inst)[:nv], list(xf)[:nv], W, H, view, proj, True)
+    cbw = orc.resolve(crx, list(inst)[:nv], list(pre)[:nv], fbw, W, H, view, proj, crx.DISPLAY_NORMAL)
+    assert np.array_equal(got_fb, fbw), describe_mismatch(got_fb, fbw, W)
+    assert np.array_equal(got_cb, cbw), describe_mismatch(got_cb, cbw, W)
+    scene.close()
+    mesh.close()
+
+
 def test_texture_through_the_scene_api(ctx):
     """crx_mesh_set_attributes and crx_scene_set_texture: per-node textures
     over shared meshes, against the oracle's frame of the same nodes; clearing

@@ -545,3 +545,48 @@ def test_mip_chain_closed_forms():
     chain, _ = _oracle_chain(ramp)
     lvl1 = chain[1024:1024 + 256].reshape(2, 128)
     assert np.array_equal(lvl1 & 0xFF, np.tile(2 * np.arange(128, dtype=np.uint32), (2, 1)))
+
+
+def test_resolve_vertex_normals():
+    """NORMAL with per-vertex normals (resolve.cu:927-943): the normals are
+    turned by world (w = 0), blended with the pixel's barycentrics,
+    normalised and written as clamp(255 N, 0, 255) per channel.  Against the
+    same steps in float64 at exact dyadic barycentrics (+-1 LSB); negative
+    components clamp to 0."""
+    W = H = 64
+    pos, idx = screen_triangle_scene([(10.0, 10.0), (42.0, 10.0), (10.0, 42.0)], W, H, depth=2.0)
+    nrm = np.array([[0.6, 0.8, 0.0], [0.0, 0.6, 0.8], [-0.8, 0.0, 0.6]], dtype=np.float32)
+    mesh = orc.HostMesh(crx, pos, idx, None, normals=nrm)
+    view, proj = crx.identity(), identity_proj(1.0)
+    inst = [mesh.cmesh(crx)]
+    fb, _ = orc.draw(crx, inst, inst, [crx.identity()], W, H, view, proj, True, threads=1)
+    cb = orc.resolve(crx, inst, [0], fb, W, H, view, proj, crx.DISPLAY_NORMAL, threads=1)
+    rgba = (cb & 0xFFFFFFFF).astype(np.uint32).view(np.uint8).reshape(H, W, 4)
+    n64 = nrm.astype(np.float64)
+    for (x, y) in [(10, 10), (26, 10), (10, 26), (18, 18), (20, 14), (12, 30)]:
+        s, t = (x - 10) / 32, (y - 10) / 32
+        N = n64[0] * (1 - s - t) + n64[1] * s + n64[2] * t
+        N = N / np.linalg.norm(N)
+        want = np.clip(N * 255.0, 0.0, 255.0)
+        got = rgba[y, x, :3].astype(float)
+        assert np.all(np.abs(got - want) <= 1.0 + 1e-9), ((x, y), got, want)
+        assert rgba[y, x, 3] == 255
+    # a scaled, rotated world turns the normals before they are blended; the
+    # positions are stored turned back (exactly: a quarter turn and a power
+    # of two), so the pixels keep their barycentrics
+    w = crx.identity()
+    w.m[0], w.m[1], w.m[4], w.m[5] = 0.0, 2.0, -2.0, 0.0     # 90 degrees about z, scale 2
+    pos2 = np.stack([pos[:, 1] * 0.5, -pos[:, 0] * 0.5, pos[:, 2]], axis=1).astype(np.float32)
+    mesh2 = orc.HostMesh(crx, pos2, idx, None, normals=nrm)
+    inst2 = [mesh2.cmesh(crx, world=w)]
+    fb2, _ = orc.draw(crx, inst2, inst2, [w], W, H, view, proj, True, threads=1)
+    assert np.array_equal(fb2, fb)
+    cb2 = orc.resolve(crx, inst2, [0], fb2, W, H, view, proj, crx.DISPLAY_NORMAL, threads=1)
+    rgba2 = (cb2 & 0xFFFFFFFF).astype(np.uint32).view(np.uint8).reshape(H, W, 4)
+    R = np.array([[0.0, -2.0, 0.0], [2.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    for (x, y) in [(10, 10), (20, 14)]:
+        s, t = (x - 10) / 32, (y - 10) / 32
+        N = (R @ n64[0]) * (1 - s - t) + (R @ n64[1]) * s + (R @ n64[2]) * t
+        N = N / np.linalg.norm(N)
+        want = np.clip(N * 255.0, 0.0, 255.0)
+        assert np.all(np.abs(rgba2[y, x, :3].astype(float) - want) <= 1.0 + 1e-9), ((x, y), rgba2[y, x], want)
