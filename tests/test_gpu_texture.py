@@ -267,6 +267,16 @@ def test_texture_through_the_scene_api(ctx):
         cbw = orc.resolve(crx, list(inst)[:nv], list(pre)[:nv], fbw, W, H, view, proj, crx.DISPLAY_TEXTURE)
         assert np.array_equal(got_fb, fbw), describe_mismatch(got_fb, fbw, W)
         assert np.array_equal(got_cb, cbw), describe_mismatch(got_cb, cbw, W)
+    # the same frame recorded into a hipGraph (crx_frame_*) takes the TEXTURE resolve too
+    print("[case] scene TEXTURE frame graph", flush=True)
+    gfb, gcb = ctx.alloc(8 * W * H), ctx.alloc(8 * W * H)
+    frame = crx.Frame(scene, [v], [gfb], [gcb], settings)
+    for _ in range(3):
+        frame.launch()
+        frame.wait()
+        assert np.array_equal(gfb.download(np.uint64, W * H), got_fb)
+        assert np.array_equal(gcb.download(np.uint64, W * H), got_cb)
+    frame.close()
     scene.close()
     mesh.close()
 
