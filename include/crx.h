@@ -124,14 +124,14 @@ typedef struct crx_cmesh {
     struct { int32_t offset; int32_t count; } instances;  /* read */
     crx_mat4 world;                     /* read by resolve */
     int32_t id;                         /* read by resolve (MESH_ID) */
-    crx_vec2* uvs;
-    crx_vec3* normals;
+    crx_vec2* uvs;                      /* read by resolve (TEXTURE, UV) */
+    crx_vec3* normals;                  /* read by resolve (NORMAL; NULL: the face normal) */
     uint32_t* colors;                   /* read by resolve (VERTEX_COLORS), RGBA8 */
     uint32_t firstTriangle;
     uint32_t numVertices;
     uint32_t index_max;
     uint64_t address;
-    crx_texture texture;
+    crx_texture texture;                /* read by resolve (TEXTURE): width, height, data, huffmanTables */
     uint8_t isLoaded;                   /* read by resolve */
     uint8_t flipTriangles;
     uint8_t compressed;                 /* read: 1 = IX_PU16 (crx_compress_mesh) */
