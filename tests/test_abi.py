@@ -153,3 +153,24 @@ def test_cpu_suite_has_no_gpu_side_effects():
     pos, idx, col = wl.sphere_mesh(rings=8, segments=16)
     assert pos.shape == (9 * 16, 3) and idx.shape == (2 * 8 * 16, 3)
     assert idx.max() < len(pos)
+
+
+def test_texture_chain_texels_host_logic():
+    """crx_texture_chain_texels (host only): the texels of levels 0 ..
+    max(floor(log2(max(w, h))) - 1, 7), each ((w + 1) / 2) x ((h + 1) / 2) of
+    the level before; non-positive sizes and 2^31 texels or more are refused."""
+    def want(w, h):
+        L = max(w, h).bit_length() - 1
+        total = 0
+        for _ in range(max(L, 8)):
+            total += w * h
+            w, h = (w + 1) // 2, (h + 1) // 2
+        return total
+    for w, h in [(1, 1), (1, 7), (5, 3), (64, 64), (255, 2), (256, 9), (1024, 3), (4096, 3), (16384, 16384),
+                 (46340, 46340), (1, 1 << 30)]:
+        assert crx.texture_chain_texels(w, h) == want(w, h), (w, h)
+    assert crx.texture_chain_texels(256, 256) == sum((256 >> k) ** 2 for k in range(8))
+    for bad in ((0, 4), (4, 0), (-3, 4), (65536, 65536), (2, 1 << 30)):
+        with pytest.raises(crx.CrxError) as e:
+            crx.texture_chain_texels(*bad)
+        assert e.value.status == 1
