@@ -61,7 +61,7 @@ def build(verbose: bool = False) -> None:
     obj_dir = LIB / "obj"
     obj_dir.mkdir(exist_ok=True)
     objs = []
-    for src in ("crx_kernels.hip", "crx_api.cpp", "crx_args.cpp", "crx_server.cpp", "crx_frame.cpp"):
+    for src in ("crx_kernels.hip", "crx_api.cpp", "crx_args.cpp", "crx_server.cpp", "crx_frame.cpp", "crx_cull.cpp"):
         obj = obj_dir / (Path(src).stem + ".o")
         cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + ["-pthread", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:

@@ -322,6 +322,8 @@ _SIGS = {
     "crx_scene_add_node": (C.c_int, [_VP, _VP, C.POINTER(Mat4), C.POINTER(C.c_uint32)]),
     "crx_scene_set_transform": (C.c_int, [_VP, C.c_uint32, C.POINTER(Mat4)]),
     "crx_scene_set_texture": (C.c_int, [_VP, C.c_uint32, C.POINTER(Texture)]),
+    "crx_frustum_cull": (C.c_int, [C.POINTER(View), C.POINTER(Box3), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
+                                   C.POINTER(C.c_uint32)]),
     "crx_mesh_set_attributes": (C.c_int, [_VP, _VP, _VP]),
     "crx_scene_num_nodes": (C.c_int, [_VP, C.POINTER(C.c_uint32)]),
     "crx_render_settings_default": (None, [C.POINTER(RenderSettings)]),
@@ -804,6 +806,17 @@ class Scene:
         if self.handle:
             lib().crx_scene_destroy(C.c_void_p(self.handle))
             self.handle = None
+
+
+def frustum_cull(view: View, boxes) -> list[int]:
+    """Indices of the world boxes the view's node cull keeps, ascending
+    (SNTriangles::update's test through the range hierarchy)."""
+    arr = _as_array(Box3, boxes)
+    n = len(arr) if arr is not None else 0
+    ids = (C.c_uint32 * max(n, 1))()
+    cnt = C.c_uint32()
+    _check(lib().crx_frustum_cull(C.byref(view), arr, n, ids, n, C.byref(cnt)))
+    return list(ids[:cnt.value])
 
 
 def texture_chain_texels(width: int, height: int) -> int:

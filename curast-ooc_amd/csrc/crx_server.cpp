@@ -131,6 +131,9 @@ struct crx_server {
     std::vector<Node> nodes;
     uint64_t totalTriangles = 0;
     bool started = false;
+    // the nodes' world boxes and the range hierarchy over them (built at start)
+    std::vector<crx_box3> wboxes;
+    NodeTree tree;
 
     // device node cache
     char* cache = nullptr;
@@ -611,9 +614,7 @@ void draw_batch(crx_server* s, crx_server::Lane& lane, const std::vector<Request
         float planes[16];
         frustum_planes(r->view, planes);
         std::vector<uint32_t>& mine = r->nodes;
-        mine.clear();
-        for (uint32_t i = 0; i < N; i++)
-            if (box_in_frustum(planes, s->nodes[i].wbox)) mine.push_back(i);
+        s->tree.visible(planes, s->wboxes.data(), mine);
         if (mine.size() > s->numSlots) {
             flush();
             group.push_back(r);
@@ -789,6 +790,9 @@ crx_status crx_server_start(crx_server* s) {
     if (s->started) return fail(CRX_INVALID_ARG, "server already started");
     if (s->totalTriangles >= (1ull << 36)) return fail(CRX_INVALID_ARG, "more than 2^36 triangles in the scene");
     CRX_HIP(hipSetDevice(s->ctx->device));
+    s->wboxes.resize(s->nodes.size());
+    for (size_t i = 0; i < s->nodes.size(); i++) s->wboxes[i] = s->nodes[i].wbox;
+    s->tree.build(s->wboxes.data(), (uint32_t)s->wboxes.size());
     size_t slot = kAlign;
     for (const Node& n : s->nodes) slot = std::max(slot, node_bytes(n));
     s->slotBytes = slot;
@@ -969,10 +973,19 @@ crx_status crx_server_view_nodes(const crx_server* s, const crx_view* view, uint
     float planes[16];
     frustum_planes(*view, planes);
     uint32_t c = 0;
-    for (uint32_t i = 0; i < s->nodes.size(); i++) {
-        if (!box_in_frustum(planes, s->nodes[i].wbox)) continue;
-        if (ids && c < capacity) ids[c] = i;
-        c++;
+    if (s->started) {
+        std::vector<uint32_t> vis;
+        s->tree.visible(planes, s->wboxes.data(), vis);
+        for (uint32_t i : vis) {
+            if (ids && c < capacity) ids[c] = i;
+            c++;
+        }
+    } else {
+        for (uint32_t i = 0; i < s->nodes.size(); i++) {
+            if (!box_in_frustum(planes, s->nodes[i].wbox)) continue;
+            if (ids && c < capacity) ids[c] = i;
+            c++;
+        }
     }
     *count = c;
     return CRX_OK;

@@ -563,6 +563,15 @@ typedef struct crx_view {
     int32_t height;
 } crx_view;
 
+/* The node cull of SNTriangles::update (SNTriangles.h:29-105) over n
+ * world-space boxes: the four normalised side planes of proj * view and the
+ * positive-vertex test.  Writes the kept box indices in ascending order to ids
+ * (at most `capacity`) and their number to *count.  Host only: the boxes go
+ * through the same range hierarchy the server culls its nodes with, whose
+ * answer equals each box's own test. */
+crx_status crx_frustum_cull(const crx_view* view, const crx_box3* boxes, uint32_t n, uint32_t* ids,
+                            uint32_t capacity, uint32_t* count);
+
 /* Scene-level switches (CuRastSettings.h:31,49,55,60). */
 typedef struct crx_render_settings {
     int32_t instanced;             /* RASTERIZER_VISBUFFER_INSTANCED vs RASTERIZER_VISBUFFER */

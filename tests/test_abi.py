@@ -174,3 +174,86 @@ def test_texture_chain_texels_host_logic():
         with pytest.raises(crx.CrxError) as e:
             crx.texture_chain_texels(*bad)
         assert e.value.status == 1
+
+
+def _views_for_cull():
+    import math
+    import workloads as wl
+    out = []
+    rng = np.random.default_rng(7)
+    for k in range(24):
+        eye = tuple(float(x) for x in rng.uniform(-300.0, 300.0, 3))
+        tgt = tuple(float(x) for x in rng.uniform(-100.0, 100.0, 3))
+        W, H = [(1920, 1080), (256, 256), (640, 200), (3840, 2160)][k % 4]
+        fovy = math.radians([60.0, 20.0, 90.0, 5.0][k % 4])
+        out.append(crx.make_view(crx.look_at(eye, tgt), wl.projection(W, H, fovy=fovy), W, H))
+    return out
+
+
+def _cull_boxes(seed):
+    """A grid of tiles (neighbouring ids are neighbours in space), scattered
+    boxes, flat, inverted and huge ones."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for r in range(30):
+        for c in range(30):
+            lo = np.array([c * 20.0 - 300.0, rng.uniform(-20, 0), r * 20.0 - 300.0], np.float32)
+            boxes.append((lo, lo + np.array([20.0, rng.uniform(0, 40), 20.0], np.float32)))
+    for _ in range(400):
+        lo = rng.uniform(-400, 400, 3).astype(np.float32)
+        boxes.append((lo, lo + rng.uniform(0, 60, 3).astype(np.float32)))
+    for _ in range(40):
+        lo = rng.uniform(-200, 200, 3).astype(np.float32)
+        boxes.append((lo, lo))                                    # a point
+        boxes.append((lo + 5.0, lo))                              # inverted
+        boxes.append((lo - 5000.0, lo + 5000.0))                  # huge
+    out = []
+    for lo, hi in boxes:
+        b = crx.Box3()
+        b.min.x, b.min.y, b.min.z = (float(v) for v in lo)
+        b.max.x, b.max.y, b.max.z = (float(v) for v in hi)
+        out.append(b)
+    return out
+
+
+def test_frustum_cull_matches_the_oracle_node_cull():
+    """crx_frustum_cull (the range hierarchy the server culls nodes with)
+    keeps exactly the boxes the oracle's SNTriangles::update test keeps (the
+    oracle rebuilds each box from its transformed corners, so the boxes here
+    are proper ones: min <= max)."""
+    L = orc.lib(crx)
+    boxes = [b for b in _cull_boxes(1) if b.min.x <= b.max.x and b.min.y <= b.max.y and b.min.z <= b.max.z]
+    n = len(boxes)
+    node_meshes = (crx.CMesh * n)()
+    node_world = (crx.Mat4 * n)()
+    for i, b in enumerate(boxes):
+        node_meshes[i].numTriangles = 1
+        node_meshes[i].aabb = b
+        node_meshes[i].isLoaded = 1
+        node_meshes[i].address = i + 1
+        node_world[i] = crx.identity()
+    for v in _views_for_cull():
+        got = crx.frustum_cull(v, boxes)
+        uniq, nu, inst = (crx.CMesh * n)(), C.c_uint32(), (crx.CMesh * n)()
+        xf, pre = (crx.Mat4 * n)(), (C.c_uint64 * n)()
+        nv = L.crx_oracle_prepare_frame(node_meshes, node_world, n, C.byref(v), 1, 1, uniq, C.byref(nu), inst, xf, pre)
+        want = sorted(int(inst[k].address) - 1 for k in range(nv))
+        assert got == want
+        assert got == sorted(got)
+
+
+def test_frustum_cull_hierarchy_equals_each_box_test():
+    """The hierarchy's shortcuts (dropping or keeping a whole range) give every
+    box the answer of its own test, also for non-finite and inverted boxes:
+    the cull of the set equals the boxes kept when each is culled alone."""
+    boxes = _cull_boxes(2)
+    nan, inf = float("nan"), float("inf")
+    for i, (lo, hi) in enumerate([((nan, 0, 0), (1, 1, 1)), ((-inf, -1, -1), (1, 1, 1)), ((0, 0, 0), (inf, inf, inf)),
+                                  ((-inf, -inf, -inf), (inf, inf, inf))]):
+        b = crx.Box3()
+        b.min.x, b.min.y, b.min.z = lo
+        b.max.x, b.max.y, b.max.z = hi
+        boxes.insert(100 + 37 * i, b)
+    for v in _views_for_cull():
+        alone = [i for i, b in enumerate(boxes) if crx.frustum_cull(v, [b]) == [0]]
+        assert crx.frustum_cull(v, boxes) == alone
