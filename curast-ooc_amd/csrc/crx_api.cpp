@@ -919,23 +919,30 @@ uint32_t scene_lists(crx_scene* scene, const crx_view* views, uint32_t numViews,
     std::vector<float> planes((size_t)numViews * 16);
     for (uint32_t v = 0; v < numViews; v++) frustum_planes(views[v], &planes[(size_t)v * 16]);
     const uint32_t N = (uint32_t)scene->nodes.size();
-    std::vector<uint8_t> vis((size_t)N * numViews, 1);
-    std::vector<uint32_t> kept;
-    kept.reserve(N);
-    for (uint32_t i = 0; i < N; i++) {
-        const crx_scene::Node& nd = scene->nodes[i];
-        bool any = true;
-        if (settings.frustum_culling) {
-            const crx_box3& wb = nd.wbox;
-            any = false;
-            for (uint32_t v = 0; v < numViews; v++) {
-                const bool in = box_in_frustum(&planes[(size_t)v * 16], wb);
-                vis[(size_t)i * numViews + v] = in ? 1 : 0;
-                any = any || in;
+    std::vector<uint8_t> vis((size_t)N * numViews, settings.frustum_culling ? 0 : 1);
+    std::vector<uint8_t> any(N, settings.frustum_culling ? 0 : 1);
+    if (settings.frustum_culling) {
+        // each view's cull through the range hierarchy (each node gets its
+        // own box test's answer), rebuilt when the nodes changed
+        if (scene->treeVersion != scene->version) {
+            scene->wboxes.resize(N);
+            for (uint32_t i = 0; i < N; i++) scene->wboxes[i] = scene->nodes[i].wbox;
+            scene->tree.build(scene->wboxes.data(), N);
+            scene->treeVersion = scene->version;
+        }
+        std::vector<uint32_t> in;
+        for (uint32_t v = 0; v < numViews; v++) {
+            scene->tree.visible(&planes[(size_t)v * 16], scene->wboxes.data(), in);
+            for (uint32_t i : in) {
+                vis[(size_t)i * numViews + v] = 1;
+                any[i] = 1;
             }
         }
-        if (any || keepAll) kept.push_back(i);
     }
+    std::vector<uint32_t> kept;
+    kept.reserve(N);
+    for (uint32_t i = 0; i < N; i++)
+        if (any[i] || keepAll) kept.push_back(i);
     // CuRast_render.h:338-345: numTriangles descending, then mesh identity;
     // instances of one mesh keep node order.
     const auto before = [&](uint32_t a, uint32_t b) {

@@ -148,6 +148,26 @@ struct DrawExtras {
     bool countersZeroed = false;
 };
 
+// A hierarchy over ranges of consecutive nodes (a binary tree whose leaves
+// hold up to 4 nodes): visible() lists, in index order, exactly the nodes
+// box_in_frustum keeps, without testing each one.  A range is dropped when
+// its box fails a plane and kept whole when its box passes every plane with
+// its nearest corner; both tests are monotone in the box (a node's box lies
+// inside its range's box, and each product and sum of the plane test rounds
+// monotonically), so every node gets the answer of its own test.  Ranges that
+// hold a non-finite coordinate are always split down to their nodes.
+struct NodeTree {
+    struct Range {
+        crx_box3 box;
+        uint32_t lo, hi;        // nodes [lo, hi)
+        uint32_t left, right;   // child ranges (0: a leaf)
+        bool finite;
+    };
+    std::vector<Range> ranges;  // ranges[0]: all nodes
+    void build(const crx_box3* boxes, uint32_t n);
+    void visible(const float planes[16], const crx_box3* boxes, std::vector<uint32_t>& out) const;
+};
+
 }  // namespace crx
 
 struct crx_context {
@@ -232,6 +252,10 @@ struct crx_scene {
     std::vector<uint32_t> listsKept;
     uint64_t listsKey = 0;            // content key of the lists (new on every rebuild)
     uint64_t listsAttrEpoch = 0;      // the context's attrEpoch the lists were built at
+    // the node cull's range hierarchy over the nodes' world boxes, as of treeVersion
+    std::vector<crx_box3> wboxes;
+    crx::NodeTree tree;
+    uint64_t treeVersion = ~0ull;
 };
 
 
@@ -273,26 +297,6 @@ crx_status check_target(const crx_render_target* t);
 void frustum_planes(const crx_view& view, float out[16]);
 crx_box3 world_box(const crx_box3& b, const crx_mat4& world);
 bool box_in_frustum(const float planes[16], const crx_box3& worldBox);
-
-// A hierarchy over ranges of consecutive nodes (a binary tree whose leaves
-// hold up to 4 nodes): visible() lists, in index order, exactly the nodes
-// box_in_frustum keeps, without testing each one.  A range is dropped when
-// its box fails a plane and kept whole when its box passes every plane with
-// its nearest corner; both tests are monotone in the box (a node's box lies
-// inside its range's box, and each product and sum of the plane test rounds
-// monotonically), so every node gets the answer of its own test.  Ranges that
-// hold a non-finite coordinate are always split down to their nodes.
-struct NodeTree {
-    struct Range {
-        crx_box3 box;
-        uint32_t lo, hi;        // nodes [lo, hi)
-        uint32_t left, right;   // child ranges (0: a leaf)
-        bool finite;
-    };
-    std::vector<Range> ranges;  // ranges[0]: all nodes
-    void build(const crx_box3* boxes, uint32_t n);
-    void visible(const float planes[16], const crx_box3* boxes, std::vector<uint32_t>& out) const;
-};
 
 
 crx_status plan_draw(crx_context* ctx, const crx_cmesh* meshes, uint32_t numMeshes, const crx_cmesh* instances,
