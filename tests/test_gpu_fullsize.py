@@ -3,7 +3,8 @@ CPU oracle bit for bit (tolerance: none).
 
 * The 400M-triangle heightfield (14142 x 14142 grid, 399,935,762 triangles)
   at 1920x1080, visibility and colour buffers.  Its index buffer is 4.8 GB,
-  so indices and positions are addressed past 2^32 bytes.
+  so indices and positions are addressed past 2^32 bytes; the far-edge view
+  is resolved with TEXTURE too.
 * 3000 instances of the 1M-triangle displaced sphere at 1920x1080: triangle
   ids reach 3.1e9, formed with the reference's 32-bit
   instanceIndex * numTriangles product (triangles_visbuffer.cu:252-254).
@@ -100,6 +101,39 @@ def test_heightfield_400m_1080p(ctx, heightfield, which):
         past = int(np.count_nonzero(ids > (1 << 32) // 12))
         say(f"{past} of {ids.size} covered pixels show triangles whose indices lie past 2^32 bytes")
         assert past > 0.99 * ids.size
+        _texture_resolve(ctx, scene, pos, idx, col, got_fb, view, proj)
+
+
+def _texture_resolve(ctx, scene, pos, idx, col, fb_host, view, proj):
+    """The TEXTURE resolve of the same visibility buffer: uvs over the whole
+    grid (repeated 64 times) and a random 4096 x 4096 mip chain, against the
+    oracle's resolve of that buffer."""
+    from crxtest import mip_chain, mip_shapes
+    say("TEXTURE resolve of the far-edge frame ...")
+    uvs = np.empty((len(pos), 2), dtype=np.float32)
+    np.multiply(pos[:, 0], np.float32(64.0 / (N - 1)), out=uvs[:, 0])
+    np.multiply(pos[:, 2], np.float32(64.0 / (N - 1)), out=uvs[:, 1])
+    rng = np.random.default_rng(5)
+    chain = mip_chain([rng.integers(0, 2 ** 32, size=s, dtype=np.uint32) for s in mip_shapes(4096, 4096)])
+    d_uvs, d_tex, d_fb = ctx.to_device(uvs), ctx.to_device(chain), ctx.to_device(fb_host)
+    d_cb = ctx.alloc(8 * W * H)
+    ctx.clear_framebuffer(d_cb, W * H)
+    host = orc.HostMesh(crx, pos, idx, col, uvs=uvs, texture=chain, texture_size=(4096, 4096))
+    m = host.cmesh(crx)
+    m_dev = crx.CMesh.from_buffer_copy(m)
+    p, i, c = scene.nodes[0][0].buffers()
+    m_dev.positions, m_dev.indices, m_dev.colors = p, i, c
+    m_dev.uvs, m_dev.texture.data = d_uvs.ptr, d_tex.ptr
+    target = crx.make_target(d_fb, d_cb, W, H, view, proj)
+    ctx.resolve_visbuffer([m_dev], [0], target, crx.DISPLAY_TEXTURE)
+    got = d_cb.download(np.uint64, W * H)
+    want = orc.resolve(crx, [m], [0], fb_host, W, H, view, proj, crx.DISPLAY_TEXTURE, threads=THREADS)
+    assert np.array_equal(got, want), "TEXTURE: " + describe_mismatch(got, want, W)
+    cov = fb_host != crx.CLEAR_PIXEL
+    say(f"TEXTURE bit-exact: {np.unique(got[cov] & 0xFFFFFFFF).size} distinct colours")
+    assert np.unique(got[cov] & 0xFFFFFFFF).size > 100_000
+    for b in (d_uvs, d_tex, d_fb, d_cb):
+        b.free()
 
 
 def test_instances_3000x1m_1080p(ctx):
