@@ -663,6 +663,9 @@ ResolveParams make_resolve_params(const crx_render_target& t, const InstanceRec*
     R.origin[0] = o.x;
     R.origin[1] = o.y;
     R.origin[2] = o.z;
+    std::memcpy(R.view, t.view.m, sizeof(R.view));
+    R.f = t.proj.m[5];
+    R.aspect = (float)t.width / (float)t.height;
     return R;
 }
 
@@ -689,7 +692,6 @@ crx_status check_resolve(const crx_render_target* targets, uint32_t numViews, cr
         CRX_TRY(check_target(targets + v));
         if (!targets[v].colorbuffer) return fail(CRX_INVALID_ARG, "target colorbuffer is NULL");
     }
-    if (attribute == CRX_DISPLAY_STAGE) return fail(CRX_UNSUPPORTED, "display attribute STAGE is not supported");
     if ((int)attribute < 0 || (int)attribute > 7) return fail(CRX_INVALID_ARG, "unknown display attribute");
     return CRX_OK;
 }
@@ -766,7 +768,7 @@ crx_status resolve_views(crx_context* ctx, const crx_cmesh* instances, uint32_t 
         CRX_HIP(launch_resolve(rv[0], stream));
     } else {
         CRX_HIP(launch_resolve_views(reinterpret_cast<const ResolveParams*>(meta + offViews), numViews, blocks,
-                                     attribute == CRX_DISPLAY_TEXTURE, stream));
+                                     full_resolve(attribute), stream));
     }
     record(ctx, T_RESOLVE_END, stream);
     return CRX_OK;
@@ -800,7 +802,7 @@ crx_status resolve_views_device(crx_context* ctx, const InstanceRec* instDev, co
     CRX_TRY(ctx->resolveStaging.mark(stream));
     record(ctx, T_RESOLVE, stream);
     CRX_HIP(launch_resolve_views(static_cast<const ResolveParams*>(ctx->resolveMeta.ptr), numViews, blocks,
-                                 attribute == CRX_DISPLAY_TEXTURE, stream));
+                                 full_resolve(attribute), stream));
     record(ctx, T_RESOLVE_END, stream);
     return CRX_OK;
 }

@@ -164,6 +164,8 @@ struct ResolveParams {
     float p00I, p11I;          // 1/proj[0][0], 1/proj[1][1]
     float viewI[16];           // inverse(view), GLM order, computed once per frame
     float origin[3];           // viewI * (0,0,0,1)
+    float view[16];            // the view matrix (STAGE: view-space vertices)
+    float f, aspect;           // proj[1][1], float(W) / float(H) (STAGE)
     uint32_t firstBlock;       // multi-view resolve: first 64x4 tile block of this view
     uint32_t ownedOnly;        // 1: leave pixels whose triangle is not in the listed instances
                                // (a resolve in passes over groups of instances)

@@ -128,7 +128,7 @@ crx_status frame_capture(crx_frame* F) {
     if (e == hipSuccess && F->settings.resolve)
         e = launch_resolve_views(reinterpret_cast<const ResolveParams*>(static_cast<char*>(F->resolveMeta.ptr) +
                                                                         F->resolveOffViews),
-                                 F->numViews, F->resolveBlocks, F->settings.attribute == CRX_DISPLAY_TEXTURE, s);
+                                 F->numViews, F->resolveBlocks, full_resolve((crx_display_attribute)F->settings.attribute), s);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(s, &g);
     if (e == hipSuccess) e = e2;

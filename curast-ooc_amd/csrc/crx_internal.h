@@ -35,8 +35,12 @@ hipError_t launch_stage2(const RasterParams& P, uint32_t grid, hipStream_t strea
 hipError_t launch_stage3(const RasterParams& P, uint32_t grid, hipStream_t stream);
 hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream);
 // views: device array of numViews records; numBlocks = total blocks over all views
-hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, bool texture,
+// full: the variant with the TEXTURE and STAGE attributes (full_resolve)
+hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, bool full,
                                 hipStream_t stream);
+// The display attributes the resolve's full variant shades (the others run
+// without their code and its registers).
+inline bool full_resolve(crx_display_attribute a) { return a == CRX_DISPLAY_TEXTURE || a == CRX_DISPLAY_STAGE; }
 hipError_t launch_clear_views(const ClearView* views, uint32_t numViews, uint32_t numBlocks, uint32_t* ctr,
                               hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, int numCUs, hipStream_t stream);

@@ -2232,8 +2232,9 @@ __device__ __forceinline__ int mip_level(float m) {
     return min(e >> 1, 7);
 }
 
-// TEX: the variant launched for the TEXTURE attribute; the other attributes
-// run without its code, whose registers would lower every resolve's occupancy.
+// TEX: the variant launched for the TEXTURE and STAGE attributes; the other
+// attributes run without their code, whose registers would lower every
+// resolve's occupancy.
 template <bool TEX>
 __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const InstanceRec& mesh, uint64_t total,
                                                 int x, int y) {
@@ -2285,6 +2286,30 @@ __device__ __forceinline__ uint32_t shade_pixel(const ResolveParams& R, const In
             color = tri * 12345678u;
         } else if (attr == 6) {                // MESH_ID
             color = ((uint32_t)mesh.id + 1u) * 12345678u;
+        } else if (TEX && attr == 7) {                 // STAGE (resolve.cu:974-1031)
+            // the stage the triangle's size and near-plane test would route it
+            // to: view space, viewToNDC, ndcToScreen (no half-pixel shift
+            // here, rasterization_helpers.cuh:95-108), the box clipped to the
+            // screen, its fragment count (an int product compared unsigned)
+            const f3 av = xform_point(R.view, aw), bv = xform_point(R.view, bw), cv = xform_point(R.view, cw);
+            const float fa = R.f / R.aspect;
+            const float da = -av.z, db = -bv.z, dc = -cv.z;
+            const float sxa = ((fa * av.x) / da * 0.5f + 0.5f) * R.Wf, sya = ((R.f * av.y) / da * 0.5f + 0.5f) * R.Hf;
+            const float sxb = ((fa * bv.x) / db * 0.5f + 0.5f) * R.Wf, syb = ((R.f * bv.y) / db * 0.5f + 0.5f) * R.Hf;
+            const float sxc = ((fa * cv.x) / dc * 0.5f + 0.5f) * R.Wf, syc = ((R.f * cv.y) / dc * 0.5f + 0.5f) * R.Hf;
+            float min_x = fminf(sxa, fminf(sxb, sxc)), max_x = fmaxf(sxa, fmaxf(sxb, sxc));
+            float min_y = fminf(sya, fminf(syb, syc)), max_y = fmaxf(sya, fmaxf(syb, syc));
+            min_x = fmaxf(min_x, 0.0f);
+            max_x = fminf(max_x, (float)(R.W - 1));
+            min_y = fmaxf(min_y, 0.0f);
+            max_y = fminf(max_y, (float)(R.H - 1));
+            const int32_t size_x = f2i(ceilf(max_x) - floorf(min_x));
+            const int32_t size_y = f2i(ceilf(max_y) - floorf(min_y));
+            const uint32_t frags = (uint32_t)size_x * (uint32_t)size_y;
+            uint32_t stage = frags <= 128u ? 0u : frags <= 4096u ? 1u : 2u;    // THRESHOLD_SMALL, THRESHOLD_LARGE
+            if (da <= 0.0f || db <= 0.0f || dc <= 0.0f) stage = 2u;
+            // SCHEME_SPECTRAL[9], [7], [3] (utils.cuh:21-33)
+            color = stage == 0u ? 0xffbd8832u : stage == 1u ? 0xffa4ddabu : 0xff61aefdu;
         } else if (TEX && attr == 1 && mesh.uvs) {    // TEXTURE (resolve.cu:783-876)
             // the neighbouring pixels' rays, hits and texture coordinates give
             // the sample's footprint, and with it the mip level
@@ -2803,10 +2828,10 @@ hipError_t launch_stage1_views(const RasterParams& P, uint32_t block, uint32_t t
     return launch_s1_any(P, block, trisPerLane, grid, pu16, true, false, stream);
 }
 
-hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, bool texture,
+hipError_t launch_resolve_views(const ResolveParams* views, uint32_t numViews, uint32_t numBlocks, bool full,
                                 hipStream_t stream) {
     if (numViews == 0 || numBlocks == 0) return hipSuccess;
-    if (texture) hipLaunchKernelGGL(k_resolve_views<true>, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
+    if (full) hipLaunchKernelGGL(k_resolve_views<true>, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
     else hipLaunchKernelGGL(k_resolve_views<false>, dim3(numBlocks), dim3(256), 0, stream, views, numViews);
     return hipGetLastError();
 }
@@ -2862,7 +2887,7 @@ hipError_t launch_check_division(uint64_t pairs, int numCUs, unsigned long long*
 
 hipError_t launch_resolve(const ResolveParams& R, hipStream_t stream) {
     const dim3 grid((uint32_t)(R.W + 63) / 64u, (uint32_t)(R.H + 3) / 4u);
-    if (R.attribute == (int32_t)CRX_DISPLAY_TEXTURE) hipLaunchKernelGGL(k_resolve<true>, grid, dim3(256), 0, stream, R);
+    if (full_resolve((crx_display_attribute)R.attribute)) hipLaunchKernelGGL(k_resolve<true>, grid, dim3(256), 0, stream, R);
     else hipLaunchKernelGGL(k_resolve<false>, grid, dim3(256), 0, stream, R);
     return hipGetLastError();
 }

@@ -170,7 +170,7 @@ typedef enum crx_display_attribute {
     CRX_DISPLAY_VERTEX_COLORS = 4,
     CRX_DISPLAY_TRIANGLE_ID = 5,
     CRX_DISPLAY_MESH_ID = 6,
-    CRX_DISPLAY_STAGE = 7          /* not supported: crx_resolve_visbuffer returns CRX_UNSUPPORTED */
+    CRX_DISPLAY_STAGE = 7          /* the stage the triangle's size and near-plane test route it to (debug colours) */
 } crx_display_attribute;
 
 /* Runtime form of the reference's compile-time constants.

@@ -901,7 +901,6 @@ int crx_oracle_draw(const crx_cmesh* meshes, uint32_t numMeshes,
 
 int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64_t* triangleCountPrefixSum,
                        const crx_render_target* target, int attribute, int nthreads) {
-    if (attribute == CRX_DISPLAY_STAGE) return -2;
     const bool texture = attribute == CRX_DISPLAY_TEXTURE;
     const int W = target->width, H = target->height;
     const M4 view = target->view;
@@ -1028,6 +1027,49 @@ int crx_oracle_resolve(const crx_cmesh* meshes, uint32_t numMeshes, const uint64
                         color = triangleIndex * 12345678u;
                     } else if (attribute == CRX_DISPLAY_MESH_ID) {
                         color = ((uint32_t)mesh.id + 1u) * 12345678u;
+                    } else if (attribute == CRX_DISPLAY_STAGE) {
+                        // resolve.cu:974-1031: the stage the triangle would be drawn by,
+                        // from its view-space vertices (resolve.cu:777-779)
+                        V3 a_view = xyz(mulmv(view, V4{a_world.x, a_world.y, a_world.z, 1.0f}));
+                        V3 b_view = xyz(mulmv(view, V4{b_world.x, b_world.y, b_world.z, 1.0f}));
+                        V3 c_view = xyz(mulmv(view, V4{c_world.x, c_world.y, c_world.z, 1.0f}));
+                        const float f = proj.m[5];
+                        const float aspect = float(W) / float(H);
+                        // viewToNDC, rasterization_helpers.cuh:95-101
+                        auto viewToNDC = [&](V3 v) {
+                            float depth = -v.z;
+                            return V3{(f / aspect) * v.x / depth, f * v.y / depth, depth};
+                        };
+                        // ndcToScreen, rasterization_helpers.cuh:103-108
+                        auto ndcToScreen = [&](V3 n) {
+                            return V2{(n.x * 0.5f + 0.5f) * float(W), (n.y * 0.5f + 0.5f) * float(H)};
+                        };
+                        V3 a_ndc = viewToNDC(a_view), b_ndc = viewToNDC(b_view), c_ndc = viewToNDC(c_view);
+                        bool isNontrivial = (a_ndc.z <= 0.0f || b_ndc.z <= 0.0f || c_ndc.z <= 0.0f);
+                        V2 a_s = ndcToScreen(a_ndc), b_s = ndcToScreen(b_ndc), c_s = ndcToScreen(c_ndc);
+                        // min / max of two floats are fminf / fmaxf in the reference's device code
+                        float min_x = std::fmin(a_s.x, std::fmin(b_s.x, c_s.x));
+                        float max_x = std::fmax(a_s.x, std::fmax(b_s.x, c_s.x));
+                        float min_y = std::fmin(a_s.y, std::fmin(b_s.y, c_s.y));
+                        float max_y = std::fmax(a_s.y, std::fmax(b_s.y, c_s.y));
+                        min_x = std::fmax(min_x, 0.0f);
+                        max_x = std::fmin(max_x, float(W - 1));
+                        min_y = std::fmax(min_y, 0.0f);
+                        max_y = std::fmin(max_y, float(H - 1));
+                        int size_x = f2i(std::ceil(max_x) - std::floor(min_x));
+                        int size_y = f2i(std::ceil(max_y) - std::floor(min_y));
+                        // int numFragments compared with the uint32_t thresholds: unsigned
+                        uint32_t numFragments = (uint32_t)size_x * (uint32_t)size_y;
+                        int stage = 0;
+                        if (numFragments <= 128u) stage = 0;          // THRESHOLD_SMALL
+                        else if (numFragments <= 4096u) stage = 1;    // THRESHOLD_LARGE
+                        else stage = 2;
+                        if (isNontrivial) stage = 2;
+                        // SCHEME_SPECTRAL (utils.cuh:21-33) entries 9, 7 and 3
+                        static const uint32_t kSpectral[11] = {0xff42019e, 0xff4f3ed5, 0xff436df4, 0xff61aefd,
+                                                               0xff8be0fe, 0xffbfffff, 0xff98f5e6, 0xffa4ddab,
+                                                               0xffa5c266, 0xffbd8832, 0xffa24f5e};
+                        color = stage == 0 ? kSpectral[9] : stage == 1 ? kSpectral[7] : kSpectral[3];
                     } else if (texture && mesh.uvs) {
                         // resolve.cu:836-882
                         const crx_vec2 uv_a = mesh.uvs[i0], uv_b = mesh.uvs[i1], uv_c = mesh.uvs[i2];

@@ -90,15 +90,17 @@ def _closeup_terrain():
     return spec
 
 
-def test_closeup_near_plane_stage2_stage3(ctx):
+@pytest.mark.parametrize("attribute", [crx.DISPLAY_VERTEX_COLORS, crx.DISPLAY_STAGE])
+def test_closeup_near_plane_stage2_stage3(ctx, attribute):
     """Camera 1.5 units above a terrain: triangles span hundreds of pixels and
     cross the camera plane, so stage 2 (wave per triangle) and stage 3 (tiles)
-    do most of the work."""
+    do most of the work; STAGE shades each pixel by the stage its triangle
+    is routed to."""
     spec = _closeup_terrain()
     W, H = 1280, 720
     view = crx.look_at((64.0, 4.5, 20.0), (64.0, 2.0, 40.0), (0.0, 1.0, 0.0))
     proj = wl.projection(W, H)
-    _check(ctx, spec, W, H, view, proj, expect_stage2=True, expect_stage3=True)
+    fb, _ = _check(ctx, spec, W, H, view, proj, attribute=attribute, expect_stage2=True, expect_stage3=True)
 
 
 @pytest.mark.parametrize("small,tile", [(32, 32), (8, 16), (256, 128), (128, 64)])
@@ -197,7 +199,7 @@ def test_heightfield_oblique(ctx):
 
 
 @pytest.mark.parametrize("attribute", [crx.DISPLAY_NONE, crx.DISPLAY_TRIANGLE_ID, crx.DISPLAY_MESH_ID,
-                                       crx.DISPLAY_NORMAL])
+                                       crx.DISPLAY_NORMAL, crx.DISPLAY_STAGE])
 def test_resolve_attributes(ctx, attribute):
     pos, idx, col = wl.sphere_mesh(rings=64, segments=128)
     worlds = list(wl.instance_transforms(5, extent=4.0, seed=3))

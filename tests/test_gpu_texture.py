@@ -330,14 +330,14 @@ def test_texture_with_a_computed_chain(ctx):
     assert np.unique(cb[fb != crx.CLEAR_PIXEL] & 0xFFFFFFFF).size > 1000
 
 
-def test_resolve_rejects_stage_and_bad_textures(ctx):
+def test_resolve_rejects_bad_attributes_and_textures(ctx):
     fb = ctx.alloc(8 * 16)
     cb = ctx.alloc(8 * 16)
     view, proj = wl.sphere_view(4, 4)
     target = crx.make_target(fb, cb, 4, 4, view, proj)
     with pytest.raises(crx.CrxError) as e:
-        ctx.resolve_visbuffer([], [], target, crx.DISPLAY_STAGE)
-    assert e.value.status == 6
+        ctx.resolve_visbuffer([], [], target, 8)          # no such display attribute
+    assert e.value.status == 1
     pos, idx, col = wl.sphere_mesh(rings=4, segments=8)
     host = orc.HostMesh(crx, pos, idx, col)
     g = GpuScene(ctx, SceneSpec().add(host, [crx.identity()]))

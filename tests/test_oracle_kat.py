@@ -590,3 +590,34 @@ def test_resolve_vertex_normals():
         N = N / np.linalg.norm(N)
         want = np.clip(N * 255.0, 0.0, 255.0)
         assert np.all(np.abs(rgba2[y, x, :3].astype(float) - want) <= 1.0 + 1e-9), ((x, y), rgba2[y, x], want)
+
+
+def test_resolve_stage_by_box_size_and_near_plane():
+    """STAGE (resolve.cu:974-1031): the triangle's screen box (ndcToScreen
+    without the half-pixel shift, so a stage-1 screen point s lands on
+    s + 0.5) clipped to the screen, ceil(max) - floor(min) per axis: at most
+    128 fragments -> SCHEME_SPECTRAL[9], at most 4096 -> [7], more -> [3];
+    a vertex at or behind the camera plane -> [3]."""
+    cases = [  # (W, legs from (10, 10) in stage-1 screen units, expected box side, colour)
+        (64, 4.0, 5, 0xFFBD8832),      # ceil(14.5) - floor(10.5) = 5: 25 fragments
+        (64, 10.0, 11, 0xFFBD8832),    # 121
+        (64, 11.0, 12, 0xFFA4DDAB),    # 144
+        (128, 63.0, 64, 0xFFA4DDAB),   # 4096
+        (128, 64.0, 65, 0xFF61AEFD),   # 4225
+    ]
+    for W, leg, side, want in cases:
+        assert side * side <= 128 or want != 0xFFBD8832
+        pos, idx = screen_triangle_scene([(10.0, 10.0), (10.0 + leg, 10.0), (10.0, 10.0 + leg)], W, W, depth=2.0)
+        fb, st, mesh, view, proj = render_one(pos, idx, W, W)
+        cb = orc.resolve(crx, [mesh.cmesh(crx)], [0], fb, W, W, view, proj, crx.DISPLAY_STAGE, threads=1)
+        cov = np.nonzero(fb != crx.CLEAR_PIXEL)[0]
+        assert cov.size > 0
+        assert np.all((cb[cov] & 0xFFFFFFFF) == want), (W, leg, hex(int(cb[cov[0]]) & 0xFFFFFFFF))
+    # a triangle reaching behind the camera: stage 2 whatever its box
+    W = 64
+    pos = np.array([[-0.5, -0.5, -2.0], [0.5, -0.5, -2.0], [0.0, 0.5, 1.0]], dtype=np.float32)
+    idx = np.array([[0, 1, 2]], dtype=np.uint32)
+    fb, st, mesh, view, proj = render_one(pos, idx, W, W)
+    cb = orc.resolve(crx, [mesh.cmesh(crx)], [0], fb, W, W, view, proj, crx.DISPLAY_STAGE, threads=1)
+    cov = np.nonzero(fb != crx.CLEAR_PIXEL)[0]
+    assert cov.size > 0 and np.all((cb[cov] & 0xFFFFFFFF) == 0xFF61AEFD)
