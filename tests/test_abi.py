@@ -257,3 +257,15 @@ def test_frustum_cull_hierarchy_equals_each_box_test():
     for v in _views_for_cull():
         alone = [i for i, b in enumerate(boxes) if crx.frustum_cull(v, [b]) == [0]]
         assert crx.frustum_cull(v, boxes) == alone
+
+
+def test_frustum_cull_non_finite_view_equals_each_box_test():
+    """A view matrix holding an infinity or a NaN gives non-finite planes;
+    the hierarchy then tests every box on its own, so the cull of the set
+    still equals the boxes kept when each is culled alone."""
+    boxes = _cull_boxes(3)
+    for k, bad in enumerate([float("inf"), float("-inf"), float("nan")]):
+        v = _views_for_cull()[k]
+        v.view.m[4 * k + 1] = bad
+        alone = [i for i, b in enumerate(boxes) if crx.frustum_cull(v, [b]) == [0]]
+        assert crx.frustum_cull(v, boxes) == alone
